@@ -1,0 +1,196 @@
+/*
+ * ash3d_tr.h -- C ABI of the MI355X transport step for Ash3d.
+ *
+ * Drop-in boundary for the reference's hot path.  The reference routines
+ * take no arguments and work on Fortran module arrays; a Fortran shim with
+ * the reference's module and routine names passes c_loc() of those arrays
+ * here once (ash3d_tr_create) and then calls the per-routine entry points.
+ * All arrays keep the reference's Fortran column-major layouts and lower
+ * bounds (src/Ash3d_VariableModules.f90:918-946):
+ *
+ *   concen_pd(-1:nx+2,-1:ny+2,-1:nz+2,1:ns,0:1)      real(8)
+ *   vx_pd, vy_pd, vz_pd, vh_pd (-1:nx+2,-1:ny+2,-1:nz+2)
+ *   vf_pd(-1:nx+2,-1:ny+2,-1:nz+2,1:ns)
+ *   kappa_pd, sigma_nx_pd, sigma_ny_pd, sigma_nz_pd (-1:nx+2,-1:ny+2,-1:nz+2)
+ *   kx, ky, kz (0:nx+1,0:ny+1,0:nz+1)                  src/Diffusion.F90:95-101
+ *   DelDxonD_cc, DelDyonD_cc (-1:nx+2,-1:ny+2)         src/Topography.f90:412-413
+ *   outflow_xz{1,2}_pd(-1:nx+2,-1:nz+2,1:ns)
+ *   outflow_yz{1,2}_pd(-1:ny+2,-1:nz+2,1:ns)
+ *   outflow_xy{1,2}_pd(-1:nx+2,-1:ny+2,1:ns)
+ *   DepositGranularity(1:nx,1:ny,1:ns)
+ *
+ * Units are the reference's: km, hours, km/h, km2/h, kg/km3.  IsAloft is a
+ * Fortran LOGICAL array, which is not interoperable; the shim passes it as
+ * an int32 0/1 mask of length ns.  Bounds are imin,imax,jmin,jmax,kmin,kmax
+ * (solution module, src/Ash3d_VariableModules.f90:885-887).
+ *
+ * Device residency: the context owns device mirrors of every array.  Between
+ * ash3d_tr_sync_from_host and ash3d_tr_sync_to_host the device copy is the
+ * authoritative one; the host must sync before it reads or writes any array
+ * (outputs, mass totals, source insertion, wind updates).  Calls on one
+ * context are asynchronous on the context's HIP stream and must not overlap
+ * (the reference calls these from one thread); several contexts may share a
+ * device.  Every entry point returns 0 on success, or an ASH3D_TR_E* code
+ * with a message available from ash3d_tr_last_error().
+ */
+#ifndef ASH3D_TR_H
+#define ASH3D_TR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASH3D_TR_ABI_VERSION 1
+
+enum {
+  ASH3D_TR_OK = 0,
+  ASH3D_TR_EINVAL = 1,    /* bad argument, bounds, shape or flag */
+  ASH3D_TR_EDEVICE = 2,   /* HIP runtime error */
+  ASH3D_TR_ENOMEM = 3,    /* device allocation failed */
+  ASH3D_TR_EBC = 4,       /* unknown Set_BC code (src/Set_BC.f90:119-124) */
+  ASH3D_TR_ESINGULAR = 5, /* a tridiagonal pivot was exactly zero (dgtsv INFO>0,
+                             ignored by the reference, reported here) */
+  ASH3D_TR_ENOLIB = 6     /* no usable HIP device */
+};
+
+/* Limiters, chosen at compile time in the reference (src/makefile:126). */
+enum {
+  ASH3D_LIM_NONE = 0,
+  ASH3D_LIM_LAXWEN = 1,
+  ASH3D_LIM_BW = 2,
+  ASH3D_LIM_FROMM = 3,
+  ASH3D_LIM_MINMOD = 4,
+  ASH3D_LIM_SUPERBEE = 5,
+  ASH3D_LIM_MC = 6
+};
+
+/* Array groups for the sync masks. */
+enum {
+  ASH3D_SYNC_CONCEN = 1u << 0,   /* concen_pd, both time levels */
+  ASH3D_SYNC_WINDS = 1u << 1,    /* vx_pd, vy_pd, vz_pd, vh_pd */
+  ASH3D_SYNC_VF = 1u << 2,       /* vf_pd */
+  ASH3D_SYNC_GEOMETRY = 1u << 3, /* kappa_pd, sigma_nx/ny/nz_pd */
+  ASH3D_SYNC_DIFFK = 1u << 4,    /* kx, ky, kz */
+  ASH3D_SYNC_TOPO = 1u << 5,     /* DelDxonD_cc, DelDyonD_cc */
+  ASH3D_SYNC_OUTFLOW = 1u << 6,  /* the six outflow_*_pd arrays */
+  ASH3D_SYNC_DEPOSIT = 1u << 7,  /* DepositGranularity */
+  ASH3D_SYNC_ALL = 0xffu
+};
+
+typedef struct ash3d_tr_desc {
+  int32_t abi_version; /* ASH3D_TR_ABI_VERSION */
+  int32_t nx, ny, nz, ns; /* nxmax, nymax, nzmax, nsmax */
+  int32_t is_periodic;    /* mesh::IsPeriodic */
+  int32_t zscaling_id;    /* mesh::ZScaling_ID: 2 = advect_sigma + topo terms */
+  int32_t limiter;        /* ASH3D_LIM_* (reference default: SUPERBEE) */
+  int32_t device;         /* HIP device ordinal */
+  /* Host arrays (may be NULL: the device copy then starts at zero, except
+   * the geometry, which starts at 1.0 as in Allocate_mesh). */
+  double *concen_pd;
+  double *vx_pd, *vy_pd, *vz_pd, *vh_pd, *vf_pd;
+  double *kappa_pd, *sigma_nx_pd, *sigma_ny_pd, *sigma_nz_pd;
+  double *kx, *ky, *kz;
+  double *deldxond_cc, *deldyond_cc;
+  double *outflow_xz1_pd, *outflow_xz2_pd;
+  double *outflow_yz1_pd, *outflow_yz2_pd;
+  double *outflow_xy1_pd, *outflow_xy2_pd;
+  double *deposit_granularity;
+} ash3d_tr_desc;
+
+typedef struct ash3d_tr_ctx ash3d_tr_ctx;
+
+/* Allocate device mirrors and upload every non-NULL host array.
+ * Replaces the allocation glue of src/Alloc_Arrays.f90:48-53 for the
+ * arrays above (the host arrays stay owned by the caller). */
+int ash3d_tr_create(const ash3d_tr_desc *desc, ash3d_tr_ctx **ctx);
+int ash3d_tr_destroy(ash3d_tr_ctx *ctx);
+
+/* Host <-> device copies of the array groups in mask (ASH3D_SYNC_*).
+ * sync_to_host blocks until the data is on the host. */
+int ash3d_tr_sync_from_host(ash3d_tr_ctx *ctx, uint32_t mask);
+int ash3d_tr_sync_to_host(ash3d_tr_ctx *ctx, uint32_t mask);
+/* Wait for all queued work of the context. */
+int ash3d_tr_synchronize(ash3d_tr_ctx *ctx);
+
+/* Set_BC(bc_code), src/Set_BC.f90:15-126. */
+int ash3d_tr_set_bc(ash3d_tr_ctx *ctx, int32_t bc_code);
+
+/* advect_x / advect_y, src/AdvectionHorz_DCU.f90:68-305 / :322-556, each
+ * starting with Set_BC(1) as the reference does.  dt in hours. */
+int ash3d_tr_advect_x(ash3d_tr_ctx *ctx, double dt, const int32_t *is_aloft,
+                      const int32_t bounds[6]);
+int ash3d_tr_advect_y(ash3d_tr_ctx *ctx, double dt, const int32_t *is_aloft,
+                      const int32_t bounds[6]);
+/* advect_z / advect_sigma incl. settling with vz+vh+vf(n), the deposit and
+ * DepositGranularity, src/AdvectionVert_DCU.f90:100-323, :341-564. */
+int ash3d_tr_advect_z(ash3d_tr_ctx *ctx, double dt, const int32_t *is_aloft,
+                      const int32_t bounds[6]);
+
+/* AdvectHorz(itoggle) with FAST_SUBGRID off, src/AdvectionHorz.f90:47-85:
+ * full-grid bounds, even itoggle = y then x, odd = x then y. */
+int ash3d_tr_advect_horz(ash3d_tr_ctx *ctx, int32_t itoggle, double dt,
+                         const int32_t *is_aloft);
+/* AdvectVert, src/AdvectionVert_DCU.f90:64-82 (full-grid bounds). */
+int ash3d_tr_advect_vert(ash3d_tr_ctx *ctx, double dt, const int32_t *is_aloft);
+
+/* One diffusion sweep along dir (0:x, 1:y, 2:z): use_cn != 0 gives
+ * diffCN_{x,y,z} with theta = imp_fac (src/Diffusion.F90:671-911,
+ * :932-1167, :1188-1428), use_cn == 0 gives the explicit diff_{x,y,z}
+ * (src/Diffusion.F90:266-388, :406-518, :536-650). */
+int ash3d_tr_diffuse_dir(ash3d_tr_ctx *ctx, int32_t dir, int32_t use_cn, double dt,
+                         double imp_fac, const int32_t *is_aloft, const int32_t bounds[6]);
+/* DiffuseHorz(itoggle) / DiffuseVert, src/Diffusion.F90:151-216 / :232-247,
+ * with FAST_SUBGRID off (full-grid bounds). */
+int ash3d_tr_diffuse_horz(ash3d_tr_ctx *ctx, int32_t itoggle, int32_t use_cn, double dt,
+                          double imp_fac, const int32_t *is_aloft);
+int ash3d_tr_diffuse_vert(ash3d_tr_ctx *ctx, int32_t use_cn, double dt, double imp_fac,
+                          const int32_t *is_aloft);
+
+/* The hot-path call sequence of one time step, src/Ash3d.F90:491-510:
+ * Set_BC(1); AdvectHorz(itime); AdvectVert; and if use_diffusion:
+ * Set_BC(2); DiffuseHorz(itime); DiffuseVert. */
+int ash3d_tr_step(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion,
+                  int32_t use_cn, double imp_fac, const int32_t *is_aloft);
+
+/* Step glue of the driver, src/Ash3d.F90:376-378:
+ * concen_pd(:,:,:,:,ts0) = concen_pd(:,:,:,:,ts1); concen_pd(...,ts1) = 0. */
+int ash3d_tr_step_glue(ash3d_tr_ctx *ctx);
+
+/* Source insertion on the vent column, src/Ash3d.F90:428-431:
+ * concen_pd(ivent,jvent,1:nz+1,1:ns,ts0) += dt * flux(1:nz+1,1:ns),
+ * flux = SourceNodeFlux in kg/km3/h, a host array (nz+1, ns). */
+int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, double dt,
+                               const double *flux);
+
+/* Kernel timing.  No counterpart in the reference, whose only timers are
+ * wall-clock reads around the whole run (src/Ash3d.F90:176-177, :727-732).
+ * ash3d_tr_profile(ctx,1) clears the records and brackets every later
+ * kernel launch of the context with HIP events on its stream;
+ * ash3d_tr_profile(ctx,0) stops recording.  ash3d_tr_profile_read sums the
+ * recorded GPU time (ms) and launch count of one kernel kind. */
+enum {
+  ASH3D_PROF_ADVECT_X = 0,
+  ASH3D_PROF_ADVECT_Y = 1,
+  ASH3D_PROF_ADVECT_Z = 2,
+  ASH3D_PROF_DIFFUSE_X = 3,
+  ASH3D_PROF_DIFFUSE_Y = 4,
+  ASH3D_PROF_DIFFUSE_Z = 5,
+  ASH3D_PROF_BC = 6,   /* Set_BC and per-direction ghost kernels */
+  ASH3D_PROF_GLUE = 7, /* step glue */
+  ASH3D_PROF_NKINDS = 8
+};
+int ash3d_tr_profile(ash3d_tr_ctx *ctx, int32_t enable);
+int ash3d_tr_profile_read(ash3d_tr_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
+
+/* Message of the last failed call on this thread. */
+const char *ash3d_tr_last_error(void);
+
+/* Library build description (compile flags, target). */
+const char *ash3d_tr_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

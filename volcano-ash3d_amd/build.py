@@ -1,0 +1,45 @@
+"""Build libash3d_tr.so (the gfx950 HIP library behind include/ash3d_tr.h).
+
+Run ``python volcano-ash3d_amd/build.py`` or call ``build()``.  The library
+is written next to this file so that it travels with the source tree.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libash3d_tr.so")
+SOURCES = ["ash3d_tr.hip", "bc.hip", "advect.hip", "diffuse.hip"]
+ARCH = os.environ.get("ASH3D_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: the kernels evaluate every expression with the
+# reference's rounding (no fused multiply-add), which makes the results
+# bitwise those of the Fortran routines.  Packed FP32 is disabled as a
+# matter of policy (the library is FP64 throughout).
+HIPFLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libash3d_tr.so")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

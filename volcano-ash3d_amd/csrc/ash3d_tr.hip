@@ -1,0 +1,615 @@
+// C ABI of the gfx950 transport step (declared in include/ash3d_tr.h).
+//
+// A context owns device mirrors of the reference's module arrays, one HIP
+// stream and the scratch the line solvers need.  Entry points enqueue
+// kernels on the stream and return; sync_to_host / synchronize wait.
+//
+// ts1 bookkeeping.  Every sweep of the reference zero-fills all of
+// concen_pd(...,ts1), writes the swept rows and copies the interior back to
+// ts0 (e.g. src/AdvectionHorz_DCU.f90:122, :285-287, :302-303), so after any
+// sweep ts1 equals ts0's interior with zero ghosts.  The kernels update ts0
+// in place and the context records that fact (TS1_MIRROR) instead of writing
+// ts1; ts1 is materialised only when the host asks for it.  The driver glue
+// ts0 <- ts1, ts1 <- 0 (src/Ash3d.F90:376-378) then only has to zero ts0's
+// ghost cells.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ash3d_tr.h"
+#include "common.hpp"
+
+using namespace ash3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ASH3D_TR_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                  __FILE__, __LINE__);                                               \
+  } while (0)
+
+enum { TS1_EXPLICIT = 0, TS1_ZERO = 1, TS1_MIRROR = 2 };
+
+}  // namespace
+
+struct ash3d_tr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Dims d{};
+  int periodic = 0, zscaling = 0, limiter = ASH3D_LIM_SUPERBEE;
+  ash3d_tr_desc host{};
+  // device arrays
+  double *c = nullptr;  // ts0 then ts1, ns*ncell each
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr, *vh = nullptr, *vf = nullptr;
+  double *kap = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  double *kx = nullptr, *ky = nullptr, *kz = nullptr;
+  double *ddx = nullptr, *ddy = nullptr;
+  double *ofxz1 = nullptr, *ofxz2 = nullptr, *ofyz1 = nullptr, *ofyz2 = nullptr;
+  double *ofxy1 = nullptr, *ofxy2 = nullptr;
+  double *dep = nullptr;
+  double *fac = nullptr;  // CN factor scratch
+  long long fac_groups = 0;
+  double *flux = nullptr;  // source column staging (nz+1)*ns
+  int *flag = nullptr;     // singular-pivot flag
+  int ts1_state = TS1_EXPLICIT;
+  int nb_adv = 4, nb_diff = 4;
+  // kernel timing (ash3d_tr_profile): event pairs per kernel kind
+  bool prof_on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_rec;  // (kind, pool index)
+
+  long long n_conc() const { return d.ncell * d.ns; }
+  long long n_k() const { return (long long)(d.nx + 2) * (d.ny + 2) * (d.nz + 2); }
+  long long n_2d() const { return (long long)(d.nx + 4) * (d.ny + 4); }
+  long long n_xz() const { return (long long)(d.nx + 4) * (d.nz + 4) * d.ns; }
+  long long n_yz() const { return (long long)(d.ny + 4) * (d.nz + 4) * d.ns; }
+  long long n_xy() const { return (long long)(d.nx + 4) * (d.ny + 4) * d.ns; }
+  long long n_dep() const { return (long long)d.nx * d.ny * d.ns; }
+  double *ts0() const { return c; }
+  double *ts1() const { return c + n_conc(); }
+};
+
+namespace {
+
+int dev_alloc(double **p, long long n) {
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(double) * (size_t)n);
+  if (e != hipSuccess) return fail(ASH3D_TR_ENOMEM, "hipMalloc of %lld doubles: %s", n, hipGetErrorString(e));
+  return 0;
+}
+
+int upload(ash3d_tr_ctx *x, double *dst, const double *src, long long n, double fill) {
+  if (src) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, x->stream));
+  } else {
+    HIPCHK(launch_fill(x->stream, dst, n, fill));
+  }
+  return 0;
+}
+
+int download(ash3d_tr_ctx *x, double *dst, const double *src, long long n) {
+  if (dst) HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, x->stream));
+  return 0;
+}
+
+int check_ctx(ash3d_tr_ctx *x) {
+  if (!x) return fail(ASH3D_TR_EINVAL, "null context");
+  HIPCHK(hipSetDevice(x->device));
+  return 0;
+}
+
+int make_mask(const ash3d_tr_ctx *x, const int32_t *is_aloft, AloftMask *m) {
+  std::memset(m, 0, sizeof *m);
+  for (int n = 0; n < x->d.ns; ++n)
+    if (!is_aloft || is_aloft[n]) m->w[n >> 5] |= (1u << (n & 31));
+  return 0;
+}
+
+int check_bounds(const ash3d_tr_ctx *x, const int32_t *b) {
+  const Dims &d = x->d;
+  if (!b) return fail(ASH3D_TR_EINVAL, "null bounds");
+  if (b[0] < 1 || b[1] > d.nx || b[2] < 1 || b[3] > d.ny || b[4] < 1 || b[5] > d.nz || b[0] > b[1] ||
+      b[2] > b[3] || b[4] > b[5])
+    return fail(ASH3D_TR_EINVAL, "bounds (%d:%d,%d:%d,%d:%d) outside the grid %dx%dx%d", b[0], b[1], b[2],
+                b[3], b[4], b[5], d.nx, d.ny, d.nz);
+  return 0;
+}
+
+void full_bounds(const ash3d_tr_ctx *x, int32_t *b) {
+  b[0] = 1; b[1] = x->d.nx; b[2] = 1; b[3] = x->d.ny; b[4] = 1; b[5] = x->d.nz;
+}
+
+// Event-bracketed launches for ash3d_tr_profile.
+struct ProfScope {
+  ash3d_tr_ctx *x;
+  size_t idx;
+  bool on;
+  ProfScope(ash3d_tr_ctx *x_, int kind) : x(x_), idx(0), on(x_->prof_on) {
+    if (!on) return;
+    if (x->ev_used == x->ev_pool.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { on = false; return; }
+      x->ev_pool.emplace_back(a, b);
+    }
+    idx = x->ev_used++;
+    x->ev_rec.emplace_back(kind, idx);
+    hipEventRecord(x->ev_pool[idx].first, x->stream);
+  }
+  ~ProfScope() {
+    if (on) hipEventRecord(x->ev_pool[idx].second, x->stream);
+  }
+};
+
+int set_bc1(ash3d_tr_ctx *x, bool velocities) {
+  ProfScope ps(x, ASH3D_PROF_BC);
+  if (velocities) HIPCHK(launch_set_bc1_vel(x->stream, x->d, x->periodic, x->vx, x->vy, x->vz, x->vf));
+  HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, x->ts0()));
+  return 0;
+}
+
+int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
+                 bool with_bc, bool bc_vel) {
+  int rc = check_bounds(x, bounds);
+  if (rc) return rc;
+  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
+  if (with_bc && (rc = set_bc1(x, bc_vel))) return rc;
+  AdvArgs a{};
+  a.d = x->d;
+  a.g = make_line_geom(x->d, dir, bounds);
+  a.c0 = x->ts0();
+  a.dt = dt;
+  a.periodic = x->periodic;
+  a.limiter = x->limiter;
+  make_mask(x, is_aloft, &a.aloft);
+  a.kap = x->kap;
+  if (dir == 0) {
+    a.vel = x->vx; a.sig = x->sx; a.of1 = x->ofyz1; a.of2 = x->ofyz2;
+    a.deld = (x->zscaling == 2) ? x->ddx : nullptr;
+  } else if (dir == 1) {
+    a.vel = x->vy; a.sig = x->sy; a.of1 = x->ofxz1; a.of2 = x->ofxz2;
+    a.deld = (x->zscaling == 2) ? x->ddy : nullptr;
+  } else {
+    a.vel = x->vz; a.vh = x->vh; a.vf = x->vf; a.sig = x->sz; a.of1 = x->ofxy1; a.of2 = x->ofxy2;
+    a.dep = x->dep;
+  }
+  {
+    ProfScope ps(x, ASH3D_PROF_ADVECT_X + dir);
+    HIPCHK(launch_advect(x->stream, a, x->nb_adv));
+  }
+  x->ts1_state = TS1_MIRROR;
+  return 0;
+}
+
+int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fac, const int32_t *is_aloft,
+                  const int32_t *bounds) {
+  int rc = check_bounds(x, bounds);
+  if (rc) return rc;
+  if (dir < 0 || dir > 2) return fail(ASH3D_TR_EINVAL, "dir must be 0, 1 or 2 (got %d)", dir);
+  // the routine's own Neumann ghosts of its direction (incl. periodic x)
+  {
+    ProfScope ps(x, ASH3D_PROF_BC);
+    HIPCHK(launch_neumann_dir(x->stream, x->d, dir, x->periodic, x->ts0()));
+  }
+  DiffArgs a{};
+  a.d = x->d;
+  a.g = make_line_geom(x->d, dir, bounds);
+  a.c0 = x->ts0();
+  a.scratch_b = x->ts1();
+  a.dt = dt;
+  a.theta = imp_fac;
+  a.flag = x->flag;
+  make_mask(x, is_aloft, &a.aloft);
+  a.kap = x->kap;
+  a.kk = dir == 0 ? x->kx : (dir == 1 ? x->ky : x->kz);
+  a.sig = dir == 0 ? x->sx : (dir == 1 ? x->sy : x->sz);
+  if (use_cn) {
+    if (a.g.rmax - a.g.rmin + 1 <= 1) {
+      // ncells <= 1: the reference zero-fills ts1 and copies nothing (:770, :909)
+      x->ts1_state = TS1_ZERO;
+      return 0;
+    }
+    const long long groups = (x->d.ns + x->nb_diff - 1) / x->nb_diff;
+    if (groups > x->fac_groups) {
+      if (x->fac) HIPCHK(hipFree(x->fac));
+      x->fac = nullptr;
+      if ((rc = dev_alloc(&x->fac, 3 * x->d.ncell * groups))) return rc;
+      x->fac_groups = groups;
+    }
+    a.fac = x->fac;
+    ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
+    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff));
+  } else {
+    ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
+    HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff));
+  }
+  x->ts1_state = TS1_MIRROR;
+  return 0;
+}
+
+int check_flag(ash3d_tr_ctx *x) {
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, x->flag, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (h) {
+    int z = 0;
+    HIPCHK(hipMemcpy(x->flag, &z, sizeof(int), hipMemcpyHostToDevice));
+    return fail(ASH3D_TR_ESINGULAR, "tridiagonal solve hit a zero pivot (dgtsv INFO=%d)", h);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ash3d_tr_last_error(void) { return g_err.c_str(); }
+
+const char *ash3d_tr_build_info(void) {
+  return "ash3d_tr: HIP, gfx950, FP64, -O3 -ffp-contract=off, ABI " "1";
+}
+
+int ash3d_tr_create(const ash3d_tr_desc *desc, ash3d_tr_ctx **out) {
+  if (!desc || !out) return fail(ASH3D_TR_EINVAL, "null descriptor or output pointer");
+  *out = nullptr;
+  if (desc->abi_version != ASH3D_TR_ABI_VERSION)
+    return fail(ASH3D_TR_EINVAL, "ABI version %d, library has %d", desc->abi_version, ASH3D_TR_ABI_VERSION);
+  if (desc->nx < 2 || desc->ny < 1 || desc->nz < 1 || desc->ns < 1)
+    return fail(ASH3D_TR_EINVAL, "bad grid %dx%dx%d with %d bins", desc->nx, desc->ny, desc->nz, desc->ns);
+  if (desc->ns > kMaxBins) return fail(ASH3D_TR_EINVAL, "at most %d grain bins (got %d)", kMaxBins, desc->ns);
+  if (desc->limiter < 0 || desc->limiter > 6) return fail(ASH3D_TR_EINVAL, "unknown limiter %d", desc->limiter);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ASH3D_TR_ENOLIB, "no HIP device");
+  if (desc->device < 0 || desc->device >= ndev) return fail(ASH3D_TR_EINVAL, "device %d of %d", desc->device, ndev);
+  auto *x = new ash3d_tr_ctx();
+  x->device = desc->device;
+  x->host = *desc;
+  x->d = make_dims(desc->nx, desc->ny, desc->nz, desc->ns);
+  x->periodic = desc->is_periodic ? 1 : 0;
+  x->zscaling = desc->zscaling_id;
+  x->limiter = desc->limiter;
+  x->nb_adv = pick_bins_per_thread(desc->ns);
+  x->nb_diff = pick_bins_per_thread(desc->ns);
+  int rc = 0;
+  if (hipSetDevice(x->device) != hipSuccess || hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete x;
+    return fail(ASH3D_TR_EDEVICE, "cannot create a stream on device %d", desc->device);
+  }
+  const Dims &d = x->d;
+  if ((rc = dev_alloc(&x->c, 2 * x->n_conc())) || (rc = dev_alloc(&x->vx, d.ncell)) ||
+      (rc = dev_alloc(&x->vy, d.ncell)) || (rc = dev_alloc(&x->vz, d.ncell)) || (rc = dev_alloc(&x->vh, d.ncell)) ||
+      (rc = dev_alloc(&x->vf, x->n_conc())) || (rc = dev_alloc(&x->kap, d.ncell)) ||
+      (rc = dev_alloc(&x->sx, d.ncell)) || (rc = dev_alloc(&x->sy, d.ncell)) || (rc = dev_alloc(&x->sz, d.ncell)) ||
+      (rc = dev_alloc(&x->kx, x->n_k())) || (rc = dev_alloc(&x->ky, x->n_k())) || (rc = dev_alloc(&x->kz, x->n_k())) ||
+      (rc = dev_alloc(&x->ddx, x->n_2d())) || (rc = dev_alloc(&x->ddy, x->n_2d())) ||
+      (rc = dev_alloc(&x->ofxz1, x->n_xz())) || (rc = dev_alloc(&x->ofxz2, x->n_xz())) ||
+      (rc = dev_alloc(&x->ofyz1, x->n_yz())) || (rc = dev_alloc(&x->ofyz2, x->n_yz())) ||
+      (rc = dev_alloc(&x->ofxy1, x->n_xy())) || (rc = dev_alloc(&x->ofxy2, x->n_xy())) ||
+      (rc = dev_alloc(&x->dep, x->n_dep())) || (rc = dev_alloc(&x->flux, (long long)(d.nz + 1) * d.ns))) {
+    std::string msg = g_err;
+    ash3d_tr_destroy(x);
+    g_err = msg;
+    return rc;
+  }
+  if (hipMalloc((void **)&x->flag, sizeof(int)) != hipSuccess || hipMemset(x->flag, 0, sizeof(int)) != hipSuccess) {
+    ash3d_tr_destroy(x);
+    return fail(ASH3D_TR_ENOMEM, "flag allocation failed");
+  }
+  rc = ash3d_tr_sync_from_host(x, ASH3D_SYNC_ALL);
+  if (rc == 0) {
+    hipError_t e = hipStreamSynchronize(x->stream);
+    if (e != hipSuccess) rc = fail(ASH3D_TR_EDEVICE, "upload: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    std::string msg = g_err;
+    ash3d_tr_destroy(x);
+    g_err = msg;
+    return rc;
+  }
+  *out = x;
+  return 0;
+}
+
+int ash3d_tr_destroy(ash3d_tr_ctx *x) {
+  if (!x) return 0;
+  hipSetDevice(x->device);
+  if (x->stream) hipStreamSynchronize(x->stream);
+  double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
+                    x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
+                    x->fac, x->flux};
+  for (double *p : ptrs)
+    if (p) hipFree(p);
+  if (x->flag) hipFree(x->flag);
+  for (auto &e : x->ev_pool) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  if (x->stream) hipStreamDestroy(x->stream);
+  delete x;
+  return 0;
+}
+
+int ash3d_tr_sync_from_host(ash3d_tr_ctx *x, uint32_t mask) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const ash3d_tr_desc &h = x->host;
+  const Dims &d = x->d;
+  if (mask & ASH3D_SYNC_CONCEN) {
+    if ((rc = upload(x, x->c, h.concen_pd, 2 * x->n_conc(), 0.0))) return rc;
+    x->ts1_state = TS1_EXPLICIT;
+  }
+  if (mask & ASH3D_SYNC_WINDS) {
+    if ((rc = upload(x, x->vx, h.vx_pd, d.ncell, 0.0)) || (rc = upload(x, x->vy, h.vy_pd, d.ncell, 0.0)) ||
+        (rc = upload(x, x->vz, h.vz_pd, d.ncell, 0.0)) || (rc = upload(x, x->vh, h.vh_pd, d.ncell, 0.0)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_VF)
+    if ((rc = upload(x, x->vf, h.vf_pd, x->n_conc(), 0.0))) return rc;
+  if (mask & ASH3D_SYNC_GEOMETRY) {
+    if ((rc = upload(x, x->kap, h.kappa_pd, d.ncell, 1.0)) || (rc = upload(x, x->sx, h.sigma_nx_pd, d.ncell, 1.0)) ||
+        (rc = upload(x, x->sy, h.sigma_ny_pd, d.ncell, 1.0)) || (rc = upload(x, x->sz, h.sigma_nz_pd, d.ncell, 1.0)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_DIFFK) {
+    if ((rc = upload(x, x->kx, h.kx, x->n_k(), 0.0)) || (rc = upload(x, x->ky, h.ky, x->n_k(), 0.0)) ||
+        (rc = upload(x, x->kz, h.kz, x->n_k(), 0.0)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_TOPO) {
+    if ((rc = upload(x, x->ddx, h.deldxond_cc, x->n_2d(), 0.0)) || (rc = upload(x, x->ddy, h.deldyond_cc, x->n_2d(), 0.0)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_OUTFLOW) {
+    if ((rc = upload(x, x->ofxz1, h.outflow_xz1_pd, x->n_xz(), 0.0)) ||
+        (rc = upload(x, x->ofxz2, h.outflow_xz2_pd, x->n_xz(), 0.0)) ||
+        (rc = upload(x, x->ofyz1, h.outflow_yz1_pd, x->n_yz(), 0.0)) ||
+        (rc = upload(x, x->ofyz2, h.outflow_yz2_pd, x->n_yz(), 0.0)) ||
+        (rc = upload(x, x->ofxy1, h.outflow_xy1_pd, x->n_xy(), 0.0)) ||
+        (rc = upload(x, x->ofxy2, h.outflow_xy2_pd, x->n_xy(), 0.0)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_DEPOSIT)
+    if ((rc = upload(x, x->dep, h.deposit_granularity, x->n_dep(), 0.0))) return rc;
+  // pageable host memory: make the copies complete before the caller may
+  // reuse its buffers
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int ash3d_tr_sync_to_host(ash3d_tr_ctx *x, uint32_t mask) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const ash3d_tr_desc &h = x->host;
+  const Dims &d = x->d;
+  if (mask & ASH3D_SYNC_CONCEN) {
+    if (x->ts1_state == TS1_MIRROR) {
+      HIPCHK(launch_mirror_interior(x->stream, d, x->ts0(), x->ts1()));
+      x->ts1_state = TS1_EXPLICIT;
+    } else if (x->ts1_state == TS1_ZERO) {
+      HIPCHK(launch_fill(x->stream, x->ts1(), x->n_conc(), 0.0));
+      x->ts1_state = TS1_EXPLICIT;
+    }
+    if ((rc = download(x, h.concen_pd, x->c, 2 * x->n_conc()))) return rc;
+  }
+  if (mask & ASH3D_SYNC_WINDS) {
+    if ((rc = download(x, h.vx_pd, x->vx, d.ncell)) || (rc = download(x, h.vy_pd, x->vy, d.ncell)) ||
+        (rc = download(x, h.vz_pd, x->vz, d.ncell)) || (rc = download(x, h.vh_pd, x->vh, d.ncell)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_VF)
+    if ((rc = download(x, h.vf_pd, x->vf, x->n_conc()))) return rc;
+  if (mask & ASH3D_SYNC_GEOMETRY) {
+    if ((rc = download(x, h.kappa_pd, x->kap, d.ncell)) || (rc = download(x, h.sigma_nx_pd, x->sx, d.ncell)) ||
+        (rc = download(x, h.sigma_ny_pd, x->sy, d.ncell)) || (rc = download(x, h.sigma_nz_pd, x->sz, d.ncell)))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_DIFFK) {
+    if ((rc = download(x, h.kx, x->kx, x->n_k())) || (rc = download(x, h.ky, x->ky, x->n_k())) ||
+        (rc = download(x, h.kz, x->kz, x->n_k())))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_TOPO) {
+    if ((rc = download(x, h.deldxond_cc, x->ddx, x->n_2d())) || (rc = download(x, h.deldyond_cc, x->ddy, x->n_2d())))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_OUTFLOW) {
+    if ((rc = download(x, h.outflow_xz1_pd, x->ofxz1, x->n_xz())) ||
+        (rc = download(x, h.outflow_xz2_pd, x->ofxz2, x->n_xz())) ||
+        (rc = download(x, h.outflow_yz1_pd, x->ofyz1, x->n_yz())) ||
+        (rc = download(x, h.outflow_yz2_pd, x->ofyz2, x->n_yz())) ||
+        (rc = download(x, h.outflow_xy1_pd, x->ofxy1, x->n_xy())) ||
+        (rc = download(x, h.outflow_xy2_pd, x->ofxy2, x->n_xy())))
+      return rc;
+  }
+  if (mask & ASH3D_SYNC_DEPOSIT)
+    if ((rc = download(x, h.deposit_granularity, x->dep, x->n_dep()))) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return check_flag(x);
+}
+
+int ash3d_tr_synchronize(ash3d_tr_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return check_flag(x);
+}
+
+int ash3d_tr_set_bc(ash3d_tr_ctx *x, int32_t bc_code) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (bc_code == 1) return set_bc1(x, true);
+  if (bc_code == 2) {
+    HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
+    return 0;
+  }
+  return fail(ASH3D_TR_EBC, "ERROR BC code not recognized (%d)", bc_code);
+}
+
+int ash3d_tr_advect_x(ash3d_tr_ctx *x, double dt, const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  return rc ? rc : sweep_advect(x, 0, dt, is_aloft, bounds, true, true);
+}
+
+int ash3d_tr_advect_y(ash3d_tr_ctx *x, double dt, const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  return rc ? rc : sweep_advect(x, 1, dt, is_aloft, bounds, true, true);
+}
+
+int ash3d_tr_advect_z(ash3d_tr_ctx *x, double dt, const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  return rc ? rc : sweep_advect(x, 2, dt, is_aloft, bounds, true, true);
+}
+
+int ash3d_tr_advect_horz(ash3d_tr_ctx *x, int32_t itoggle, double dt, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  const int first = (itoggle % 2 == 0) ? 1 : 0;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, true, true))) return rc;
+  return sweep_advect(x, 1 - first, dt, is_aloft, b, true, false);
+}
+
+int ash3d_tr_advect_vert(ash3d_tr_ctx *x, double dt, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  return sweep_advect(x, 2, dt, is_aloft, b, true, true);
+}
+
+int ash3d_tr_diffuse_dir(ash3d_tr_ctx *x, int32_t dir, int32_t use_cn, double dt, double imp_fac,
+                         const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  return rc ? rc : sweep_diffuse(x, dir, use_cn, dt, imp_fac, is_aloft, bounds);
+}
+
+int ash3d_tr_diffuse_horz(ash3d_tr_ctx *x, int32_t itoggle, int32_t use_cn, double dt, double imp_fac,
+                          const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  const int first = (itoggle % 2 == 0) ? 0 : 1;  // even: x then y (:187-199)
+  if ((rc = sweep_diffuse(x, first, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+  return sweep_diffuse(x, 1 - first, use_cn, dt, imp_fac, is_aloft, b);
+}
+
+int ash3d_tr_diffuse_vert(ash3d_tr_ctx *x, int32_t use_cn, double dt, double imp_fac, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  return sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b);
+}
+
+int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                  double imp_fac, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  // Set_BC(1) (src/Ash3d.F90:491), then AdvectHorz / AdvectVert.  Their own
+  // Set_BC(1) calls repeat velocity ghosts that cannot have changed, and
+  // re-zero concentration ghosts that the sweeps never write; only the
+  // periodic x copies change, so only a periodic grid repeats the pass.
+  if ((rc = set_bc1(x, true))) return rc;
+  const int first = (itime % 2 == 0) ? 1 : 0;
+  const bool again = x->periodic != 0;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false))) return rc;
+  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false))) return rc;
+  if (use_diffusion) {
+    {
+      ProfScope ps(x, ASH3D_PROF_BC);
+      HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
+    }
+    const int dfirst = (itime % 2 == 0) ? 0 : 1;
+    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+    if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+    if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+  }
+  return 0;
+}
+
+int ash3d_tr_step_glue(ash3d_tr_ctx *x) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  ProfScope ps(x, ASH3D_PROF_GLUE);
+  if (x->ts1_state == TS1_MIRROR) {
+    HIPCHK(launch_zero_ghosts(x->stream, x->d, x->ts0()));
+  } else if (x->ts1_state == TS1_ZERO) {
+    HIPCHK(launch_fill(x->stream, x->ts0(), x->n_conc(), 0.0));
+  } else {
+    HIPCHK(hipMemcpyAsync(x->ts0(), x->ts1(), sizeof(double) * (size_t)x->n_conc(), hipMemcpyDeviceToDevice,
+                          x->stream));
+  }
+  x->ts1_state = TS1_ZERO;
+  return 0;
+}
+
+int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, double dt, const double *flux) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!flux) return fail(ASH3D_TR_EINVAL, "null flux");
+  if (ivent < 1 || ivent > x->d.nx || jvent < 1 || jvent > x->d.ny)
+    return fail(ASH3D_TR_EINVAL, "vent (%d,%d) outside the grid", ivent, jvent);
+  HIPCHK(hipMemcpyAsync(x->flux, flux, sizeof(double) * (size_t)(x->d.nz + 1) * x->d.ns, hipMemcpyHostToDevice,
+                        x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(launch_source_add(x->stream, x->d, x->ts0(), ivent, jvent, dt, x->flux));
+  return 0;
+}
+
+int ash3d_tr_profile(ash3d_tr_ctx *x, int32_t enable) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->prof_on = enable != 0;
+  if (enable) {
+    x->ev_used = 0;
+    x->ev_rec.clear();
+  }
+  return 0;
+}
+
+int ash3d_tr_profile_read(ash3d_tr_ctx *x, int32_t kind, double *total_ms, int64_t *launches) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!total_ms || !launches) return fail(ASH3D_TR_EINVAL, "null output");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  double t = 0.0;
+  int64_t n = 0;
+  for (auto &r : x->ev_rec) {
+    if (r.first != kind) continue;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, x->ev_pool[r.second].first, x->ev_pool[r.second].second));
+    t += ms;
+    ++n;
+  }
+  *total_ms = t;
+  *launches = n;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Ghost-cell kernels: Set_BC (src/Set_BC.f90:15-126), the per-direction
+// Neumann ghosts the diffusion routines set (src/Diffusion.F90:317-327,
+// :457-460, :587-590, :756-768, :1018-1021, :1277-1280), the driver's step
+// glue (src/Ash3d.F90:376-378) and the vent-column source add (:428-431).
+// All of these touch O(surface) cells; every thread writes one cell and
+// reads only interior cells, so no ordering between threads is needed.
+#include "common.hpp"
+
+namespace ash3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// The t-th ghost cell of plane k.  In the four ghost planes every cell is a
+// ghost; in an interior plane the ghosts are the ring of 4 rows of nx+4
+// cells (j = -1,0,ny+1,ny+2) and 4 columns of ny cells (i = -1,0,nx+1,nx+2).
+__device__ inline bool ghost_cell(const Dims &d, int k, long long t, int &i, int &j) {
+  const int W = d.nx + 4;
+  const bool gplane = (k < 1 || k > d.nz);
+  if (gplane) {
+    if (t >= (long long)W * (d.ny + 4)) return false;
+    i = (int)(t % W) - 1;
+    j = (int)(t / W) - 1;
+    return true;
+  }
+  const long long rowcells = 4LL * W;
+  if (t < rowcells) {
+    const int r = (int)(t / W);
+    i = (int)(t % W) - 1;
+    j = (r < 2) ? (r - 1) : (d.ny + (r - 1));  // -1, 0, ny+1, ny+2
+    return true;
+  }
+  t -= rowcells;
+  if (t >= 4LL * d.ny) return false;
+  const int col = (int)(t / d.ny);
+  j = (int)(t % d.ny) + 1;
+  i = (col < 2) ? (col - 1) : (d.nx + (col - 1));
+  return true;
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// x index of the Neumann / periodic source of ghost column i (interior i maps to itself)
+__device__ inline int xmap(int i, int nx, int periodic) {
+  if (i >= 1 && i <= nx) return i;
+  if (periodic) return i < 1 ? i + nx : i - nx;  // -1->nx-1, 0->nx, nx+1->1, nx+2->2
+  return i < 1 ? 1 : nx;
+}
+
+enum { GH_ZERO = 0, GH_BC1 = 1, GH_BC2 = 2 };
+
+// One thread per ghost cell of plane blockIdx.y-1 (k) of bin blockIdx.z.
+// mode GH_ZERO: 0; GH_BC1: Set_BC(1) concentration rule (:77-91); GH_BC2:
+// Set_BC(2) (:98-118) -- the sequential x (periodic override), y, z mirrors
+// leave every ghost equal to c(xmap(i), clamp(j), clamp(k)).
+__global__ __launch_bounds__(kBlock) void k_ghosts(Dims d, int periodic, int mode, double *c0,
+                                                  int kplane0) {
+  const int k = (int)blockIdx.y + kplane0;
+  const int n = (int)blockIdx.z;
+  int i, j;
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (!ghost_cell(d, k, t, i, j)) return;
+  double *c = c0 + d.ncell * n;
+  double v = 0.0;
+  if (mode == GH_BC1) {
+    if (periodic && j >= 1 && j <= d.ny && k >= 1 && k <= d.nz && (i < 1 || i > d.nx))
+      v = c[d.p3(xmap(i, d.nx, 1), j, k)];
+  } else if (mode == GH_BC2) {
+    v = c[d.p3(xmap(i, d.nx, periodic), clampi(j, 1, d.ny), clampi(k, 1, d.nz))];
+  }
+  c[d.p3(i, j, k)] = v;
+}
+
+hipError_t launch_ghosts(hipStream_t s, const Dims &d, int periodic, int mode, double *c0) {
+  // ghost planes k = -1,0 and nz+1,nz+2: full planes
+  const long long full = (long long)(d.nx + 4) * (d.ny + 4);
+  const long long ring = 4LL * (d.nx + 4) + 4LL * d.ny;
+  dim3 gfull((unsigned)((full + kBlock - 1) / kBlock), 2, d.ns);
+  hipLaunchKernelGGL(k_ghosts, gfull, dim3(kBlock), 0, s, d, periodic, mode, c0, -1);
+  hipLaunchKernelGGL(k_ghosts, gfull, dim3(kBlock), 0, s, d, periodic, mode, c0, d.nz + 1);
+  dim3 gring((unsigned)((ring + kBlock - 1) / kBlock), d.nz, d.ns);
+  hipLaunchKernelGGL(k_ghosts, gring, dim3(kBlock), 0, s, d, periodic, mode, c0, 1);
+  return hipGetLastError();
+}
+
+// Set_BC(1) velocities (:44-70): zero-order extrapolation into the ghost
+// layers of vx (x, periodic wrap), vy (y), vz and every vf bin (z).
+__global__ __launch_bounds__(kBlock) void k_vel_x(Dims d, int periodic, double *vx) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long nl = (long long)(d.ny + 4) * (d.nz + 4);
+  if (t >= nl) return;
+  const int j = (int)(t % (d.ny + 4)) - 1, k = (int)(t / (d.ny + 4)) - 1;
+  const int nx = d.nx;
+  if (periodic) {
+    vx[d.p3(-1, j, k)] = vx[d.p3(nx - 1, j, k)];
+    vx[d.p3(0, j, k)] = vx[d.p3(nx, j, k)];
+    vx[d.p3(nx + 1, j, k)] = vx[d.p3(1, j, k)];
+    vx[d.p3(nx + 2, j, k)] = vx[d.p3(2, j, k)];
+  } else {
+    vx[d.p3(-1, j, k)] = vx[d.p3(1, j, k)];
+    vx[d.p3(0, j, k)] = vx[d.p3(1, j, k)];
+    vx[d.p3(nx + 1, j, k)] = vx[d.p3(nx, j, k)];
+    vx[d.p3(nx + 2, j, k)] = vx[d.p3(nx, j, k)];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vel_y(Dims d, double *vy) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long nl = (long long)(d.nx + 4) * (d.nz + 4);
+  if (t >= nl) return;
+  const int i = (int)(t % (d.nx + 4)) - 1, k = (int)(t / (d.nx + 4)) - 1;
+  const int ny = d.ny;
+  vy[d.p3(i, -1, k)] = vy[d.p3(i, 1, k)];
+  vy[d.p3(i, 0, k)] = vy[d.p3(i, 1, k)];
+  vy[d.p3(i, ny + 1, k)] = vy[d.p3(i, ny, k)];
+  vy[d.p3(i, ny + 2, k)] = vy[d.p3(i, ny, k)];
+}
+
+// vz when blockIdx.y == 0, vf bin blockIdx.y-1 otherwise
+__global__ __launch_bounds__(kBlock) void k_vel_z(Dims d, double *vz, double *vf) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long nl = (long long)(d.nx + 4) * (d.ny + 4);
+  if (t >= nl) return;
+  const int i = (int)(t % (d.nx + 4)) - 1, j = (int)(t / (d.nx + 4)) - 1;
+  double *v = blockIdx.y == 0 ? vz : vf + d.ncell * (long long)(blockIdx.y - 1);
+  const int nz = d.nz;
+  v[d.p3(i, j, -1)] = v[d.p3(i, j, 1)];
+  v[d.p3(i, j, 0)] = v[d.p3(i, j, 1)];
+  v[d.p3(i, j, nz + 1)] = v[d.p3(i, j, nz)];
+  v[d.p3(i, j, nz + 2)] = v[d.p3(i, j, nz)];
+}
+
+// Neumann ghosts along one direction for all bins of ts0, all transverse
+// indices including ghosts; x applies the periodic override.
+__global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodic, double *c0) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int n = (int)blockIdx.y;
+  int A, B, N;
+  if (dir == 0) { A = d.ny + 4; B = d.nz + 4; N = d.nx; }
+  else if (dir == 1) { A = d.nx + 4; B = d.nz + 4; N = d.ny; }
+  else { A = d.nx + 4; B = d.ny + 4; N = d.nz; }
+  if (t >= (long long)A * B) return;
+  const int a = (int)(t % A) - 1, b = (int)(t / A) - 1;
+  double *c = c0 + d.ncell * n;
+  const long long base = line_base(d, dir, a, b), st = line_stride(d, dir);
+  const int p = (dir == 0) ? periodic : 0;
+  const int g[4] = {-1, 0, N + 1, N + 2};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    int src;
+    if (p) src = g[u] < 1 ? g[u] + N : g[u] - N;
+    else src = g[u] < 1 ? 1 : N;
+    c[base + st * (g[u] + 1)] = c[base + st * (src + 1)];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double v) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long q = t; q < n; q += stride) p[q] = v;
+}
+
+// ts1 = interior(ts0), zero ghosts: the content the reference leaves in ts1
+// after any sweep (zero fill, interior rows written, copied to ts0).
+__global__ __launch_bounds__(kBlock) void k_mirror(Dims d, const double *c0, double *c1) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long tot = d.ncell * d.ns;
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long q = t; q < tot; q += stride) {
+    const long long cell = q % d.ncell;
+    const int i = (int)(cell % d.sy) - 1;
+    const int j = (int)((cell / d.sy) % (d.ny + 4)) - 1;
+    const int k = (int)(cell / d.sz) - 1;
+    const bool in = i >= 1 && i <= d.nx && j >= 1 && j <= d.ny && k >= 1 && k <= d.nz;
+    c1[q] = in ? c0[q] : 0.0;
+  }
+}
+
+// concen(iv,jv,k,n,ts0) += dt*flux(k,n), k = 1..nz+1 (src/Ash3d.F90:428-431)
+__global__ __launch_bounds__(kBlock) void k_source(Dims d, double *c0, int iv, int jv, double dt,
+                                                  const double *flux) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long tot = (long long)(d.nz + 1) * d.ns;
+  if (t >= tot) return;
+  const int k = (int)(t % (d.nz + 1)) + 1, n = (int)(t / (d.nz + 1));
+  const long long o = d.p3(iv, jv, k) + d.ncell * n;
+  c0[o] = c0[o] + dt * flux[t];
+}
+
+unsigned blocks_for(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > 65535LL * 16) b = 65535LL * 16;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double *vx, double *vy,
+                              double *vz, double *vf) {
+  const long long nx_l = (long long)(d.ny + 4) * (d.nz + 4);
+  const long long ny_l = (long long)(d.nx + 4) * (d.nz + 4);
+  const long long nz_l = (long long)(d.nx + 4) * (d.ny + 4);
+  hipLaunchKernelGGL(k_vel_x, dim3(blocks_for(nx_l)), dim3(kBlock), 0, s, d, periodic, vx);
+  hipLaunchKernelGGL(k_vel_y, dim3(blocks_for(ny_l)), dim3(kBlock), 0, s, d, vy);
+  hipLaunchKernelGGL(k_vel_z, dim3(blocks_for(nz_l), 1 + d.ns), dim3(kBlock), 0, s, d, vz, vf);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_bc1_conc(hipStream_t s, const Dims &d, int periodic, double *c0) {
+  return launch_ghosts(s, d, periodic, GH_BC1, c0);
+}
+
+hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0) {
+  return launch_ghosts(s, d, periodic, GH_BC2, c0);
+}
+
+hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0) {
+  return launch_ghosts(s, d, 0, GH_ZERO, c0);
+}
+
+hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0) {
+  long long nl;
+  if (dir == 0) nl = (long long)(d.ny + 4) * (d.nz + 4);
+  else if (dir == 1) nl = (long long)(d.nx + 4) * (d.nz + 4);
+  else nl = (long long)(d.nx + 4) * (d.ny + 4);
+  hipLaunchKernelGGL(k_neumann, dim3(blocks_for(nl), d.ns), dim3(kBlock), 0, s, d, dir, periodic, c0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, double *p, long long n, double v) {
+  if (n <= 0) return hipSuccess;
+  unsigned b = blocks_for(n);
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_fill, dim3(b), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_mirror_interior(hipStream_t s, const Dims &d, const double *c0, double *c1) {
+  unsigned b = blocks_for(d.ncell * d.ns);
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_mirror, dim3(b), dim3(kBlock), 0, s, d, c0, c1);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_add(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt,
+                             const double *flux) {
+  const long long tot = (long long)(d.nz + 1) * d.ns;
+  hipLaunchKernelGGL(k_source, dim3(blocks_for(tot)), dim3(kBlock), 0, s, d, c0, iv, jv, dt, flux);
+  return hipGetLastError();
+}
+
+}  // namespace ash3d

@@ -1,0 +1,128 @@
+// Shared definitions of the gfx950 transport-step library: padded-grid
+// indexing in the reference's Fortran layout, kernel argument blocks and the
+// internal launchers the C ABI (ash3d_tr.hip) drives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ash3d {
+
+// concen_pd(-1:nx+2,-1:ny+2,-1:nz+2,1:ns,0:1) and the other padded arrays
+// (src/Ash3d_VariableModules.f90:918-946): x fastest, lower bound -1.
+struct Dims {
+  int nx, ny, nz, ns;
+  long long sy;     // nx+4
+  long long sz;     // (nx+4)*(ny+4)
+  long long ncell;  // (nx+4)*(ny+4)*(nz+4)
+  __host__ __device__ inline long long p3(int i, int j, int k) const {
+    return (long long)(i + 1) + sy * (long long)(j + 1) + sz * (long long)(k + 1);
+  }
+};
+
+inline Dims make_dims(int nx, int ny, int nz, int ns) {
+  Dims d;
+  d.nx = nx; d.ny = ny; d.nz = nz; d.ns = ns;
+  d.sy = nx + 4;
+  d.sz = (long long)(nx + 4) * (ny + 4);
+  d.ncell = d.sz * (nz + 4);
+  return d;
+}
+
+// IsAloft as a bit mask passed by value (no per-call upload).
+constexpr int kMaxBins = 1024;
+struct AloftMask {
+  uint32_t w[kMaxBins / 32];
+  __host__ __device__ inline bool on(int n) const { return (w[n >> 5] >> (n & 31)) & 1u; }
+};
+
+// Geometry of one sweep direction: lines of length N along dir, transverse
+// interior extents Na (fastest, mapped to lanes) and Nb.
+struct LineGeom {
+  int dir;          // 0:x 1:y 2:z
+  int N, Na, Nb;
+  int rmin, rmax;   // active range along the line
+  int a0, a1, b0, b1;  // active transverse ranges
+};
+
+__host__ __device__ inline LineGeom make_line_geom(const Dims &d, int dir, const int32_t *bnd) {
+  LineGeom g;
+  g.dir = dir;
+  if (dir == 0) {
+    g.N = d.nx; g.Na = d.ny; g.Nb = d.nz;
+    g.rmin = bnd[0]; g.rmax = bnd[1]; g.a0 = bnd[2]; g.a1 = bnd[3]; g.b0 = bnd[4]; g.b1 = bnd[5];
+  } else if (dir == 1) {
+    g.N = d.ny; g.Na = d.nx; g.Nb = d.nz;
+    g.rmin = bnd[2]; g.rmax = bnd[3]; g.a0 = bnd[0]; g.a1 = bnd[1]; g.b0 = bnd[4]; g.b1 = bnd[5];
+  } else {
+    g.N = d.nz; g.Na = d.nx; g.Nb = d.ny;
+    g.rmin = bnd[4]; g.rmax = bnd[5]; g.a0 = bnd[0]; g.a1 = bnd[1]; g.b0 = bnd[2]; g.b1 = bnd[3];
+  }
+  return g;
+}
+
+// Offset of cell -1 of the line through transverse (a,b) and its stride.
+__host__ __device__ inline long long line_base(const Dims &d, int dir, int a, int b) {
+  return dir == 0 ? d.p3(-1, a, b) : (dir == 1 ? d.p3(a, -1, b) : d.p3(a, b, -1));
+}
+__host__ __device__ inline long long line_stride(const Dims &d, int dir) {
+  return dir == 0 ? 1LL : (dir == 1 ? d.sy : d.sz);
+}
+// Same for the diffusivity arrays kx/ky/kz(0:nx+1,0:ny+1,0:nz+1): offset of cell 0.
+__host__ __device__ inline long long kline_base(const Dims &d, int dir, int a, int b) {
+  const long long kx = d.nx + 2, kxy = (long long)(d.nx + 2) * (d.ny + 2);
+  return dir == 0 ? (kx * a + kxy * b) : (dir == 1 ? (a + kxy * b) : (a + kx * b));
+}
+__host__ __device__ inline long long kline_stride(const Dims &d, int dir) {
+  return dir == 0 ? 1LL : (dir == 1 ? (long long)(d.nx + 2) : (long long)(d.nx + 2) * (d.ny + 2));
+}
+
+struct AdvArgs {
+  Dims d;
+  LineGeom g;
+  double *c0;          // concen ts0
+  const double *vel;   // vx_pd / vy_pd / vz_pd
+  const double *vh;    // vh_pd (z)
+  const double *vf;    // vf_pd (z)
+  const double *sig;   // sigma_n{x,y,z}_pd
+  const double *kap;   // kappa_pd
+  const double *deld;  // DelD{x,y}onD_cc (x,y with ZScaling_ID == 2)
+  double *of1, *of2;   // outflow on the low / high side of the line
+  double *dep;         // DepositGranularity (z)
+  double dt;
+  int periodic;
+  int limiter;
+  AloftMask aloft;
+};
+
+struct DiffArgs {
+  Dims d;
+  LineGeom g;
+  double *c0;          // concen ts0 (in/out)
+  double *scratch_b;   // per-bin scratch, same layout as concen ts0 (the ts1 slot)
+  double *fac;         // per-line factor scratch: 3 * ncell per bin group
+  const double *kk;    // kx / ky / kz
+  const double *sig;
+  const double *kap;
+  double dt, theta;
+  int *flag;           // set to the first singular pivot row (dgtsv INFO)
+  AloftMask aloft;
+};
+
+// launchers (defined next to their kernels)
+hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double *vx, double *vy,
+                              double *vz, double *vf);
+hipError_t launch_set_bc1_conc(hipStream_t s, const Dims &d, int periodic, double *c0);
+hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0);
+hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0);
+hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0);
+hipError_t launch_fill(hipStream_t s, double *p, long long n, double v);
+hipError_t launch_mirror_interior(hipStream_t s, const Dims &d, const double *c0, double *c1);
+hipError_t launch_source_add(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt,
+                             const double *flux);
+hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);
+hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
+hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
+int pick_bins_per_thread(int ns);
+
+}  // namespace ash3d

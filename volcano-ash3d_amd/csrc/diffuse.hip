@@ -1,0 +1,344 @@
+// Diffusion sweeps: explicit diff_{x,y,z} (src/Diffusion.F90:266-650) and
+// the theta-scheme diffCN_{x,y,z} (src/Diffusion.F90:671-1428) with the
+// tridiagonal solve of LAPACK dgtsv (Gaussian elimination with partial
+// pivoting), which the reference calls once per line and bin.
+//
+// One thread owns one line for NB bins.  The matrix of a line does not
+// depend on the bin, so its coefficients (three divisions per cell) and its
+// LU factorisation are formed once per line and applied to the NB
+// right-hand sides; the elimination is streamed along the line in exactly
+// dgtsv's order, the U factors go to a per-line scratch and the eliminated
+// right-hand sides to the ts1 slot (which the reference zero-fills anyway),
+// and the back substitution streams the line in reverse writing ts0.
+// No contraction (-ffp-contract=off): bitwise the reference arithmetic.
+#include "common.hpp"
+
+namespace ash3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int DIR, int NB>
+__global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int n0 = (int)blockIdx.y * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  double *c = A.c0;
+  bool exists[NB], on[NB];
+  bool any_on = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    exists[u] = (n0 + u) < d.ns;
+    on[u] = exists[u] && line_on && A.aloft.on(n0 + u);
+    any_on = any_on || on[u];
+  }
+  if (any_on) {
+    const double dt = A.dt;
+    // window: q_{r-1}, kap_{r-1}, K_{r-1}; interface r-1 quantities
+    double qm[NB], dqm[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) qm[u] = dqm[u] = 0.0;
+    double kapm = 1.0, Km = 0.0, sigm = 0.0, Am = 0.0;
+    for (int r = rmin - 1; r <= rmax + 1; ++r) {
+      const long long cell = base + sL * (long long)(r + 1);
+      const double kap0 = A.kap[cell];
+      const double K0 = A.kk[kB + kS * (long long)r];
+      double q0[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) q0[u] = exists[u] ? c[cell + d.ncell * (long long)(n0 + u)] : 0.0;
+      if (r >= rmin) {
+        // interface I = r (:357-359)
+        const double sig0 = A.sig[cell];
+        const double ds = 2.0 * sig0 / (kapm + kap0);
+        const double kds = 0.5 * (Km + K0) * ds;
+        const double A0 = dt * kds;
+        const int cc = r - 1;
+        if (cc >= rmin) {
+          // cell cc = r-1 (:367-372): interfaces cc (=r-1) and cc+1 (=r)
+          const double sR = sig0 / kapm;  // sig_{c+1}/kap_c
+          const double sLf = sigm / kapm; // sig_c/kap_c
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const double dq0 = q0[u] - qm[u];
+            if (on[u]) {
+              const double LF = A0 * dq0 * sR;
+              const double RF = -(Am * dqm[u] * sLf);
+              c[base + sL * (long long)(cc + 1) + d.ncell * (long long)(n0 + u)] = qm[u] + (LF + RF);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) dqm[u] = q0[u] - qm[u];
+        Am = A0;
+        sigm = sig0;
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) qm[u] = q0[u];
+      kapm = kap0;
+      Km = K0;
+    }
+  }
+  // zero fill of ts1 then copy (:314, :385-386)
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    if (!exists[u]) continue;
+    const long long bofs = d.ncell * (long long)(n0 + u);
+    if (!on[u]) {
+      for (int r = 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    } else {
+      for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+      for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    }
+  }
+}
+
+// theta-scheme line solve.  Row l (cell c = rmin-1+l) of the system
+// (:807-853):
+//   L = dt*kds_c*sig_c/kap_c, R = dt*kds_{c+1}*sig_{c+1}/kap_c, C = L+R
+//   first row: D = (1+th*C) - th*L, DU = -(th*R)
+//   interior:  DL = -(th*L), D = 1+th*C, DU = -(th*R)
+//   last row:  DL = -(th*L), D = (1+th*C) - th*R
+//   B = ((1-th)*L*q_left + (1-(1-th)*C)*q_c) + (1-th)*R*q_right
+// with q_left = q_c on the first row and q_right = q_c on the last.
+template <int DIR, int NB>
+__global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int n0 = (int)blockIdx.y * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int n = rmax - rmin + 1;  // nlineq
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  double *c = A.c0;
+  double *sb = A.scratch_b;
+  // per-line U factors: D, DU, DL(2nd super-diagonal), one slab per bin group
+  double *fD = A.fac + 3 * d.ncell * (long long)blockIdx.y;
+  double *fDU = fD + d.ncell;
+  double *fDL = fDU + d.ncell;
+  bool exists[NB], on[NB];
+  bool any_on = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    exists[u] = (n0 + u) < d.ns;
+    on[u] = exists[u] && line_on && A.aloft.on(n0 + u);
+    any_on = any_on || on[u];
+  }
+  if (any_on) {
+    const double dt = A.dt, th = A.theta;
+    // ---- coefficients of row 1 need interfaces rmin and rmin+1
+    // streaming state: kap_{c-1}, kap_c; K_{c-1}, K_c; sig_c; q_{c-1}, q_c
+    auto kapat = [&](int r) { return A.kap[base + sL * (long long)(r + 1)]; };
+    auto Kat = [&](int r) { return A.kk[kB + kS * (long long)r]; };
+    auto sigat = [&](int r) { return A.sig[base + sL * (long long)(r + 1)]; };
+    auto qat = [&](int u, int r) { return c[base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u)]; };
+    // interface I: kds_I = kavg*(sig/vavg) (:794-801)
+    auto kds_at = [&](double kapl, double kapr, double Kl, double Kr, double sg) {
+      const double vavg = 0.5 * (kapl + kapr);
+      const double kavg = 0.5 * (Kl + Kr);
+      const double ds = sg / vavg;
+      return kavg * ds;
+    };
+    double kap_c = kapat(rmin), K_c = Kat(rmin);
+    double kap_l = kapat(rmin - 1), K_l = Kat(rmin - 1);
+    double sig_c = sigat(rmin);
+    double kds_c = kds_at(kap_l, kap_c, K_l, K_c, sig_c);
+    double q_l[NB], q_c[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      q_c[u] = exists[u] ? qat(u, rmin) : 0.0;
+      q_l[u] = q_c[u];  // BC_left_t0 = q(rmin) (:822)
+    }
+    // elimination state: current row i (modified D, DU, B) and the
+    // original sub-diagonal entry that row i+1 carries
+    double Dc = 0.0, DUc = 0.0, Bc[NB];
+    int singular = 0;
+    for (int l = 1; l <= n; ++l) {
+      const int cc = rmin - 1 + l;
+      // cell cc+1 (right neighbour) data
+      const double kap_r = kapat(cc + 1), K_r = Kat(cc + 1), sig_r = sigat(cc + 1);
+      const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
+      const double LeftFac = dt * kds_c * sig_c / kap_c;
+      const double RightFac = dt * kds_r * sig_r / kap_c;
+      const double CenterFac = LeftFac + RightFac;
+      double Dn, DUn = 0.0, DLn = 0.0;
+      if (l == 1) {
+        Dn = 1.0 + (th)*CenterFac - (th)*LeftFac;
+        DUn = -((th)*RightFac);
+      } else if (l < n) {
+        DLn = -((th)*LeftFac);
+        Dn = 1.0 + (th)*CenterFac;
+        DUn = -((th)*RightFac);
+      } else {
+        DLn = -((th)*LeftFac);
+        Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
+      }
+      double Bn[NB], q_r[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        q_r[u] = (l < n) ? (exists[u] ? qat(u, cc + 1) : 0.0) : q_c[u];  // BC_right_t0 (:845)
+        Bn[u] = (1.0 - th) * LeftFac * q_l[u] + (1.0 - (1.0 - th) * CenterFac) * q_c[u] +
+                (1.0 - th) * RightFac * q_r[u];
+      }
+      if (l == 1) {
+        Dc = Dn;
+        DUc = DUn;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
+      } else {
+        // dgtsv elimination step i = l-1 with row i+1 = l (NRHS = 1 path)
+        const int i = l - 1;
+        const long long fo = base + sL * (long long)(rmin - 1 + i + 1);  // cell of row i
+        double DLo;  // 2nd super-diagonal of U in row i
+        if (fabs(Dc) >= fabs(DLn)) {
+          if (Dc == 0.0) { if (!singular) singular = i; }
+          const double fact = DLn / Dc;
+          Dn = Dn - fact * DUc;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) Bn[u] = Bn[u] - fact * Bc[u];
+          DLo = (i <= n - 2) ? 0.0 : DLn;
+        } else {
+          const double fact = Dc / DLn;
+          const double newDi = DLn;
+          const double temp = Dn;
+          Dn = DUc - fact * temp;
+          if (i <= n - 2) {
+            DLo = DUn;
+            DUn = -fact * DLo;
+          } else {
+            DLo = DLn;
+          }
+          DUc = temp;
+          Dc = newDi;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const double tb = Bc[u];
+            Bc[u] = Bn[u];
+            Bn[u] = tb - fact * Bn[u];
+          }
+        }
+        // row i is final: store its U entries and eliminated RHS
+        fD[fo] = Dc;
+        fDU[fo] = DUc;
+        fDL[fo] = DLo;
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+          if (on[u]) sb[fo + d.ncell * (long long)(n0 + u)] = Bc[u];
+        Dc = Dn;
+        DUc = DUn;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
+      }
+      // advance the window
+      kap_l = kap_c; kap_c = kap_r;
+      K_l = K_c; K_c = K_r;
+      sig_c = sig_r;
+      kds_c = kds_r;
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        q_l[u] = q_c[u];
+        q_c[u] = q_r[u];
+      }
+    }
+    if (Dc == 0.0 && !singular) singular = n;
+    if (singular) atomicCAS(A.flag, 0, singular);
+    // ---- back substitution (dgtsv, NRHS <= 2 path), row n first
+    double x1[NB], x2[NB];
+    {
+      const double Dn_ = Dc;
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        x1[u] = Bc[u] / Dn_;
+        x2[u] = 0.0;
+        if (on[u]) c[base + sL * (long long)(rmax + 1) + d.ncell * (long long)(n0 + u)] = x1[u];
+      }
+    }
+    for (int i = n - 1; i >= 1; --i) {
+      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      const double Di = fD[fo], DUi = fDU[fo], DLi = fDL[fo];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const double Bi = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+        double xi;
+        if (i == n - 1) xi = (Bi - DUi * x1[u]) / Di;
+        else xi = (Bi - DUi * x1[u] - DLi * x2[u]) / Di;
+        x2[u] = x1[u];
+        x1[u] = xi;
+        if (on[u]) c[fo + d.ncell * (long long)(n0 + u)] = xi;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    if (!exists[u]) continue;
+    const long long bofs = d.ncell * (long long)(n0 + u);
+    if (!on[u]) {
+      for (int r = 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    } else {
+      for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+      for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    }
+  }
+}
+
+template <template <int, int> class K, int NB>
+struct dummy {};
+
+template <int NB>
+hipError_t launch_explicit_nb(hipStream_t s, const DiffArgs &a) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  switch (a.g.dir) {
+    case 0: hipLaunchKernelGGL((k_diff_explicit<0, NB>), grid, dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_diff_explicit<1, NB>), grid, dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_diff_explicit<2, NB>), grid, dim3(kBlock), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+template <int NB>
+hipError_t launch_cn_nb(hipStream_t s, const DiffArgs &a) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  switch (a.g.dir) {
+    case 0: hipLaunchKernelGGL((k_diff_cn<0, NB>), grid, dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_diff_cn<1, NB>), grid, dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_diff_cn<2, NB>), grid, dim3(kBlock), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb) {
+  switch (nb) {
+    case 1: return launch_explicit_nb<1>(s, a);
+    case 2: return launch_explicit_nb<2>(s, a);
+    case 4: return launch_explicit_nb<4>(s, a);
+    default: return launch_explicit_nb<8>(s, a);
+  }
+}
+
+hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb) {
+  switch (nb) {
+    case 1: return launch_cn_nb<1>(s, a);
+    case 2: return launch_cn_nb<2>(s, a);
+    case 4: return launch_cn_nb<4>(s, a);
+    default: return launch_cn_nb<8>(s, a);
+  }
+}
+
+}  // namespace ash3d

@@ -1,0 +1,265 @@
+"""ctypes mirror of the C ABI in include/ash3d_tr.h.
+
+``Transport`` plays the role of the reference's transport modules for one
+``TransportState``: it owns a device context (created from the state's
+host arrays), and its methods carry the reference's routine names and
+argument meaning -- ``Set_BC(bc_code)``, ``advect_x()``, ``advect_y()``,
+``AdvectHorz(itoggle)``, ``AdvectVert()``, ``diff_x/y/z()``,
+``diffCN_x/y/z()``, ``DiffuseHorz(i)``, ``DiffuseVert()`` -- reading dt,
+IsAloft, Imp_fac and the imin..kmax bounds from the state the way the
+Fortran routines read them from the modules.  Results stay on the device
+until ``sync_to_host()``.
+
+Errors from the library raise ``Ash3dError``; a missing library raises at
+load time (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libash3d_tr.so")
+ABI_VERSION = 1
+
+SYNC_CONCEN, SYNC_WINDS, SYNC_VF, SYNC_GEOMETRY = 1, 2, 4, 8
+SYNC_DIFFK, SYNC_TOPO, SYNC_OUTFLOW, SYNC_DEPOSIT = 16, 32, 64, 128
+SYNC_ALL = 0xFF
+
+ERRORS = {1: "EINVAL", 2: "EDEVICE", 3: "ENOMEM", 4: "EBC", 5: "ESINGULAR", 6: "ENOLIB"}
+
+
+class Ash3dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ash3d_tr error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("ns", ctypes.c_int32),
+        ("is_periodic", ctypes.c_int32), ("zscaling_id", ctypes.c_int32), ("limiter", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("concen_pd", _dp),
+        ("vx_pd", _dp), ("vy_pd", _dp), ("vz_pd", _dp), ("vh_pd", _dp), ("vf_pd", _dp),
+        ("kappa_pd", _dp), ("sigma_nx_pd", _dp), ("sigma_ny_pd", _dp), ("sigma_nz_pd", _dp),
+        ("kx", _dp), ("ky", _dp), ("kz", _dp),
+        ("deldxond_cc", _dp), ("deldyond_cc", _dp),
+        ("outflow_xz1_pd", _dp), ("outflow_xz2_pd", _dp),
+        ("outflow_yz1_pd", _dp), ("outflow_yz2_pd", _dp),
+        ("outflow_xy1_pd", _dp), ("outflow_xy2_pd", _dp),
+        ("deposit_granularity", _dp),
+    ]
+
+
+# every exported entry point and its argument types
+_SIGS = {
+    "ash3d_tr_create": [ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_void_p)],
+    "ash3d_tr_destroy": [ctypes.c_void_p],
+    "ash3d_tr_sync_from_host": [ctypes.c_void_p, ctypes.c_uint32],
+    "ash3d_tr_sync_to_host": [ctypes.c_void_p, ctypes.c_uint32],
+    "ash3d_tr_synchronize": [ctypes.c_void_p],
+    "ash3d_tr_set_bc": [ctypes.c_void_p, ctypes.c_int32],
+    "ash3d_tr_advect_x": [ctypes.c_void_p, ctypes.c_double, _ip, _ip],
+    "ash3d_tr_advect_y": [ctypes.c_void_p, ctypes.c_double, _ip, _ip],
+    "ash3d_tr_advect_z": [ctypes.c_void_p, ctypes.c_double, _ip, _ip],
+    "ash3d_tr_advect_horz": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _ip],
+    "ash3d_tr_advect_vert": [ctypes.c_void_p, ctypes.c_double, _ip],
+    "ash3d_tr_diffuse_dir": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                             ctypes.c_double, _ip, _ip],
+    "ash3d_tr_diffuse_horz": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                              ctypes.c_double, _ip],
+    "ash3d_tr_diffuse_vert": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _ip],
+    "ash3d_tr_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                      ctypes.c_double, _ip],
+    "ash3d_tr_step_glue": [ctypes.c_void_p],
+    "ash3d_tr_source_column_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp],
+    "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
+    "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_int64)],
+}
+PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue")
+EXPORTS = tuple(_SIGS) + ("ash3d_tr_last_error", "ash3d_tr_build_info")
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} is missing: build it with `python volcano-ash3d_amd/build.py` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        lib.ash3d_tr_last_error.restype = ctypes.c_char_p
+        lib.ash3d_tr_build_info.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class Transport:
+    """Device context for one TransportState, with the reference's API."""
+
+    def __init__(self, st, device: int = 0):
+        self.lib = load()
+        self.st = st
+        d = Desc()
+        d.abi_version = ABI_VERSION
+        d.nx, d.ny, d.nz, d.ns = st.nx, st.ny, st.nz, st.ns
+        d.is_periodic = int(bool(st.is_periodic))
+        d.zscaling_id = int(st.zscaling_id)
+        d.limiter = int(st.limiter)
+        d.device = device
+        for name in st.array_names():
+            setattr(d, name, _ptr(getattr(st, name)))
+        self._desc = d
+        self._ctx = ctypes.c_void_p()
+        self._check(self.lib.ash3d_tr_create(ctypes.byref(d), ctypes.byref(self._ctx)))
+
+    # -- plumbing ----------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise Ash3dError(rc, self.lib.ash3d_tr_last_error().decode())
+
+    def _aloft(self):
+        self.st.is_aloft = np.ascontiguousarray(self.st.is_aloft, dtype=np.int32)
+        return self.st.is_aloft.ctypes.data_as(_ip)
+
+    def _bounds(self):
+        self._b = np.ascontiguousarray(self.st.bounds, dtype=np.int32)
+        return self._b.ctypes.data_as(_ip)
+
+    def _full_bounds(self):
+        st = self.st
+        st.bounds = (1, st.nx, 1, st.ny, 1, st.nz)
+
+    def close(self):
+        if self._ctx:
+            self.lib.ash3d_tr_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync_to_host(self, mask: int = SYNC_ALL):
+        self._check(self.lib.ash3d_tr_sync_to_host(self._ctx, mask))
+
+    def sync_from_host(self, mask: int = SYNC_ALL):
+        self._check(self.lib.ash3d_tr_sync_from_host(self._ctx, mask))
+
+    def synchronize(self):
+        self._check(self.lib.ash3d_tr_synchronize(self._ctx))
+
+    # -- the reference routines -------------------------------------------
+    def Set_BC(self, bc_code: int):
+        self._check(self.lib.ash3d_tr_set_bc(self._ctx, bc_code))
+
+    def advect_x(self):
+        self._check(self.lib.ash3d_tr_advect_x(self._ctx, self.st.dt, self._aloft(), self._bounds()))
+
+    def advect_y(self):
+        self._check(self.lib.ash3d_tr_advect_y(self._ctx, self.st.dt, self._aloft(), self._bounds()))
+
+    def advect_z(self):
+        self._check(self.lib.ash3d_tr_advect_z(self._ctx, self.st.dt, self._aloft(), self._bounds()))
+
+    def AdvectHorz(self, itoggle: int):
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_advect_horz(self._ctx, itoggle, self.st.dt, self._aloft()))
+
+    def AdvectVert(self):
+        self._check(self.lib.ash3d_tr_advect_vert(self._ctx, self.st.dt, self._aloft()))
+
+    def _diffuse(self, dirn: int, use_cn: bool):
+        self._check(self.lib.ash3d_tr_diffuse_dir(self._ctx, dirn, int(use_cn), self.st.dt,
+                                                  self.st.imp_fac, self._aloft(), self._bounds()))
+
+    def diff_x(self):
+        self._diffuse(0, False)
+
+    def diff_y(self):
+        self._diffuse(1, False)
+
+    def diff_z(self):
+        self._diffuse(2, False)
+
+    def diffCN_x(self):
+        self._diffuse(0, True)
+
+    def diffCN_y(self):
+        self._diffuse(1, True)
+
+    def diffCN_z(self):
+        self._diffuse(2, True)
+
+    def DiffuseHorz(self, itoggle: int):
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_diffuse_horz(self._ctx, itoggle, int(bool(self.st.use_cn)), self.st.dt,
+                                                   self.st.imp_fac, self._aloft()))
+
+    def DiffuseVert(self):
+        self._check(self.lib.ash3d_tr_diffuse_vert(self._ctx, int(bool(self.st.use_cn)), self.st.dt,
+                                                   self.st.imp_fac, self._aloft()))
+
+    # -- driver-level entry points ------------------------------------------
+    def transport_step(self, itime: int, use_diffusion: bool):
+        """Set_BC(1); AdvectHorz(itime); AdvectVert; [Set_BC(2);
+        DiffuseHorz(itime); DiffuseVert] (src/Ash3d.F90:491-510)."""
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_step(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                           int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft()))
+
+    def step_glue(self):
+        self._check(self.lib.ash3d_tr_step_glue(self._ctx))
+
+    def source_column_add(self, ivent: int, jvent: int, flux: np.ndarray):
+        flux = np.asfortranarray(flux, dtype=np.float64)
+        assert flux.shape == (self.st.nz + 1, self.st.ns)
+        self._check(self.lib.ash3d_tr_source_column_add(self._ctx, ivent, jvent, self.st.dt,
+                                                        flux.ctypes.data_as(_dp)))
+
+
+    # -- kernel timing ----------------------------------------------------
+    def profile(self, enable: bool):
+        self._check(self.lib.ash3d_tr_profile(self._ctx, int(bool(enable))))
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, name in enumerate(PROF_KINDS):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            self._check(self.lib.ash3d_tr_profile_read(self._ctx, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+
+def build_info() -> str:
+    return load().ash3d_tr_build_info().decode()
