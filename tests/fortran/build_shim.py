@@ -1,0 +1,38 @@
+"""Build the shim test driver: the drop-in Fortran modules of
+volcano-ash3d_amd/fortran compiled against the test host modules, linked to
+libash3d_tr.so.  Two variants, as the reference's DIFFMETH switch:
+_build/cn (-DCRANKNIC) and _build/ex (explicit diffusion)."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+PKG = os.path.join(ROOT, "volcano-ash3d_amd")
+SHIMS = ["ash3d_tr_c.f90", "Diffusion.F90", "ash3d_tr_attach.F90", "AdvectionHorz_DCU.F90",
+         "AdvectionHorz.F90", "AdvectionVert_DCU.F90", "Set_BC.F90"]
+FC = os.environ.get("ASH3D_FC", "/opt/rocm/llvm/bin/amdflang")
+
+
+def build(verbose=False):
+    outs = {}
+    for variant, defs in (("cn", ["-DCRANKNIC", "-DLIM_SUPERBEE"]), ("ex", ["-DLIM_SUPERBEE"])):
+        bdir = os.path.join(HERE, "_build", variant)
+        os.makedirs(bdir, exist_ok=True)
+        srcs = [os.path.join(HERE, "host_modules.F90")] + [os.path.join(PKG, "fortran", s) for s in SHIMS] + \
+               [os.path.join(HERE, "shim_driver.F90")]
+        exe = os.path.join(bdir, "shim_driver")
+        cmd = [FC, "-O2", "-cpp", *defs, "-J", bdir, "-I", bdir, *srcs, "-o", exe,
+               "-L", PKG, "-lash3d_tr", f"-Wl,-rpath,{PKG}"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("building the shim driver failed")
+        outs[variant] = exe
+    return outs
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))
