@@ -164,6 +164,17 @@ int ash3d_tr_step_glue(ash3d_tr_ctx *ctx);
 int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, double dt,
                                const double *flux);
 
+/* Per-bin mass budget in kg, evaluated on the device from the current
+ * state (deterministic FP64 reduction), as the sums of Calc_AshVol_Aloft,
+ * Calc_AshVol_Deposit and Calc_AshVol_Outflow (src/Output_Vars.f90:1255-1293,
+ * :1375-1404, :1419-1459) before their division by MagmaDensity*1e9:
+ *   aloft[n]   = sum concen(1:nx,1:ny,1:nz,n,ts0)*kappa(1:nx,1:ny,1:nz)
+ *   deposit[n] = sum DepositGranularity(:,:,n)*kappa(1:nx,1:ny,0)
+ *   outflow[n] = yz1, yz2, xz1, xz2, xy2 outflows times their ghost kappa.
+ * Each output is a host array of ns doubles (may be NULL).  Blocks until
+ * the values are on the host. */
+int ash3d_tr_mass_totals(ash3d_tr_ctx *ctx, double *aloft, double *deposit, double *outflow);
+
 /* Kernel timing.  No counterpart in the reference, whose only timers are
  * wall-clock reads around the whole run (src/Ash3d.F90:176-177, :727-732).
  * ash3d_tr_profile(ctx,1) clears the records and brackets every later

@@ -222,6 +222,7 @@ def test_large_lonlat_mass_conservation(pkg, tr):
     """Full-size property: 10 steps of the forecast grid conserve total mass
     (aloft + deposit + outflow) to 1e-13 and keep concentrations >= 0."""
     st = pkg.workloads.config2_state()
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]  # the glue copies ts1 into ts0
     m0 = interior_mass(st).sum()
     with tr.Transport(st) as T:
         for itime in range(1, 11):
@@ -231,3 +232,32 @@ def test_large_lonlat_mass_conservation(pkg, tr):
     m1 = (interior_mass(st) + outflow_mass(st)).sum()
     assert abs(m1 - m0) <= 1e-13 * m0
     assert st.interior().min() >= -1e-9 * st.interior().max()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_mass_totals_on_device(pkg, tr, periodic):
+    """The device mass budget equals the definitions of Calc_AshVol_Aloft /
+    _Deposit / _Outflow (before the volume conversion) evaluated on the
+    synced host arrays, to 1e-13, and is bitwise repeatable."""
+    st = pkg.workloads.random_state(17, 13, 11, 4, seed=81, latlon=True, periodic=periodic)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    with tr.Transport(st) as T:
+        for itime in (1, 2, 3):
+            T.step_glue()
+            T.transport_step(itime, True)
+        a1, d1, o1 = T.mass_totals()
+        a2, d2, o2 = T.mass_totals()
+        T.sync_to_host()
+    nx, ny, nz = st.nx, st.ny, st.nz
+    kap = st.kappa_pd
+    ref_a = interior_mass(st)
+    ref_d = np.einsum("ijn,ij->n", st.deposit_granularity, kap[2:nx + 2, 2:ny + 2, 1])
+    ref_o = (np.einsum("jkn,jk->n", st.outflow_yz1_pd[2:ny + 2, 2:nz + 2], kap[1, 2:ny + 2, 2:nz + 2]) +
+             np.einsum("jkn,jk->n", st.outflow_yz2_pd[2:ny + 2, 2:nz + 2], kap[nx + 2, 2:ny + 2, 2:nz + 2]) +
+             np.einsum("ikn,ik->n", st.outflow_xz1_pd[2:nx + 2, 2:nz + 2], kap[2:nx + 2, 1, 2:nz + 2]) +
+             np.einsum("ikn,ik->n", st.outflow_xz2_pd[2:nx + 2, 2:nz + 2], kap[2:nx + 2, ny + 2, 2:nz + 2]) +
+             np.einsum("ijn,ij->n", st.outflow_xy2_pd[2:nx + 2, 2:ny + 2], kap[2:nx + 2, 2:ny + 2, nz + 2]))
+    np.testing.assert_allclose(a1, ref_a, rtol=1e-13)
+    np.testing.assert_allclose(d1, ref_d, rtol=1e-13)
+    np.testing.assert_allclose(o1, ref_o, rtol=1e-13, atol=1e-13 * ref_a.max())
+    assert np.array_equal(a1, a2) and np.array_equal(d1, d2) and np.array_equal(o1, o2)

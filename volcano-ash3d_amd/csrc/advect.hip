@@ -14,60 +14,37 @@
 // contraction: the library is built with -ffp-contract=off), so results are
 // bitwise those of the Fortran routines.
 #include "common.hpp"
+#include "dcu.hpp"
 
 namespace ash3d {
 
 namespace {
 
-constexpr double EPS_THRESH = 1.0e-10;  // src/Ash3d_VariableModules.f90:273
 constexpr int kBlock = 256;
 
-// Limited Delta Q, src/AdvectionHorz_DCU.f90:186-226 (limiter is uniform
-// across the launch, so the switch does not diverge).
-__device__ inline double limited_dq(int lim, double dq, double dqu, double theta) {
-  switch (lim) {
-    case 1: return dq;                                      // Lax-Wendroff
-    case 2: return dqu;                                     // Beam-Warming
-    case 3: return dq * (0.5 * (1.0 + theta));              // Fromm
-    case 4: return dq * fmax(0.0, fmin(1.0, theta));        // minmod
-    case 5: return dq * fmax(fmax(0.0, fmin(1.0, 2.0 * theta)), fmin(2.0, theta));  // superbee
-    case 6: return dq * fmax(0.0, fmin(fmin((1.0 + theta) / 2.0, 2.0), 2.0 * theta));  // MC
-    default: return 0.0;
-  }
-}
-
-// fss_I, the limited second-order flux correction at interface I
-// (:179-231): dq0 = dq_I, dqm = dq_{I-1}, dqp = dq_{I+1}, u = usig_I,
-// dtv = dt/kappa of cell I.
-__device__ inline double limited_flux(int lim, double dqm, double dq0, double dqp, double u,
-                                      double dtv) {
-  if (fabs(dq0) <= EPS_THRESH) return 0.0;
-  double dqu = 0.0, theta = 0.0;
-  if (lim != 1) {
-    dqu = (u > 0.0) ? dqm : dqp;
-    if (fabs(dqu) > 3.0 * fabs(dq0) || fabs(dq0) < EPS_THRESH)
-      theta = copysign(3.0, dqu * dq0);
-    else
-      theta = dqu / dq0;
-  }
-  const double ldq = limited_dq(lim, dq0, dqu, theta);
-  const double aus = fabs(u);
-  return 0.5 * aus * (1.0 - dtv * aus) * ldq;
-}
-
+// Block = 64 lanes (consecutive transverse index a, i.e. coalesced for y
+// and z lines) x W segments of the same lines.  Segment w emits the cell
+// updates of cc in [lo, hi] of the line (cc = 0 and N+1 are the ghost
+// updates that feed the outflow/deposit); it streams cells lo-2 .. hi+2.
+// The two cells below lo and above hi are also written by the neighbouring
+// segments, so they are read into registers before a workgroup barrier and
+// never re-read from memory.
 template <int DIR, int NB, bool TOPO>
-__global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
+__global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (gid >= (long long)g.Na * g.Nb) return;
-  const int a = (int)(gid % g.Na) + 1;
-  const int b = (int)(gid / g.Na) + 1;
+  const int ntile = (g.Na + 63) / 64;
+  const int a = (int)(blockIdx.x % (unsigned)ntile) * 64 + (int)threadIdx.x + 1;
+  const int b = (int)(blockIdx.x / (unsigned)ntile) + 1;
+  const bool lane_ok = a <= g.Na;
+  const int W = blockDim.y, w = threadIdx.y;
   const int n0 = (int)blockIdx.y * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int seg = (N + 2 + W - 1) / W;
+  const int lo = w * seg, hi = min(N + 1, lo + seg - 1);
   const long long sL = line_stride(d, DIR);
-  const long long base = line_base(d, DIR, a, b);
-  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const long long base = lane_ok ? line_base(d, DIR, a, b) : 0;
+  const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *c = A.c0;
 
   bool exists[NB], on[NB];
@@ -75,16 +52,29 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     const int n = n0 + u;
-    exists[u] = n < d.ns;
+    exists[u] = lane_ok && n < d.ns;
     on[u] = exists[u] && line_on && A.aloft.on(n);
     any_on = any_on || on[u];
   }
   // outflow entry of this line: outflow(a,b,n) with the transverse extents
   const long long ofs0 = (long long)(a + 1) + (long long)(g.Na + 4) * (long long)(b + 1);
   const long long ofs_bin = (long long)(g.Na + 4) * (g.Nb + 4);
+  const int ldlo = rmin - 2, ldhi = rmax + 2;  // cells gathered by the reference
+
+  // halo cells shared with the neighbouring segments
+  double hq[4][NB];
+  const int hr[4] = {lo - 2, lo - 1, hi + 1, hi + 2};
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      hq[h][u] = 0.0;
+      if (any_on && exists[u] && hr[h] >= ldlo && hr[h] <= ldhi)
+        hq[h][u] = c[base + sL * (long long)(hr[h] + 1) + d.ncell * (long long)(n0 + u)];
+    }
+  if (W > 1) __syncthreads();
 
   if (any_on) {
-    const int lo = rmin - 2, hi = rmax + 2;  // cells gathered by the reference
     const int ncells = rmax - rmin + 1;
     const double dt = A.dt;
     const int lim = A.limiter;
@@ -97,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
       q0[u] = q1[u] = q2[u] = dq0[u] = dq1[u] = dq2[u] = fsr2[u] = fss2[u] = tp[u] = 0.0;
       vb1[u] = ub0[u] = ub1[u] = ub2[u] = 0.0;
     }
-    for (int r = -1; r <= N + 3; ++r) {
-      const bool ld = (r >= lo && r <= hi);
+    for (int r = lo - 2; r <= hi + 2; ++r) {
+      const bool ld = (r >= ldlo && r <= ldhi);
       const long long cell = base + sL * (long long)(r + 1);
       // ---- gather cell r (:143-146 / :171-176)
       double vel0 = 0.0, sig0 = 0.0, kap0 = 1.0, vzh0 = 0.0, D0 = 0.0;
@@ -112,10 +102,12 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
           D0 = (DIR == 0) ? A.deld[(long long)(r + 1) + (long long)(d.nx + 4) * (a + 1)]
                           : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (r + 1)];
       }
+      const int h = (r == lo - 2) ? 0 : (r == lo - 1) ? 1 : (r == hi + 1) ? 2 : (r == hi + 2) ? 3 : -1;
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         q0[u] = 0.0;
-        if (ld && exists[u]) q0[u] = c[cell + d.ncell * (long long)(n0 + u)];
+        if (h >= 0) q0[u] = hq[h][u];
+        else if (ld && exists[u]) q0[u] = c[cell + d.ncell * (long long)(n0 + u)];
       }
       const bool uin = (r >= rmin - 1 && r <= rmax + 1);
       // dt/kappa (:155-156) -- bin independent
@@ -138,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
       const bool fs_in = I >= rmin - 1 && I <= rmax + 1;
       // ---- cell c = r-2
       const int cc = r - 2;
-      const bool upd_in = cc >= rmin - 1 && cc <= rmax + 1;
+      const bool upd_in = cc >= lo && cc <= hi && cc >= rmin - 1 && cc <= rmax + 1;
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const double uI = (DIR == 2) ? ub1[u] : u1;
@@ -209,18 +201,16 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
   }
   // cells the reference leaves at zero: inactive bins and lines, and cells
   // outside [rmin, rmax] (zero fill of ts1 then copy, :122, :302-303)
+  const int zlo = max(lo, 1), zhi = min(hi, N);
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (!exists[u]) continue;
     const long long bofs = d.ncell * (long long)(n0 + u);
-    if (!on[u]) {
-      for (int r = 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
-    } else {
-      for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
-      for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
-    }
-    if (DIR == 2) {
-      // DepositGranularity = outflow_xy1 (src/AdvectionVert_DCU.f90:321)
+    for (int r = zlo; r <= zhi; ++r)
+      if (!on[u] || r < rmin || r > rmax) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    if (DIR == 2 && w == 0) {
+      // DepositGranularity = outflow_xy1 (src/AdvectionVert_DCU.f90:321);
+      // segment 0 owns the bottom ghost update
       const long long of = ofs0 + ofs_bin * (long long)(n0 + u);
       A.dep[(long long)(a - 1) + (long long)d.nx * ((long long)(b - 1) + (long long)d.ny * (n0 + u))] =
           A.of1[of];
@@ -230,12 +220,20 @@ __global__ __launch_bounds__(kBlock) void k_advect(AdvArgs A) {
 
 template <int DIR, int NB>
 hipError_t launch_dir(hipStream_t s, const AdvArgs &a) {
-  const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  const int ntile = (a.g.Na + 63) / 64;
+  const long long groups = (a.d.ns + NB - 1) / NB;
+  // segments per line: enough waves to fill the chip, segments >= 16 cells,
+  // at most 4 (one wave per SIMD of the CU holding the workgroup)
+  const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
+  int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
+  W = W < 1 ? 1 : (W > 4 ? 4 : W);
+  while (W > 1 && (a.g.N + 2) / W < 16) --W;
+  dim3 grid((unsigned)((long long)ntile * a.g.Nb), (unsigned)groups);
+  dim3 block(64, W);
   if (DIR != 2 && a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect<DIR, NB, true>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_advect<DIR, NB, true>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((k_advect<DIR, NB, false>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_advect<DIR, NB, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -258,6 +256,11 @@ int pick_bins_per_thread(int ns) {
 }
 
 hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb) {
+  if (a.g.dir == 0) {
+    hipError_t e = launch_advect_x_rows(s, a);
+    if (e != hipErrorInvalidValue) return e;
+    (void)hipGetLastError();  // row too long for one workgroup: stream it
+  }
   switch (nb) {
     case 1: return launch_nb<1>(s, a);
     case 2: return launch_nb<2>(s, a);

@@ -71,6 +71,7 @@ struct ash3d_tr_ctx {
   long long fac_groups = 0;
   double *flux = nullptr;  // source column staging (nz+1)*ns
   int *flag = nullptr;     // singular-pivot flag
+  double *tot = nullptr;   // mass-total partial sums and results
   int ts1_state = TS1_EXPLICIT;
   int nb_adv = 4, nb_diff = 4;
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
@@ -336,7 +337,7 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   if (x->stream) hipStreamSynchronize(x->stream);
   double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
-                    x->fac, x->flux};
+                    x->fac, x->flux, x->tot};
   for (double *p : ptrs)
     if (p) hipFree(p);
   if (x->flag) hipFree(x->flag);
@@ -609,6 +610,26 @@ int ash3d_tr_profile_read(ash3d_tr_ctx *x, int32_t kind, double *total_ms, int64
   }
   *total_ms = t;
   *launches = n;
+  return 0;
+}
+
+int ash3d_tr_mass_totals(ash3d_tr_ctx *x, double *aloft, double *deposit, double *outflow) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const Dims &d = x->d;
+  const long long np = mass_partials_size(d);
+  if (!x->tot && (rc = dev_alloc(&x->tot, np + 3LL * d.ns))) return rc;
+  double *out = x->tot + np;
+  HIPCHK(launch_mass_totals(x->stream, d, x->ts0(), x->kap, x->dep, x->ofyz1, x->ofyz2, x->ofxz1, x->ofxz2,
+                            x->ofxy2, x->tot, out));
+  std::vector<double> h(3 * (size_t)d.ns);
+  HIPCHK(hipMemcpyAsync(h.data(), out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int n = 0; n < d.ns; ++n) {
+    if (aloft) aloft[n] = h[n];
+    if (deposit) deposit[n] = h[d.ns + n];
+    if (outflow) outflow[n] = h[2 * d.ns + n];
+  }
   return 0;
 }
 

@@ -81,6 +81,7 @@ _SIGS = {
                       ctypes.c_double, _ip],
     "ash3d_tr_step_glue": [ctypes.c_void_p],
     "ash3d_tr_source_column_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp],
+    "ash3d_tr_mass_totals": [ctypes.c_void_p, _dp, _dp, _dp],
     "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_int64)],
@@ -247,6 +248,15 @@ class Transport:
         self._check(self.lib.ash3d_tr_source_column_add(self._ctx, ivent, jvent, self.st.dt,
                                                         flux.ctypes.data_as(_dp)))
 
+
+    def mass_totals(self):
+        """Per-bin (aloft, deposit, outflow) masses in kg, computed on the
+        device (src/Output_Vars.f90:1255-1459 before the volume conversion)."""
+        ns = self.st.ns
+        a, dpo, o = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        self._check(self.lib.ash3d_tr_mass_totals(self._ctx, a.ctypes.data_as(_dp), dpo.ctypes.data_as(_dp),
+                                                  o.ctypes.data_as(_dp)))
+        return a, dpo, o
 
     # -- kernel timing ----------------------------------------------------
     def profile(self, enable: bool):
