@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--config", choices=("forecast", "cartesian"), default="forecast")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transport step shards "
@@ -113,6 +114,13 @@ def main():
         T.transport_step(itime, use_diffusion)
     T.synchronize()
     log(f"[bench] warm-up {args.warmup} steps done")
+    tuning = None
+    if not args.no_tune:
+        at = importlib.import_module("volcano-ash3d_amd.autotune")
+        tuning = at.autotune(T, use_diffusion)
+        log("[bench] autotune: " + json.dumps({k: [v[0], {str(a): round(b, 4) for a, b in (v[1] or {}).items()}]
+                                                for k, v in tuning.items()}))
+        itime += 1000
 
     T.profile(True)
     T.synchronize()
@@ -166,7 +174,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "nx": st.nx, "ny": st.ny, "nz": st.nz, "bins": st.ns,
                    "dt_h": st.dt, "diffusion": "crank-nicolson" if use_diffusion else "off",
-                   "parallelism": "single gpu"},
+                   "parallelism": "single gpu",
+                   "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults"},
         "step_hbm": {"bytes_per_bincell": round(bpb, 2), "frac_of_peak": round(step_frac, 4)},
         "roofline": roofline,
         "kernels": kernels,

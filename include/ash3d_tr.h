@@ -175,6 +175,20 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, 
  * the values are on the host. */
 int ash3d_tr_mass_totals(ash3d_tr_ctx *ctx, double *aloft, double *deposit, double *outflow);
 
+/* Launch tuning (no counterpart in the reference): the number of grain
+ * bins one thread carries through a sweep (1, 2, 4 or 8).  More bins share
+ * the per-cell terms (dt/kappa, interface speeds, line factorisations);
+ * fewer keep more waves resident.  Results do not depend on it. */
+enum {
+  ASH3D_TUNE_NB_ADVECT_X = 0, /* used when a row is too long for one workgroup */
+  ASH3D_TUNE_NB_ADVECT_Y = 1,
+  ASH3D_TUNE_NB_ADVECT_Z = 2,
+  ASH3D_TUNE_NB_DIFFUSE_X = 3,
+  ASH3D_TUNE_NB_DIFFUSE_Y = 4,
+  ASH3D_TUNE_NB_DIFFUSE_Z = 5
+};
+int ash3d_tr_tune(ash3d_tr_ctx *ctx, int32_t param, int32_t value);
+
 /* Kernel timing.  No counterpart in the reference, whose only timers are
  * wall-clock reads around the whole run (src/Ash3d.F90:176-177, :727-732).
  * ash3d_tr_profile(ctx,1) clears the records and brackets every later
@@ -190,7 +204,8 @@ enum {
   ASH3D_PROF_DIFFUSE_Z = 5,
   ASH3D_PROF_BC = 6,   /* Set_BC and per-direction ghost kernels */
   ASH3D_PROF_GLUE = 7, /* step glue */
-  ASH3D_PROF_NKINDS = 8
+  ASH3D_PROF_TRANSPOSE = 8, /* x<->y transposes around the x-line solves */
+  ASH3D_PROF_NKINDS = 9
 };
 int ash3d_tr_profile(ash3d_tr_ctx *ctx, int32_t enable);
 int ash3d_tr_profile_read(ash3d_tr_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);

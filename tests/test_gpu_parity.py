@@ -261,3 +261,20 @@ def test_mass_totals_on_device(pkg, tr, periodic):
     np.testing.assert_allclose(d1, ref_d, rtol=1e-13)
     np.testing.assert_allclose(o1, ref_o, rtol=1e-13, atol=1e-13 * ref_a.max())
     assert np.array_equal(a1, a2) and np.array_equal(d1, d2) and np.array_equal(o1, o2)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 4, 8])
+def test_tuning_does_not_change_results(pkg, oracle, tr, nb):
+    """Every bins-per-thread launch setting gives the oracle's bits."""
+    st = pkg.workloads.random_state(19, 14, 10, 7, seed=91, latlon=True, periodic=True)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    T.tune(**{p: nb for p in tr.Transport.TUNE_PARAMS})
+    for itime in (1, 2):
+        for X in (O, T):
+            X.step_glue()
+            X.transport_step(itime, True)
+    T.sync_to_host()
+    T.close()
+    assert_same(a, b)

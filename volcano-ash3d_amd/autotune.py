@@ -1,0 +1,57 @@
+"""Launch autotuner for the transport step.
+
+For each sweep kind it times every bins-per-thread setting (the number of
+grain bins one GPU thread carries along a line) with the library's HIP-event
+kernel timers, on the live context, and keeps the fastest.  The tuning
+steps advance the context's state like ordinary steps; results of a step do
+not depend on the setting (the kernels are bitwise identical across it).
+Choices can be cached in a JSON file keyed by grid shape and bin count.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+PARAMS = (("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("nb_diffuse_x", "diffuse_x"),
+          ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
+
+
+def _key(st):
+    return f"{st.nx}x{st.ny}x{st.nz}x{st.ns}"
+
+
+def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), itime0: int = 1000,
+             cache_path: str | None = None) -> dict:
+    st = T.st
+    if cache_path and os.path.exists(cache_path):
+        cached = json.load(open(cache_path)).get(_key(st))
+        if cached:
+            T.tune(**{p: v for p, v in cached.items()})
+            return {p: (v, None) for p, v in cached.items()}
+    chosen = {}
+    itime = itime0
+    for param, kind in PARAMS:
+        if kind.startswith("diffuse") and not use_diffusion:
+            continue
+        times = {}
+        for nb in candidates:
+            if nb > 1 and nb // 2 >= st.ns:
+                continue
+            T.tune(**{param: nb})
+            T.profile(True)
+            for _ in range(steps):
+                itime += 1
+                T.step_glue()
+                T.transport_step(itime, use_diffusion)
+            prof = T.profile_read()
+            T.profile(False)
+            ms, n = prof[kind]
+            times[nb] = ms / max(n, 1)
+        best = min(times, key=times.get)
+        T.tune(**{param: best})
+        chosen[param] = (best, times)
+    if cache_path:
+        data = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+        data[_key(st)] = {p: v[0] for p, v in chosen.items()}
+        json.dump(data, open(cache_path, "w"), indent=1)
+    return chosen

@@ -61,17 +61,21 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const long long ofs_bin = (long long)(g.Na + 4) * (g.Nb + 4);
   const int ldlo = rmin - 2, ldhi = rmax + 2;  // cells gathered by the reference
 
-  // halo cells shared with the neighbouring segments
-  double hq[4][NB];
-  const int hr[4] = {lo - 2, lo - 1, hi + 1, hi + 2};
+  // halo cells shared with the neighbouring segments (four named arrays so
+  // that they stay in registers)
+  double hq0[NB], hq1[NB], hq2[NB], hq3[NB];
+  auto halo = [&](int r, int u) {
+    return (any_on && exists[u] && r >= ldlo && r <= ldhi)
+               ? c[base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u)]
+               : 0.0;
+  };
 #pragma unroll
-  for (int h = 0; h < 4; ++h)
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      hq[h][u] = 0.0;
-      if (any_on && exists[u] && hr[h] >= ldlo && hr[h] <= ldhi)
-        hq[h][u] = c[base + sL * (long long)(hr[h] + 1) + d.ncell * (long long)(n0 + u)];
-    }
+  for (int u = 0; u < NB; ++u) {
+    hq0[u] = halo(lo - 2, u);
+    hq1[u] = halo(lo - 1, u);
+    hq2[u] = halo(hi + 1, u);
+    hq3[u] = halo(hi + 2, u);
+  }
   if (W > 1) __syncthreads();
 
   if (any_on) {
@@ -87,28 +91,50 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
       q0[u] = q1[u] = q2[u] = dq0[u] = dq1[u] = dq2[u] = fsr2[u] = fss2[u] = tp[u] = 0.0;
       vb1[u] = ub0[u] = ub1[u] = ub2[u] = 0.0;
     }
-    for (int r = lo - 2; r <= hi + 2; ++r) {
+    // ---- gather of cell r (:143-146 / :171-176), issued one cell ahead so
+    // the loads of cell r+1 are in flight while cell r is computed
+    double pf_sig = 0.0, pf_kap = 1.0, pf_vel = 0.0, pf_D = 0.0, pf_q[NB], pf_vf[NB];
+    auto gather = [&](int r) {
       const bool ld = (r >= ldlo && r <= ldhi);
       const long long cell = base + sL * (long long)(r + 1);
-      // ---- gather cell r (:143-146 / :171-176)
-      double vel0 = 0.0, sig0 = 0.0, kap0 = 1.0, vzh0 = 0.0, D0 = 0.0;
+      pf_sig = 0.0; pf_kap = 1.0; pf_vel = 0.0; pf_D = 0.0;
       if (ld) {
-        sig0 = A.sig[cell];
-        kap0 = A.kap[cell];
-        if (DIR == 2) vzh0 = A.vel[cell] + A.vh[cell];
-        else vel0 = A.vel[cell];
+        pf_sig = A.sig[cell];
+        pf_kap = A.kap[cell];
+        if (DIR == 2) pf_vel = A.vel[cell] + A.vh[cell];
+        else pf_vel = A.vel[cell];
         // DelDxonD_cc(r,j) for x lines, DelDyonD_cc(i,r) for y lines (:163-166, :414-417)
         if (TOPO && r >= rmin - 1 && r <= rmax + 1)
-          D0 = (DIR == 0) ? A.deld[(long long)(r + 1) + (long long)(d.nx + 4) * (a + 1)]
-                          : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (r + 1)];
+          pf_D = (DIR == 0) ? A.deld[(long long)(r + 1) + (long long)(d.nx + 4) * (a + 1)]
+                            : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (r + 1)];
       }
-      const int h = (r == lo - 2) ? 0 : (r == lo - 1) ? 1 : (r == hi + 1) ? 2 : (r == hi + 2) ? 3 : -1;
+      const bool halo_cell = (r == lo - 2) || (r == lo - 1) || (r == hi + 1) || (r == hi + 2);
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        q0[u] = 0.0;
-        if (h >= 0) q0[u] = hq[h][u];
-        else if (ld && exists[u]) q0[u] = c[cell + d.ncell * (long long)(n0 + u)];
+        pf_q[u] = 0.0;
+        pf_vf[u] = 0.0;
+        if (!halo_cell && ld && exists[u]) pf_q[u] = c[cell + d.ncell * (long long)(n0 + u)];
+        if (DIR == 2 && ld && exists[u]) pf_vf[u] = A.vf[cell + d.ncell * (long long)(n0 + u)];
       }
+    };
+    gather(lo - 2);
+    for (int r = lo - 2; r <= hi + 2; ++r) {
+      const bool ld = (r >= ldlo && r <= ldhi);
+      const double sig0 = pf_sig, kap0 = pf_kap, D0 = pf_D;
+      const double vel0 = (DIR == 2) ? 0.0 : pf_vel, vzh0 = (DIR == 2) ? pf_vel : 0.0;
+      double vfc[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        // the halo cells come from the registers filled before the barrier
+        double v = pf_q[u];
+        v = (r == lo - 2) ? hq0[u] : v;
+        v = (r == lo - 1) ? hq1[u] : v;
+        v = (r == hi + 1) ? hq2[u] : v;
+        v = (r == hi + 2) ? hq3[u] : v;
+        q0[u] = v;
+        vfc[u] = pf_vf[u];
+      }
+      if (r + 1 <= hi + 2) gather(r + 1);
       const bool uin = (r >= rmin - 1 && r <= rmax + 1);
       // dt/kappa (:155-156) -- bin independent
       const double dtv0 = uin ? dt / kap0 : 0.0;
@@ -119,7 +145,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
       for (int u = 0; u < NB; ++u) {
         vb0[u] = 0.0;
         if (DIR == 2) {
-          if (ld && exists[u]) vb0[u] = vzh0 + A.vf[cell + d.ncell * (long long)(n0 + u)];
+          if (ld && exists[u]) vb0[u] = vzh0 + vfc[u];
           ub0[u] = uin ? 0.5 * (vb1[u] + vb0[u]) * sig0 : 0.0;
         }
         dq0[u] = q0[u] - q1[u];  // (:168-169), used only for r >= rmin-1
@@ -248,11 +274,14 @@ hipError_t launch_nb(hipStream_t s, const AdvArgs &a) {
 
 }  // namespace
 
-int pick_bins_per_thread(int ns) {
-  if (ns >= 8) return 4;
-  if (ns >= 4) return 4;
-  if (ns >= 2) return 2;
-  return 1;
+// Default bins per thread (kind 0 advection, 1 diffusion; dir 0..2):
+// more bins share the per-cell terms, fewer keep more waves resident.
+int pick_bins_per_thread(int ns, int kind, int dir) {
+  // measured on the 360x240x50x15 forecast grid (bench autotuner)
+  const int cap = (kind == 0) ? 2 : (dir == 0 ? 4 : 8);
+  int nb = 1;
+  while (nb * 2 <= cap && nb * 2 <= ns) nb *= 2;
+  return nb;
 }
 
 hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb) {

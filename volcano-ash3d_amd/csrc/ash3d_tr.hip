@@ -72,8 +72,12 @@ struct ash3d_tr_ctx {
   double *flux = nullptr;  // source column staging (nz+1)*ns
   int *flag = nullptr;     // singular-pivot flag
   double *tot = nullptr;   // mass-total partial sums and results
+  // x-line solves run on an x<->y transposed copy: state, and the cached
+  // transposed kappa, sigma_nx and kx (rebuilt after a geometry/K upload)
+  double *tq = nullptr, *tkap = nullptr, *tsx = nullptr, *tkx = nullptr;
+  bool tgeom_valid = false;
   int ts1_state = TS1_EXPLICIT;
-  int nb_adv = 4, nb_diff = 4;
+  int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -196,7 +200,7 @@ int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, c
   }
   {
     ProfScope ps(x, ASH3D_PROF_ADVECT_X + dir);
-    HIPCHK(launch_advect(x->stream, a, x->nb_adv));
+    HIPCHK(launch_advect(x->stream, a, x->nb_adv[dir]));
   }
   x->ts1_state = TS1_MIRROR;
   return 0;
@@ -224,13 +228,40 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
   a.kap = x->kap;
   a.kk = dir == 0 ? x->kx : (dir == 1 ? x->ky : x->kz);
   a.sig = dir == 0 ? x->sx : (dir == 1 ? x->sy : x->sz);
-  if (use_cn) {
-    if (a.g.rmax - a.g.rmin + 1 <= 1) {
-      // ncells <= 1: the reference zero-fills ts1 and copies nothing (:770, :909)
-      x->ts1_state = TS1_ZERO;
-      return 0;
+  if (use_cn && a.g.rmax - a.g.rmin + 1 <= 1) {
+    // ncells <= 1: the reference zero-fills ts1 and copies nothing (:770, :909)
+    x->ts1_state = TS1_ZERO;
+    return 0;
+  }
+  const bool transposed = (dir == 0);
+  if (transposed) {
+    // x lines as the y lines of the transposed grid (x and y swapped)
+    const Dims &d = x->d;
+    if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
+    if (!x->tgeom_valid) {
+      if (!x->tkap && ((rc = dev_alloc(&x->tkap, d.ncell)) || (rc = dev_alloc(&x->tsx, d.ncell)) ||
+                       (rc = dev_alloc(&x->tkx, x->n_k()))))
+        return rc;
+      ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+      HIPCHK(launch_transpose_xy(x->stream, x->kap, x->tkap, d.nx + 4, d.ny + 4, d.nz + 4));
+      HIPCHK(launch_transpose_xy(x->stream, x->sx, x->tsx, d.nx + 4, d.ny + 4, d.nz + 4));
+      HIPCHK(launch_transpose_xy(x->stream, x->kx, x->tkx, d.nx + 2, d.ny + 2, d.nz + 2));
+      x->tgeom_valid = true;
     }
-    const long long groups = (x->d.ns + x->nb_diff - 1) / x->nb_diff;
+    {
+      ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+      HIPCHK(launch_transpose_xy(x->stream, x->ts0(), x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));
+    }
+    const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
+    a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
+    a.g = make_line_geom(a.d, 1, tb);
+    a.c0 = x->tq;
+    a.kap = x->tkap;
+    a.sig = x->tsx;
+    a.kk = x->tkx;
+  }
+  if (use_cn) {
+    const long long groups = (x->d.ns + x->nb_diff[dir] - 1) / x->nb_diff[dir];
     if (groups > x->fac_groups) {
       if (x->fac) HIPCHK(hipFree(x->fac));
       x->fac = nullptr;
@@ -239,10 +270,15 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     }
     a.fac = x->fac;
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
-    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff));
+    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
   } else {
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
-    HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff));
+    HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff[dir]));
+  }
+  if (transposed) {
+    ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+    HIPCHK(launch_transpose_xy(x->stream, x->tq, x->ts0(), x->d.ny + 4, x->d.nx + 4,
+                               (long long)(x->d.nz + 4) * x->d.ns));
   }
   x->ts1_state = TS1_MIRROR;
   return 0;
@@ -289,8 +325,10 @@ int ash3d_tr_create(const ash3d_tr_desc *desc, ash3d_tr_ctx **out) {
   x->periodic = desc->is_periodic ? 1 : 0;
   x->zscaling = desc->zscaling_id;
   x->limiter = desc->limiter;
-  x->nb_adv = pick_bins_per_thread(desc->ns);
-  x->nb_diff = pick_bins_per_thread(desc->ns);
+  for (int k = 0; k < 3; ++k) {
+    x->nb_adv[k] = pick_bins_per_thread(desc->ns, 0, k);
+    x->nb_diff[k] = pick_bins_per_thread(desc->ns, 1, k);
+  }
   int rc = 0;
   if (hipSetDevice(x->device) != hipSuccess || hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
     delete x;
@@ -337,7 +375,7 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   if (x->stream) hipStreamSynchronize(x->stream);
   double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
-                    x->fac, x->flux, x->tot};
+                    x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx};
   for (double *p : ptrs)
     if (p) hipFree(p);
   if (x->flag) hipFree(x->flag);
@@ -366,6 +404,7 @@ int ash3d_tr_sync_from_host(ash3d_tr_ctx *x, uint32_t mask) {
   }
   if (mask & ASH3D_SYNC_VF)
     if ((rc = upload(x, x->vf, h.vf_pd, x->n_conc(), 0.0))) return rc;
+  if (mask & (ASH3D_SYNC_GEOMETRY | ASH3D_SYNC_DIFFK)) x->tgeom_valid = false;
   if (mask & ASH3D_SYNC_GEOMETRY) {
     if ((rc = upload(x, x->kap, h.kappa_pd, d.ncell, 1.0)) || (rc = upload(x, x->sx, h.sigma_nx_pd, d.ncell, 1.0)) ||
         (rc = upload(x, x->sy, h.sigma_ny_pd, d.ncell, 1.0)) || (rc = upload(x, x->sz, h.sigma_nz_pd, d.ncell, 1.0)))
@@ -579,6 +618,22 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, do
                         x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(launch_source_add(x->stream, x->d, x->ts0(), ivent, jvent, dt, x->flux));
+  return 0;
+}
+
+int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (value != 1 && value != 2 && value != 4 && value != 8)
+    return fail(ASH3D_TR_EINVAL, "bins per thread must be 1, 2, 4 or 8 (got %d)", value);
+  if (param >= ASH3D_TUNE_NB_ADVECT_X && param <= ASH3D_TUNE_NB_ADVECT_Z) {
+    x->nb_adv[param - ASH3D_TUNE_NB_ADVECT_X] = value;
+  } else if (param >= ASH3D_TUNE_NB_DIFFUSE_X && param <= ASH3D_TUNE_NB_DIFFUSE_Z) {
+    HIPCHK(hipStreamSynchronize(x->stream));  // the CN scratch is sized by it
+    x->nb_diff[param - ASH3D_TUNE_NB_DIFFUSE_X] = value;
+  } else {
+    return fail(ASH3D_TR_EINVAL, "unknown tuning parameter %d", param);
+  }
   return 0;
 }
 

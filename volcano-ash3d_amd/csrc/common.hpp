@@ -124,8 +124,9 @@ hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
 hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
-int pick_bins_per_thread(int ns);
+int pick_bins_per_thread(int ns, int kind, int dir);
 long long mass_partials_size(const Dims &d);
+hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes);
 hipError_t launch_mass_totals(hipStream_t s, const Dims &d, const double *c0, const double *kap,
                               const double *dep, const double *yz1, const double *yz2, const double *xz1,
                               const double *xz2, const double *xy2, double *partial, double *out);

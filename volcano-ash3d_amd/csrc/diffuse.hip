@@ -166,10 +166,24 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     // original sub-diagonal entry that row i+1 carries
     double Dc = 0.0, DUc = 0.0, Bc[NB];
     int singular = 0;
+    // right-neighbour data of row l, fetched one row ahead
+    double pf_kap = kapat(rmin + 1), pf_K = Kat(rmin + 1), pf_sig = sigat(rmin + 1), pf_q[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) pf_q[u] = (1 < n && exists[u]) ? qat(u, rmin + 1) : 0.0;
     for (int l = 1; l <= n; ++l) {
       const int cc = rmin - 1 + l;
       // cell cc+1 (right neighbour) data
-      const double kap_r = kapat(cc + 1), K_r = Kat(cc + 1), sig_r = sigat(cc + 1);
+      const double kap_r = pf_kap, K_r = pf_K, sig_r = pf_sig;
+      double q_next[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) q_next[u] = pf_q[u];
+      if (l < n) {
+        pf_kap = kapat(cc + 2);
+        pf_K = Kat(cc + 2);
+        pf_sig = sigat(cc + 2);
+#pragma unroll
+        for (int u = 0; u < NB; ++u) pf_q[u] = (l + 1 < n && exists[u]) ? qat(u, cc + 2) : 0.0;
+      }
       const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
       const double LeftFac = dt * kds_c * sig_c / kap_c;
       const double RightFac = dt * kds_r * sig_r / kap_c;
@@ -189,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
       double Bn[NB], q_r[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        q_r[u] = (l < n) ? (exists[u] ? qat(u, cc + 1) : 0.0) : q_c[u];  // BC_right_t0 (:845)
+        q_r[u] = (l < n) ? q_next[u] : q_c[u];  // BC_right_t0 (:845)
         Bn[u] = (1.0 - th) * LeftFac * q_l[u] + (1.0 - (1.0 - th) * CenterFac) * q_c[u] +
                 (1.0 - th) * RightFac * q_r[u];
       }
@@ -266,12 +280,27 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         if (on[u]) c[base + sL * (long long)(rmax + 1) + d.ncell * (long long)(n0 + u)] = x1[u];
       }
     }
+    // row i's factors and eliminated right-hand sides, fetched one row ahead
+    double pf_D = 0.0, pf_DU = 0.0, pf_DL = 0.0, pf_B[NB];
+    auto fetch_row = [&](int i) {
+      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      pf_D = fD[fo];
+      pf_DU = fDU[fo];
+      pf_DL = fDL[fo];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) pf_B[u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+    };
+    if (n - 1 >= 1) fetch_row(n - 1);
     for (int i = n - 1; i >= 1; --i) {
       const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
-      const double Di = fD[fo], DUi = fDU[fo], DLi = fDL[fo];
+      const double Di = pf_D, DUi = pf_DU, DLi = pf_DL;
+      double Bcur[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) Bcur[u] = pf_B[u];
+      if (i - 1 >= 1) fetch_row(i - 1);
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const double Bi = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+        const double Bi = Bcur[u];
         double xi;
         if (i == n - 1) xi = (Bi - DUi * x1[u]) / Di;
         else xi = (Bi - DUi * x1[u] - DLi * x2[u]) / Di;

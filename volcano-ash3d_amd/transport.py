@@ -82,11 +82,13 @@ _SIGS = {
     "ash3d_tr_step_glue": [ctypes.c_void_p],
     "ash3d_tr_source_column_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp],
     "ash3d_tr_mass_totals": [ctypes.c_void_p, _dp, _dp, _dp],
+    "ash3d_tr_tune": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_int64)],
 }
-PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue")
+PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue",
+              "transpose")
 EXPORTS = tuple(_SIGS) + ("ash3d_tr_last_error", "ash3d_tr_build_info")
 
 _lib = None
@@ -257,6 +259,14 @@ class Transport:
         self._check(self.lib.ash3d_tr_mass_totals(self._ctx, a.ctypes.data_as(_dp), dpo.ctypes.data_as(_dp),
                                                   o.ctypes.data_as(_dp)))
         return a, dpo, o
+
+    # -- launch tuning ----------------------------------------------------
+    TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z")
+
+    def tune(self, **kw):
+        """Set bins-per-thread launch parameters, e.g. tune(nb_advect_z=2)."""
+        for name, value in kw.items():
+            self._check(self.lib.ash3d_tr_tune(self._ctx, self.TUNE_PARAMS.index(name), int(value)))
 
     # -- kernel timing ----------------------------------------------------
     def profile(self, enable: bool):
