@@ -72,6 +72,63 @@ def cpu_baseline(st, use_diffusion, steps):
                       f"{st.nx}x{st.ny}x{st.nz}x{st.ns} workload, {dt:.1f} s"}
 
 
+def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
+    """Time a source-inversion gradient on the same grid: forward run with
+    checkpoints, then the checkpointed reverse sweep.  Returns the
+    adjoint/forward time ratio and the per-step times of ash3d_tr_adj_step
+    and ash3d_tr_step on resident states."""
+    adj = importlib.import_module("volcano-ash3d_amd.adjoint")
+    import numpy as np
+    st.concen_pd[...] = 0.0
+    dz = np.full(st.nz, 0.5)
+    H = min(40, st.nz)
+    frac = np.full(st.ns, 1.0 / st.ns)
+    inv = adj.SourceInversion(st, st.nx // 4, st.ny // 2, n_heights=H, n_periods=periods,
+                              steps_per_period=steps_per_period, frac=frac, dz=dz, obs_every=steps_per_period // 2,
+                              use_diffusion=use_diffusion)
+    rng = np.random.default_rng(0)
+    s = 1e6 * rng.random((periods, H))
+    w = rng.random((len(inv.obs_steps), st.nx, st.ny))
+    inv.forward(s)  # warm-up (allocations, first-touch)
+    inv.gradient(w)
+    inv.T.synchronize()
+    t0 = time.perf_counter()
+    inv.forward(s)
+    inv.T.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = inv.gradient(w)
+    inv.T.synchronize()
+    t_grad = time.perf_counter() - t0
+    # kernel-level: one adjoint step vs one forward step on a resident state
+    T = inv.T
+    T.profile(True)
+    reps = 5
+    T.checkpoint_load(0)
+    T.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        T.adj_step(1 + r, use_diffusion)
+    T.synchronize()
+    t_adj = (time.perf_counter() - t0) / reps
+    prof = T.profile_read()
+    T.profile(False)
+    T.checkpoint_load(0)
+    T.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        T.transport_step(1 + r, use_diffusion)
+    T.synchronize()
+    t_step = (time.perf_counter() - t0) / reps
+    inv.close()
+    return {"workload": f"source inversion, {periods} periods x {steps_per_period} steps, {H} source heights, "
+                        f"column loads every {steps_per_period // 2} steps, checkpoint segment {inv.K}",
+            "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
+            "adj_step_ms": round(1e3 * t_adj, 3), "step_ms": round(1e3 * t_step, 3),
+            "step_ratio": round(t_adj / t_step, 3), "grad_norm": float(np.abs(g).max()),
+            "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +138,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--adj-periods", type=int, default=2)
+    ap.add_argument("--adj-steps-per-period", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transport step shards "
@@ -103,6 +163,7 @@ def main():
         workload = "projected 100x100x30, 5 km, uniform wind, 1 bin, no diffusion, superbee"
     st.concen_pd[..., 1] = st.concen_pd[..., 0]  # the glue copies ts1 into ts0 each step
     cpu_state = None if args.no_cpu_baseline else st.copy()
+    adj_state = None if args.no_adjoint else st.copy()
     log(f"[bench] state built in {time.perf_counter() - t0:.1f} s, dt={st.dt} h, "
         f"{st.nbytes() / 1e9:.2f} GB host arrays")
 
@@ -162,6 +223,11 @@ def main():
     log("[bench] kernels: " + json.dumps(kernels))
 
     T.close()
+    adjoint = None
+    if adj_state is not None:
+        log("[bench] adjoint leg ...")
+        adjoint = adjoint_leg(pkg, adj_state, use_diffusion, args.adj_periods, args.adj_steps_per_period)
+        log("[bench] adjoint: " + json.dumps(adjoint))
     cpu = None
     if cpu_state is not None:
         log("[bench] timing the CPU oracle ...")
@@ -179,6 +245,7 @@ def main():
         "step_hbm": {"bytes_per_bincell": round(bpb, 2), "frac_of_peak": round(step_frac, 4)},
         "roofline": roofline,
         "kernels": kernels,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (rate / cpu["value"]) if cpu else None,
     }

@@ -205,10 +205,65 @@ enum {
   ASH3D_PROF_BC = 6,   /* Set_BC and per-direction ghost kernels */
   ASH3D_PROF_GLUE = 7, /* step glue */
   ASH3D_PROF_TRANSPOSE = 8, /* x<->y transposes around the x-line solves */
-  ASH3D_PROF_NKINDS = 9
+  ASH3D_PROF_ADJ_ADVECT = 9,   /* adjoint advection sweeps */
+  ASH3D_PROF_ADJ_DIFFUSE = 10, /* adjoint diffusion sweeps */
+  ASH3D_PROF_COPY = 11,        /* state snapshots and checkpoints */
+  ASH3D_PROF_NKINDS = 12
 };
 int ash3d_tr_profile(ash3d_tr_ctx *ctx, int32_t enable);
 int ash3d_tr_profile_read(ash3d_tr_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
+
+/* ---- Checkpoints ------------------------------------------------------
+ * No counterpart in the reference (which only writes output files).  Slots
+ * of device memory, each one copy of concen_pd(:,:,:,:,ts0).  Save copies
+ * ts0 into a slot; load copies a slot into ts0 and leaves ts1 equal to its
+ * interior with zero ghosts, as after a sweep, so that a following
+ * ash3d_tr_step_glue or ash3d_tr_step continues from the saved state. */
+enum { ASH3D_CKPT_SAVE = 0, ASH3D_CKPT_LOAD = 1 };
+int ash3d_tr_checkpoints(ash3d_tr_ctx *ctx, int32_t nslots);
+int ash3d_tr_checkpoint(ash3d_tr_ctx *ctx, int32_t slot, int32_t op);
+
+/* ---- Discrete adjoint --------------------------------------------------
+ * No counterpart in the reference.  The adjoint variable lambda has the
+ * layout of concen_pd(:,:,:,:,ts0) (ghost entries stay zero).  Each adjoint
+ * entry point applies the transpose of the Jacobian of the matching forward
+ * routine, linearised at the forward input that is in ts0 when it is
+ * called: lambda <- J(ts0)^T lambda.  For the limited DCU sweeps the
+ * limiter branches are those the forward sweep takes on that input, so the
+ * forward map is F(q) = J(q) q and the adjoint is exact for it; the
+ * diffusion sweeps are linear.  ts0 is read, not changed (except that the
+ * advection entry points set its Set_BC(1) ghosts, as the forward does).
+ * Outflow and deposit are not differentiated. */
+/* Allocates the adjoint buffers, zeroes lambda and n_source_slots source
+ * gradient slots of (nz+1)*ns values. */
+int ash3d_tr_adj_init(ash3d_tr_ctx *ctx, int32_t n_source_slots);
+/* Host copies of lambda (ns*(nx+4)*(ny+4)*(nz+4) values); set(NULL) zeroes. */
+int ash3d_tr_adj_set(ash3d_tr_ctx *ctx, const double *lambda);
+int ash3d_tr_adj_get(ash3d_tr_ctx *ctx, double *lambda);
+/* Adjoints of advect_x/y/z (dir 0/1/2; forward: src/AdvectionHorz_DCU.f90:
+ * 68-305, :322-556; src/AdvectionVert_DCU.f90:100-323). */
+int ash3d_tr_adj_advect(ash3d_tr_ctx *ctx, int32_t dir, double dt, const int32_t *is_aloft,
+                        const int32_t bounds[6]);
+/* Adjoints of diff_x/y/z and diffCN_x/y/z (src/Diffusion.F90:266-650,
+ * :671-1428).  Linear, so ts0 is not used. */
+int ash3d_tr_adj_diffuse(ash3d_tr_ctx *ctx, int32_t dir, int32_t use_cn, double dt, double imp_fac,
+                         const int32_t *is_aloft, const int32_t bounds[6]);
+/* Adjoint of ash3d_tr_step with its input state in ts0: the forward
+ * advection sweeps are recomputed from ts0 into private snapshots, then the
+ * adjoint sweeps run in reverse order. */
+int ash3d_tr_adj_step(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                      double imp_fac, const int32_t *is_aloft);
+/* Source gradient: slot(k,n) += dt * lambda(ivent,jvent,k,n), k = 1..nz+1,
+ * the adjoint of ash3d_tr_source_column_add; read returns all slots
+ * (slot-major, then bin, then k). */
+int ash3d_tr_adj_source(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, double dt, int32_t slot);
+int ash3d_tr_adj_source_read(ash3d_tr_ctx *ctx, double *out);
+/* Column load L(i,j) = sum_k sum_n concen(i,j,k,n,ts0) * dz(k) / 1e3
+ * (t/km2, ConcentrationCalculator, src/Output_Vars.f90:1058-1064, without
+ * its display threshold), out(nx,ny) x fastest; dz(1:nz) in km.  The
+ * adjoint adds L^T w to lambda. */
+int ash3d_tr_column_load(ash3d_tr_ctx *ctx, const double *dz, double *out);
+int ash3d_tr_adj_column_load(ash3d_tr_ctx *ctx, const double *dz, const double *w);
 
 /* Message of the last failed call on this thread. */
 const char *ash3d_tr_last_error(void);

@@ -10,7 +10,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 PKG = os.path.join(ROOT, "volcano-ash3d_amd")
 SHIMS = ["ash3d_tr_c.f90", "Diffusion.F90", "ash3d_tr_attach.F90", "AdvectionHorz_DCU.F90",
-         "AdvectionHorz.F90", "AdvectionVert_DCU.F90", "Set_BC.F90"]
+         "AdvectionHorz.F90", "AdvectionVert_DCU.F90", "Set_BC.F90", "Ash3d_Adjoint.F90"]
+# test programs: the reference driver's hot-path sequence, and a source
+# inversion through the adjoint entry points (Crank-Nicolson build only)
+PROGRAMS = {"shim_driver": ("cn", "ex"), "inversion_driver": ("cn",)}
 FC = os.environ.get("ASH3D_FC", "/opt/rocm/llvm/bin/amdflang")
 
 
@@ -19,18 +22,21 @@ def build(verbose=False):
     for variant, defs in (("cn", ["-DCRANKNIC", "-DLIM_SUPERBEE"]), ("ex", ["-DLIM_SUPERBEE"])):
         bdir = os.path.join(HERE, "_build", variant)
         os.makedirs(bdir, exist_ok=True)
-        srcs = [os.path.join(HERE, "host_modules.F90")] + [os.path.join(PKG, "fortran", s) for s in SHIMS] + \
-               [os.path.join(HERE, "shim_driver.F90")]
-        exe = os.path.join(bdir, "shim_driver")
-        cmd = [FC, "-O2", "-cpp", *defs, "-J", bdir, "-I", bdir, *srcs, "-o", exe,
-               "-L", PKG, "-lash3d_tr", f"-Wl,-rpath,{PKG}"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError("building the shim driver failed")
-        outs[variant] = exe
+        for prog, variants in PROGRAMS.items():
+            if variant not in variants:
+                continue
+            srcs = [os.path.join(HERE, "host_modules.F90")] + \
+                [os.path.join(PKG, "fortran", s) for s in SHIMS] + [os.path.join(HERE, prog + ".F90")]
+            exe = os.path.join(bdir, prog)
+            cmd = [FC, "-O2", "-ffp-contract=off", "-cpp", *defs, "-J", bdir, "-I", bdir, *srcs, "-o", exe,
+                   "-L", PKG, "-lash3d_tr", f"-Wl,-rpath,{PKG}"]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError(f"building {prog} failed")
+            outs[(prog, variant)] = exe
     return outs
 
 

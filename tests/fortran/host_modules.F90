@@ -35,6 +35,7 @@
 #else
       logical, parameter       :: useCN   = .false.
 #endif
+      logical                  :: useDiffusion     = .true.
       logical                  :: useVarDiffH      = .false.
       logical                  :: useVarDiffV      = .false.
       end module global_param
@@ -51,6 +52,7 @@
       real(kind=ip),dimension(:,:,:),allocatable :: sigma_nx_pd
       real(kind=ip),dimension(:,:,:),allocatable :: sigma_ny_pd
       real(kind=ip),dimension(:,:,:),allocatable :: sigma_nz_pd
+      real(kind=ip),dimension(:)    ,allocatable :: dz_vec_pd
       end module mesh
 
       module solution

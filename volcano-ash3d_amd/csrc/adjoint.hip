@@ -1,0 +1,683 @@
+// Discrete adjoint of the transport step.
+//
+// Advection.  For fixed limiter branches a DCU sweep is linear in q: the
+// limited difference is ldq = alpha*dq_I + beta*dqu_I, with (alpha, beta)
+// fixed by the branch phi(theta) takes (superbee: (0,0), (0,2), (1,0),
+// (0,1), (2,0); the clamped theta = +-3 gives (phi(+-3), 0)), and every
+// other term of the update (src/AdvectionHorz_DCU.f90:238-281) is linear.
+// The adjoint kernels recompute the branches from the pre-sweep state
+// exactly as the forward sweep does and apply the transpose of that linear
+// map to lambda, the adjoint of the post-sweep interior.  With
+// mu_c = -dt/kappa_c * lambda_c on the updated cells, the update pairs
+// fs_l(I) with mu_{I-1}, fs_r(I) with mu_I and fss(I) with mu_{I-1} - mu_I;
+// interface I contributes to the inputs q_{I-2} .. q_{I+1}.  Ghost inputs
+// are zero (Set_BC(1)) except the periodic x copies, whose adjoint is
+// folded back onto cells 1, 2, nx-1, nx.  Interior cells outside the
+// active ranges leave the sweep as zero, so only their stencil terms
+// remain.
+//
+// Diffusion.  A theta-scheme line maps q to x = A^{-1} B q
+// (src/Diffusion.F90:807-890); its adjoint is lambda' = B^T A^{-T} lambda:
+// a tridiagonal solve with the transposed matrix (the elimination with
+// partial pivoting of dgtsv) and the transposed right-hand-side stencil.
+// The row coefficients are recomputed from kappa, sigma and K on both
+// passes instead of being stored.  The explicit scheme
+// (src/Diffusion.F90:335-377) is a three-point stencil whose Neumann (or
+// periodic x) ghosts fold back onto the end cells.
+#include "common.hpp"
+#include "dcu.hpp"
+
+namespace ash3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// (alpha, beta) of ldq = alpha*dq + beta*dqu for the branch the forward
+// limiter (dcu.hpp limited_flux) takes; (0,0) where it skips the interface.
+__device__ inline void limiter_coeffs(int lim, double dqm, double dq0, double dqp, double u, double &alpha,
+                                      double &beta) {
+  alpha = 0.0;
+  beta = 0.0;
+  if (lim == 0 || fabs(dq0) <= EPS_THRESH) return;
+  if (lim == 1) { alpha = 1.0; return; }
+  const double dqu = (u > 0.0) ? dqm : dqp;
+  const bool clamped = fabs(dqu) > 3.0 * fabs(dq0) || fabs(dq0) < EPS_THRESH;
+  const double theta = clamped ? copysign(3.0, dqu * dq0) : dqu / dq0;
+  // ldq = dq*phi(theta); on a branch phi = p0 + p1*theta, so
+  // ldq = p0*dq + p1*dqu, or dq*phi(theta) with theta constant when clamped
+  double p0 = 0.0, p1 = 0.0;
+  switch (lim) {
+    case 2: beta = 1.0; return;  // Beam-Warming: ldq = dqu
+    case 3: p0 = 0.5; p1 = 0.5; break;  // Fromm
+    case 4:  // minmod
+      if (theta <= 0.0) {} else if (theta < 1.0) p1 = 1.0; else p0 = 1.0;
+      break;
+    case 5:  // superbee
+      if (theta <= 0.0) {}
+      else if (theta <= 0.5) p1 = 2.0;
+      else if (theta <= 1.0) p0 = 1.0;
+      else if (theta <= 2.0) p1 = 1.0;
+      else p0 = 2.0;
+      break;
+    case 6: {  // MC
+      const double a = (1.0 + theta) / 2.0, b2 = 2.0 * theta;
+      if (theta <= 0.0) {}
+      else if (b2 <= a && b2 <= 2.0) p1 = 2.0;
+      else if (a <= 2.0) { p0 = 0.5; p1 = 0.5; }
+      else p0 = 2.0;
+      break;
+    }
+    default: return;
+  }
+  if (clamped) alpha = p0 + p1 * theta;
+  else { alpha = p0; beta = p1; }
+}
+
+// Contributions of interface I to the adjoints of q_{I-2}, q_{I-1}, q_I,
+// q_{I+1} (e0..e3).  cl = mu_{I-1}, cr = mu_I; g = 0.5|u|(1 - dtv_I|u|);
+// tfD and tfDm1 the topographic factors tf*D of cells I and I-1.
+__device__ inline void interface_adjoint(int lim, double qm2, double qm1, double q0, double qp1, double u,
+                                         double divp, double divm, double g, double tfD, double tfDm1,
+                                         double cl, double cr, double &e0, double &e1, double &e2,
+                                         double &e3) {
+  e0 = 0.0;
+  e3 = 0.0;
+  // fs_l(I) = min(0,u)(q_I - q_{I-1}) + q_{I-1} divp - tf_{I-1} q_{I-1} D_{I-1}
+  // fs_r(I) = max(0,u)(q_I - q_{I-1}) + q_I divm + tf_I q_I D_I
+  e2 = fmin(0.0, u) * cl + (fmax(0.0, u) + divm + tfD) * cr;
+  e1 = (divp - fmin(0.0, u) - tfDm1) * cl - fmax(0.0, u) * cr;
+  double al, be;
+  limiter_coeffs(lim, qm1 - qm2, q0 - qm1, qp1 - q0, u, al, be);
+  const double w = g * (cl - cr);
+  e2 += al * w;
+  e1 -= al * w;
+  if (be != 0.0) {
+    if (u > 0.0) { e1 += be * w; e0 -= be * w; }
+    else { e3 += be * w; e2 -= be * w; }
+  }
+}
+
+// ---- y and z: one lane per line, streaming along it -------------------
+template <int DIR, bool TOPO>
+__global__ __launch_bounds__(kBlock) void k_adj_advect_lines(AdvArgs A, const double *__restrict__ qpre,
+                                                             double *lam, int nb) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const int lim = A.limiter;
+  const double dt = A.dt;
+  auto cell = [&](int r) { return base + sL * (long long)(r + 1); };
+  auto ld_ok = [&](int r) { return r >= rmin - 2 && r <= rmax + 2; };
+  auto uin = [&](int r) { return r >= rmin - 1 && r <= rmax + 1; };
+
+  for (int n = (int)blockIdx.y * nb; n < d.ns && n < ((int)blockIdx.y + 1) * nb; ++n) {
+    const long long bofs = d.ncell * (long long)n;
+    if (!(line_on && A.aloft.on(n))) {
+      for (int r = 1; r <= N; ++r) lam[cell(r) + bofs] = 0.0;
+      continue;
+    }
+    auto qv = [&](int r) { return ld_ok(r) ? qpre[cell(r) + bofs] : 0.0; };
+    auto velv = [&](int r) -> double {
+      if (!ld_ok(r)) return 0.0;
+      if (DIR == 2) return A.vel[cell(r)] + A.vh[cell(r)] + A.vf[cell(r) + bofs];
+      return A.vel[cell(r)];
+    };
+    auto sigv = [&](int r) { return ld_ok(r) ? A.sig[cell(r)] : 0.0; };
+    auto uv = [&](int I) { return uin(I) ? 0.5 * (velv(I - 1) + velv(I)) * sigv(I) : 0.0; };
+    auto tfDv = [&](int r) -> double {
+      if (!TOPO || !uin(r)) return 0.0;
+      const double D = (DIR == 0) ? A.deld[(long long)(r + 1) + (long long)(d.nx + 4) * (a + 1)]
+                                  : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (r + 1)];
+      return 0.5 * velv(r) * (sigv(r) + sigv(r + 1)) * D;
+    };
+    auto muv = [&](int c) {
+      return (c >= rmin && c <= rmax) ? -(dt / A.kap[cell(c)]) * lam[cell(c) + bofs] : 0.0;
+    };
+    double ghost[4] = {0.0, 0.0, 0.0, 0.0};  // adjoint reaching cells -1, 0, N+1, N+2
+    // final adjoint of input cell r; lam at r is read here for the last time
+    auto emit = [&](int r, double v) {
+      if (r >= 1 && r <= N) {
+        const double direct = (r >= rmin && r <= rmax) ? lam[cell(r) + bofs] : 0.0;
+        lam[cell(r) + bofs] = direct + v;
+      } else if (r == -1) ghost[0] += v;
+      else if (r == 0) ghost[1] += v;
+      else if (r == N + 1) ghost[2] += v;
+      else if (r == N + 2) ghost[3] += v;
+    };
+    for (int r = 1; r < rmin - 2 && r <= N; ++r) emit(r, 0.0);
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;  // pending adjoints of I-2, I-1, I
+    double q_m2 = qv(rmin - 2), q_m1 = qv(rmin - 1), q_0 = qv(rmin), q_p1 = qv(rmin + 1);
+    double u_m1 = uv(rmin - 1), u_0 = uv(rmin), u_p1 = uv(rmin + 1);
+    double mu_m1 = 0.0, mu_0 = muv(rmin);
+    double tfD_m1 = tfDv(rmin - 1), tfD_0 = tfDv(rmin);
+    for (int I = rmin; I <= rmax + 1; ++I) {
+      const double divp = fmax(0.0, u_0) - fmax(0.0, u_m1);
+      const double divm = fmin(0.0, u_p1) - fmin(0.0, u_0);
+      const double aus = fabs(u_0);
+      const double gI = 0.5 * aus * (1.0 - (dt / A.kap[cell(I)]) * aus);
+      double e0, e1, e2, e3;
+      interface_adjoint(lim, q_m2, q_m1, q_0, q_p1, u_0, divp, divm, gI, tfD_0, tfD_m1, mu_m1, mu_0, e0, e1,
+                        e2, e3);
+      emit(I - 2, acc0 + e0);
+      acc0 = acc1 + e1;
+      acc1 = acc2 + e2;
+      acc2 = e3;
+      q_m2 = q_m1; q_m1 = q_0; q_0 = q_p1; q_p1 = qv(I + 2);
+      u_m1 = u_0; u_0 = u_p1; u_p1 = uv(I + 2);
+      mu_m1 = mu_0; mu_0 = muv(I + 1);
+      tfD_m1 = tfD_0; tfD_0 = tfDv(I + 1);
+    }
+    emit(rmax, acc0);
+    emit(rmax + 1, acc1);
+    emit(rmax + 2, acc2);
+    for (int r = rmax + 3; r <= N; ++r) emit(r, 0.0);
+    if (A.periodic) {
+      // Set_BC(1) copies: cell -1 <- N-1, 0 <- N, N+1 <- 1, N+2 <- 2
+      lam[cell(N - 1) + bofs] += ghost[0];
+      lam[cell(N) + bofs] += ghost[1];
+      lam[cell(1) + bofs] += ghost[2];
+      lam[cell(2) + bofs] += ghost[3];
+    }
+  }
+}
+
+// ---- x: one workgroup per row, gather through LDS ---------------------
+template <int CPT, bool TOPO>
+__global__ __launch_bounds__(256) void k_adj_advect_x_rows(AdvArgs A, const double *__restrict__ qpre,
+                                                           double *lam) {
+  extern __shared__ double smem[];
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int nx = d.nx;
+  const int W = nx + 4;  // cells -1..nx+2 at LDS index c+1
+  double *s_q = smem, *s_mu = smem + W;
+  double *s_e0 = smem + 2 * W, *s_e1 = smem + 3 * W, *s_e2 = smem + 4 * W, *s_e3 = smem + 5 * W;
+
+  const int j = (int)(blockIdx.x % (unsigned)d.ny) + 1;
+  const int k = (int)(blockIdx.x / (unsigned)d.ny) + 1;
+  const int rmin = g.rmin, rmax = g.rmax;
+  const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
+  const long long base = d.p3(-1, j, k);
+  const int t = threadIdx.x, BT = blockDim.x;
+  const int lim = A.limiter;
+
+  int cc[CPT];
+  bool valid[CPT];
+#pragma unroll
+  for (int m = 0; m < CPT; ++m) {
+    cc[m] = -1 + t + BT * m;
+    valid[m] = cc[m] <= nx + 2;
+  }
+  bool any_on = false;
+  if (row_on)
+    for (int n = 0; n < d.ns; ++n) any_on = any_on || A.aloft.on(n);
+
+  // bin-independent terms at interface / cell c (as the forward row kernel)
+  double u[CPT], dtv[CPT], divp[CPT], divm[CPT], gI[CPT], tfD[CPT], tfDm1[CPT];
+  if (any_on) {
+    double vel[CPT], sig[CPT], kap[CPT], D[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = cc[m];
+      vel[m] = 0.0; sig[m] = 0.0; kap[m] = 1.0; D[m] = 0.0;
+      if (valid[m] && c >= rmin - 2 && c <= rmax + 2) {
+        const long long o = base + (c + 1);
+        vel[m] = A.vel[o];
+        sig[m] = A.sig[o];
+        kap[m] = A.kap[o];
+        if (TOPO && c >= rmin - 1 && c <= rmax + 1)
+          D[m] = A.deld[(long long)(c + 1) + (long long)(nx + 4) * (j + 1)];
+      }
+      if (valid[m]) { s_q[c + 1] = vel[m]; s_mu[c + 1] = sig[m]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = cc[m], p = c + 1;
+      u[m] = 0.0; dtv[m] = 0.0; tfD[m] = 0.0; gI[m] = 0.0;
+      if (!valid[m]) continue;
+      const bool uin = c >= rmin - 1 && c <= rmax + 1;
+      const double velm1 = p >= 1 ? s_q[p - 1] : 0.0;
+      const double sigp1 = p + 1 < W ? s_mu[p + 1] : 0.0;
+      u[m] = uin ? 0.5 * (velm1 + vel[m]) * sig[m] : 0.0;
+      dtv[m] = uin ? A.dt / kap[m] : 0.0;
+      if (TOPO) tfD[m] = 0.5 * vel[m] * (sig[m] + sigp1) * D[m];
+      const double aus = fabs(u[m]);
+      gI[m] = 0.5 * aus * (1.0 - dtv[m] * aus);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CPT; ++m)
+      if (valid[m]) { s_q[cc[m] + 1] = u[m]; s_mu[cc[m] + 1] = tfD[m]; }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      divp[m] = divm[m] = tfDm1[m] = 0.0;
+      if (!valid[m]) continue;
+      const int p = cc[m] + 1;
+      const double um1 = p >= 1 ? s_q[p - 1] : 0.0;
+      const double up1 = p + 1 < W ? s_q[p + 1] : 0.0;
+      divp[m] = fmax(0.0, u[m]) - fmax(0.0, um1);
+      divm[m] = fmin(0.0, up1) - fmin(0.0, u[m]);
+      if (TOPO && p >= 1) tfDm1[m] = s_mu[p - 1];
+    }
+    __syncthreads();
+  }
+
+  for (int n = 0; n < d.ns; ++n) {
+    const long long bofs = d.ncell * (long long)n;
+    if (!(row_on && A.aloft.on(n))) {
+#pragma unroll
+      for (int m = 0; m < CPT; ++m)
+        if (cc[m] >= 1 && cc[m] <= nx) lam[base + (cc[m] + 1) + bofs] = 0.0;
+      continue;
+    }
+    double lm[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = cc[m];
+      lm[m] = 0.0;
+      if (!valid[m]) continue;
+      const bool act = c >= rmin && c <= rmax;
+      lm[m] = act ? lam[base + (c + 1) + bofs] : 0.0;
+      s_q[c + 1] = (c >= rmin - 2 && c <= rmax + 2) ? qpre[base + (c + 1) + bofs] : 0.0;
+      s_mu[c + 1] = act ? -dtv[m] * lm[m] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = cc[m], p = c + 1;
+      if (!valid[m]) continue;
+      double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+      if (c >= rmin && c <= rmax + 1)
+        interface_adjoint(lim, s_q[p - 2], s_q[p - 1], s_q[p], s_q[p + 1], u[m], divp[m], divm[m], gI[m], tfD[m],
+                          tfDm1[m], s_mu[p - 1], s_mu[p], e0, e1, e2, e3);
+      s_e0[p] = e0; s_e1[p] = e1; s_e2[p] = e2; s_e3[p] = e3;
+    }
+    __syncthreads();
+    double v[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int p = cc[m] + 1;
+      v[m] = 0.0;
+      if (!valid[m]) continue;
+      // interfaces r+2, r+1, r, r-1 reach cell r as their I-2, I-1, I, I+1
+      v[m] = (p + 2 < W ? s_e0[p + 2] : 0.0) + (p + 1 < W ? s_e1[p + 1] : 0.0) + s_e2[p] +
+             (p >= 1 ? s_e3[p - 1] : 0.0);
+    }
+    if (A.periodic) {
+      __syncthreads();  // s_q is free: hold the ghost adjoints there
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        const int c = cc[m];
+        if (valid[m] && (c <= 0 || c >= nx + 1)) s_q[c + 1] = v[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        const int c = cc[m];
+        if (c == 1) v[m] += s_q[nx + 2];  // from cell nx+1
+        if (c == 2) v[m] += s_q[nx + 3];  // from cell nx+2
+        if (c == nx - 1) v[m] += s_q[0];  // from cell -1
+        if (c == nx) v[m] += s_q[1];      // from cell 0
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = cc[m];
+      if (c >= 1 && c <= nx) lam[base + (c + 1) + bofs] = lm[m] + v[m];
+    }
+    __syncthreads();  // before the next bin overwrites the LDS rows
+  }
+}
+
+// ---- theta-scheme lines: lambda' = B^T A^{-T} lambda -------------------
+struct RowC {
+  double a, b, c;     // B row: a q_{l-1} + b q_l + c q_{l+1}
+  double dl, dg, du;  // A row: dl x_{l-1} + dg x_l + du x_{l+1}
+};
+
+template <int DIR, int NB>
+__global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int n0 = (int)blockIdx.y * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int n = rmax - rmin + 1;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  double *uD = A.fac + 3 * d.ncell * (long long)blockIdx.y;
+  double *uDU = uD + d.ncell, *uDL = uD + 2 * d.ncell;
+  double *sb = A.scratch_b;
+  auto cell = [&](int r) { return base + sL * (long long)(r + 1); };
+  auto row = [&](int l) { return cell(rmin - 1 + l); };
+  bool on[NB], any_on = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    on[u] = (n0 + u) < d.ns && line_on && A.aloft.on(n0 + u);
+    any_on = any_on || on[u];
+  }
+  if (any_on) {
+    const double dt = A.dt, th = A.theta;
+    // interface factor kds(I) (:797-801) and the row coefficients (:807-853)
+    auto kds_at = [&](int I) {
+      const double vavg = 0.5 * (A.kap[cell(I - 1)] + A.kap[cell(I)]);
+      const double kavg = 0.5 * (A.kk[kB + kS * (long long)(I - 1)] + A.kk[kB + kS * (long long)I]);
+      return kavg * (A.sig[cell(I)] / vavg);
+    };
+    auto rowc = [&](int l) {
+      RowC r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if (l < 1 || l > n) return r;
+      const int c = rmin - 1 + l;
+      const double kap = A.kap[cell(c)];
+      const double L = dt * kds_at(c) * A.sig[cell(c)] / kap;
+      const double R = dt * kds_at(c + 1) * A.sig[cell(c + 1)] / kap;
+      const double C = L + R;
+      r.a = (1.0 - th) * L;
+      r.b = 1.0 - (1.0 - th) * C;
+      r.c = (1.0 - th) * R;
+      r.dl = -(th * L);
+      r.dg = (l == 1) ? 1.0 + th * C - th * L : (l == n ? 1.0 + th * C - th * R : 1.0 + th * C);
+      r.du = -(th * R);
+      return r;
+    };
+    // A^T y = lambda.  Row l of A^T: sub-diagonal du_{l-1}, diagonal dg_l,
+    // super-diagonal dl_{l+1}.  Streamed elimination with partial pivoting.
+    RowC rprev = rowc(1), rcur = rowc(2);
+    double Dc = rprev.dg, DUc = rcur.dl, Bc[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) Bc[u] = on[u] ? lam[row(1) + d.ncell * (long long)(n0 + u)] : 0.0;
+    for (int l = 2; l <= n; ++l) {
+      const int i = l - 1;
+      const RowC rnext = rowc(l + 1);
+      const double DLn = rprev.du;
+      double Dn = rcur.dg;
+      double DUn = (l < n) ? rnext.dl : 0.0;
+      double Bn[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) Bn[u] = on[u] ? lam[row(l) + d.ncell * (long long)(n0 + u)] : 0.0;
+      double DLo = 0.0;
+      if (fabs(Dc) >= fabs(DLn)) {
+        const double fact = DLn / Dc;
+        Dn = Dn - fact * DUc;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Bn[u] = Bn[u] - fact * Bc[u];
+      } else {
+        const double fact = Dc / DLn;
+        const double temp = Dn;
+        Dn = DUc - fact * temp;
+        if (i <= n - 2) {
+          DLo = DUn;
+          DUn = -fact * DLo;
+        }
+        DUc = temp;
+        Dc = DLn;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const double tb = Bc[u];
+          Bc[u] = Bn[u];
+          Bn[u] = tb - fact * Bn[u];
+        }
+      }
+      const long long o = row(i);
+      uD[o] = Dc; uDU[o] = DUc; uDL[o] = DLo;
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+        if (on[u]) sb[o + d.ncell * (long long)(n0 + u)] = Bc[u];
+      Dc = Dn; DUc = DUn;
+#pragma unroll
+      for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
+      rprev = rcur;
+      rcur = rnext;
+    }
+    // back substitution downward; row i+1 of lambda' = B^T y is final once
+    // y_i is known:
+    //   lambda'_l = c_{l-1} y_{l-1} + (b_l + [l=1] a_l + [l=n] c_l) y_l + a_{l+1} y_{l+1}
+    double y1[NB], y2[NB];  // y_{i+1}, y_{i+2}
+#pragma unroll
+    for (int u = 0; u < NB; ++u) { y1[u] = Bc[u] / Dc; y2[u] = 0.0; }
+    RowC rhi{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // row i+2
+    RowC rmid = rowc(n);                      // row i+1
+    for (int i = n - 1; i >= 0; --i) {
+      double yi[NB];
+      RowC rlo{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if (i >= 1) {
+        const long long o = row(i);
+        const double ud = uD[o], udu = uDU[o], udl = uDL[o];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const double Bi = on[u] ? sb[o + d.ncell * (long long)(n0 + u)] : 0.0;
+          yi[u] = (i == n - 1) ? (Bi - udu * y1[u]) / ud : (Bi - udu * y1[u] - udl * y2[u]) / ud;
+        }
+        rlo = rowc(i);
+      } else {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) yi[u] = 0.0;
+      }
+      const int l = i + 1;
+      const double bd = rmid.b + (l == 1 ? rmid.a : 0.0) + (l == n ? rmid.c : 0.0);
+      const long long ol = row(l);
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+        if (on[u]) lam[ol + d.ncell * (long long)(n0 + u)] = rlo.c * yi[u] + bd * y1[u] + rhi.a * y2[u];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) { y2[u] = y1[u]; y1[u] = yi[u]; }
+      rhi = rmid;
+      rmid = rlo;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    if (n0 + u >= d.ns) break;
+    const long long bofs = d.ncell * (long long)(n0 + u);
+    for (int r = 1; r <= N; ++r)
+      if (!on[u] || r < rmin || r > rmax) lam[cell(r) + bofs] = 0.0;
+  }
+}
+
+// ---- explicit diffusion lines -----------------------------------------
+// q'_c = q_c + Ep_c (q_{c+1} - q_c) - Em_c (q_c - q_{c-1}), c in [rmin,rmax],
+// Ep_c = dt kds_{c+1} sig_{c+1}/kappa_c, Em_c = dt kds_c sig_c/kappa_c.
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void k_adj_diff_explicit(DiffArgs A, double *lam, int nb, int periodic) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const double dt = A.dt;
+  auto cell = [&](int r) { return base + sL * (long long)(r + 1); };
+  auto kds_at = [&](int I) {
+    const double ds = 2.0 * A.sig[cell(I)] / (A.kap[cell(I - 1)] + A.kap[cell(I)]);
+    return 0.5 * (A.kk[kB + kS * (long long)(I - 1)] + A.kk[kB + kS * (long long)I]) * ds;
+  };
+  for (int n = (int)blockIdx.y * nb; n < d.ns && n < ((int)blockIdx.y + 1) * nb; ++n) {
+    const long long bofs = d.ncell * (long long)n;
+    if (!(line_on && A.aloft.on(n))) {
+      for (int r = 1; r <= N; ++r) lam[cell(r) + bofs] = 0.0;
+      continue;
+    }
+    auto act = [&](int c) { return c >= rmin && c <= rmax; };
+    auto lv = [&](int c) { return act(c) ? lam[cell(c) + bofs] : 0.0; };
+    auto epv = [&](int c) { return act(c) ? dt * kds_at(c + 1) * (A.sig[cell(c + 1)] / A.kap[cell(c)]) : 0.0; };
+    auto emv = [&](int c) { return act(c) ? dt * kds_at(c) * (A.sig[cell(c)] / A.kap[cell(c)]) : 0.0; };
+    for (int r = 1; r < rmin - 1; ++r) lam[cell(r) + bofs] = 0.0;
+    // sliding over r: lambda and Ep at r-1, lambda, Ep, Em at r, lambda and Em at r+1
+    double l_m1 = lv(rmin - 2), l_0 = lv(rmin - 1), l_p1 = lv(rmin);
+    double ep_m1 = epv(rmin - 2), ep_0 = epv(rmin - 1);
+    double em_0 = emv(rmin - 1), em_p1 = emv(rmin);
+    double glo = 0.0, ghi = 0.0;  // adjoints of ghost cells 0 and N+1
+    for (int r = rmin - 1; r <= rmax + 1; ++r) {
+      const double v = (1.0 - ep_0 - em_0) * l_0 + ep_m1 * l_m1 + em_p1 * l_p1;
+      if (r == 0) glo = v;
+      else if (r == N + 1) ghi = v;
+      else lam[cell(r) + bofs] = v;  // r = rmin-1, rmax+1 (inactive) keep only the stencil terms
+      l_m1 = l_0; l_0 = l_p1; l_p1 = lv(r + 2);
+      ep_m1 = ep_0; ep_0 = epv(r + 1);
+      em_0 = em_p1; em_p1 = emv(r + 2);
+    }
+    for (int r = rmax + 2; r <= N; ++r) lam[cell(r) + bofs] = 0.0;
+    // Neumann ghosts: 0 <- cell 1, N+1 <- cell N; periodic x: 0 <- N, N+1 <- 1
+    if (rmin == 1) lam[cell(periodic ? N : 1) + bofs] += glo;
+    if (rmax == N) lam[cell(periodic ? 1 : N) + bofs] += ghi;
+  }
+}
+
+// L(i,j) = sum_k (sum_n c(i,j,k,n)) * dz(k) / 1e3 (src/Output_Vars.f90:1061-1066)
+__global__ void k_column_load(Dims d, const double *c0, const double *dz, double *out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)d.nx * d.ny) return;
+  const int i = (int)(t % d.nx) + 1, j = (int)(t / d.nx) + 1;
+  double L = 0.0;
+  for (int k = 1; k <= d.nz; ++k) {
+    double s = 0.0;
+    for (int n = 0; n < d.ns; ++n) s += c0[d.p3(i, j, k) + d.ncell * (long long)n];
+    L = L + s * dz[k - 1] / 1.0e3;
+  }
+  out[t] = L;
+}
+
+// lambda(i,j,k,n) += w(i,j) * dz(k) / 1e3
+__global__ void k_adj_column(Dims d, double *lam, const double *dz, const double *w) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)d.nx * d.ny * d.nz) return;
+  const int i = (int)(t % d.nx) + 1, j = (int)((t / d.nx) % d.ny) + 1;
+  const int k = (int)(t / ((long long)d.nx * d.ny)) + 1;
+  const double wk = w[(long long)(i - 1) + (long long)d.nx * (j - 1)] * dz[k - 1] / 1.0e3;
+  for (int n = 0; n < d.ns; ++n) {
+    const long long o = d.p3(i, j, k) + d.ncell * (long long)n;
+    lam[o] = lam[o] + wk;
+  }
+}
+
+// slot(k,n) += dt * lambda(iv,jv,k,n), k = 1..nz+1
+__global__ void k_adj_source(Dims d, const double *lam, int iv, int jv, double dt, double *slot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (d.nz + 1) * d.ns) return;
+  const int k = t % (d.nz + 1) + 1, n = t / (d.nz + 1);
+  slot[t] = slot[t] + dt * lam[d.p3(iv, jv, k) + d.ncell * (long long)n];
+}
+
+template <int CPT>
+hipError_t adj_x_rows_cpt(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int bt) {
+  const size_t shmem = sizeof(double) * 6 * (size_t)(a.d.nx + 4);
+  dim3 grid((unsigned)((long long)a.d.ny * a.d.nz));
+  const void *kern = a.deld != nullptr ? (const void *)k_adj_advect_x_rows<CPT, true>
+                                       : (const void *)k_adj_advect_x_rows<CPT, false>;
+  if (shmem > 65536) {
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+  }
+  if (a.deld != nullptr)
+    hipLaunchKernelGGL((k_adj_advect_x_rows<CPT, true>), grid, dim3(bt), shmem, s, a, qpre, lam);
+  else
+    hipLaunchKernelGGL((k_adj_advect_x_rows<CPT, false>), grid, dim3(bt), shmem, s, a, qpre, lam);
+  return hipGetLastError();
+}
+
+template <int DIR>
+hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + nb - 1) / nb));
+  if (DIR != 2 && a.deld != nullptr)
+    hipLaunchKernelGGL((k_adj_advect_lines<DIR, true>), grid, dim3(kBlock), 0, s, a, qpre, lam, nb);
+  else
+    hipLaunchKernelGGL((k_adj_advect_lines<DIR, false>), grid, dim3(kBlock), 0, s, a, qpre, lam, nb);
+  return hipGetLastError();
+}
+
+template <int DIR, int NB>
+hipError_t adj_cn_dir(hipStream_t s, const DiffArgs &a, double *lam) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  hipLaunchKernelGGL((k_adj_cn<DIR, NB>), grid, dim3(kBlock), 0, s, a, lam);
+  return hipGetLastError();
+}
+
+template <int DIR>
+hipError_t adj_cn_nb(hipStream_t s, const DiffArgs &a, double *lam, int nb) {
+  switch (nb) {
+    case 1: return adj_cn_dir<DIR, 1>(s, a, lam);
+    case 2: return adj_cn_dir<DIR, 2>(s, a, lam);
+    case 4: return adj_cn_dir<DIR, 4>(s, a, lam);
+    default: return adj_cn_dir<DIR, 8>(s, a, lam);
+  }
+}
+
+}  // namespace
+
+// x rows through LDS (rows of up to 8*256-4 cells, longer rows streamed
+// per lane), y and z lines streamed; a.periodic selects the periodic x fold.
+hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
+  if (a.g.dir == 0) {
+    const int cells = a.d.nx + 4;
+    int cpt = (cells + 255) / 256;
+    if (cpt > 8) return adj_lines<0>(s, a, qpre, lam, nb);
+    if (cpt == 3) cpt = 4;
+    if (cpt > 4) cpt = 8;
+    int bt = (cells + cpt - 1) / cpt;
+    bt = ((bt + 63) / 64) * 64;
+    switch (cpt) {
+      case 1: return adj_x_rows_cpt<1>(s, a, qpre, lam, bt);
+      case 2: return adj_x_rows_cpt<2>(s, a, qpre, lam, bt);
+      case 4: return adj_x_rows_cpt<4>(s, a, qpre, lam, bt);
+      default: return adj_x_rows_cpt<8>(s, a, qpre, lam, bt);
+    }
+  }
+  if (a.g.dir == 1) return adj_lines<1>(s, a, qpre, lam, nb);
+  return adj_lines<2>(s, a, qpre, lam, nb);
+}
+
+// a.g.dir is 1 or 2 (x lines run on the transposed grid as dir 1)
+hipError_t launch_adj_cn(hipStream_t s, const DiffArgs &a, double *lam, int nb) {
+  if (a.g.dir == 2) return adj_cn_nb<2>(s, a, lam, nb);
+  return adj_cn_nb<1>(s, a, lam, nb);
+}
+
+hipError_t launch_adj_diff_explicit(hipStream_t s, const DiffArgs &a, double *lam, int nb, int periodic) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + nb - 1) / nb));
+  if (a.g.dir == 2)
+    hipLaunchKernelGGL((k_adj_diff_explicit<2>), grid, dim3(kBlock), 0, s, a, lam, nb, periodic);
+  else
+    hipLaunchKernelGGL((k_adj_diff_explicit<1>), grid, dim3(kBlock), 0, s, a, lam, nb, periodic);
+  return hipGetLastError();
+}
+
+hipError_t launch_column_load(hipStream_t s, const Dims &d, const double *c0, const double *dz, double *out) {
+  const long long t = (long long)d.nx * d.ny;
+  hipLaunchKernelGGL(k_column_load, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, d, c0, dz, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_column(hipStream_t s, const Dims &d, double *lam, const double *dz, const double *w) {
+  const long long t = (long long)d.nx * d.ny * d.nz;
+  hipLaunchKernelGGL(k_adj_column, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, d, lam, dz, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_source(hipStream_t s, const Dims &d, const double *lam, int iv, int jv, double dt,
+                             double *slot) {
+  const int t = (d.nz + 1) * d.ns;
+  hipLaunchKernelGGL(k_adj_source, dim3((t + 255) / 256), dim3(256), 0, s, d, lam, iv, jv, dt, slot);
+  return hipGetLastError();
+}
+
+}  // namespace ash3d

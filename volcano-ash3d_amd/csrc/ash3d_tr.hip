@@ -16,6 +16,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -76,6 +77,13 @@ struct ash3d_tr_ctx {
   // transposed kappa, sigma_nx and kx (rebuilt after a geometry/K upload)
   double *tq = nullptr, *tkap = nullptr, *tsx = nullptr, *tkx = nullptr;
   bool tgeom_valid = false;
+  // adjoint: lambda, snapshots of the recomputed sweep inputs, line-solve
+  // scratch, an outflow sink for recomputed sweeps, source-gradient slots
+  double *lam = nullptr, *snap_b = nullptr, *snap_z = nullptr, *adj_sb = nullptr, *sink = nullptr;
+  double *src_grad = nullptr;
+  int n_src_slots = 0;
+  double *col = nullptr;  // column-load staging: nz dz values, then nx*ny loads / weights
+  std::vector<double *> ckpt;  // checkpoint slots
   int ts1_state = TS1_EXPLICIT;
   int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
@@ -166,23 +174,21 @@ struct ProfScope {
   }
 };
 
-int set_bc1(ash3d_tr_ctx *x, bool velocities) {
+// Set_BC(1) on a state buffer (ts0 unless given).
+int set_bc1(ash3d_tr_ctx *x, bool velocities, double *state = nullptr) {
   ProfScope ps(x, ASH3D_PROF_BC);
   if (velocities) HIPCHK(launch_set_bc1_vel(x->stream, x->d, x->periodic, x->vx, x->vy, x->vz, x->vf));
-  HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, x->ts0()));
+  HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, state ? state : x->ts0()));
   return 0;
 }
 
-int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
-                 bool with_bc, bool bc_vel) {
-  int rc = check_bounds(x, bounds);
-  if (rc) return rc;
-  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
-  if (with_bc && (rc = set_bc1(x, bc_vel))) return rc;
+// Argument block of one advection sweep over state c0.
+AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
+                    double *c0) {
   AdvArgs a{};
   a.d = x->d;
   a.g = make_line_geom(x->d, dir, bounds);
-  a.c0 = x->ts0();
+  a.c0 = c0;
   a.dt = dt;
   a.periodic = x->periodic;
   a.limiter = x->limiter;
@@ -198,11 +204,59 @@ int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, c
     a.vel = x->vz; a.vh = x->vh; a.vf = x->vf; a.sig = x->sz; a.of1 = x->ofxy1; a.of2 = x->ofxy2;
     a.dep = x->dep;
   }
+  return a;
+}
+
+// One advection sweep on ts0, or on another state buffer whose outflow and
+// deposit go to the sink (recomputation for the adjoint).
+int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
+                 bool with_bc, bool bc_vel, double *state = nullptr) {
+  int rc = check_bounds(x, bounds);
+  if (rc) return rc;
+  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
+  if (with_bc && (rc = set_bc1(x, bc_vel, state))) return rc;
+  AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, state ? state : x->ts0());
+  if (state) {
+    const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
+    a.of1 = x->sink;
+    a.of2 = x->sink + face;
+    a.dep = x->sink + 2 * face;
+  }
   {
     ProfScope ps(x, ASH3D_PROF_ADVECT_X + dir);
     HIPCHK(launch_advect(x->stream, a, x->nb_adv[dir]));
   }
-  x->ts1_state = TS1_MIRROR;
+  if (!state) x->ts1_state = TS1_MIRROR;
+  return 0;
+}
+
+// Transposed kappa, sigma_nx and kx for the x-line solves.
+int ensure_tgeom(ash3d_tr_ctx *x) {
+  int rc = 0;
+  const Dims &d = x->d;
+  if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
+  if (x->tgeom_valid) return 0;
+  if (!x->tkap && ((rc = dev_alloc(&x->tkap, d.ncell)) || (rc = dev_alloc(&x->tsx, d.ncell)) ||
+                   (rc = dev_alloc(&x->tkx, x->n_k()))))
+    return rc;
+  ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+  HIPCHK(launch_transpose_xy(x->stream, x->kap, x->tkap, d.nx + 4, d.ny + 4, d.nz + 4));
+  HIPCHK(launch_transpose_xy(x->stream, x->sx, x->tsx, d.nx + 4, d.ny + 4, d.nz + 4));
+  HIPCHK(launch_transpose_xy(x->stream, x->kx, x->tkx, d.nx + 2, d.ny + 2, d.nz + 2));
+  x->tgeom_valid = true;
+  return 0;
+}
+
+// CN factor scratch for bin groups of nb
+int ensure_fac(ash3d_tr_ctx *x, int nb) {
+  const long long groups = (x->d.ns + nb - 1) / nb;
+  if (groups <= x->fac_groups) return 0;
+  if (x->fac) HIPCHK(hipFree(x->fac));
+  x->fac = nullptr;
+  x->fac_groups = 0;
+  int rc = dev_alloc(&x->fac, 3 * x->d.ncell * groups);
+  if (rc) return rc;
+  x->fac_groups = groups;
   return 0;
 }
 
@@ -237,17 +291,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
   if (transposed) {
     // x lines as the y lines of the transposed grid (x and y swapped)
     const Dims &d = x->d;
-    if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
-    if (!x->tgeom_valid) {
-      if (!x->tkap && ((rc = dev_alloc(&x->tkap, d.ncell)) || (rc = dev_alloc(&x->tsx, d.ncell)) ||
-                       (rc = dev_alloc(&x->tkx, x->n_k()))))
-        return rc;
-      ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
-      HIPCHK(launch_transpose_xy(x->stream, x->kap, x->tkap, d.nx + 4, d.ny + 4, d.nz + 4));
-      HIPCHK(launch_transpose_xy(x->stream, x->sx, x->tsx, d.nx + 4, d.ny + 4, d.nz + 4));
-      HIPCHK(launch_transpose_xy(x->stream, x->kx, x->tkx, d.nx + 2, d.ny + 2, d.nz + 2));
-      x->tgeom_valid = true;
-    }
+    if ((rc = ensure_tgeom(x))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
       HIPCHK(launch_transpose_xy(x->stream, x->ts0(), x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));
@@ -261,13 +305,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.kk = x->tkx;
   }
   if (use_cn) {
-    const long long groups = (x->d.ns + x->nb_diff[dir] - 1) / x->nb_diff[dir];
-    if (groups > x->fac_groups) {
-      if (x->fac) HIPCHK(hipFree(x->fac));
-      x->fac = nullptr;
-      if ((rc = dev_alloc(&x->fac, 3 * x->d.ncell * groups))) return rc;
-      x->fac_groups = groups;
-    }
+    if ((rc = ensure_fac(x, x->nb_diff[dir]))) return rc;
     a.fac = x->fac;
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
@@ -281,6 +319,81 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
                                (long long)(x->d.nz + 4) * x->d.ns));
   }
   x->ts1_state = TS1_MIRROR;
+  return 0;
+}
+
+int need_adjoint(ash3d_tr_ctx *x) {
+  if (!x->lam) return fail(ASH3D_TR_EINVAL, "adjoint buffers not allocated (call ash3d_tr_adj_init)");
+  return 0;
+}
+
+int copy_state(ash3d_tr_ctx *x, double *dst, const double *src) {
+  ProfScope ps(x, ASH3D_PROF_COPY);
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)x->n_conc(), hipMemcpyDeviceToDevice, x->stream));
+  return 0;
+}
+
+// lambda <- J_adv(qpre)^T lambda for one direction; qpre carries the
+// Set_BC(1) ghosts the forward sweep saw.
+int adj_advect_sweep(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
+                     double *qpre) {
+  AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, qpre);
+  a.periodic = (dir == 0) ? x->periodic : 0;
+  ProfScope ps(x, ASH3D_PROF_ADJ_ADVECT);
+  HIPCHK(launch_adj_advect(x->stream, a, qpre, x->lam, x->nb_adv[dir]));
+  return 0;
+}
+
+// lambda <- (A^{-1} B)^T lambda, or the explicit stencil's transpose
+int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fac, const int32_t *is_aloft,
+                      const int32_t *bounds) {
+  int rc = check_bounds(x, bounds);
+  if (rc) return rc;
+  if (dir < 0 || dir > 2) return fail(ASH3D_TR_EINVAL, "dir must be 0, 1 or 2 (got %d)", dir);
+  DiffArgs a{};
+  a.d = x->d;
+  a.g = make_line_geom(x->d, dir, bounds);
+  a.dt = dt;
+  a.theta = imp_fac;
+  a.flag = x->flag;
+  make_mask(x, is_aloft, &a.aloft);
+  a.kap = x->kap;
+  a.kk = dir == 0 ? x->kx : (dir == 1 ? x->ky : x->kz);
+  a.sig = dir == 0 ? x->sx : (dir == 1 ? x->sy : x->sz);
+  if (use_cn && a.g.rmax - a.g.rmin + 1 <= 1) return 0;  // the forward leaves ts0 as it is
+  double *lam = x->lam;
+  const bool transposed = (dir == 0);
+  if (transposed) {
+    const Dims &d = x->d;
+    if ((rc = ensure_tgeom(x))) return rc;
+    {
+      ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+      HIPCHK(launch_transpose_xy(x->stream, x->lam, x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));
+    }
+    const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
+    a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
+    a.g = make_line_geom(a.d, 1, tb);
+    a.kap = x->tkap;
+    a.sig = x->tsx;
+    a.kk = x->tkx;
+    lam = x->tq;
+  }
+  {
+    ProfScope ps(x, ASH3D_PROF_ADJ_DIFFUSE);
+    if (use_cn) {
+      if ((rc = ensure_fac(x, x->nb_diff[dir]))) return rc;
+      a.fac = x->fac;
+      a.scratch_b = x->adj_sb;
+      HIPCHK(launch_adj_cn(x->stream, a, lam, x->nb_diff[dir]));
+    } else {
+      HIPCHK(launch_adj_diff_explicit(x->stream, a, lam, x->nb_diff[dir], (dir == 0) ? x->periodic : 0));
+    }
+  }
+  if (transposed) {
+    ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+    HIPCHK(launch_transpose_xy(x->stream, x->tq, x->lam, x->d.ny + 4, x->d.nx + 4,
+                               (long long)(x->d.nz + 4) * x->d.ns));
+  }
   return 0;
 }
 
@@ -375,8 +488,11 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   if (x->stream) hipStreamSynchronize(x->stream);
   double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
-                    x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx};
+                    x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx, x->lam, x->snap_b,
+                    x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col};
   for (double *p : ptrs)
+    if (p) hipFree(p);
+  for (double *p : x->ckpt)
     if (p) hipFree(p);
   if (x->flag) hipFree(x->flag);
   for (auto &e : x->ev_pool) {
@@ -685,6 +801,196 @@ int ash3d_tr_mass_totals(ash3d_tr_ctx *x, double *aloft, double *deposit, double
     if (deposit) deposit[n] = h[d.ns + n];
     if (outflow) outflow[n] = h[2 * d.ns + n];
   }
+  return 0;
+}
+
+int ash3d_tr_checkpoints(ash3d_tr_ctx *x, int32_t nslots) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (nslots < 0) return fail(ASH3D_TR_EINVAL, "negative checkpoint count %d", nslots);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  while ((int)x->ckpt.size() > nslots) {
+    HIPCHK(hipFree(x->ckpt.back()));
+    x->ckpt.pop_back();
+  }
+  while ((int)x->ckpt.size() < nslots) {
+    double *p = nullptr;
+    if ((rc = dev_alloc(&p, x->n_conc()))) return rc;
+    x->ckpt.push_back(p);
+  }
+  return 0;
+}
+
+int ash3d_tr_checkpoint(ash3d_tr_ctx *x, int32_t slot, int32_t op) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (slot < 0 || slot >= (int)x->ckpt.size())
+    return fail(ASH3D_TR_EINVAL, "checkpoint slot %d of %d", slot, (int)x->ckpt.size());
+  if (op == ASH3D_CKPT_SAVE) return copy_state(x, x->ckpt[slot], x->ts0());
+  if (op == ASH3D_CKPT_LOAD) {
+    if ((rc = copy_state(x, x->ts0(), x->ckpt[slot]))) return rc;
+    x->ts1_state = TS1_MIRROR;
+    return 0;
+  }
+  return fail(ASH3D_TR_EINVAL, "unknown checkpoint operation %d", op);
+}
+
+int ash3d_tr_adj_init(ash3d_tr_ctx *x, int32_t n_source_slots) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (n_source_slots < 0) return fail(ASH3D_TR_EINVAL, "negative source slot count %d", n_source_slots);
+  const Dims &d = x->d;
+  const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
+  if (!x->lam) {
+    if ((rc = dev_alloc(&x->lam, x->n_conc())) || (rc = dev_alloc(&x->snap_b, x->n_conc())) ||
+        (rc = dev_alloc(&x->snap_z, x->n_conc())) || (rc = dev_alloc(&x->adj_sb, x->n_conc())) ||
+        (rc = dev_alloc(&x->sink, 2 * face + x->n_dep())))
+      return rc;
+    HIPCHK(launch_fill(x->stream, x->sink, 2 * face + x->n_dep(), 0.0));
+  }
+  const long long per = (long long)(d.nz + 1) * d.ns;
+  if (n_source_slots > x->n_src_slots || !x->src_grad) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (x->src_grad) HIPCHK(hipFree(x->src_grad));
+    x->src_grad = nullptr;
+    x->n_src_slots = 0;
+    if ((rc = dev_alloc(&x->src_grad, per * std::max(1, (int)n_source_slots)))) return rc;
+  }
+  x->n_src_slots = n_source_slots;
+  HIPCHK(launch_fill(x->stream, x->lam, x->n_conc(), 0.0));
+  HIPCHK(launch_fill(x->stream, x->src_grad, per * std::max(1, (int)n_source_slots), 0.0));
+  return 0;
+}
+
+int ash3d_tr_adj_set(ash3d_tr_ctx *x, const double *lambda) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  if (!lambda) {
+    HIPCHK(launch_fill(x->stream, x->lam, x->n_conc(), 0.0));
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(x->lam, lambda, sizeof(double) * (size_t)x->n_conc(), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(launch_zero_ghosts(x->stream, x->d, x->lam));
+  return 0;
+}
+
+int ash3d_tr_adj_get(ash3d_tr_ctx *x, double *lambda) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  if (!lambda) return fail(ASH3D_TR_EINVAL, "null output");
+  HIPCHK(hipMemcpyAsync(lambda, x->lam, sizeof(double) * (size_t)x->n_conc(), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int ash3d_tr_adj_advect(ash3d_tr_ctx *x, int32_t dir, double dt, const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x)) || (rc = check_bounds(x, bounds))) return rc;
+  if (dir < 0 || dir > 2) return fail(ASH3D_TR_EINVAL, "dir must be 0, 1 or 2 (got %d)", dir);
+  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
+  if ((rc = set_bc1(x, true))) return rc;
+  return adj_advect_sweep(x, dir, dt, is_aloft, bounds, x->ts0());
+}
+
+int ash3d_tr_adj_diffuse(ash3d_tr_ctx *x, int32_t dir, int32_t use_cn, double dt, double imp_fac,
+                         const int32_t *is_aloft, const int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  return adj_diffuse_sweep(x, dir, use_cn, dt, imp_fac, is_aloft, bounds);
+}
+
+int ash3d_tr_adj_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                      double imp_fac, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
+  int32_t b[6];
+  full_bounds(x, b);
+  if (use_diffusion && use_cn && x->d.nz <= 1) {
+    // diffCN_z with one layer zero-fills ts1 and copies nothing: the step's
+    // output (ts1) is zero whatever its input
+    HIPCHK(launch_fill(x->stream, x->lam, x->n_conc(), 0.0));
+    return 0;
+  }
+  // recompute the inputs of the second horizontal sweep and of the vertical
+  // sweep from the step input (ts0), as ash3d_tr_step computes them
+  if ((rc = set_bc1(x, true))) return rc;
+  const int first = (itime % 2 == 0) ? 1 : 0;
+  if ((rc = copy_state(x, x->snap_b, x->ts0()))) return rc;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->snap_b))) return rc;
+  if (x->periodic && (rc = set_bc1(x, false, x->snap_b))) return rc;
+  if ((rc = copy_state(x, x->snap_z, x->snap_b))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, false, false, x->snap_z))) return rc;
+  if (x->periodic && (rc = set_bc1(x, false, x->snap_z))) return rc;
+  // adjoint sweeps in reverse order (Set_BC(2) ghosts are not read by the
+  // diffusion sweeps, which set their own)
+  if (use_diffusion) {
+    const int dfirst = (itime % 2 == 0) ? 0 : 1;
+    if ((rc = adj_diffuse_sweep(x, 2, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+    if ((rc = adj_diffuse_sweep(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+    if ((rc = adj_diffuse_sweep(x, dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+  }
+  if ((rc = adj_advect_sweep(x, 2, dt, is_aloft, b, x->snap_z))) return rc;
+  if ((rc = adj_advect_sweep(x, 1 - first, dt, is_aloft, b, x->snap_b))) return rc;
+  return adj_advect_sweep(x, first, dt, is_aloft, b, x->ts0());
+}
+
+int ash3d_tr_adj_source(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, double dt, int32_t slot) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  if (ivent < 1 || ivent > x->d.nx || jvent < 1 || jvent > x->d.ny)
+    return fail(ASH3D_TR_EINVAL, "vent (%d,%d) outside the grid", ivent, jvent);
+  if (slot < 0 || slot >= x->n_src_slots)
+    return fail(ASH3D_TR_EINVAL, "source slot %d of %d", slot, x->n_src_slots);
+  HIPCHK(launch_adj_source(x->stream, x->d, x->lam, ivent, jvent, dt,
+                           x->src_grad + (long long)slot * (x->d.nz + 1) * x->d.ns));
+  return 0;
+}
+
+int ash3d_tr_adj_source_read(ash3d_tr_ctx *x, double *out) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x))) return rc;
+  if (!out) return fail(ASH3D_TR_EINVAL, "null output");
+  const size_t n = (size_t)x->n_src_slots * (size_t)(x->d.nz + 1) * x->d.ns;
+  if (n) HIPCHK(hipMemcpyAsync(out, x->src_grad, sizeof(double) * n, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+static int ensure_col(ash3d_tr_ctx *x) {
+  if (x->col) return 0;
+  return dev_alloc(&x->col, (long long)x->d.nz + (long long)x->d.nx * x->d.ny);
+}
+
+int ash3d_tr_column_load(ash3d_tr_ctx *x, const double *dz, double *out) {
+  int rc = check_ctx(x);
+  if (rc || (rc = ensure_col(x))) return rc;
+  if (!dz || !out) return fail(ASH3D_TR_EINVAL, "null dz or output");
+  const Dims &d = x->d;
+  const size_t nxy = (size_t)d.nx * d.ny;
+  if (x->ts1_state == TS1_ZERO) {  // ConcentrationCalculator reads ts1
+    std::memset(out, 0, sizeof(double) * nxy);
+    return 0;
+  }
+  const double *src = (x->ts1_state == TS1_EXPLICIT) ? x->ts1() : x->ts0();
+  HIPCHK(hipMemcpyAsync(x->col, dz, sizeof(double) * (size_t)d.nz, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(launch_column_load(x->stream, d, src, x->col, x->col + d.nz));
+  HIPCHK(hipMemcpyAsync(out, x->col + d.nz, sizeof(double) * nxy, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int ash3d_tr_adj_column_load(ash3d_tr_ctx *x, const double *dz, const double *w) {
+  int rc = check_ctx(x);
+  if (rc || (rc = need_adjoint(x)) || (rc = ensure_col(x))) return rc;
+  if (!dz || !w) return fail(ASH3D_TR_EINVAL, "null dz or weights");
+  const Dims &d = x->d;
+  HIPCHK(hipStreamSynchronize(x->stream));  // the staging buffer may still be read
+  HIPCHK(hipMemcpyAsync(x->col, dz, sizeof(double) * (size_t)d.nz, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->col + d.nz, w, sizeof(double) * (size_t)d.nx * d.ny, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(launch_adj_column(x->stream, d, x->lam, x->col, x->col + d.nz));
   return 0;
 }
 

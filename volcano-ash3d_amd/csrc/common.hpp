@@ -130,5 +130,13 @@ hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, in
 hipError_t launch_mass_totals(hipStream_t s, const Dims &d, const double *c0, const double *kap,
                               const double *dep, const double *yz1, const double *yz2, const double *xz1,
                               const double *xz2, const double *xy2, double *partial, double *out);
+// adjoint (adjoint.hip)
+hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb);
+hipError_t launch_adj_cn(hipStream_t s, const DiffArgs &a, double *lam, int nb);
+hipError_t launch_adj_diff_explicit(hipStream_t s, const DiffArgs &a, double *lam, int nb, int periodic);
+hipError_t launch_column_load(hipStream_t s, const Dims &d, const double *c0, const double *dz, double *out);
+hipError_t launch_adj_column(hipStream_t s, const Dims &d, double *lam, const double *dz, const double *w);
+hipError_t launch_adj_source(hipStream_t s, const Dims &d, const double *lam, int iv, int jv, double dt,
+                             double *slot);
 
 }  // namespace ash3d

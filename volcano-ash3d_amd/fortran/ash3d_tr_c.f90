@@ -30,7 +30,10 @@
              c_ash3d_tr_sync_to_host, c_ash3d_tr_set_bc, c_ash3d_tr_advect_x, &
              c_ash3d_tr_advect_y, c_ash3d_tr_advect_z, c_ash3d_tr_diffuse_dir, &
              c_ash3d_tr_step, c_ash3d_tr_step_glue, c_ash3d_tr_source_column_add, &
-             c_ash3d_tr_last_error, ASH3D_TR_ABI_VERSION, ASH3D_SYNC_ALL
+             c_ash3d_tr_last_error, ASH3D_TR_ABI_VERSION, ASH3D_SYNC_ALL, &
+             c_ash3d_tr_checkpoints, c_ash3d_tr_checkpoint, c_ash3d_tr_adj_init, &
+             c_ash3d_tr_adj_step, c_ash3d_tr_adj_source, c_ash3d_tr_adj_source_read, &
+             c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load
 
       integer(c_int32_t), parameter :: ASH3D_TR_ABI_VERSION = 1
       integer(c_int32_t), parameter :: ASH3D_SYNC_ALL = 255
@@ -127,6 +130,51 @@
           integer(c_int32_t), value :: iv, jv
           real(c_double), value :: dt
           real(c_double), intent(in) :: flux(*)
+        end function
+        integer(c_int) function c_ash3d_tr_checkpoints(ctx, nslots) bind(C, name='ash3d_tr_checkpoints')
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: nslots
+        end function
+        integer(c_int) function c_ash3d_tr_checkpoint(ctx, slot, op) bind(C, name='ash3d_tr_checkpoint')
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: slot, op
+        end function
+        integer(c_int) function c_ash3d_tr_adj_init(ctx, nslots) bind(C, name='ash3d_tr_adj_init')
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: nslots
+        end function
+        integer(c_int) function c_ash3d_tr_adj_step(ctx, itime, dt, use_diff, use_cn, imp_fac, aloft) &
+            bind(C, name='ash3d_tr_adj_step')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: itime, use_diff, use_cn
+          real(c_double), value :: dt, imp_fac
+          integer(c_int32_t), intent(in) :: aloft(*)
+        end function
+        integer(c_int) function c_ash3d_tr_adj_source(ctx, iv, jv, dt, slot) bind(C, name='ash3d_tr_adj_source')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: iv, jv, slot
+          real(c_double), value :: dt
+        end function
+        integer(c_int) function c_ash3d_tr_adj_source_read(ctx, out) bind(C, name='ash3d_tr_adj_source_read')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function c_ash3d_tr_column_load(ctx, dz, out) bind(C, name='ash3d_tr_column_load')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: dz(*)
+          real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function c_ash3d_tr_adj_column_load(ctx, dz, w) bind(C, name='ash3d_tr_adj_column_load')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: dz(*), w(*)
         end function
         type(c_ptr) function c_ash3d_tr_last_error() bind(C, name='ash3d_tr_last_error')
           import :: c_ptr

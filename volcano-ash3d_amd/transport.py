@@ -86,9 +86,24 @@ _SIGS = {
     "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_int64)],
+    "ash3d_tr_checkpoints": [ctypes.c_void_p, ctypes.c_int32],
+    "ash3d_tr_checkpoint": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
+    "ash3d_tr_adj_init": [ctypes.c_void_p, ctypes.c_int32],
+    "ash3d_tr_adj_set": [ctypes.c_void_p, _dp],
+    "ash3d_tr_adj_get": [ctypes.c_void_p, _dp],
+    "ash3d_tr_adj_advect": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _ip, _ip],
+    "ash3d_tr_adj_diffuse": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                             ctypes.c_double, _ip, _ip],
+    "ash3d_tr_adj_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_double, _ip],
+    "ash3d_tr_adj_source": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32],
+    "ash3d_tr_adj_source_read": [ctypes.c_void_p, _dp],
+    "ash3d_tr_column_load": [ctypes.c_void_p, _dp, _dp],
+    "ash3d_tr_adj_column_load": [ctypes.c_void_p, _dp, _dp],
 }
 PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue",
-              "transpose")
+              "transpose", "adj_advect", "adj_diffuse", "copy")
+CKPT_SAVE, CKPT_LOAD = 0, 1
 EXPORTS = tuple(_SIGS) + ("ash3d_tr_last_error", "ash3d_tr_build_info")
 
 _lib = None
@@ -259,6 +274,74 @@ class Transport:
         self._check(self.lib.ash3d_tr_mass_totals(self._ctx, a.ctypes.data_as(_dp), dpo.ctypes.data_as(_dp),
                                                   o.ctypes.data_as(_dp)))
         return a, dpo, o
+
+    # -- checkpoints -------------------------------------------------------
+    def checkpoints(self, nslots: int):
+        """Resize the device checkpoint store to nslots copies of ts0."""
+        self._check(self.lib.ash3d_tr_checkpoints(self._ctx, int(nslots)))
+
+    def checkpoint_save(self, slot: int):
+        self._check(self.lib.ash3d_tr_checkpoint(self._ctx, int(slot), CKPT_SAVE))
+
+    def checkpoint_load(self, slot: int):
+        self._check(self.lib.ash3d_tr_checkpoint(self._ctx, int(slot), CKPT_LOAD))
+
+    # -- observation operator ----------------------------------------------
+    def column_load(self, dz: np.ndarray) -> np.ndarray:
+        """Cloud load (t/km2) of every column, (nx, ny) Fortran order."""
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        assert dz.shape == (self.st.nz,)
+        out = np.zeros((self.st.nx, self.st.ny), order="F")
+        self._check(self.lib.ash3d_tr_column_load(self._ctx, dz.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+        return out
+
+    # -- discrete adjoint --------------------------------------------------
+    def adj_init(self, n_source_slots: int = 0):
+        self._check(self.lib.ash3d_tr_adj_init(self._ctx, int(n_source_slots)))
+
+    def adj_set(self, lam):
+        """lam: array of concen_pd(:,:,:,:,ts0)'s shape, or None for zero."""
+        if lam is None:
+            self._check(self.lib.ash3d_tr_adj_set(self._ctx, None))
+            return
+        lam = np.asfortranarray(lam, dtype=np.float64)
+        assert lam.shape == self.st.concen_pd.shape[:4]
+        self._check(self.lib.ash3d_tr_adj_set(self._ctx, lam.ctypes.data_as(_dp)))
+
+    def adj_get(self) -> np.ndarray:
+        out = np.zeros(self.st.concen_pd.shape[:4], order="F")
+        self._check(self.lib.ash3d_tr_adj_get(self._ctx, out.ctypes.data_as(_dp)))
+        return out
+
+    def adj_advect(self, dirn: int):
+        """lambda <- J^T lambda for advect_x/y/z (dirn 0/1/2) at the state in ts0."""
+        self._check(self.lib.ash3d_tr_adj_advect(self._ctx, int(dirn), self.st.dt, self._aloft(), self._bounds()))
+
+    def adj_diffuse(self, dirn: int, use_cn: bool):
+        self._check(self.lib.ash3d_tr_adj_diffuse(self._ctx, int(dirn), int(bool(use_cn)), self.st.dt,
+                                                  self.st.imp_fac, self._aloft(), self._bounds()))
+
+    def adj_step(self, itime: int, use_diffusion: bool):
+        """Adjoint of transport_step(itime) with the step's input state in ts0."""
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_adj_step(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                               int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft()))
+
+    def adj_source(self, ivent: int, jvent: int, slot: int):
+        self._check(self.lib.ash3d_tr_adj_source(self._ctx, ivent, jvent, self.st.dt, int(slot)))
+
+    def adj_source_read(self, n_slots: int) -> np.ndarray:
+        """Source gradients, shape (n_slots, ns, nz+1)."""
+        out = np.zeros((max(n_slots, 1), self.st.ns, self.st.nz + 1))
+        self._check(self.lib.ash3d_tr_adj_source_read(self._ctx, out.ctypes.data_as(_dp)))
+        return out[:n_slots]
+
+    def adj_column_load(self, dz: np.ndarray, w: np.ndarray):
+        """lambda += L^T w for the column load; w of shape (nx, ny)."""
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        w = np.asfortranarray(w, dtype=np.float64)
+        assert w.shape == (self.st.nx, self.st.ny)
+        self._check(self.lib.ash3d_tr_adj_column_load(self._ctx, dz.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
 
     # -- launch tuning ----------------------------------------------------
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z")
