@@ -53,6 +53,26 @@ def step_bytes_per_bincell(st, use_diffusion):
     return per_cell / st.ns
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_forecast_360x240x50x15.md")
+
+
+def measured_traffic(kind, st, tuning):
+    """HBM bytes per launch of a kernel kind from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
+    on this grid with the same bins per thread; else None."""
+    if (st.nx, st.ny, st.nz, st.ns) != (360, 240, 50, 15) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    text = open(TRAFFIC_FILE).read()
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    rec = json.loads(block).get(kind)
+    if rec is None:
+        return None
+    tuned = (tuning or {}).get("nb_" + kind, [None])[0]
+    if rec["bins_per_thread"] is not None and tuned not in (None, rec["bins_per_thread"]):
+        return None
+    return rec["traffic_bytes"]
+
+
 def cpu_baseline(st, use_diffusion, steps):
     """Time the oracle (C port of the reference path, OpenMP rows in the
     advection sweeps as in the reference) on a bounded sample."""
@@ -83,23 +103,33 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
     dz = np.full(st.nz, 0.5)
     H = min(40, st.nz)
     frac = np.full(st.ns, 1.0 / st.ns)
-    inv = adj.SourceInversion(st, st.nx // 4, st.ny // 2, n_heights=H, n_periods=periods,
-                              steps_per_period=steps_per_period, frac=frac, dz=dz, obs_every=steps_per_period // 2,
-                              use_diffusion=use_diffusion)
+    n_steps = periods * steps_per_period
     rng = np.random.default_rng(0)
     s = 1e6 * rng.random((periods, H))
-    w = rng.random((len(inv.obs_steps), st.nx, st.ny))
-    inv.forward(s)  # warm-up (allocations, first-touch)
-    inv.gradient(w)
-    inv.T.synchronize()
-    t0 = time.perf_counter()
-    inv.forward(s)
-    inv.T.synchronize()
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    g = inv.gradient(w)
-    inv.T.synchronize()
-    t_grad = time.perf_counter() - t0
+
+    def timed(segment):
+        inv = adj.SourceInversion(st, st.nx // 4, st.ny // 2, n_heights=H, n_periods=periods,
+                                  steps_per_period=steps_per_period, frac=frac, dz=dz,
+                                  obs_every=steps_per_period // 2, use_diffusion=use_diffusion, segment=segment)
+        w = np.random.default_rng(1).random((len(inv.obs_steps), st.nx, st.ny))
+        inv.forward(s)  # warm-up (allocations, first touch)
+        inv.gradient(w)
+        inv.T.synchronize()
+        t0 = time.perf_counter()
+        inv.forward(s)
+        inv.T.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = inv.gradient(w)
+        inv.T.synchronize()
+        return inv, g, t_fwd, time.perf_counter() - t0
+
+    # square-root checkpointing (replays every segment but the last)
+    K_sqrt = max(1, int(round(n_steps ** 0.5)))
+    inv, _, f_rep, g_rep = timed(K_sqrt)
+    inv.close()
+    # default: segment length from the device-memory budget (all step inputs fit here)
+    inv, g, t_fwd, t_grad = timed(None)
     # kernel-level: one adjoint step vs one forward step on a resident state
     T = inv.T
     T.profile(True)
@@ -122,8 +152,9 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
     t_step = (time.perf_counter() - t0) / reps
     inv.close()
     return {"workload": f"source inversion, {periods} periods x {steps_per_period} steps, {H} source heights, "
-                        f"column loads every {steps_per_period // 2} steps, checkpoint segment {inv.K}",
+                        f"column loads every {steps_per_period // 2} steps, every step input kept on the device",
             "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
+            "ratio_sqrt_checkpointing": round(g_rep / f_rep, 3), "sqrt_segment": K_sqrt,
             "adj_step_ms": round(1e3 * t_adj, 3), "step_ms": round(1e3 * t_step, 3),
             "step_ratio": round(t_adj / t_step, 3), "grad_norm": float(np.abs(g).max()),
             "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
@@ -139,6 +170,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no HIP-event kernel timing (for hardware-counter runs); no per-kernel figures")
     ap.add_argument("--adj-periods", type=int, default=2)
     ap.add_argument("--adj-steps-per-period", type=int, default=10)
     args = ap.parse_args()
@@ -183,7 +216,8 @@ def main():
                                                 for k, v in tuning.items()}))
         itime += 1000
 
-    T.profile(True)
+    if not args.no_profile:
+        T.profile(True)
     T.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -192,9 +226,13 @@ def main():
         T.transport_step(itime, use_diffusion)
     T.synchronize()
     wall = time.perf_counter() - t0
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
+    if args.no_profile:
+        T.close()
+        print(json.dumps({"steps": args.steps, "ms_per_step": 1e3 * wall / args.steps, "profile": None}))
+        return
     prof = T.profile_read()
     T.profile(False)
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
 
     bincells = st.ns * st.nx * st.ny * st.nz
     rate = bincells * args.steps / wall
@@ -217,9 +255,15 @@ def main():
                    key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dominant]
     achieved = dk["gbps"]
+    traffic = measured_traffic(dominant, st, tuning)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "avg_ms": dk["avg_ms"], "algorithmic_bytes": algorithmic_bytes(dominant, st)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "avg_ms": dk["avg_ms"], "algorithmic_bytes": algorithmic_bytes(dominant, st),
+                "traffic_source": "profiles/hbm_traffic_forecast_360x240x50x15.md" if traffic else None}
+    for k, v in kernels.items():
+        t = measured_traffic(k, st, tuning)
+        if t:
+            v["traffic_gbps"] = round(t / (v["avg_ms"] * 1e-3) / 1e9, 1)
     log("[bench] kernels: " + json.dumps(kernels))
 
     T.close()

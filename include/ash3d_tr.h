@@ -160,7 +160,9 @@ int ash3d_tr_step_glue(ash3d_tr_ctx *ctx);
 
 /* Source insertion on the vent column, src/Ash3d.F90:428-431:
  * concen_pd(ivent,jvent,1:nz+1,1:ns,ts0) += dt * flux(1:nz+1,1:ns),
- * flux = SourceNodeFlux in kg/km3/h, a host array (nz+1, ns). */
+ * flux = SourceNodeFlux in kg/km3/h, a host array (nz+1, ns).  The array is
+ * copied before the call returns; the upload and the update are
+ * asynchronous (the host may run up to four insertions ahead). */
 int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, double dt,
                                const double *flux);
 

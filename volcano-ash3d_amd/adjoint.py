@@ -14,11 +14,15 @@ returns the loads; SourceInversion.gradient(w) returns dJ/ds for
 J = sum_t <w_t, L_t>, by one reverse sweep of the discrete adjoint.
 
 Checkpointing.  Every adjoint step needs the input state of its forward
-step.  The forward run stores the step input every K steps (K about the
-square root of the step count); the reverse sweep replays each segment of K
-steps from its checkpoint, keeping every step input of the segment, and
-then walks the segment backwards.  Device memory: (n_steps/K + K) states;
-work: one extra forward step per step.
+step.  The steps form segments of K; the forward run stores the input of
+the first step of every segment but the last, and the input of every step
+of the last segment.  The reverse sweep walks the last segment backwards
+directly, and replays every other segment from its checkpoint (keeping
+each step input) before walking it backwards.  Device memory: n_seg + K + 1
+states (the +1 holds the initial state); extra work: one forward step per
+step outside the last segment.  A run whose step inputs all fit the device-memory
+budget is a single segment and never replays; otherwise K is the longest
+segment length whose states fit.
 
 SourceToLoads wraps the pair as a torch.autograd.Function, so an inversion
 can use torch optimisers on s.
@@ -34,7 +38,7 @@ from .state import TransportState
 class SourceInversion:
     def __init__(self, st: TransportState, ivent: int, jvent: int, n_heights: int, n_periods: int,
                  steps_per_period: int, frac, dz, obs_every: int = 1, use_diffusion: bool = True,
-                 segment: int = None, itime0: int = 1, device: int = 0):
+                 segment: int = None, itime0: int = 1, device: int = 0, mem_budget: float = 64e9):
         if not 1 <= n_heights <= st.nz + 1:
             raise ValueError(f"n_heights must be in 1..nz+1 (got {n_heights})")
         self.st = st
@@ -49,11 +53,22 @@ class SourceInversion:
         self.n_steps = self.P * self.spp
         self.obs_steps = [t for t in range(self.n_steps) if (t + 1) % obs_every == 0]
         self._obs_index = {t: i for i, t in enumerate(self.obs_steps)}
-        self.K = int(segment) if segment else max(1, int(math.ceil(math.sqrt(self.n_steps))))
+        if segment:
+            self.K = int(segment)
+        else:
+            state_bytes = 8.0 * st.ns * (st.nx + 4) * (st.ny + 4) * (st.nz + 4)
+            slots = max(3, int(mem_budget // state_bytes) - 1)
+            if self.n_steps + 1 <= slots:
+                self.K = self.n_steps  # every step input stored: no replay
+            else:  # the longest segments whose states fit
+                self.K = max([K for K in range(1, self.n_steps + 1) if -(-self.n_steps // K) + K <= slots]
+                             or [max(1, int(math.ceil(math.sqrt(self.n_steps))))])
         self.n_seg = (self.n_steps + self.K - 1) // self.K
-        self.initial = st.concen_pd.copy(order="F")
         self.T = _tr.Transport(st, device=device)
-        self.T.checkpoints(self.n_seg + self.K)
+        # slot 0: initial state; 1..n_seg: segment inputs; then K step inputs
+        self.T.checkpoints(1 + self.n_seg + self.K)
+        self.T.step_glue()  # ts0 <- the initial ts1, as the first step's glue would
+        self.T.checkpoint_save(0)
         self.T.adj_init(self.P)
         self._s = None  # source table of the last forward run
 
@@ -67,9 +82,8 @@ class SourceInversion:
         f[:self.H, :] = np.outer(s[p, :self.H], self.frac)
         return f
 
-    def _reset(self):
-        self.st.concen_pd[...] = self.initial
-        self.T.sync_from_host(_tr.SYNC_CONCEN)
+    def _inner(self, t):
+        return 1 + self.n_seg + t % self.K
 
     def _begin_step(self, s, t):
         """Glue and source insertion of step t (the forward step's input)."""
@@ -81,12 +95,15 @@ class SourceInversion:
         s = np.asarray(s, dtype=np.float64)
         assert s.shape == (self.P, self.H)
         self._s = s.copy()
-        self._reset()
+        self.T.checkpoint_load(0)
         loads = np.zeros((len(self.obs_steps), self.st.nx, self.st.ny))
+        last = self.n_seg - 1
         for t in range(self.n_steps):
             self._begin_step(s, t)
-            if t % self.K == 0:
-                self.T.checkpoint_save(t // self.K)
+            if t // self.K == last:
+                self.T.checkpoint_save(self._inner(t))
+            elif t % self.K == 0:
+                self.T.checkpoint_save(1 + t // self.K)
             self.T.transport_step(self.itime0 + t, self.use_diffusion)
             if t in self._obs_index:
                 loads[self._obs_index[t]] = self.T.column_load(self.dz)
@@ -103,18 +120,19 @@ class SourceInversion:
         T.adj_init(self.P)
         for seg in reversed(range(self.n_seg)):
             t0, t1 = seg * self.K, min((seg + 1) * self.K, self.n_steps)
-            # replay: the step inputs of the segment into slots n_seg + (t - t0)
-            T.checkpoint_load(seg)
-            for t in range(t0, t1):
-                if t > t0:
-                    self._begin_step(s, t)
-                T.checkpoint_save(self.n_seg + t - t0)
-                if t < t1 - 1:
-                    T.transport_step(self.itime0 + t, self.use_diffusion)
+            if seg < self.n_seg - 1:
+                # replay the segment from its checkpoint, keeping every step input
+                T.checkpoint_load(1 + seg)
+                for t in range(t0, t1):
+                    if t > t0:
+                        self._begin_step(s, t)
+                    T.checkpoint_save(self._inner(t))
+                    if t < t1 - 1:
+                        T.transport_step(self.itime0 + t, self.use_diffusion)
             for t in reversed(range(t0, t1)):
                 if t in self._obs_index:
                     T.adj_column_load(self.dz, w[self._obs_index[t]])
-                T.checkpoint_load(self.n_seg + t - t0)
+                T.checkpoint_load(self._inner(t))
                 T.adj_step(self.itime0 + t, self.use_diffusion)
                 T.adj_source(self.ivent, self.jvent, t // self.spp)
         G = T.adj_source_read(self.P)  # (P, ns, nz+1): sum over steps of dt * lambda(vent)

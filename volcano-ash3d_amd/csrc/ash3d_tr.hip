@@ -71,6 +71,13 @@ struct ash3d_tr_ctx {
   double *fac = nullptr;  // CN factor scratch
   long long fac_groups = 0;
   double *flux = nullptr;  // source column staging (nz+1)*ns
+  // pinned host ring for the source columns: the caller's buffer is copied
+  // into a slot, so the upload is asynchronous and the host may run ahead
+  static constexpr int kFluxRing = 4;
+  double *flux_pinned[kFluxRing] = {};
+  hipEvent_t flux_ev[kFluxRing] = {};
+  bool flux_used[kFluxRing] = {};
+  int flux_slot = 0;
   int *flag = nullptr;     // singular-pivot flag
   double *tot = nullptr;   // mass-total partial sums and results
   // x-line solves run on an x<->y transposed copy: state, and the cached
@@ -494,6 +501,10 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
     if (p) hipFree(p);
   for (double *p : x->ckpt)
     if (p) hipFree(p);
+  for (int k = 0; k < ash3d_tr_ctx::kFluxRing; ++k) {
+    if (x->flux_pinned[k]) hipHostFree(x->flux_pinned[k]);
+    if (x->flux_ev[k]) hipEventDestroy(x->flux_ev[k]);
+  }
   if (x->flag) hipFree(x->flag);
   for (auto &e : x->ev_pool) {
     hipEventDestroy(e.first);
@@ -730,9 +741,18 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, do
   if (!flux) return fail(ASH3D_TR_EINVAL, "null flux");
   if (ivent < 1 || ivent > x->d.nx || jvent < 1 || jvent > x->d.ny)
     return fail(ASH3D_TR_EINVAL, "vent (%d,%d) outside the grid", ivent, jvent);
-  HIPCHK(hipMemcpyAsync(x->flux, flux, sizeof(double) * (size_t)(x->d.nz + 1) * x->d.ns, hipMemcpyHostToDevice,
-                        x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
+  const size_t bytes = sizeof(double) * (size_t)(x->d.nz + 1) * x->d.ns;
+  const int k = x->flux_slot;
+  if (!x->flux_pinned[k]) {
+    HIPCHK(hipHostMalloc((void **)&x->flux_pinned[k], bytes, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&x->flux_ev[k], hipEventDisableTiming));
+  }
+  if (x->flux_used[k]) HIPCHK(hipEventSynchronize(x->flux_ev[k]));  // its last upload has been read
+  std::memcpy(x->flux_pinned[k], flux, bytes);
+  HIPCHK(hipMemcpyAsync(x->flux, x->flux_pinned[k], bytes, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipEventRecord(x->flux_ev[k], x->stream));
+  x->flux_used[k] = true;
+  x->flux_slot = (k + 1) % ash3d_tr_ctx::kFluxRing;
   HIPCHK(launch_source_add(x->stream, x->d, x->ts0(), ivent, jvent, dt, x->flux));
   return 0;
 }
