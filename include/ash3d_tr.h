@@ -267,6 +267,45 @@ int ash3d_tr_adj_source_read(ash3d_tr_ctx *ctx, double *out);
 int ash3d_tr_column_load(ash3d_tr_ctx *ctx, const double *dz, double *out);
 int ash3d_tr_adj_column_load(ash3d_tr_ctx *ctx, const double *dz, const double *w);
 
+/* ---- Driver work between the hot-path calls ---------------------------
+ * Met steps: the two bracketing wind fields of the wind_grid module
+ * (v*_meso_{last,next}_step_sp, m/s, FP32, unpadded (1:nx,1:ny,1:nz) and
+ * vf (1:nx,1:ny,1:nz,1:ns)).  push makes the current "next" step the "last"
+ * one and uploads a new "next" (vf may be NULL); interp evaluates
+ * src/MesoInterpolater.f90:203-230 into vx/vy/vz_pd (and vf_pd if use_vf)
+ * at Interval_Frac = frac. */
+int ash3d_tr_met_push(ash3d_tr_ctx *ctx, const float *vx, const float *vy, const float *vz, const float *vf);
+int ash3d_tr_met_interp(ash3d_tr_ctx *ctx, double frac, int32_t use_vf);
+
+/* Adjust_DT, src/Adjust_dt.f90:89-333.  use_met_next selects the met-step
+ * branch (Adjust_DT(mesostep=.true.), reading the "next" met step); else the
+ * current vx/vy/vz/vf_pd.  The diffusion restriction is evaluated on every
+ * call (the reference caches it after the first call unless the
+ * diffusivity varies).  Returns dt (hours) and diag = vxmax_dx, vymax_dy,
+ * vzmax_dz, time_diffuse (diag may be NULL). */
+typedef struct ash3d_tr_dt_params {
+  int32_t is_latlon, use_met_next, n_gs_aloft, use_diffusion, use_cn;
+  double dx, dy;      /* km (projected grids) */
+  const double *dz;   /* dz_vec_pd(1:nz), km, host */
+  double imp_dt_fac, cfl, dt_min, dt_max, eps_small;
+  double time, next_write_time, simtime; /* hours */
+} ash3d_tr_dt_params;
+int ash3d_tr_adjust_dt(ash3d_tr_ctx *ctx, const ash3d_tr_dt_params *p, double *dt, double *diag);
+
+/* get_minmax_index (FAST_SUBGRID), src/Calc_Mesh.f90:432-542, on ts0 with
+ * CLOUDCON_GRID_THRESH = thresh (1e-7 in the reference,
+ * src/Output_Vars.f90:120).  bounds = imin,imax,jmin,jmax,kmin,kmax.
+ * kmax < kmin is ASH3D_TR_EINVAL (the reference stops). */
+int ash3d_tr_minmax_index(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, int32_t kvent, double thresh,
+                          int32_t bounds[6]);
+/* ash3d_tr_step built with FAST_SUBGRID: AdvectHorz takes the box of
+ * get_minmax_index (src/AdvectionHorz.f90:60-66), AdvectVert the same box,
+ * DiffuseHorz/DiffuseVert the box grown by one cell
+ * (src/Diffusion.F90:168-175).  bounds returns the advection box. */
+int ash3d_tr_step_subgrid(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                          double imp_fac, const int32_t *is_aloft, int32_t ivent, int32_t jvent, int32_t kvent,
+                          double thresh, int32_t bounds[6]);
+
 /* Message of the last failed call on this thread. */
 const char *ash3d_tr_last_error(void);
 

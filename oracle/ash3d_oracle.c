@@ -909,3 +909,211 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+
+/* ---- wind time interpolation : src/MesoInterpolater.f90:203-230 -------- */
+/* v_pd = (real(last,8) + real(next - last,8) * frac) * MPS_2_KMPHR; the
+ * difference is taken in FP32 (both operands are _sp arrays). */
+void orc_met_interp(orc_state *s, const orc_met_step *last, const orc_met_step *next, double frac,
+                    int use_vf) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  const double MPS_2_KMPHR = 3.6; /* src/Ash3d_VariableModules.f90:286 */
+  int i, j, k, n;
+  for (k = 1; k <= nz; k++)
+    for (j = 1; j <= ny; j++)
+      for (i = 1; i <= nx; i++) {
+        size_t m = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+        size_t o = P3(s, i, j, k);
+        float dvx = next->vx[m] - last->vx[m], dvy = next->vy[m] - last->vy[m], dvz = next->vz[m] - last->vz[m];
+        s->vx_pd[o] = ((double)last->vx[m] + (double)dvx * frac) * MPS_2_KMPHR;
+        s->vy_pd[o] = ((double)last->vy[m] + (double)dvy * frac) * MPS_2_KMPHR;
+        s->vz_pd[o] = ((double)last->vz[m] + (double)dvz * frac) * MPS_2_KMPHR;
+      }
+  if (!use_vf) return;
+  for (n = 1; n <= s->ns; n++)
+    for (k = 1; k <= nz; k++)
+      for (j = 1; j <= ny; j++)
+        for (i = 1; i <= nx; i++) {
+          size_t m = (size_t)(i - 1) +
+                     (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * (size_t)(n - 1)));
+          float dv = next->vf[m] - last->vf[m];
+          s->vf_pd[PF(s, i, j, k, n)] = ((double)last->vf[m] + (double)dv * frac) * MPS_2_KMPHR;
+        }
+}
+
+/* ---- Adjust_DT : src/Adjust_dt.f90:89-333 ------------------------------ */
+double orc_adjust_dt(const orc_state *s, const orc_dt_params *p, const orc_met_step *next, double *diag) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz, ns = s->ns;
+  const double MPS_2_KMPHR = 3.6;
+  double vxmax_dx = 0.0, vymax_dy = 0.0, vzmax_dz = 0.0, time_advect, time_diffuse, dt_tmp, dt;
+  int i, j, k, n;
+  if (p->use_met_next) { /* :96-136 */
+    if (!p->is_latlon) {
+      float mx = 0.0f, my = 0.0f;
+      for (size_t m = 0; m < (size_t)nx * ny * nz; m++) {
+        if (fabsf(next->vx[m]) > mx) mx = fabsf(next->vx[m]);
+        if (fabsf(next->vy[m]) > my) my = fabsf(next->vy[m]);
+      }
+      vxmax_dx = (double)mx * MPS_2_KMPHR / p->dx;
+      vymax_dy = (double)my * MPS_2_KMPHR / p->dy;
+    }
+    for (i = 1; i <= nx; i++)
+      for (j = 1; j <= ny; j++)
+        for (k = 1; k <= nz; k++) {
+          size_t m = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+          double t3;
+          float vfm = 0.0f;
+          if (p->is_latlon) {
+            double ms = fmin(s->sigma_nx_pd[P3(s, i, j, k)], s->sigma_nx_pd[P3(s, i + 1, j, k)]);
+            double t1 = (double)fabsf(next->vx[m]) * MPS_2_KMPHR * ms / s->kappa_pd[P3(s, i, j, k)];
+            double t2;
+            if (t1 > vxmax_dx) vxmax_dx = t1;
+            ms = fmin(s->sigma_ny_pd[P3(s, i, j, k)], s->sigma_ny_pd[P3(s, i, j + 1, k)]);
+            t2 = (double)fabsf(next->vy[m]) * MPS_2_KMPHR * ms / s->kappa_pd[P3(s, i, j, k)];
+            if (t2 > vymax_dy) vymax_dy = t2;
+          }
+          for (n = 0; n < ns; n++) {
+            float a = fabsf(next->vf[m + (size_t)nx * ny * nz * (size_t)n]);
+            if (a > vfm) vfm = a;
+          }
+          t3 = (double)(fabsf(next->vz[m]) + vfm) * MPS_2_KMPHR / p->dz[k - 1];
+          if (t3 > vzmax_dz) vzmax_dz = t3;
+        }
+  } else { /* :137-172 */
+    if (!p->is_latlon) {
+      double mx = 0.0, my = 0.0;
+      for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+          for (i = 1; i <= nx; i++) {
+            if (fabs(s->vx_pd[P3(s, i, j, k)]) > mx) mx = fabs(s->vx_pd[P3(s, i, j, k)]);
+            if (fabs(s->vy_pd[P3(s, i, j, k)]) > my) my = fabs(s->vy_pd[P3(s, i, j, k)]);
+          }
+      vxmax_dx = mx / p->dx;
+      vymax_dy = my / p->dy;
+    }
+    for (i = 1; i <= nx; i++)
+      for (j = 1; j <= ny; j++)
+        for (k = 1; k <= nz; k++) {
+          double t3, vfm = 0.0;
+          if (p->is_latlon) {
+            double ms = fmin(s->sigma_nx_pd[P3(s, i, j, k)], s->sigma_nx_pd[P3(s, i + 1, j, k)]);
+            double t1 = fabs(s->vx_pd[P3(s, i, j, k)]) * ms / s->kappa_pd[P3(s, i, j, k)], t2;
+            if (t1 > vxmax_dx) vxmax_dx = t1;
+            ms = fmin(s->sigma_ny_pd[P3(s, i, j, k)], s->sigma_ny_pd[P3(s, i, j + 1, k)]);
+            t2 = fabs(s->vy_pd[P3(s, i, j, k)]) * ms / s->kappa_pd[P3(s, i, j, k)];
+            if (t2 > vymax_dy) vymax_dy = t2;
+          }
+          for (n = 1; n <= ns; n++) {
+            double a = fabs(s->vf_pd[PF(s, i, j, k, n)]);
+            if (a > vfm) vfm = a;
+          }
+          t3 = (fabs(s->vz_pd[P3(s, i, j, k)]) + vfm) / p->dz[k - 1];
+          if (t3 > vzmax_dz) vzmax_dz = t3;
+        }
+  }
+  if (p->n_gs_aloft == 0) vzmax_dz = 0.0; /* :176 */
+  time_advect = 1.0 / fmax(fmax(vxmax_dx, vymax_dy), vzmax_dz);
+  time_diffuse = p->dt_max;
+  if (p->use_diffusion) { /* :182-246 (computed on every call here) */
+    for (i = 1; i <= nx; i++)
+      for (j = 1; j <= ny; j++)
+        for (k = 1; k <= nz; k++) {
+          double kap = s->kappa_pd[P3(s, i, j, k)], ms, t1, t2, t3, r;
+          ms = fmin(s->sigma_nx_pd[P3(s, i, j, k)], s->sigma_nx_pd[P3(s, i + 1, j, k)]);
+          r = kap / ms;
+          t1 = (r * r) / s->kx[PK(s, i, j, k)];
+          ms = fmin(s->sigma_ny_pd[P3(s, i, j, k)], s->sigma_ny_pd[P3(s, i, j + 1, k)]);
+          r = kap / ms;
+          t2 = (r * r) / s->ky[PK(s, i, j, k)];
+          ms = fmin(s->sigma_nz_pd[P3(s, i, j, k)], s->sigma_nz_pd[P3(s, i, j, k + 1)]);
+          r = kap / ms;
+          t3 = (r * r) / s->kz[PK(s, i, j, k)];
+          if (time_diffuse > t1 + t2 + t3) time_diffuse = t1 + t2 + t3;
+        }
+    time_diffuse = time_diffuse * (p->use_cn ? p->imp_dt_fac : 0.5);
+  }
+  dt_tmp = fmin(time_advect, time_diffuse); /* :296-323 */
+  if (dt_tmp < p->dt_min) dt = p->dt_min;
+  else if (dt_tmp > p->dt_max) dt = p->dt_max;
+  else dt = fmin(1.0, p->cfl) * dt_tmp;
+  dt = p->dt_min * (double)(int)(dt / p->dt_min);
+  if ((p->next_write_time - p->time) > p->eps_small && (p->next_write_time - p->time) < dt)
+    dt = p->next_write_time - p->time;
+  if (p->time + dt > p->simtime) dt = p->simtime - p->time;
+  if (diag) {
+    diag[0] = vxmax_dx; diag[1] = vymax_dy; diag[2] = vzmax_dz; diag[3] = time_diffuse;
+  }
+  return dt;
+}
+
+/* ---- get_minmax_index : src/Calc_Mesh.f90:432-542 ---------------------- */
+static double slab_max(const orc_state *s, int dir, int idx) {
+  double m = -INFINITY;
+  int a, b, n;
+  const int A = dir == 0 ? s->ny : s->nx, B = dir == 2 ? s->ny : s->nz;
+  for (n = 1; n <= s->ns; n++)
+    for (b = 1; b <= B; b++)
+      for (a = 1; a <= A; a++) {
+        int i = dir == 0 ? idx : a, j = dir == 0 ? a : (dir == 1 ? idx : b), k = dir == 2 ? idx : b;
+        double v = s->concen_pd[PC(s, i, j, k, n, TS0)];
+        if (v > m) m = v;
+      }
+  return m;
+}
+
+int orc_minmax_index(const orc_state *s, int ivent, int jvent, int kvent, double thresh, int32_t b[6]) {
+  const int N[3] = {s->nx, s->ny, s->nz}, vent[3] = {ivent, jvent, kvent};
+  int d, i;
+  for (d = 0; d < 3; d++) {
+    int lo = 1, hi = N[d];
+    for (i = 2; i <= vent[d] - 1; i++) {
+      if (slab_max(s, d, i) < thresh) lo = i;
+      else break;
+    }
+    for (i = N[d]; i >= lo + 2; i--) {
+      if (slab_max(s, d, i) < thresh) hi = i;
+      else break;
+    }
+    if (d == 0 && s->is_periodic && (lo == 1 || hi == N[0])) { lo = 1; hi = N[0]; }
+    b[2 * d] = lo;
+    b[2 * d + 1] = hi;
+  }
+  if (b[5] < b[4]) return 1;
+  for (d = 0; d < 3; d++)
+    if (b[2 * d + 1] - b[2 * d] < 2) {
+      b[2 * d + 1] = b[2 * d + 1] + 1 < N[d] ? b[2 * d + 1] + 1 : N[d];
+      b[2 * d] = b[2 * d + 1] - 2;
+    }
+  return 0;
+}
+
+/* src/Ash3d.F90:491-510 built with FAST_SUBGRID: AdvectHorz takes the box of
+ * get_minmax_index (src/AdvectionHorz.f90:60-66), AdvectVert keeps it, and
+ * DiffuseHorz grows it by one cell (src/Diffusion.F90:168-175). */
+int orc_transport_step_subgrid(orc_state *s, int itime, int use_diffusion, int ivent, int jvent, int kvent,
+                               double thresh, int32_t bounds[6]) {
+  int32_t b[6];
+  if (orc_set_bc(s, 1)) return 1;
+  if (orc_minmax_index(s, ivent, jvent, kvent, thresh, b)) return 2;
+  if (bounds) memcpy(bounds, b, sizeof b);
+  s->imin = b[0]; s->imax = b[1]; s->jmin = b[2]; s->jmax = b[3]; s->kmin = b[4]; s->kmax = b[5];
+  if (itime % 2 == 0) { orc_advect_y(s); orc_advect_x(s); }
+  else { orc_advect_x(s); orc_advect_y(s); }
+  orc_advect_z(s);
+  if (use_diffusion) {
+    orc_set_bc(s, 2);
+    s->imin = s->imin - 1 > 1 ? s->imin - 1 : 1;
+    s->imax = s->imax + 1 < s->nx ? s->imax + 1 : s->nx;
+    s->jmin = s->jmin - 1 > 1 ? s->jmin - 1 : 1;
+    s->jmax = s->jmax + 1 < s->ny ? s->jmax + 1 : s->ny;
+    s->kmin = s->kmin - 1 > 1 ? s->kmin - 1 : 1;
+    s->kmax = s->kmax + 1 < s->nz ? s->kmax + 1 : s->nz;
+    if (s->use_cn) {
+      if (itime % 2 == 0) { orc_diffcn_x(s); orc_diffcn_y(s); } else { orc_diffcn_y(s); orc_diffcn_x(s); }
+      orc_diffcn_z(s);
+    } else {
+      if (itime % 2 == 0) { orc_diff_x(s); orc_diff_y(s); } else { orc_diff_y(s); orc_diff_x(s); }
+      orc_diff_z(s);
+    }
+  }
+  return 0;
+}

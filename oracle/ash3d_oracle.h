@@ -21,6 +21,9 @@
  *                          algorithm, Gaussian elimination with partial
  *                          pivoting)
  *   step glue              src/Ash3d.F90:376-378, :428-431, :491-510
+ *   wind time interpolation src/MesoInterpolater.f90:203-230
+ *   Adjust_DT              src/Adjust_dt.f90:89-333
+ *   get_minmax_index       src/Calc_Mesh.f90:432-542 (and the FAST_SUBGRID step)
  *
  * Arrays use the reference's Fortran column-major layouts and lower bounds
  * (src/Ash3d_VariableModules.f90:918-946, src/Diffusion.F90:95-101):
@@ -94,6 +97,41 @@ void orc_step_glue(orc_state *s);
 void orc_source_column_add(orc_state *s, int ivent, int jvent,
                            const double *flux /* (1:nz+1,1:ns) */);
 int orc_transport_step(orc_state *s, int itime, int use_diffusion);
+
+/* Met-step winds in m/s, FP32, unpadded (1:nx,1:ny,1:nz) and vf
+ * (1:nx,1:ny,1:nz,1:ns): the wind_grid module's v*_meso_{last,next}_step_sp. */
+typedef struct {
+  const float *vx, *vy, *vz, *vf;
+} orc_met_step;
+
+/* vx/vy/vz_pd(1:nx,1:ny,1:nz) and, if use_vf, vf_pd(1:nx,1:ny,1:nz,:) from
+ * the bracketing met steps at Interval_Frac = frac. */
+void orc_met_interp(orc_state *s, const orc_met_step *last, const orc_met_step *next, double frac,
+                    int use_vf);
+
+typedef struct {
+  int32_t is_latlon;      /* mesh::IsLatLon */
+  int32_t use_met_next;   /* Adjust_DT(mesostep) present: the met-step (FAST_DT) branch */
+  int32_t n_gs_aloft;     /* Tephra::n_gs_aloft */
+  int32_t use_diffusion, use_cn;
+  double dx, dy;          /* mesh::dx, dy (km) */
+  const double *dz;       /* dz_vec_pd(1:nz) */
+  double imp_dt_fac, cfl, dt_min, dt_max, eps_small;
+  double time, next_write_time, simtime; /* hours */
+} orc_dt_params;
+
+/* Adjust_DT: returns dt (hours); diag[0..3] = vxmax_dx, vymax_dy, vzmax_dz,
+ * time_diffuse. */
+double orc_adjust_dt(const orc_state *s, const orc_dt_params *p, const orc_met_step *next, double *diag);
+
+/* get_minmax_index with the vent (ivent,jvent,kvent) and CLOUDCON_GRID_THRESH
+ * = thresh; returns 0, or 1 for kmax < kmin (the reference stops). */
+int orc_minmax_index(const orc_state *s, int ivent, int jvent, int kvent, double thresh, int32_t bounds[6]);
+
+/* ash3d step with FAST_SUBGRID (src/Ash3d.F90:491-510, src/AdvectionHorz.f90:60-66,
+ * src/Diffusion.F90:168-175); returns 2 if get_minmax_index stops. */
+int orc_transport_step_subgrid(orc_state *s, int itime, int use_diffusion, int ivent, int jvent, int kvent,
+                               double thresh, int32_t bounds[6]);
 
 void orc_dgtsv(int n, int nrhs, double *dl, double *d, double *du, double *b,
                int ldb, int *info);

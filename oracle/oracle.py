@@ -66,6 +66,34 @@ class _OrcState(ctypes.Structure):
     ]
 
 
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class _MetStep(ctypes.Structure):
+    _fields_ = [("vx", _fp), ("vy", _fp), ("vz", _fp), ("vf", _fp)]
+
+
+class DtParams(ctypes.Structure):
+    """orc_dt_params (Adjust_DT inputs); the same field order as
+    ash3d_tr_dt_params of the library ABI."""
+    _fields_ = [("is_latlon", ctypes.c_int32), ("use_met_next", ctypes.c_int32), ("n_gs_aloft", ctypes.c_int32),
+                ("use_diffusion", ctypes.c_int32), ("use_cn", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", _dp),
+                ("imp_dt_fac", ctypes.c_double), ("cfl", ctypes.c_double), ("dt_min", ctypes.c_double),
+                ("dt_max", ctypes.c_double), ("eps_small", ctypes.c_double), ("time", ctypes.c_double),
+                ("next_write_time", ctypes.c_double), ("simtime", ctypes.c_double)]
+
+
+def _met(step):
+    """(vx, vy, vz, vf) float32 Fortran arrays -> _MetStep (vf may be None)."""
+    m = _MetStep()
+    for name, a in zip(("vx", "vy", "vz", "vf"), step):
+        if a is not None:
+            assert a.dtype == np.float32 and a.flags.f_contiguous
+            setattr(m, name, a.ctypes.data_as(_fp))
+    return m
+
+
 _lib = None
 
 
@@ -96,6 +124,16 @@ def load():
         lib.orc_limiter_phi.argtypes = [ctypes.c_int, ctypes.c_double]
         lib.orc_limiter_phi.restype = ctypes.c_double
         lib.orc_num_threads.restype = ctypes.c_int
+        M = ctypes.POINTER(_MetStep)
+        lib.orc_met_interp.argtypes = [P, M, M, ctypes.c_double, ctypes.c_int]
+        lib.orc_met_interp.restype = None
+        lib.orc_adjust_dt.argtypes = [P, ctypes.POINTER(DtParams), M, _dp]
+        lib.orc_adjust_dt.restype = ctypes.c_double
+        lib.orc_minmax_index.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip]
+        lib.orc_minmax_index.restype = ctypes.c_int
+        lib.orc_transport_step_subgrid.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_double, _ip]
+        lib.orc_transport_step_subgrid.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -191,6 +229,32 @@ class Oracle:
 
     def transport_step(self, itime: int, use_diffusion: bool) -> int:
         return self._call("orc_transport_step", itime, int(bool(use_diffusion)))
+
+    # driver work between the hot-path calls
+    def met_interp(self, last, nxt, frac: float, use_vf: bool):
+        """MesoInterpolater's wind time interpolation (src/MesoInterpolater.f90:203-230)."""
+        a, b = _met(last), _met(nxt)
+        self._call("orc_met_interp", ctypes.byref(a), ctypes.byref(b), float(frac), int(bool(use_vf)))
+
+    def adjust_dt(self, params: DtParams, nxt=None):
+        """Adjust_DT (src/Adjust_dt.f90:89-333): (dt, [vxmax_dx, vymax_dy, vzmax_dz, time_diffuse])."""
+        m = _met(nxt) if nxt is not None else _MetStep()
+        diag = np.zeros(4)
+        dt = self._call("orc_adjust_dt", ctypes.byref(params), ctypes.byref(m), _ptr(diag))
+        return dt, diag
+
+    def minmax_index(self, ivent: int, jvent: int, kvent: int, thresh: float = 1e-7):
+        """get_minmax_index (src/Calc_Mesh.f90:432-542): (status, bounds)."""
+        b = np.zeros(6, dtype=np.int32)
+        rc = self._call("orc_minmax_index", ivent, jvent, kvent, float(thresh), b.ctypes.data_as(_ip))
+        return rc, tuple(int(v) for v in b)
+
+    def transport_step_subgrid(self, itime: int, use_diffusion: bool, ivent: int, jvent: int, kvent: int,
+                               thresh: float = 1e-7):
+        b = np.zeros(6, dtype=np.int32)
+        rc = self._call("orc_transport_step_subgrid", itime, int(bool(use_diffusion)), ivent, jvent, kvent,
+                        float(thresh), b.ctypes.data_as(_ip))
+        return rc, tuple(int(v) for v in b)
 
     @property
     def last_info(self) -> int:

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -91,6 +92,12 @@ struct ash3d_tr_ctx {
   int n_src_slots = 0;
   double *col = nullptr;  // column-load staging: nz dz values, then nx*ny loads / weights
   std::vector<double *> ckpt;  // checkpoint slots
+  // met steps: two slots of vx, vy, vz (n3) and vf (n3*ns) in FP32
+  float *met[2][4] = {};
+  bool met_vf[2] = {false, false};
+  int met_next = 1, met_count = 0;
+  double *dtp = nullptr;                 // Adjust_DT partials, then dz
+  unsigned long long *prof = nullptr;    // get_minmax_index slab maxima
   int ts1_state = TS1_EXPLICIT;
   int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
@@ -501,6 +508,11 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
     if (p) hipFree(p);
   for (double *p : x->ckpt)
     if (p) hipFree(p);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 4; ++b)
+      if (x->met[a][b]) hipFree(x->met[a][b]);
+  if (x->dtp) hipFree(x->dtp);
+  if (x->prof) hipFree(x->prof);
   for (int k = 0; k < ash3d_tr_ctx::kFluxRing; ++k) {
     if (x->flux_pinned[k]) hipHostFree(x->flux_pinned[k]);
     if (x->flux_ev[k]) hipEventDestroy(x->flux_ev[k]);
@@ -1011,6 +1023,177 @@ int ash3d_tr_adj_column_load(ash3d_tr_ctx *x, const double *dz, const double *w)
   HIPCHK(hipMemcpyAsync(x->col + d.nz, w, sizeof(double) * (size_t)d.nx * d.ny, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(launch_adj_column(x->stream, d, x->lam, x->col, x->col + d.nz));
+  return 0;
+}
+
+int ash3d_tr_met_push(ash3d_tr_ctx *x, const float *vx, const float *vy, const float *vz, const float *vf) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!vx || !vy || !vz) return fail(ASH3D_TR_EINVAL, "null met-step wind");
+  const Dims &d = x->d;
+  const size_t n3 = (size_t)d.nx * d.ny * d.nz;
+  if (!x->met[0][0]) {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const size_t n = (b == 3) ? n3 * d.ns : n3;
+        hipError_t e = hipMalloc((void **)&x->met[a][b], sizeof(float) * n);
+        if (e != hipSuccess) return fail(ASH3D_TR_ENOMEM, "met-step allocation: %s", hipGetErrorString(e));
+      }
+  }
+  const int slot = 1 - x->met_next;  // the old "last": overwritten by the new "next"
+  const float *src[4] = {vx, vy, vz, vf};
+  for (int b = 0; b < 4; ++b) {
+    if (!src[b]) continue;
+    const size_t n = (b == 3) ? n3 * d.ns : n3;
+    HIPCHK(hipMemcpyAsync(x->met[slot][b], src[b], sizeof(float) * n, hipMemcpyHostToDevice, x->stream));
+  }
+  HIPCHK(hipStreamSynchronize(x->stream));  // pageable sources
+  x->met_vf[slot] = vf != nullptr;
+  x->met_next = slot;
+  ++x->met_count;
+  return 0;
+}
+
+int ash3d_tr_met_interp(ash3d_tr_ctx *x, double frac, int32_t use_vf) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (x->met_count < 2) return fail(ASH3D_TR_EINVAL, "two met steps are needed (have %d)", x->met_count);
+  const int nxt = x->met_next, lst = 1 - nxt;
+  if (use_vf && !(x->met_vf[0] && x->met_vf[1])) return fail(ASH3D_TR_EINVAL, "fall velocities were not pushed");
+  const float *last[4] = {x->met[lst][0], x->met[lst][1], x->met[lst][2], x->met[lst][3]};
+  const float *next[4] = {x->met[nxt][0], x->met[nxt][1], x->met[nxt][2], x->met[nxt][3]};
+  HIPCHK(launch_met_interp(x->stream, x->d, last, next, frac, use_vf != 0, x->vx, x->vy, x->vz, x->vf));
+  return 0;
+}
+
+int ash3d_tr_adjust_dt(ash3d_tr_ctx *x, const ash3d_tr_dt_params *p, double *dt_out, double *diag) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!p || !dt_out || !p->dz) return fail(ASH3D_TR_EINVAL, "null parameters, dz or output");
+  if (p->use_met_next && x->met_count < 1) return fail(ASH3D_TR_EINVAL, "no met step pushed");
+  if (p->use_met_next && !x->met_vf[x->met_next]) return fail(ASH3D_TR_EINVAL, "the next met step has no vf");
+  const Dims &d = x->d;
+  const int nb = dt_partial_blocks(d);
+  if (!x->dtp && (rc = dev_alloc(&x->dtp, 4LL * 2048 + d.nz))) return rc;
+  double *dzd = x->dtp + 4LL * 2048;
+  HIPCHK(hipMemcpyAsync(dzd, p->dz, sizeof(double) * (size_t)d.nz, hipMemcpyHostToDevice, x->stream));
+  const int nxt = x->met_next;
+  const float *mn[4] = {x->met[nxt][0], x->met[nxt][1], x->met[nxt][2], x->met[nxt][3]};
+  HIPCHK(launch_dt_partials(x->stream, d, p->is_latlon, p->use_met_next, p->use_diffusion, x->vx, x->vy, x->vz, x->vf,
+                            x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz, dzd, mn, x->dtp));
+  std::vector<double> h(4 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(h.data(), x->dtp, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = -INFINITY;
+  for (int b = 0; b < nb; ++b) {
+    r0 = std::fmax(r0, h[4 * b]);
+    r1 = std::fmax(r1, h[4 * b + 1]);
+    r2 = std::fmax(r2, h[4 * b + 2]);
+    r3 = std::fmax(r3, h[4 * b + 3]);
+  }
+  const double MPS_2_KMPHR = 3.6;
+  double vxmax_dx = r0, vymax_dy = r1, vzmax_dz = r2;
+  if (!p->is_latlon) {  // src/Adjust_dt.f90:100-106, :141-146
+    vxmax_dx = p->use_met_next ? r0 * MPS_2_KMPHR / p->dx : r0 / p->dx;
+    vymax_dy = p->use_met_next ? r1 * MPS_2_KMPHR / p->dy : r1 / p->dy;
+  }
+  if (p->n_gs_aloft == 0) vzmax_dz = 0.0;  // :176
+  const double time_advect = 1.0 / std::fmax(std::fmax(vxmax_dx, vymax_dy), vzmax_dz);
+  double time_diffuse = p->dt_max;
+  if (p->use_diffusion) {  // :182-246
+    if (time_diffuse > -r3) time_diffuse = -r3;
+    time_diffuse = time_diffuse * (p->use_cn ? p->imp_dt_fac : 0.5);
+  }
+  const double dt_tmp = std::fmin(time_advect, time_diffuse);  // :296-333
+  double dt;
+  if (dt_tmp < p->dt_min) dt = p->dt_min;
+  else if (dt_tmp > p->dt_max) dt = p->dt_max;
+  else dt = std::fmin(1.0, p->cfl) * dt_tmp;
+  dt = p->dt_min * (double)(int)(dt / p->dt_min);
+  if ((p->next_write_time - p->time) > p->eps_small && (p->next_write_time - p->time) < dt)
+    dt = p->next_write_time - p->time;
+  if (p->time + dt > p->simtime) dt = p->simtime - p->time;
+  *dt_out = dt;
+  if (diag) {
+    diag[0] = vxmax_dx; diag[1] = vymax_dy; diag[2] = vzmax_dz; diag[3] = time_diffuse;
+  }
+  return 0;
+}
+
+static int minmax_index(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, int32_t kvent, double thresh, int32_t *b) {
+  const Dims &d = x->d;
+  const int np = d.nx + d.ny + d.nz;
+  int rc = 0;
+  if (!x->prof) {
+    hipError_t e = hipMalloc((void **)&x->prof, sizeof(unsigned long long) * (size_t)np);
+    if (e != hipSuccess) return fail(ASH3D_TR_ENOMEM, "profile allocation: %s", hipGetErrorString(e));
+  }
+  HIPCHK(launch_slab_max(x->stream, d, x->ts0(), x->prof));
+  std::vector<double> pm(np);
+  HIPCHK(hipMemcpyAsync(pm.data(), x->prof, sizeof(double) * (size_t)np, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  // src/Calc_Mesh.f90:459-541, scanning the slab maxima
+  const int N[3] = {d.nx, d.ny, d.nz}, vent[3] = {ivent, jvent, kvent};
+  const double *slab[3] = {pm.data(), pm.data() + d.nx, pm.data() + d.nx + d.ny};
+  for (int q = 0; q < 3; ++q) {
+    int lo = 1, hi = N[q];
+    for (int i = 2; i <= vent[q] - 1; ++i) {
+      if (slab[q][i - 1] < thresh) lo = i;
+      else break;
+    }
+    for (int i = N[q]; i >= lo + 2; --i) {
+      if (slab[q][i - 1] < thresh) hi = i;
+      else break;
+    }
+    if (q == 0 && x->periodic && (lo == 1 || hi == N[0])) { lo = 1; hi = N[0]; }
+    b[2 * q] = lo;
+    b[2 * q + 1] = hi;
+  }
+  if (b[5] < b[4]) return fail(ASH3D_TR_EINVAL, "ERROR: kmax<kmin in get_minmax_index");
+  for (int q = 0; q < 3; ++q)
+    if (b[2 * q + 1] - b[2 * q] < 2) {
+      b[2 * q + 1] = std::min(b[2 * q + 1] + 1, N[q]);
+      b[2 * q] = b[2 * q + 1] - 2;
+    }
+  (void)rc;
+  return 0;
+}
+
+int ash3d_tr_minmax_index(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, int32_t kvent, double thresh,
+                          int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!bounds) return fail(ASH3D_TR_EINVAL, "null bounds");
+  return minmax_index(x, ivent, jvent, kvent, thresh, bounds);
+}
+
+int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                          double imp_fac, const int32_t *is_aloft, int32_t ivent, int32_t jvent, int32_t kvent,
+                          double thresh, int32_t bounds[6]) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  int32_t b[6];
+  if ((rc = set_bc1(x, true))) return rc;  // src/Ash3d.F90:491
+  if ((rc = minmax_index(x, ivent, jvent, kvent, thresh, b))) return rc;
+  if (bounds)
+    for (int q = 0; q < 6; ++q) bounds[q] = b[q];
+  const int first = (itime % 2 == 0) ? 1 : 0;
+  const bool again = x->periodic != 0;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false))) return rc;
+  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false))) return rc;
+  if (use_diffusion) {
+    {
+      ProfScope ps(x, ASH3D_PROF_BC);
+      HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
+    }
+    const int32_t g[6] = {std::max(1, b[0] - 1), std::min(x->d.nx, b[1] + 1), std::max(1, b[2] - 1),
+                          std::min(x->d.ny, b[3] + 1), std::max(1, b[4] - 1), std::min(x->d.nz, b[5] + 1)};
+    const int dfirst = (itime % 2 == 0) ? 0 : 1;
+    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, g))) return rc;
+    if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, g))) return rc;
+    if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, g))) return rc;
+  }
   return 0;
 }
 

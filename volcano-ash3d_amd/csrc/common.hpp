@@ -130,6 +130,16 @@ hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, in
 hipError_t launch_mass_totals(hipStream_t s, const Dims &d, const double *c0, const double *kap,
                               const double *dep, const double *yz1, const double *yz2, const double *xz1,
                               const double *xz2, const double *xy2, double *partial, double *out);
+// driver work on the device (meteo.hip)
+hipError_t launch_met_interp(hipStream_t s, const Dims &d, const float *const last[4], const float *const next[4],
+                             double frac, bool with_vf, double *vx, double *vy, double *vz, double *vf);
+int dt_partial_blocks(const Dims &d);
+hipError_t launch_dt_partials(hipStream_t s, const Dims &d, int latlon, int met, int diffusion, const double *vx,
+                              const double *vy, const double *vz, const double *vf, const double *kap,
+                              const double *sx, const double *sy, const double *sz, const double *kx,
+                              const double *ky, const double *kz, const double *dz, const float *const met_next[4],
+                              double *partial);
+hipError_t launch_slab_max(hipStream_t s, const Dims &d, const double *c0, unsigned long long *prof);
 // adjoint (adjoint.hip)
 hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb);
 hipError_t launch_adj_cn(hipStream_t s, const DiffArgs &a, double *lam, int nb);

@@ -59,6 +59,18 @@ class Desc(ctypes.Structure):
     ]
 
 
+class DtParams(ctypes.Structure):
+    """ash3d_tr_dt_params: the inputs of Adjust_DT (src/Adjust_dt.f90:89-333)."""
+    _fields_ = [("is_latlon", ctypes.c_int32), ("use_met_next", ctypes.c_int32), ("n_gs_aloft", ctypes.c_int32),
+                ("use_diffusion", ctypes.c_int32), ("use_cn", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.POINTER(ctypes.c_double)),
+                ("imp_dt_fac", ctypes.c_double), ("cfl", ctypes.c_double), ("dt_min", ctypes.c_double),
+                ("dt_max", ctypes.c_double), ("eps_small", ctypes.c_double), ("time", ctypes.c_double),
+                ("next_write_time", ctypes.c_double), ("simtime", ctypes.c_double)]
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
 # every exported entry point and its argument types
 _SIGS = {
     "ash3d_tr_create": [ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_void_p)],
@@ -100,6 +112,13 @@ _SIGS = {
     "ash3d_tr_adj_source_read": [ctypes.c_void_p, _dp],
     "ash3d_tr_column_load": [ctypes.c_void_p, _dp, _dp],
     "ash3d_tr_adj_column_load": [ctypes.c_void_p, _dp, _dp],
+    "ash3d_tr_met_push": [ctypes.c_void_p, _fp, _fp, _fp, _fp],
+    "ash3d_tr_met_interp": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32],
+    "ash3d_tr_adjust_dt": [ctypes.c_void_p, ctypes.POINTER(DtParams), _dp, _dp],
+    "ash3d_tr_minmax_index": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _ip],
+    "ash3d_tr_step_subgrid": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                              ctypes.c_double, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                              ctypes.c_double, _ip],
 }
 PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue",
               "transpose", "adj_advect", "adj_diffuse", "copy")
@@ -342,6 +361,49 @@ class Transport:
         w = np.asfortranarray(w, dtype=np.float64)
         assert w.shape == (self.st.nx, self.st.ny)
         self._check(self.lib.ash3d_tr_adj_column_load(self._ctx, dz.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+
+    # -- driver work between the hot-path calls ------------------------------
+    def met_push(self, vx, vy, vz, vf=None):
+        """Upload the next met step (m/s, float32 Fortran arrays (nx,ny,nz); vf (nx,ny,nz,ns))."""
+        st = self.st
+        arrs = []
+        for a, shape in ((vx, (st.nx, st.ny, st.nz)), (vy, (st.nx, st.ny, st.nz)), (vz, (st.nx, st.ny, st.nz)),
+                         (vf, (st.nx, st.ny, st.nz, st.ns))):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.asfortranarray(a, dtype=np.float32)
+            assert a.shape == shape
+            arrs.append(a)
+        self._met_keep = arrs
+        self._check(self.lib.ash3d_tr_met_push(self._ctx, *[None if a is None else a.ctypes.data_as(_fp)
+                                                            for a in arrs]))
+
+    def met_interp(self, frac: float, use_vf: bool):
+        self._check(self.lib.ash3d_tr_met_interp(self._ctx, float(frac), int(bool(use_vf))))
+
+    def adjust_dt(self, params: DtParams):
+        """(dt, [vxmax_dx, vymax_dy, vzmax_dz, time_diffuse])."""
+        dt = ctypes.c_double()
+        diag = np.zeros(4)
+        self._check(self.lib.ash3d_tr_adjust_dt(self._ctx, ctypes.byref(params), ctypes.byref(dt),
+                                                diag.ctypes.data_as(_dp)))
+        return dt.value, diag
+
+    def minmax_index(self, ivent: int, jvent: int, kvent: int, thresh: float = 1e-7):
+        b = np.zeros(6, dtype=np.int32)
+        self._check(self.lib.ash3d_tr_minmax_index(self._ctx, ivent, jvent, kvent, float(thresh),
+                                                   b.ctypes.data_as(_ip)))
+        return tuple(int(v) for v in b)
+
+    def transport_step_subgrid(self, itime: int, use_diffusion: bool, ivent: int, jvent: int, kvent: int,
+                               thresh: float = 1e-7):
+        """The step as built with FAST_SUBGRID; returns the advection box."""
+        b = np.zeros(6, dtype=np.int32)
+        self._check(self.lib.ash3d_tr_step_subgrid(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                                   int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft(),
+                                                   ivent, jvent, kvent, float(thresh), b.ctypes.data_as(_ip)))
+        return tuple(int(v) for v in b)
 
     # -- launch tuning ----------------------------------------------------
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z")
