@@ -160,6 +160,54 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
             "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
 
 
+def ensemble_leg(pkg, tr, members, steps, size_scale):
+    """Ensemble throughput: one context (own stream) per member on the one
+    GPU, all members stepped together; eager launches against replays of
+    the captured glue + step graphs."""
+    import numpy as np
+    t0 = time.perf_counter()
+    states = [pkg.workloads.ensemble_member(m, size_scale=size_scale) for m in range(members)]
+    log(f"[bench] ensemble: {members} members built in {time.perf_counter() - t0:.1f} s")
+    Ts = [tr.Transport(st) for st in states]
+    work = sum(st.ns * st.nx * st.ny * st.nz for st in states)
+    for T in Ts:
+        T.st.concen_pd[..., 1] = T.st.concen_pd[..., 0]
+        T.sync_from_host(tr.SYNC_CONCEN)
+        T.step_glue()
+        T.transport_step(1, True)
+    for T in Ts:
+        T.synchronize()
+
+    def run(mode, it0):
+        t = time.perf_counter()
+        for s_ in range(steps):
+            it = it0 + s_
+            for T in Ts:
+                if mode == "graph":
+                    T.graph_launch(it)
+                else:
+                    T.step_glue()
+                    T.transport_step(it, True)
+        for T in Ts:
+            T.synchronize()
+        return time.perf_counter() - t
+
+    run("eager", 2)  # warm-up
+    t_eager = run("eager", 2 + steps)
+    for T in Ts:
+        T.graph_capture(2, True)
+        T.graph_capture(3, True)
+    run("graph", 2 + 2 * steps)  # warm-up
+    t_graph = run("graph", 2 + 3 * steps)
+    sizes = [(st.nx, st.ny, st.nz) for st in states]
+    for T in Ts:
+        T.close()
+    return {"members": members, "steps": steps, "bin_cells_per_step": work,
+            "grid_sizes": f"{min(sizes)}..{max(sizes)}, 10 bins",
+            "eager_rate": work * steps / t_eager, "graph_rate": work * steps / t_graph,
+            "eager_ms_per_ensemble_step": 1e3 * t_eager / steps, "graph_ms_per_ensemble_step": 1e3 * t_graph / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +220,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-profile", action="store_true",
                     help="no HIP-event kernel timing (for hardware-counter runs); no per-kernel figures")
+    ap.add_argument("--ensemble-members", type=int, default=16,
+                    help="members of the ensemble leg (0: skip); each on its own context and stream")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-scale", type=float, default=1.0, help="horizontal size factor of the members")
     ap.add_argument("--adj-periods", type=int, default=2)
     ap.add_argument("--adj-steps-per-period", type=int, default=10)
     args = ap.parse_args()
@@ -267,6 +319,11 @@ def main():
     log("[bench] kernels: " + json.dumps(kernels))
 
     T.close()
+    ensemble = None
+    if args.ensemble_members > 0 and not args.no_profile:
+        log("[bench] ensemble leg ...")
+        ensemble = ensemble_leg(pkg, tr, args.ensemble_members, args.ensemble_steps, args.ensemble_scale)
+        log("[bench] ensemble: " + json.dumps(ensemble))
     adjoint = None
     if adj_state is not None:
         log("[bench] adjoint leg ...")
@@ -290,6 +347,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "adjoint": adjoint,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (rate / cpu["value"]) if cpu else None,
     }

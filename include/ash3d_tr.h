@@ -215,6 +215,18 @@ enum {
 int ash3d_tr_profile(ash3d_tr_ctx *ctx, int32_t enable);
 int ash3d_tr_profile_read(ash3d_tr_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 
+/* ---- Captured steps -----------------------------------------------------
+ * No counterpart in the reference.  graph_capture records
+ * ash3d_tr_step_glue followed by ash3d_tr_step(itime, ...) as a HIP graph
+ * for the parity of itime (the x/y order alternates); graph_launch replays
+ * it for a later step of the same parity, with the arguments captured
+ * (dt, flags, IsAloft, the full-grid bounds).  Capture after at least one
+ * eager step (scratch allocated, ts1 mirrored), with kernel timing off;
+ * a synchronize-free replay costs one launch instead of about twenty. */
+int ash3d_tr_graph_capture(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                           double imp_fac, const int32_t *is_aloft);
+int ash3d_tr_graph_launch(ash3d_tr_ctx *ctx, int32_t itime);
+
 /* ---- Checkpoints ------------------------------------------------------
  * No counterpart in the reference (which only writes output files).  Slots
  * of device memory, each one copy of concen_pd(:,:,:,:,ts0).  Save copies

@@ -48,6 +48,7 @@
       logical            :: IsPeriodic   = .false.
       integer            :: ZScaling_ID  = 0
       integer :: nxmax, nymax, nzmax, nsmax
+      integer :: ivent, jvent, kvent
       real(kind=ip),dimension(:,:,:),allocatable :: kappa_pd
       real(kind=ip),dimension(:,:,:),allocatable :: sigma_nx_pd
       real(kind=ip),dimension(:,:,:),allocatable :: sigma_ny_pd
@@ -76,6 +77,12 @@
       integer :: jmin,jmax
       integer :: kmin,kmax
       end module solution
+
+      module Output_Vars
+      use precis_param
+      implicit none
+      real(kind=ip), parameter :: CLOUDCON_GRID_THRESH = 1.0e-7_ip
+      end module Output_Vars
 
       module time_data
       use precis_param

@@ -42,8 +42,8 @@
       end interface
 
       character(len=512) :: fin, fout
-      integer :: hdr(9), ihdr(6), nsteps, itime, n, h, p, iobs, nobs
-      integer :: ivent, jvent, nh, np, spp, obs_every
+      integer :: hdr(12), ihdr(6), nsteps, itime, n, h, p, iobs, nobs
+      integer :: nh, np, spp, obs_every
       real(kind=8) :: rhdr(2), J
       integer, allocatable :: aloft(:)
       real(kind=8), allocatable :: frac(:), dz(:), s(:,:), w(:,:,:), loads(:,:,:), G(:,:,:), grad(:,:), flux(:,:)

@@ -6,7 +6,7 @@
 !
 ! usage: shim_driver in.bin out.bin
 ! in.bin (stream): int32 nx,ny,nz,ns,periodic,zscaling,use_diffusion,nsteps,
-!   resident; float64 dt, imp_fac; int32 aloft(ns); then concen_pd, vx_pd,
+!   resident,ivent,jvent,kvent; float64 dt, imp_fac; int32 aloft(ns); then concen_pd, vx_pd,
 !   vy_pd, vz_pd, vh_pd, vf_pd, kappa_pd, sigma_nx/ny/nz_pd, kx, ky, kz,
 !   DelDxonD_cc, DelDyonD_cc, outflow_xz1/2, outflow_yz1/2, outflow_xy1/2,
 !   DepositGranularity.  out.bin: the same arrays after the run.
@@ -40,7 +40,7 @@
       end interface
 
       character(len=512) :: fin, fout
-      integer :: hdr(9), nsteps, itime, n, use_diffusion, resident
+      integer :: hdr(12), nsteps, itime, n, use_diffusion, resident
       real(kind=8) :: rhdr(2)
       integer, allocatable :: aloft(:)
 
@@ -54,6 +54,7 @@
       use_diffusion = hdr(7)
       nsteps = hdr(8)
       resident = hdr(9)
+      ivent = hdr(10); jvent = hdr(11); kvent = hdr(12)
       read(10) rhdr
       dt = rhdr(1)
       dt_ip = real(dt, ip)

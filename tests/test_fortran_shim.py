@@ -19,10 +19,10 @@ ORDER = ("concen_pd", "vx_pd", "vy_pd", "vz_pd", "vh_pd", "vf_pd", "kappa_pd", "
          "outflow_yz1_pd", "outflow_yz2_pd", "outflow_xy1_pd", "outflow_xy2_pd", "deposit_granularity")
 
 
-def write_case(path, st, use_diffusion, nsteps, resident):
+def write_case(path, st, use_diffusion, nsteps, resident, vent=(1, 1, 1)):
     with open(path, "wb") as f:
         np.array([st.nx, st.ny, st.nz, st.ns, int(st.is_periodic), st.zscaling_id, int(use_diffusion), nsteps,
-                  int(resident)], dtype=np.int32).tofile(f)
+                  int(resident), *vent], dtype=np.int32).tofile(f)
         np.array([st.dt, st.imp_fac], dtype=np.float64).tofile(f)
         np.asarray(st.is_aloft, dtype=np.int32).tofile(f)
         for name in ORDER:
@@ -120,3 +120,33 @@ def test_fortran_inversion_matches_python(pkg, tmp_path):
     assert np.array_equal(grad_f, grad)
     J = float(np.sum(loads * w))
     assert abs(J_f - J) <= 1e-12 * J
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("resident", [False, True])
+def test_shim_driver_fast_subgrid(pkg, oracle, tmp_path, resident):
+    """The drivers built with -DFAST_SUBGRID (AdvectHorz takes the device
+    get_minmax_index box, DiffuseHorz grows it) against the oracle's
+    FAST_SUBGRID step."""
+    exe = os.path.join(HERE, "fortran", "_build", "sg", "shim_driver")
+    assert os.path.exists(exe), "build the shim driver (tests/fortran/build_shim.py)"
+    st = pkg.workloads.random_state(26, 20, 12, 3, seed=12, latlon=True, periodic=True)
+    c = st.concen_pd[..., 0].copy()
+    st.concen_pd[...] = 0.0
+    st.concen_pd[9:17, 7:14, 4:9, :, 0] = c[9:17, 7:14, 4:9, :]
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    vent = (12, 10, 5)
+    ref = st.copy()
+    O = oracle.Oracle(ref)
+    for itime in range(1, 4):
+        O.step_glue()
+        rc, _ = O.transport_step_subgrid(itime, True, *vent)
+        assert rc == 0
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_case(fin, st, True, 3, resident, vent)
+    r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = st.copy()
+    read_case(fout, out)
+    for name in ORDER:
+        assert np.array_equal(getattr(out, name), getattr(ref, name)), name

@@ -97,6 +97,8 @@ struct ash3d_tr_ctx {
   bool met_vf[2] = {false, false};
   int met_next = 1, met_count = 0;
   double *dtp = nullptr;                 // Adjust_DT partials, then dz
+  // captured glue + step, one graph per step parity
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   unsigned long long *prof = nullptr;    // get_minmax_index slab maxima
   int ts1_state = TS1_EXPLICIT;
   int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
@@ -512,6 +514,8 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
     for (int b = 0; b < 4; ++b)
       if (x->met[a][b]) hipFree(x->met[a][b]);
   if (x->dtp) hipFree(x->dtp);
+  for (int k = 0; k < 2; ++k)
+    if (x->graph[k]) hipGraphExecDestroy(x->graph[k]);
   if (x->prof) hipFree(x->prof);
   for (int k = 0; k < ash3d_tr_ctx::kFluxRing; ++k) {
     if (x->flux_pinned[k]) hipHostFree(x->flux_pinned[k]);
@@ -581,12 +585,12 @@ int ash3d_tr_sync_to_host(ash3d_tr_ctx *x, uint32_t mask) {
   const ash3d_tr_desc &h = x->host;
   const Dims &d = x->d;
   if (mask & ASH3D_SYNC_CONCEN) {
+    // materialise ts1; the recorded state stays valid (ts1 now holds
+    // exactly what the record says), so captured steps may still follow
     if (x->ts1_state == TS1_MIRROR) {
       HIPCHK(launch_mirror_interior(x->stream, d, x->ts0(), x->ts1()));
-      x->ts1_state = TS1_EXPLICIT;
     } else if (x->ts1_state == TS1_ZERO) {
       HIPCHK(launch_fill(x->stream, x->ts1(), x->n_conc(), 0.0));
-      x->ts1_state = TS1_EXPLICIT;
     }
     if ((rc = download(x, h.concen_pd, x->c, 2 * x->n_conc()))) return rc;
   }
@@ -1023,6 +1027,50 @@ int ash3d_tr_adj_column_load(ash3d_tr_ctx *x, const double *dz, const double *w)
   HIPCHK(hipMemcpyAsync(x->col + d.nz, w, sizeof(double) * (size_t)d.nx * d.ny, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(launch_adj_column(x->stream, d, x->lam, x->col, x->col + d.nz));
+  return 0;
+}
+
+int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                           double imp_fac, const int32_t *is_aloft) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (x->ts1_state != TS1_MIRROR)
+    return fail(ASH3D_TR_EINVAL, "capture after an eager step (the glue must be the ghost-zeroing kernel)");
+  if (x->prof_on) return fail(ASH3D_TR_EINVAL, "kernel timing is on; switch it off before capturing");
+  if (use_diffusion && use_cn && !x->fac)
+    return fail(ASH3D_TR_EINVAL, "capture after an eager step (line-solve scratch not allocated)");
+  if (use_diffusion && !x->tgeom_valid)
+    return fail(ASH3D_TR_EINVAL, "capture after an eager step (transposed geometry not built)");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  const int slot = itime % 2;
+  if (x->graph[slot]) {
+    HIPCHK(hipGraphExecDestroy(x->graph[slot]));
+    x->graph[slot] = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  int rc2 = ash3d_tr_step_glue(x);
+  if (!rc2) rc2 = ash3d_tr_step(x, itime, dt, use_diffusion, use_cn, imp_fac, is_aloft);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(x->stream, &g);
+  if (rc2) {
+    if (g) hipGraphDestroy(g);
+    return rc2;
+  }
+  if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "stream capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&x->graph[slot], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "graph instantiation: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ash3d_tr_graph_launch(ash3d_tr_ctx *x, int32_t itime) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const int slot = itime % 2;
+  if (!x->graph[slot]) return fail(ASH3D_TR_EINVAL, "no graph captured for step parity %d", slot);
+  if (x->ts1_state != TS1_MIRROR) return fail(ASH3D_TR_EINVAL, "the state changed since the capture (sync or load)");
+  HIPCHK(hipGraphLaunch(x->graph[slot], x->stream));
+  x->ts1_state = TS1_MIRROR;
   return 0;
 }
 

@@ -33,7 +33,8 @@
              c_ash3d_tr_last_error, ASH3D_TR_ABI_VERSION, ASH3D_SYNC_ALL, &
              c_ash3d_tr_checkpoints, c_ash3d_tr_checkpoint, c_ash3d_tr_adj_init, &
              c_ash3d_tr_adj_step, c_ash3d_tr_adj_source, c_ash3d_tr_adj_source_read, &
-             c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load
+             c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load, c_ash3d_tr_minmax_index, &
+             c_ash3d_tr_met_push, c_ash3d_tr_met_interp
 
       integer(c_int32_t), parameter :: ASH3D_TR_ABI_VERSION = 1
       integer(c_int32_t), parameter :: ASH3D_SYNC_ALL = 255
@@ -175,6 +176,25 @@
           import :: c_int, c_ptr, c_double
           type(c_ptr), value :: ctx
           real(c_double), intent(in) :: dz(*), w(*)
+        end function
+        integer(c_int) function c_ash3d_tr_minmax_index(ctx, iv, jv, kv, thresh, bnd) &
+            bind(C, name='ash3d_tr_minmax_index')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: iv, jv, kv
+          real(c_double), value :: thresh
+          integer(c_int32_t), intent(out) :: bnd(6)
+        end function
+        integer(c_int) function c_ash3d_tr_met_push(ctx, vx, vy, vz, vf) bind(C, name='ash3d_tr_met_push')
+          import :: c_int, c_ptr, c_float
+          type(c_ptr), value :: ctx
+          real(c_float), intent(in) :: vx(*), vy(*), vz(*), vf(*)
+        end function
+        integer(c_int) function c_ash3d_tr_met_interp(ctx, frac, use_vf) bind(C, name='ash3d_tr_met_interp')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), value :: frac
+          integer(c_int32_t), value :: use_vf
         end function
         type(c_ptr) function c_ash3d_tr_last_error() bind(C, name='ash3d_tr_last_error')
           import :: c_ptr

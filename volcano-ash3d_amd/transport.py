@@ -112,6 +112,9 @@ _SIGS = {
     "ash3d_tr_adj_source_read": [ctypes.c_void_p, _dp],
     "ash3d_tr_column_load": [ctypes.c_void_p, _dp, _dp],
     "ash3d_tr_adj_column_load": [ctypes.c_void_p, _dp, _dp],
+    "ash3d_tr_graph_capture": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_double, _ip],
+    "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_met_push": [ctypes.c_void_p, _fp, _fp, _fp, _fp],
     "ash3d_tr_met_interp": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32],
     "ash3d_tr_adjust_dt": [ctypes.c_void_p, ctypes.POINTER(DtParams), _dp, _dp],
@@ -293,6 +296,17 @@ class Transport:
         self._check(self.lib.ash3d_tr_mass_totals(self._ctx, a.ctypes.data_as(_dp), dpo.ctypes.data_as(_dp),
                                                   o.ctypes.data_as(_dp)))
         return a, dpo, o
+
+    # -- captured steps ----------------------------------------------------
+    def graph_capture(self, itime: int, use_diffusion: bool):
+        """Capture step_glue + transport_step(itime) for itime's parity."""
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_graph_capture(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                                    int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft()))
+
+    def graph_launch(self, itime: int):
+        """step_glue + transport_step(itime) from the captured graph."""
+        self._check(self.lib.ash3d_tr_graph_launch(self._ctx, itime))
 
     # -- checkpoints -------------------------------------------------------
     def checkpoints(self, nslots: int):

@@ -247,6 +247,30 @@ def config2_state(seed: int = 2, ns: int = 15, nx: int = 360, ny: int = 240, nz:
     return st
 
 
+def ensemble_member(member: int, ns: int = 10, size_scale: float = 1.0, seed: int = 3) -> TransportState:
+    """One member of the ensemble workload: a lon-lat 0.25 deg grid whose
+    size is drawn uniformly in [120,400] x [80,300] x [40,60] cells (times
+    size_scale horizontally), dz 0.5 km, ns bins with |vf| log-spaced
+    0.02-3 m/s, the zonal jet with the member's own perturbations (sigma
+    2 m/s), Kh = Kz = 1000 m2/s, Crank-Nicolson, and a cloud aloft around a
+    plume height drawn in 8-20 km."""
+    rng = np.random.default_rng([seed, member])
+    nx = max(8, int(rng.integers(120, 401) * size_scale))
+    ny = max(8, int(rng.integers(80, 301) * size_scale))
+    nz = int(rng.integers(40, 61))
+    st = TransportState(nx, ny, nz, ns, limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5)
+    lonlat_mesh(st, 180.0, 25.0, 0.25, 0.25, np.full(nz, 0.5))
+    zonal_jet_winds(st, rng, sigma_mps=2.0)
+    set_uniform_fall(st, fall_velocities(ns, 0.02, 3.0))
+    set_diffusivity(st, 1000.0, 1000.0)
+    st.use_diffusion = True
+    st.dt = adjust_dt(st, use_diffusion=True)
+    height_km = rng.uniform(8.0, 20.0)
+    k = min(nz - 3.0, height_km / 0.5)
+    gaussian_blob(st, (nx * 0.3, ny * 0.5, k), (nx * 0.08, ny * 0.1, nz * 0.12), 1.0e3)
+    return st
+
+
 def random_state(nx, ny, nz, ns, seed, latlon=False, periodic=False, zscaling_id=0,
                  limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5, diffusivity=True,
                  cfl=0.8) -> TransportState:
