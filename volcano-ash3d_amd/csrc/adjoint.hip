@@ -104,7 +104,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_lines(AdvArgs A, const do
                                                              double *lam, int nb) {
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + nb - 1) / nb);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_lines(AdvArgs A, const do
   auto ld_ok = [&](int r) { return r >= rmin - 2 && r <= rmax + 2; };
   auto uin = [&](int r) { return r >= rmin - 1 && r <= rmax + 1; };
 
-  for (int n = (int)blockIdx.y * nb; n < d.ns && n < ((int)blockIdx.y + 1) * nb; ++n) {
+  for (int n = tg.group * nb; n < d.ns && n < (tg.group + 1) * nb; ++n) {
     const long long bofs = d.ncell * (long long)n;
     if (!(line_on && A.aloft.on(n))) {
       for (int r = 1; r <= N; ++r) lam[cell(r) + bofs] = 0.0;
@@ -348,17 +349,18 @@ template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
-  const int n0 = (int)blockIdx.y * NB;
+  const int n0 = tg.group * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
   const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
-  double *uD = A.fac + 3 * d.ncell * (long long)blockIdx.y;
+  double *uD = A.fac;  // shared by the bin groups (identical values)
   double *uDU = uD + d.ncell, *uDL = uD + 2 * d.ncell;
   double *sb = A.scratch_b;
   auto cell = [&](int r) { return base + sL * (long long)(r + 1); };
@@ -494,7 +496,8 @@ template <int DIR>
 __global__ __launch_bounds__(kBlock) void k_adj_diff_explicit(DiffArgs A, double *lam, int nb, int periodic) {
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + nb - 1) / nb);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_diff_explicit(DiffArgs A, double
     const double ds = 2.0 * A.sig[cell(I)] / (A.kap[cell(I - 1)] + A.kap[cell(I)]);
     return 0.5 * (A.kk[kB + kS * (long long)(I - 1)] + A.kk[kB + kS * (long long)I]) * ds;
   };
-  for (int n = (int)blockIdx.y * nb; n < d.ns && n < ((int)blockIdx.y + 1) * nb; ++n) {
+  for (int n = tg.group * nb; n < d.ns && n < (tg.group + 1) * nb; ++n) {
     const long long bofs = d.ncell * (long long)n;
     if (!(line_on && A.aloft.on(n))) {
       for (int r = 1; r <= N; ++r) lam[cell(r) + bofs] = 0.0;
@@ -595,7 +598,7 @@ hipError_t adj_x_rows_cpt(hipStream_t s, const AdvArgs &a, const double *qpre, d
 template <int DIR>
 hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + nb - 1) / nb));
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + nb - 1) / nb)));
   if (DIR != 2 && a.deld != nullptr)
     hipLaunchKernelGGL((k_adj_advect_lines<DIR, true>), grid, dim3(kBlock), 0, s, a, qpre, lam, nb);
   else
@@ -606,7 +609,7 @@ hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double
 template <int DIR, int NB>
 hipError_t adj_cn_dir(hipStream_t s, const DiffArgs &a, double *lam) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
   hipLaunchKernelGGL((k_adj_cn<DIR, NB>), grid, dim3(kBlock), 0, s, a, lam);
   return hipGetLastError();
 }
@@ -653,7 +656,7 @@ hipError_t launch_adj_cn(hipStream_t s, const DiffArgs &a, double *lam, int nb) 
 
 hipError_t launch_adj_diff_explicit(hipStream_t s, const DiffArgs &a, double *lam, int nb, int periodic) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + nb - 1) / nb));
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + nb - 1) / nb)));
   if (a.g.dir == 2)
     hipLaunchKernelGGL((k_adj_diff_explicit<2>), grid, dim3(kBlock), 0, s, a, lam, nb, periodic);
   else

@@ -34,11 +34,13 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const int ntile = (g.Na + 63) / 64;
-  const int a = (int)(blockIdx.x % (unsigned)ntile) * 64 + (int)threadIdx.x + 1;
-  const int b = (int)(blockIdx.x / (unsigned)ntile) + 1;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
+  const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
+  const int b = (int)(tg.tile / ntile) + 1;
   const bool lane_ok = a <= g.Na;
   const int W = blockDim.y, w = threadIdx.y;
-  const int n0 = (int)blockIdx.y * NB;
+  const int n0 = tg.group * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int seg = (N + 2 + W - 1) / W;
   const int lo = w * seg, hi = min(N + 1, lo + seg - 1);
@@ -254,7 +256,7 @@ hipError_t launch_dir(hipStream_t s, const AdvArgs &a) {
   int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
   W = W < 1 ? 1 : (W > 4 ? 4 : W);
   while (W > 1 && (a.g.N + 2) / W < 16) --W;
-  dim3 grid((unsigned)((long long)ntile * a.g.Nb), (unsigned)groups);
+  dim3 grid((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
   dim3 block(64, W);
   if (DIR != 2 && a.deld != nullptr)
     hipLaunchKernelGGL((k_advect<DIR, NB, true>), grid, block, 0, s, a);

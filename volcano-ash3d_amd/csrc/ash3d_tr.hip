@@ -69,7 +69,7 @@ struct ash3d_tr_ctx {
   double *ofxz1 = nullptr, *ofxz2 = nullptr, *ofyz1 = nullptr, *ofyz2 = nullptr;
   double *ofxy1 = nullptr, *ofxy2 = nullptr;
   double *dep = nullptr;
-  double *fac = nullptr;  // CN factor scratch
+  double *fac = nullptr;  // line-solve U factors (3 * ncell)
   long long fac_groups = 0;
   double *flux = nullptr;  // source column staging (nz+1)*ns
   // pinned host ring for the source columns: the caller's buffer is copied
@@ -263,16 +263,13 @@ int ensure_tgeom(ash3d_tr_ctx *x) {
   return 0;
 }
 
-// CN factor scratch for bin groups of nb
+// Line-solve U factors (one set per line, shared by the bin groups)
 int ensure_fac(ash3d_tr_ctx *x, int nb) {
-  const long long groups = (x->d.ns + nb - 1) / nb;
-  if (groups <= x->fac_groups) return 0;
-  if (x->fac) HIPCHK(hipFree(x->fac));
-  x->fac = nullptr;
-  x->fac_groups = 0;
-  int rc = dev_alloc(&x->fac, 3 * x->d.ncell * groups);
+  (void)nb;
+  if (x->fac) return 0;
+  int rc = dev_alloc(&x->fac, 3 * x->d.ncell);
   if (rc) return rc;
-  x->fac_groups = groups;
+  x->fac_groups = 1;
   return 0;
 }
 

@@ -77,6 +77,24 @@ __host__ __device__ inline long long kline_stride(const Dims &d, int dir) {
   return dir == 0 ? 1LL : (dir == 1 ? (long long)(d.nx + 2) : (long long)(d.nx + 2) * (d.ny + 2));
 }
 
+// XCD-aware order of a flattened (tile, bin group) grid.  Workgroups are
+// dispatched round-robin over the XCDs (workgroup b to XCD b mod 8), each
+// XCD with its own L2.  The m-th workgroup dispatched to XCD x takes tile
+// (m / G) * 8 + x and bin group m mod G, so the G bin groups of a tile run
+// back to back on one XCD and read the tile's per-cell fields (winds,
+// geometry, K, line factors) through the same L2.  A launch has
+// padded_tiles(T) * G workgroups; those with tile >= T return at once.
+constexpr int kXcds = 8;
+__host__ __device__ inline long long padded_tiles(long long T) { return (T + kXcds - 1) / kXcds * kXcds; }
+struct TileGroup {
+  long long tile;
+  int group;
+};
+__device__ inline TileGroup xcd_tile_group(long long b, int G) {
+  const long long x = b % kXcds, m = b / kXcds;
+  return TileGroup{(m / G) * kXcds + x, (int)(m % G)};
+}
+
 struct AdvArgs {
   Dims d;
   LineGeom g;
@@ -100,7 +118,7 @@ struct DiffArgs {
   LineGeom g;
   double *c0;          // concen ts0 (in/out)
   double *scratch_b;   // per-bin scratch, same layout as concen ts0 (the ts1 slot)
-  double *fac;         // per-line factor scratch: 3 * ncell per bin group
+  double *fac;         // per-line U factors, 3 * ncell, shared by the bin groups
   const double *kk;    // kx / ky / kz
   const double *sig;
   const double *kap;

@@ -23,11 +23,12 @@ template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
-  const int n0 = (int)blockIdx.y * NB;
+  const int n0 = tg.group * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
   const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
@@ -111,13 +112,15 @@ __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
 // with q_left = q_c on the first row and q_right = q_c on the last.
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
+  constexpr int PF = 4;  // rows in flight per stream
   const Dims d = A.d;
   const LineGeom g = A.g;
-  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
-  const int n0 = (int)blockIdx.y * NB;
+  const int n0 = tg.group * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;  // nlineq
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
@@ -125,8 +128,11 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *c = A.c0;
   double *sb = A.scratch_b;
-  // per-line U factors: D, DU, DL(2nd super-diagonal), one slab per bin group
-  double *fD = A.fac + 3 * d.ncell * (long long)blockIdx.y;
+  // per-line U factors: D, DU, DL (2nd super-diagonal).  They depend on the
+  // line only, so every bin group writes the same values to the same slab
+  // (identical bits whatever the interleaving) and the groups of a tile,
+  // which run together on one XCD, share it through L2.
+  double *fD = A.fac;
   double *fDU = fD + d.ncell;
   double *fDL = fDU + d.ncell;
   bool exists[NB], on[NB];
@@ -166,24 +172,34 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     // original sub-diagonal entry that row i+1 carries
     double Dc = 0.0, DUc = 0.0, Bc[NB];
     int singular = 0;
-    // right-neighbour data of row l, fetched one row ahead
-    double pf_kap = kapat(rmin + 1), pf_K = Kat(rmin + 1), pf_sig = sigat(rmin + 1), pf_q[NB];
+    // Row l needs the data of its right neighbour, cell rmin+l (kappa, K,
+    // sigma; q while that cell is active).  A ring of PF rows is kept in
+    // flight: slot p holds row l0+p of the current block of PF rows and is
+    // refilled with row l+PF when row l consumes it, so PF loads per stream
+    // are outstanding while the recurrence runs.
+    double rk[PF], rK[PF], rs[PF], rq[PF][NB];
+    auto fetch_right = [&](int slot, int l) {
+      const int cell = rmin + l;
+      rk[slot] = kapat(cell);
+      rK[slot] = Kat(cell);
+      rs[slot] = sigat(cell);
 #pragma unroll
-    for (int u = 0; u < NB; ++u) pf_q[u] = (1 < n && exists[u]) ? qat(u, rmin + 1) : 0.0;
-    for (int l = 1; l <= n; ++l) {
-      const int cc = rmin - 1 + l;
+      for (int u = 0; u < NB; ++u) rq[slot][u] = (l < n && exists[u]) ? qat(u, cell) : 0.0;
+    };
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+      if (1 + p <= n) fetch_right(p, 1 + p);
+    for (int l0 = 1; l0 <= n; l0 += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int l = l0 + p;
+      if (l > n) break;
       // cell cc+1 (right neighbour) data
-      const double kap_r = pf_kap, K_r = pf_K, sig_r = pf_sig;
+      const double kap_r = rk[p], K_r = rK[p], sig_r = rs[p];
       double q_next[NB];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) q_next[u] = pf_q[u];
-      if (l < n) {
-        pf_kap = kapat(cc + 2);
-        pf_K = Kat(cc + 2);
-        pf_sig = sigat(cc + 2);
-#pragma unroll
-        for (int u = 0; u < NB; ++u) pf_q[u] = (l + 1 < n && exists[u]) ? qat(u, cc + 2) : 0.0;
-      }
+      for (int u = 0; u < NB; ++u) q_next[u] = rq[p][u];
+      if (l + PF <= n) fetch_right(p, l + PF);
       const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
       const double LeftFac = dt * kds_c * sig_c / kap_c;
       const double RightFac = dt * kds_r * sig_r / kap_c;
@@ -267,6 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         q_c[u] = q_r[u];
       }
     }
+    }
     if (Dc == 0.0 && !singular) singular = n;
     if (singular) atomicCAS(A.flag, 0, singular);
     // ---- back substitution (dgtsv, NRHS <= 2 path), row n first
@@ -280,24 +297,31 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         if (on[u]) c[base + sL * (long long)(rmax + 1) + d.ncell * (long long)(n0 + u)] = x1[u];
       }
     }
-    // row i's factors and eliminated right-hand sides, fetched one row ahead
-    double pf_D = 0.0, pf_DU = 0.0, pf_DL = 0.0, pf_B[NB];
-    auto fetch_row = [&](int i) {
+    // rows' factors and eliminated right-hand sides, PF rows in flight
+    // (slot p holds row i0-p of the current block)
+    double bD[PF], bDU[PF], bDL[PF], bB[PF][NB];
+    auto fetch_row = [&](int slot, int i) {
       const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
-      pf_D = fD[fo];
-      pf_DU = fDU[fo];
-      pf_DL = fDL[fo];
+      bD[slot] = fD[fo];
+      bDU[slot] = fDU[fo];
+      bDL[slot] = fDL[fo];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) pf_B[u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+      for (int u = 0; u < NB; ++u) bB[slot][u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
     };
-    if (n - 1 >= 1) fetch_row(n - 1);
-    for (int i = n - 1; i >= 1; --i) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+      if (n - 1 - p >= 1) fetch_row(p, n - 1 - p);
+    for (int i0 = n - 1; i0 >= 1; i0 -= PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int i = i0 - p;
+      if (i < 1) break;
       const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
-      const double Di = pf_D, DUi = pf_DU, DLi = pf_DL;
+      const double Di = bD[p], DUi = bDU[p], DLi = bDL[p];
       double Bcur[NB];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) Bcur[u] = pf_B[u];
-      if (i - 1 >= 1) fetch_row(i - 1);
+      for (int u = 0; u < NB; ++u) Bcur[u] = bB[p][u];
+      if (i - PF >= 1) fetch_row(p, i - PF);
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const double Bi = Bcur[u];
@@ -308,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         x1[u] = xi;
         if (on[u]) c[fo + d.ncell * (long long)(n0 + u)] = xi;
       }
+    }
     }
   }
 #pragma unroll
@@ -329,7 +354,7 @@ struct dummy {};
 template <int NB>
 hipError_t launch_explicit_nb(hipStream_t s, const DiffArgs &a) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
   switch (a.g.dir) {
     case 0: hipLaunchKernelGGL((k_diff_explicit<0, NB>), grid, dim3(kBlock), 0, s, a); break;
     case 1: hipLaunchKernelGGL((k_diff_explicit<1, NB>), grid, dim3(kBlock), 0, s, a); break;
@@ -341,7 +366,7 @@ hipError_t launch_explicit_nb(hipStream_t s, const DiffArgs &a) {
 template <int NB>
 hipError_t launch_cn_nb(hipStream_t s, const DiffArgs &a) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
-  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock), (unsigned)((a.d.ns + NB - 1) / NB));
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
   switch (a.g.dir) {
     case 0: hipLaunchKernelGGL((k_diff_cn<0, NB>), grid, dim3(kBlock), 0, s, a); break;
     case 1: hipLaunchKernelGGL((k_diff_cn<1, NB>), grid, dim3(kBlock), 0, s, a); break;
