@@ -250,7 +250,6 @@ int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, c
 int ensure_tgeom(ash3d_tr_ctx *x) {
   int rc = 0;
   const Dims &d = x->d;
-  if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
   if (x->tgeom_valid) return 0;
   if (!x->tkap && ((rc = dev_alloc(&x->tkap, d.ncell)) || (rc = dev_alloc(&x->tsx, d.ncell)) ||
                    (rc = dev_alloc(&x->tkx, x->n_k()))))
@@ -300,11 +299,31 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     x->ts1_state = TS1_ZERO;
     return 0;
   }
+  if (dir == 0 && use_cn) {
+    // x lines: geometry, K, factors and scratch in the x<->y transposed
+    // layout, the state read and written in place through LDS row tiles
+    const Dims &d = x->d;
+    if ((rc = ensure_tgeom(x)) || (rc = ensure_fac(x, x->nb_diff[0]))) return rc;
+    const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
+    a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
+    a.g = make_line_geom(a.d, 1, tb);
+    a.kap = x->tkap;
+    a.sig = x->tsx;
+    a.kk = x->tkx;
+    a.fac = x->fac;
+    {
+      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);
+      HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0]));
+    }
+    x->ts1_state = TS1_MIRROR;
+    return 0;
+  }
   const bool transposed = (dir == 0);
   if (transposed) {
-    // x lines as the y lines of the transposed grid (x and y swapped)
+    // explicit x lines as the y lines of the transposed grid (x and y swapped)
     const Dims &d = x->d;
     if ((rc = ensure_tgeom(x))) return rc;
+    if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
       HIPCHK(launch_transpose_xy(x->stream, x->ts0(), x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));
@@ -379,6 +398,7 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
   if (transposed) {
     const Dims &d = x->d;
     if ((rc = ensure_tgeom(x))) return rc;
+    if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
       HIPCHK(launch_transpose_xy(x->stream, x->lam, x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));

@@ -142,6 +142,9 @@ hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
 hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
+// x-line solves on the original state layout (geometry, K and scratch in
+// the x<->y transposed layout described by a)
+hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb);
 int pick_bins_per_thread(int ns, int kind, int dir);
 long long mass_partials_size(const Dims &d);
 hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes);

@@ -348,6 +348,320 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
   }
 }
 
+// The x-line solve without transposing the state.  Lines run along x,
+// which is contiguous in memory, so a lane per line would read a different
+// row per lane.  One workgroup of 64 lanes owns 64 lines (consecutive j at
+// one k); the state is staged through LDS in tiles of TX cells of all 64
+// rows (each tile row is a coalesced 8*TX-byte segment), in both
+// directions: q tiles are loaded ahead of the elimination, solution tiles
+// are stored behind the back substitution.  Geometry, K, the U factors and
+// the eliminated right-hand sides use the x<->y transposed layout (cached
+// transposed kappa, sigma_nx, kx; scratch), where consecutive lanes are
+// consecutive addresses.  The arithmetic is k_diff_cn's, row for row.
+struct XState {
+  Dims d;     // original dims
+  double *c;  // concen ts0, original layout
+};
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
+  constexpr int PF = 4;                    // geometry rows in flight
+  constexpr int TX = (NB >= 8) ? 8 : 16;   // cells per LDS tile
+  __shared__ double s_t[NB][64][TX + 1];
+  __shared__ long long s_row[64];          // original offset of cell -1 of each lane's line (-1: none)
+  __shared__ unsigned char s_on[64];       // line_on of each lane
+  const Dims d = A.d;  // transposed dims
+  const Dims o = X.d;
+  const LineGeom g = A.g;
+  const int G = (d.ns + NB - 1) / NB;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, G);
+  const long long lines = (long long)g.Na * g.Nb;
+  if (tg.tile * 64 >= lines) return;  // padding workgroup (uniform)
+  const int t = threadIdx.x;
+  const long long gid = tg.tile * 64 + t;
+  const bool lane_ok = gid < lines;
+  const int a = lane_ok ? (int)(gid % g.Na) + 1 : 1;  // j
+  const int b = lane_ok ? (int)(gid / g.Na) + 1 : 1;  // k
+  const int n0 = tg.group * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int n = rmax - rmin + 1;
+  const long long sL = line_stride(d, 1), base = line_base(d, 1, a, b);
+  const long long kS = kline_stride(d, 1), kB = kline_base(d, 1, a, b);
+  const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const long long obase = o.p3(-1, a, b);
+  double *c = X.c;
+  double *sb = A.scratch_b;
+  double *fD = A.fac;
+  double *fDU = fD + d.ncell;
+  double *fDL = fDU + d.ncell;
+  bool exists[NB], bin_on[NB], on[NB];
+  bool any_on = false, any_bin = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    exists[u] = (n0 + u) < d.ns;
+    bin_on[u] = exists[u] && A.aloft.on(n0 + u);
+    on[u] = bin_on[u] && line_on;
+    any_on = any_on || on[u];
+    any_bin = any_bin || bin_on[u];
+  }
+  s_row[t] = lane_ok ? obase : -1;
+  s_on[t] = line_on ? 1 : 0;
+  __syncthreads();
+  // the block works if one of its lines is active for one of its bins
+  const bool block_on = __syncthreads_or(any_on ? 1 : 0) != 0;
+
+  // q of cells t0 .. t0+TX-1 (active cells only) of all 64 lines
+  auto load_tile = [&](int t0) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!bin_on[u]) continue;  // uniform
+#pragma unroll 4
+      for (int m = 0; m < TX; ++m) {
+        const int f = m * 64 + t, row = f / TX, col = f % TX, cell = t0 + col;
+        const long long rb = s_row[row];
+        s_t[u][row][col] = (rb >= 0 && cell >= rmin && cell <= rmax)
+                               ? c[rb + (cell + 1) + o.ncell * (long long)(n0 + u)]
+                               : 0.0;
+      }
+    }
+  };
+  // store the solution of cells lo .. lo+TX-1 (active cells of active lines)
+  auto store_tile = [&](int lo) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!bin_on[u]) continue;  // uniform
+#pragma unroll 4
+      for (int m = 0; m < TX; ++m) {
+        const int f = m * 64 + t, row = f / TX, col = f % TX, cell = lo + col;
+        const long long rb = s_row[row];
+        if (rb >= 0 && s_on[row] && cell >= rmin && cell <= rmax)
+          c[rb + (cell + 1) + o.ncell * (long long)(n0 + u)] = s_t[u][row][col];
+      }
+    }
+  };
+
+  if (block_on && n > 1) {
+    const double dt = A.dt, th = A.theta;
+    auto kapat = [&](int r) { return A.kap[base + sL * (long long)(r + 1)]; };
+    auto Kat = [&](int r) { return A.kk[kB + kS * (long long)r]; };
+    auto sigat = [&](int r) { return A.sig[base + sL * (long long)(r + 1)]; };
+    auto kds_at = [&](double kapl, double kapr, double Kl, double Kr, double sg) {
+      const double vavg = 0.5 * (kapl + kapr);
+      const double kavg = 0.5 * (Kl + Kr);
+      const double ds = sg / vavg;
+      return kavg * ds;
+    };
+    load_tile(rmin);
+    __syncthreads();
+    int tlo = rmin;  // first cell of the tile in LDS
+    double kap_c = kapat(rmin), K_c = Kat(rmin);
+    double kap_l = kapat(rmin - 1), K_l = Kat(rmin - 1);
+    double sig_c = sigat(rmin);
+    double kds_c = kds_at(kap_l, kap_c, K_l, K_c, sig_c);
+    double q_l[NB], q_c[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      q_c[u] = s_t[u][t][0];
+      q_l[u] = q_c[u];  // BC_left_t0 = q(rmin) (:822)
+    }
+    double Dc = 0.0, DUc = 0.0, Bc[NB];
+    int singular = 0;
+    double rk[PF], rK[PF], rs[PF];
+    auto fetch_right = [&](int slot, int l) {
+      const int cell = rmin + l;
+      rk[slot] = kapat(cell);
+      rK[slot] = Kat(cell);
+      rs[slot] = sigat(cell);
+    };
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+      if (1 + p <= n) fetch_right(p, 1 + p);
+    for (int l0 = 1; l0 <= n; l0 += PF) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const int l = l0 + p;
+        if (l > n) break;
+        const double kap_r = rk[p], K_r = rK[p], sig_r = rs[p];
+        if (l + PF <= n) fetch_right(p, l + PF);
+        double q_next[NB];
+        if (l < n) {
+          if (rmin + l >= tlo + TX) {  // next tile (uniform: l and n are)
+            __syncthreads();
+            tlo += TX;
+            load_tile(tlo);
+            __syncthreads();
+          }
+#pragma unroll
+          for (int u = 0; u < NB; ++u) q_next[u] = s_t[u][t][rmin + l - tlo];
+        } else {
+#pragma unroll
+          for (int u = 0; u < NB; ++u) q_next[u] = 0.0;
+        }
+        const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
+        const double LeftFac = dt * kds_c * sig_c / kap_c;
+        const double RightFac = dt * kds_r * sig_r / kap_c;
+        const double CenterFac = LeftFac + RightFac;
+        double Dn, DUn = 0.0, DLn = 0.0;
+        if (l == 1) {
+          Dn = 1.0 + (th)*CenterFac - (th)*LeftFac;
+          DUn = -((th)*RightFac);
+        } else if (l < n) {
+          DLn = -((th)*LeftFac);
+          Dn = 1.0 + (th)*CenterFac;
+          DUn = -((th)*RightFac);
+        } else {
+          DLn = -((th)*LeftFac);
+          Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
+        }
+        double Bn[NB], q_r[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          q_r[u] = (l < n) ? q_next[u] : q_c[u];  // BC_right_t0 (:845)
+          Bn[u] = (1.0 - th) * LeftFac * q_l[u] + (1.0 - (1.0 - th) * CenterFac) * q_c[u] +
+                  (1.0 - th) * RightFac * q_r[u];
+        }
+        if (l == 1) {
+          Dc = Dn;
+          DUc = DUn;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
+        } else {
+          const int i = l - 1;
+          const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+          double DLo;
+          if (fabs(Dc) >= fabs(DLn)) {
+            if (Dc == 0.0) { if (!singular) singular = i; }
+            const double fact = DLn / Dc;
+            Dn = Dn - fact * DUc;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) Bn[u] = Bn[u] - fact * Bc[u];
+            DLo = (i <= n - 2) ? 0.0 : DLn;
+          } else {
+            const double fact = Dc / DLn;
+            const double newDi = DLn;
+            const double temp = Dn;
+            Dn = DUc - fact * temp;
+            if (i <= n - 2) {
+              DLo = DUn;
+              DUn = -fact * DLo;
+            } else {
+              DLo = DLn;
+            }
+            DUc = temp;
+            Dc = newDi;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+              const double tb = Bc[u];
+              Bc[u] = Bn[u];
+              Bn[u] = tb - fact * Bn[u];
+            }
+          }
+          if (any_on) {
+            fD[fo] = Dc;
+            fDU[fo] = DUc;
+            fDL[fo] = DLo;
+          }
+#pragma unroll
+          for (int u = 0; u < NB; ++u)
+            if (on[u]) sb[fo + d.ncell * (long long)(n0 + u)] = Bc[u];
+          Dc = Dn;
+          DUc = DUn;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
+        }
+        kap_l = kap_c; kap_c = kap_r;
+        K_l = K_c; K_c = K_r;
+        sig_c = sig_r;
+        kds_c = kds_r;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          q_l[u] = q_c[u];
+          q_c[u] = q_r[u];
+        }
+      }
+    }
+    if (Dc == 0.0 && !singular) singular = n;
+    if (singular && any_on) atomicCAS(A.flag, 0, singular);
+    // ---- back substitution; solutions collect in an LDS tile covering
+    // cells [thi-TX+1, thi], stored when the sweep leaves it
+    __syncthreads();  // the q tile is no longer read
+    int thi = rmax;
+    double x1[NB], x2[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      x1[u] = Bc[u] / Dc;
+      x2[u] = 0.0;
+      s_t[u][t][TX - 1] = x1[u];  // cell rmax at column TX-1 of tile [rmax-TX+1, rmax]
+    }
+    double bD[PF], bDU[PF], bDL[PF], bB[PF][NB];
+    auto fetch_row = [&](int slot, int i) {
+      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      bD[slot] = fD[fo];
+      bDU[slot] = fDU[fo];
+      bDL[slot] = fDL[fo];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) bB[slot][u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+    };
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+      if (n - 1 - p >= 1) fetch_row(p, n - 1 - p);
+    for (int i0 = n - 1; i0 >= 1; i0 -= PF) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const int i = i0 - p;
+        if (i < 1) break;
+        const int cell = rmin - 1 + i;
+        if (cell < thi - TX + 1) {  // left the tile (uniform)
+          __syncthreads();
+          store_tile(thi - TX + 1);
+          __syncthreads();
+          thi -= TX;
+        }
+        const double Di = bD[p], DUi = bDU[p], DLi = bDL[p];
+        double Bcur[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Bcur[u] = bB[p][u];
+        if (i - PF >= 1) fetch_row(p, i - PF);
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const double Bi = Bcur[u];
+          double xi;
+          if (i == n - 1) xi = (Bi - DUi * x1[u]) / Di;
+          else xi = (Bi - DUi * x1[u] - DLi * x2[u]) / Di;
+          x2[u] = x1[u];
+          x1[u] = xi;
+          s_t[u][t][cell - (thi - TX + 1)] = xi;
+        }
+      }
+    }
+    __syncthreads();
+    store_tile(thi - TX + 1);
+  }
+  // cells the reference leaves at zero (inactive bins and lines, cells
+  // outside [rmin, rmax]); a one-cell line copies nothing (:770, :909)
+  if (n > 1 && lane_ok) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!exists[u]) continue;
+      const long long bofs = o.ncell * (long long)(n0 + u);
+      if (!on[u]) {
+        for (int r = 1; r <= N; ++r) c[obase + (r + 1) + bofs] = 0.0;
+      } else {
+        for (int r = 1; r < rmin; ++r) c[obase + (r + 1) + bofs] = 0.0;
+        for (int r = rmax + 1; r <= N; ++r) c[obase + (r + 1) + bofs] = 0.0;
+      }
+    }
+  }
+}
+
+template <int NB>
+hipError_t launch_cn_xrows_nb(hipStream_t s, const DiffArgs &a, const XState &x) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)(padded_tiles((lines + 63) / 64) * ((a.d.ns + NB - 1) / NB)));
+  hipLaunchKernelGGL((k_diff_cn_xrows<NB>), grid, dim3(64), 0, s, a, x);
+  return hipGetLastError();
+}
+
 template <template <int, int> class K, int NB>
 struct dummy {};
 
@@ -383,6 +697,16 @@ hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb) {
     case 2: return launch_explicit_nb<2>(s, a);
     case 4: return launch_explicit_nb<4>(s, a);
     default: return launch_explicit_nb<8>(s, a);
+  }
+}
+
+hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb) {
+  const XState x{orig, c0};
+  switch (nb) {
+    case 1: return launch_cn_xrows_nb<1>(s, a, x);
+    case 2: return launch_cn_xrows_nb<2>(s, a, x);
+    case 4: return launch_cn_xrows_nb<4>(s, a, x);
+    default: return launch_cn_xrows_nb<8>(s, a, x);
   }
 }
 
