@@ -187,7 +187,8 @@ enum {
   ASH3D_TUNE_NB_ADVECT_Z = 2,
   ASH3D_TUNE_NB_DIFFUSE_X = 3,
   ASH3D_TUNE_NB_DIFFUSE_Y = 4,
-  ASH3D_TUNE_NB_DIFFUSE_Z = 5
+  ASH3D_TUNE_NB_DIFFUSE_Z = 5,
+  ASH3D_TUNE_X_TILE = 6 /* cells per LDS tile of the x-line solves: 8, 16 or 32 */
 };
 int ash3d_tr_tune(ash3d_tr_ctx *ctx, int32_t param, int32_t value);
 

@@ -1,8 +1,8 @@
 """Launch autotuner for the transport step.
 
 For each sweep kind it times every bins-per-thread setting (the number of
-grain bins one GPU thread carries along a line) with the library's HIP-event
-kernel timers, on the live context, and keeps the fastest.  The tuning
+grain bins one GPU thread carries along a line), and for the x-line solves
+the LDS tile width, with the library's HIP-event kernel timers, on the live context, and keeps the fastest.  The tuning
 steps advance the context's state like ordinary steps; results of a step do
 not depend on the setting (the kernels are bitwise identical across it).
 Choices can be cached in a JSON file keyed by grid shape and bin count.
@@ -13,7 +13,8 @@ import json
 import os
 
 PARAMS = (("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("nb_diffuse_x", "diffuse_x"),
-          ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
+          ("x_tile", "diffuse_x"), ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
+X_TILES = (8, 16, 32)
 
 
 def _key(st):
@@ -34,8 +35,8 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), it
         if kind.startswith("diffuse") and not use_diffusion:
             continue
         times = {}
-        for nb in candidates:
-            if nb > 1 and nb // 2 >= st.ns:
+        for nb in (X_TILES if param == "x_tile" else candidates):
+            if param != "x_tile" and nb > 1 and nb // 2 >= st.ns:
                 continue
             T.tune(**{param: nb})
             T.profile(True)

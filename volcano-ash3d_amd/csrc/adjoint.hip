@@ -20,8 +20,8 @@
 // (src/Diffusion.F90:807-890); its adjoint is lambda' = B^T A^{-T} lambda:
 // a tridiagonal solve with the transposed matrix (the elimination with
 // partial pivoting of dgtsv) and the transposed right-hand-side stencil.
-// The row coefficients are recomputed from kappa, sigma and K on both
-// passes instead of being stored.  The explicit scheme
+// The row coefficients are rebuilt on both passes from the row factors L, R
+// the forward sweeps cache.  The explicit scheme
 // (src/Diffusion.F90:335-377) is a three-point stencil whose Neumann (or
 // periodic x) ghosts fold back onto the end cells.
 #include "common.hpp"
@@ -358,7 +358,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
-  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *uD = A.fac;  // shared by the bin groups (identical values)
   double *uDU = uD + d.ncell, *uDL = uD + 2 * d.ncell;
@@ -372,20 +371,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
     any_on = any_on || on[u];
   }
   if (any_on) {
-    const double dt = A.dt, th = A.theta;
-    // interface factor kds(I) (:797-801) and the row coefficients (:807-853)
-    auto kds_at = [&](int I) {
-      const double vavg = 0.5 * (A.kap[cell(I - 1)] + A.kap[cell(I)]);
-      const double kavg = 0.5 * (A.kk[kB + kS * (long long)(I - 1)] + A.kk[kB + kS * (long long)I]);
-      return kavg * (A.sig[cell(I)] / vavg);
-    };
+    const double th = A.theta;
+    // the row coefficients (:807-853) from the cached row factors L, R
     auto rowc = [&](int l) {
       RowC r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       if (l < 1 || l > n) return r;
-      const int c = rmin - 1 + l;
-      const double kap = A.kap[cell(c)];
-      const double L = dt * kds_at(c) * A.sig[cell(c)] / kap;
-      const double R = dt * kds_at(c + 1) * A.sig[cell(c + 1)] / kap;
+      const long long o = cell(rmin - 1 + l);
+      const double L = A.lr[o];
+      const double R = A.lr[o + d.ncell];
       const double C = L + R;
       r.a = (1.0 - th) * L;
       r.b = 1.0 - (1.0 - th) * C;

@@ -70,6 +70,10 @@ struct ash3d_tr_ctx {
   double *ofxy1 = nullptr, *ofxy2 = nullptr;
   double *dep = nullptr;
   double *fac = nullptr;  // line-solve U factors (3 * ncell)
+  // theta-scheme row factors L, R per direction (2 * ncell each), valid for lr_dt
+  double *lr[3] = {nullptr, nullptr, nullptr};
+  double lr_dt[3] = {0.0, 0.0, 0.0};
+  bool lr_ok[3] = {false, false, false};
   long long fac_groups = 0;
   double *flux = nullptr;  // source column staging (nz+1)*ns
   // pinned host ring for the source columns: the caller's buffer is copied
@@ -102,6 +106,7 @@ struct ash3d_tr_ctx {
   unsigned long long *prof = nullptr;    // get_minmax_index slab maxima
   int ts1_state = TS1_EXPLICIT;
   int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
+  int x_tile = 16;  // cells per LDS tile of the x-line solves
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -262,6 +267,20 @@ int ensure_tgeom(ash3d_tr_ctx *x) {
   return 0;
 }
 
+// Row factors L, R of the theta-scheme lines of one direction, recomputed
+// when dt changes (the geometry and K uploads invalidate them)
+int ensure_lr(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
+  int rc = 0;
+  if (!x->lr[dir] && (rc = dev_alloc(&x->lr[dir], 2 * x->d.ncell))) return rc;
+  if (!x->lr_ok[dir] || x->lr_dt[dir] != a.dt) {
+    HIPCHK(launch_line_lr(x->stream, a, x->lr[dir]));
+    x->lr_ok[dir] = true;
+    x->lr_dt[dir] = a.dt;
+  }
+  a.lr = x->lr[dir];
+  return 0;
+}
+
 // Line-solve U factors (one set per line, shared by the bin groups)
 int ensure_fac(ash3d_tr_ctx *x, int nb) {
   (void)nb;
@@ -311,9 +330,10 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.sig = x->tsx;
     a.kk = x->tkx;
     a.fac = x->fac;
+    if ((rc = ensure_lr(x, 0, a))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);
-      HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0]));
+      HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
     }
     x->ts1_state = TS1_MIRROR;
     return 0;
@@ -337,7 +357,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.kk = x->tkx;
   }
   if (use_cn) {
-    if ((rc = ensure_fac(x, x->nb_diff[dir]))) return rc;
+    if ((rc = ensure_fac(x, x->nb_diff[dir])) || (rc = ensure_lr(x, dir, a))) return rc;
     a.fac = x->fac;
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
@@ -414,7 +434,7 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
   {
     ProfScope ps(x, ASH3D_PROF_ADJ_DIFFUSE);
     if (use_cn) {
-      if ((rc = ensure_fac(x, x->nb_diff[dir]))) return rc;
+      if ((rc = ensure_fac(x, x->nb_diff[dir])) || (rc = ensure_lr(x, dir, a))) return rc;
       a.fac = x->fac;
       a.scratch_b = x->adj_sb;
       HIPCHK(launch_adj_cn(x->stream, a, lam, x->nb_diff[dir]));
@@ -522,7 +542,7 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
                     x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx, x->lam, x->snap_b,
-                    x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col};
+                    x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col, x->lr[0], x->lr[1], x->lr[2]};
   for (double *p : ptrs)
     if (p) hipFree(p);
   for (double *p : x->ckpt)
@@ -564,7 +584,10 @@ int ash3d_tr_sync_from_host(ash3d_tr_ctx *x, uint32_t mask) {
   }
   if (mask & ASH3D_SYNC_VF)
     if ((rc = upload(x, x->vf, h.vf_pd, x->n_conc(), 0.0))) return rc;
-  if (mask & (ASH3D_SYNC_GEOMETRY | ASH3D_SYNC_DIFFK)) x->tgeom_valid = false;
+  if (mask & (ASH3D_SYNC_GEOMETRY | ASH3D_SYNC_DIFFK)) {
+    x->tgeom_valid = false;
+    x->lr_ok[0] = x->lr_ok[1] = x->lr_ok[2] = false;
+  }
   if (mask & ASH3D_SYNC_GEOMETRY) {
     if ((rc = upload(x, x->kap, h.kappa_pd, d.ncell, 1.0)) || (rc = upload(x, x->sx, h.sigma_nx_pd, d.ncell, 1.0)) ||
         (rc = upload(x, x->sy, h.sigma_ny_pd, d.ncell, 1.0)) || (rc = upload(x, x->sz, h.sigma_nz_pd, d.ncell, 1.0)))
@@ -793,6 +816,12 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, do
 int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
   int rc = check_ctx(x);
   if (rc) return rc;
+  if (param == ASH3D_TUNE_X_TILE) {
+    if (value != 8 && value != 16 && value != 32)
+      return fail(ASH3D_TR_EINVAL, "x-line tile must be 8, 16 or 32 cells (got %d)", value);
+    x->x_tile = value;
+    return 0;
+  }
   if (value != 1 && value != 2 && value != 4 && value != 8)
     return fail(ASH3D_TR_EINVAL, "bins per thread must be 1, 2, 4 or 8 (got %d)", value);
   if (param >= ASH3D_TUNE_NB_ADVECT_X && param <= ASH3D_TUNE_NB_ADVECT_Z) {

@@ -122,6 +122,9 @@ struct DiffArgs {
   const double *kk;    // kx / ky / kz
   const double *sig;
   const double *kap;
+  // theta-scheme row factors of each cell c (src/Diffusion.F90:807-816):
+  // lr[c] = L = dt*kds_c*sig_c/kap_c, lr[c + ncell] = R = dt*kds_{c+1}*sig_{c+1}/kap_c
+  const double *lr;
   double dt, theta;
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
   AloftMask aloft;
@@ -142,9 +145,11 @@ hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
 hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
+// the row factors L, R of every cell of the sweep's lines (bin-independent)
+hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
-hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb);
+hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);
 int pick_bins_per_thread(int ns, int kind, int dir);
 long long mass_partials_size(const Dims &d);
 hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes);

@@ -19,6 +19,40 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Row factors of the theta-scheme lines, one thread per cell (lanes along
+// the contiguous transverse index).  For the row of cell c (:807-816):
+//   kds_I = (0.5*(K_{I-1}+K_I)) * (sig_I / (0.5*(kap_{I-1}+kap_I)))
+//   L = dt*kds_c*sig_c/kap_c,  R = dt*kds_{c+1}*sig_{c+1}/kap_c
+// They depend on dt, the geometry and K only, so the context keeps them
+// between steps; the line solves then read two numbers per row instead of
+// evaluating three divisions in every bin group.
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)g.Na * g.N * g.Nb) return;
+  const int a = (int)(t % g.Na) + 1;
+  const long long rest = t / g.Na;
+  const int r = (int)(rest % g.N) + 1;
+  const int b = (int)(rest / g.N) + 1;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
+  auto kapat = [&](int q) { return A.kap[base + sL * (long long)(q + 1)]; };
+  auto Kat = [&](int q) { return A.kk[kB + kS * (long long)q]; };
+  auto sigat = [&](int q) { return A.sig[base + sL * (long long)(q + 1)]; };
+  auto kds_at = [&](int I) {
+    const double vavg = 0.5 * (kapat(I - 1) + kapat(I));
+    const double kavg = 0.5 * (Kat(I - 1) + Kat(I));
+    const double ds = sigat(I) / vavg;
+    return kavg * ds;
+  };
+  const double dt = A.dt, kap_c = kapat(r);
+  const long long o = base + sL * (long long)(r + 1);
+  lr[o] = dt * kds_at(r) * sigat(r) / kap_c;
+  lr[o + d.ncell] = dt * kds_at(r + 1) * sigat(r + 1) / kap_c;
+}
+
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
   const Dims d = A.d;
@@ -144,24 +178,9 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     any_on = any_on || on[u];
   }
   if (any_on) {
-    const double dt = A.dt, th = A.theta;
-    // ---- coefficients of row 1 need interfaces rmin and rmin+1
-    // streaming state: kap_{c-1}, kap_c; K_{c-1}, K_c; sig_c; q_{c-1}, q_c
-    auto kapat = [&](int r) { return A.kap[base + sL * (long long)(r + 1)]; };
-    auto Kat = [&](int r) { return A.kk[kB + kS * (long long)r]; };
-    auto sigat = [&](int r) { return A.sig[base + sL * (long long)(r + 1)]; };
+    const double th = A.theta;
     auto qat = [&](int u, int r) { return c[base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u)]; };
-    // interface I: kds_I = kavg*(sig/vavg) (:794-801)
-    auto kds_at = [&](double kapl, double kapr, double Kl, double Kr, double sg) {
-      const double vavg = 0.5 * (kapl + kapr);
-      const double kavg = 0.5 * (Kl + Kr);
-      const double ds = sg / vavg;
-      return kavg * ds;
-    };
-    double kap_c = kapat(rmin), K_c = Kat(rmin);
-    double kap_l = kapat(rmin - 1), K_l = Kat(rmin - 1);
-    double sig_c = sigat(rmin);
-    double kds_c = kds_at(kap_l, kap_c, K_l, K_c, sig_c);
+    const double *Lf = A.lr, *Rf = A.lr + d.ncell;
     double q_l[NB], q_c[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
@@ -172,37 +191,31 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     // original sub-diagonal entry that row i+1 carries
     double Dc = 0.0, DUc = 0.0, Bc[NB];
     int singular = 0;
-    // Row l needs the data of its right neighbour, cell rmin+l (kappa, K,
-    // sigma; q while that cell is active).  A ring of PF rows is kept in
-    // flight: slot p holds row l0+p of the current block of PF rows and is
-    // refilled with row l+PF when row l consumes it, so PF loads per stream
-    // are outstanding while the recurrence runs.
-    double rk[PF], rK[PF], rs[PF], rq[PF][NB];
-    auto fetch_right = [&](int slot, int l) {
-      const int cell = rmin + l;
-      rk[slot] = kapat(cell);
-      rK[slot] = Kat(cell);
-      rs[slot] = sigat(cell);
+    // Row l needs its factors L, R (cell c = rmin-1+l) and the q of its
+    // right neighbour (cell c+1, while it is active).  A ring of PF rows is
+    // kept in flight: slot p holds row l0+p of the current block of PF rows
+    // and is refilled with row l+PF when row l consumes it.
+    double rL[PF], rR[PF], rq[PF][NB];
+    auto fetch_row_in = [&](int slot, int l) {
+      const long long o = base + sL * (long long)(rmin + l);  // cell rmin-1+l
+      rL[slot] = Lf[o];
+      rR[slot] = Rf[o];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) rq[slot][u] = (l < n && exists[u]) ? qat(u, cell) : 0.0;
+      for (int u = 0; u < NB; ++u) rq[slot][u] = (l < n && exists[u]) ? qat(u, rmin + l) : 0.0;
     };
 #pragma unroll
     for (int p = 0; p < PF; ++p)
-      if (1 + p <= n) fetch_right(p, 1 + p);
+      if (1 + p <= n) fetch_row_in(p, 1 + p);
     for (int l0 = 1; l0 <= n; l0 += PF) {
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
       const int l = l0 + p;
       if (l > n) break;
-      // cell cc+1 (right neighbour) data
-      const double kap_r = rk[p], K_r = rK[p], sig_r = rs[p];
+      const double LeftFac = rL[p], RightFac = rR[p];
       double q_next[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) q_next[u] = rq[p][u];
-      if (l + PF <= n) fetch_right(p, l + PF);
-      const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
-      const double LeftFac = dt * kds_c * sig_c / kap_c;
-      const double RightFac = dt * kds_r * sig_r / kap_c;
+      if (l + PF <= n) fetch_row_in(p, l + PF);
       const double CenterFac = LeftFac + RightFac;
       double Dn, DUn = 0.0, DLn = 0.0;
       if (l == 1) {
@@ -273,10 +286,6 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
       }
       // advance the window
-      kap_l = kap_c; kap_c = kap_r;
-      K_l = K_c; K_c = K_r;
-      sig_c = sig_r;
-      kds_c = kds_r;
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         q_l[u] = q_c[u];
@@ -363,10 +372,9 @@ struct XState {
   double *c;  // concen ts0, original layout
 };
 
-template <int NB>
+template <int NB, int TX>
 __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
-  constexpr int PF = 4;                    // geometry rows in flight
-  constexpr int TX = (NB >= 8) ? 8 : 16;   // cells per LDS tile
+  constexpr int PF = 4;  // row factors in flight; TX cells per LDS tile
   __shared__ double s_t[NB][64][TX + 1];
   __shared__ long long s_row[64];          // original offset of cell -1 of each lane's line (-1: none)
   __shared__ unsigned char s_on[64];       // line_on of each lane
@@ -441,23 +449,11 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
   };
 
   if (block_on && n > 1) {
-    const double dt = A.dt, th = A.theta;
-    auto kapat = [&](int r) { return A.kap[base + sL * (long long)(r + 1)]; };
-    auto Kat = [&](int r) { return A.kk[kB + kS * (long long)r]; };
-    auto sigat = [&](int r) { return A.sig[base + sL * (long long)(r + 1)]; };
-    auto kds_at = [&](double kapl, double kapr, double Kl, double Kr, double sg) {
-      const double vavg = 0.5 * (kapl + kapr);
-      const double kavg = 0.5 * (Kl + Kr);
-      const double ds = sg / vavg;
-      return kavg * ds;
-    };
+    const double th = A.theta;
+    const double *Lf = A.lr, *Rf = A.lr + d.ncell;
     load_tile(rmin);
     __syncthreads();
     int tlo = rmin;  // first cell of the tile in LDS
-    double kap_c = kapat(rmin), K_c = Kat(rmin);
-    double kap_l = kapat(rmin - 1), K_l = Kat(rmin - 1);
-    double sig_c = sigat(rmin);
-    double kds_c = kds_at(kap_l, kap_c, K_l, K_c, sig_c);
     double q_l[NB], q_c[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
@@ -466,23 +462,22 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
     }
     double Dc = 0.0, DUc = 0.0, Bc[NB];
     int singular = 0;
-    double rk[PF], rK[PF], rs[PF];
-    auto fetch_right = [&](int slot, int l) {
-      const int cell = rmin + l;
-      rk[slot] = kapat(cell);
-      rK[slot] = Kat(cell);
-      rs[slot] = sigat(cell);
+    double rL[PF], rR[PF];
+    auto fetch_lr = [&](int slot, int l) {
+      const long long o = base + sL * (long long)(rmin + l);  // cell rmin-1+l
+      rL[slot] = Lf[o];
+      rR[slot] = Rf[o];
     };
 #pragma unroll
     for (int p = 0; p < PF; ++p)
-      if (1 + p <= n) fetch_right(p, 1 + p);
+      if (1 + p <= n) fetch_lr(p, 1 + p);
     for (int l0 = 1; l0 <= n; l0 += PF) {
 #pragma unroll
       for (int p = 0; p < PF; ++p) {
         const int l = l0 + p;
         if (l > n) break;
-        const double kap_r = rk[p], K_r = rK[p], sig_r = rs[p];
-        if (l + PF <= n) fetch_right(p, l + PF);
+        const double LeftFac = rL[p], RightFac = rR[p];
+        if (l + PF <= n) fetch_lr(p, l + PF);
         double q_next[NB];
         if (l < n) {
           if (rmin + l >= tlo + TX) {  // next tile (uniform: l and n are)
@@ -497,9 +492,6 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
 #pragma unroll
           for (int u = 0; u < NB; ++u) q_next[u] = 0.0;
         }
-        const double kds_r = kds_at(kap_c, kap_r, K_c, K_r, sig_r);
-        const double LeftFac = dt * kds_c * sig_c / kap_c;
-        const double RightFac = dt * kds_r * sig_r / kap_c;
         const double CenterFac = LeftFac + RightFac;
         double Dn, DUn = 0.0, DLn = 0.0;
         if (l == 1) {
@@ -569,10 +561,6 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
 #pragma unroll
           for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
         }
-        kap_l = kap_c; kap_c = kap_r;
-        K_l = K_c; K_c = K_r;
-        sig_c = sig_r;
-        kds_c = kds_r;
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
           q_l[u] = q_c[u];
@@ -654,12 +642,21 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
   }
 }
 
-template <int NB>
+template <int NB, int TX>
 hipError_t launch_cn_xrows_nb(hipStream_t s, const DiffArgs &a, const XState &x) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)(padded_tiles((lines + 63) / 64) * ((a.d.ns + NB - 1) / NB)));
-  hipLaunchKernelGGL((k_diff_cn_xrows<NB>), grid, dim3(64), 0, s, a, x);
+  hipLaunchKernelGGL((k_diff_cn_xrows<NB, TX>), grid, dim3(64), 0, s, a, x);
   return hipGetLastError();
+}
+
+template <int NB>
+hipError_t launch_cn_xrows_tx(hipStream_t s, const DiffArgs &a, const XState &x, int tx) {
+  // LDS per workgroup NB*64*(TX+1) doubles: at most 34 KiB for NB <= 2 and
+  // TX = 32, 35 KiB for NB = 4 and TX = 16, 37 KiB for NB = 8 and TX = 8
+  if (tx <= 8 || NB >= 8) return launch_cn_xrows_nb<NB, 8>(s, a, x);
+  if (tx <= 16 || NB >= 4) return launch_cn_xrows_nb<NB, 16>(s, a, x);
+  return launch_cn_xrows_nb<NB, 32>(s, a, x);
 }
 
 template <template <int, int> class K, int NB>
@@ -700,13 +697,24 @@ hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb) {
   }
 }
 
-hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb) {
+hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
+  const long long cells = (long long)a.g.Na * a.g.N * a.g.Nb;
+  dim3 grid((unsigned)((cells + kBlock - 1) / kBlock));
+  switch (a.g.dir) {
+    case 0: hipLaunchKernelGGL((k_line_lr<0>), grid, dim3(kBlock), 0, s, a, lr); break;
+    case 1: hipLaunchKernelGGL((k_line_lr<1>), grid, dim3(kBlock), 0, s, a, lr); break;
+    default: hipLaunchKernelGGL((k_line_lr<2>), grid, dim3(kBlock), 0, s, a, lr); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx) {
   const XState x{orig, c0};
   switch (nb) {
-    case 1: return launch_cn_xrows_nb<1>(s, a, x);
-    case 2: return launch_cn_xrows_nb<2>(s, a, x);
-    case 4: return launch_cn_xrows_nb<4>(s, a, x);
-    default: return launch_cn_xrows_nb<8>(s, a, x);
+    case 1: return launch_cn_xrows_tx<1>(s, a, x, tx);
+    case 2: return launch_cn_xrows_tx<2>(s, a, x, tx);
+    case 4: return launch_cn_xrows_tx<4>(s, a, x, tx);
+    default: return launch_cn_xrows_tx<8>(s, a, x, tx);
   }
 }
 

@@ -420,7 +420,8 @@ class Transport:
         return tuple(int(v) for v in b)
 
     # -- launch tuning ----------------------------------------------------
-    TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z")
+    TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z",
+                   "x_tile")
 
     def tune(self, **kw):
         """Set bins-per-thread launch parameters, e.g. tune(nb_advect_z=2)."""
