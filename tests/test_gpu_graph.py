@@ -53,6 +53,49 @@ def test_graph_replays_match_eager(pkg, tr, case):
     G.close()
 
 
+def test_graph_replay_after_factor_changes(pkg, tr):
+    """The captured line solves read cached row and LU factors: an eager
+    step at another dt (which refactors the lines) and a diffusivity upload
+    between replays must not leak into the replays."""
+    st = pkg.workloads.random_state(21, 17, 10, 4, seed=33, latlon=True, periodic=False, use_cn=True)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    a, b = st.copy(), st.copy()
+    E, G = tr.Transport(a), tr.Transport(b)
+    dt0 = st.dt
+    for T in (E, G):
+        T.step_glue()
+        T.transport_step(1, True)
+    G.graph_capture(2, True)
+    G.graph_capture(3, True)
+
+    def check(itime):
+        E.sync_to_host()
+        G.sync_to_host()
+        assert np.array_equal(a.concen_pd, b.concen_pd), itime
+
+    for X, T in ((a, E), (b, G)):  # eager step at half the dt
+        X.dt = 0.5 * dt0
+        T.step_glue()
+        T.transport_step(2, True)
+        X.dt = dt0
+    check(2)
+    E.step_glue()
+    E.transport_step(3, True)
+    G.graph_launch(3)
+    check(3)
+    for X, T in ((a, E), (b, G)):  # new diffusivities
+        X.kx *= 1.5
+        X.kz *= 0.5
+        T.sync_from_host(tr.SYNC_DIFFK)
+    for itime in (4, 5):
+        E.step_glue()
+        E.transport_step(itime, True)
+        G.graph_launch(itime)
+        check(itime)
+    E.close()
+    G.close()
+
+
 def test_graph_capture_needs_an_eager_step(pkg, tr):
     st = pkg.workloads.random_state(9, 8, 6, 2, seed=2)
     T = tr.Transport(st.copy())

@@ -74,6 +74,24 @@ struct ash3d_tr_ctx {
   double *lr[3] = {nullptr, nullptr, nullptr};
   double lr_dt[3] = {0.0, 0.0, 0.0};
   bool lr_ok[3] = {false, false, false};
+  // LU factors of the theta-scheme lines per direction (one allocation:
+  // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags and
+  // one INFO per line), valid for the key (dt, theta, rmin, rmax)
+  double *lu[3] = {nullptr, nullptr, nullptr};
+  uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
+  int *lu_sing[3] = {nullptr, nullptr, nullptr};
+  double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
+  int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
+  bool lu_ok[3] = {false, false, false};
+  double *junk = nullptr;  // kJunk doubles that masked-off lanes store to
+  // the sweep arguments the factors of each direction were last requested
+  // with, and those of the sweeps in each captured graph (graph_launch
+  // brings the caches back to them before a replay)
+  DiffArgs lu_args[3] = {};
+  bool lu_args_set[3] = {false, false, false};
+  DiffArgs graph_lu[2][3] = {};
+  bool graph_lu_on[2][3] = {};
+  bool graph_tgeom[2] = {false, false};  // the graph reads the transposed geometry
   long long fac_groups = 0;
   double *flux = nullptr;  // source column staging (nz+1)*ns
   // pinned host ring for the source columns: the caller's buffer is copied
@@ -276,8 +294,38 @@ int ensure_lr(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
     HIPCHK(launch_line_lr(x->stream, a, x->lr[dir]));
     x->lr_ok[dir] = true;
     x->lr_dt[dir] = a.dt;
+    x->lu_ok[dir] = false;
   }
   a.lr = x->lr[dir];
+  return 0;
+}
+
+// LU factors of the theta-scheme lines of one direction (after ensure_lr),
+// refactored when dt, theta or the active range along the lines changes
+int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
+  int rc = 0;
+  const long long nc = x->d.ncell, lines = (long long)a.g.Na * a.g.Nb;
+  if (!x->lu[dir]) {
+    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2))) return rc;
+    x->lu_piv[dir] = (uint8_t *)(x->lu[dir] + 4 * nc);
+    x->lu_sing[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8);
+  }
+  if (!x->junk && (rc = dev_alloc(&x->junk, kJunk))) return rc;
+  if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
+      x->lu_rmax[dir] != a.g.rmax) {
+    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir]));
+    x->lu_ok[dir] = true;
+    x->lu_dt[dir] = a.dt;
+    x->lu_th[dir] = a.theta;
+    x->lu_rmin[dir] = a.g.rmin;
+    x->lu_rmax[dir] = a.g.rmax;
+  }
+  a.lu = x->lu[dir];
+  a.piv = x->lu_piv[dir];
+  a.sing = x->lu_sing[dir];
+  a.junk = x->junk;
+  x->lu_args[dir] = a;
+  x->lu_args_set[dir] = true;
   return 0;
 }
 
@@ -322,15 +370,14 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     // x lines: geometry, K, factors and scratch in the x<->y transposed
     // layout, the state read and written in place through LDS row tiles
     const Dims &d = x->d;
-    if ((rc = ensure_tgeom(x)) || (rc = ensure_fac(x, x->nb_diff[0]))) return rc;
+    if ((rc = ensure_tgeom(x))) return rc;
     const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
     a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
     a.g = make_line_geom(a.d, 1, tb);
     a.kap = x->tkap;
     a.sig = x->tsx;
     a.kk = x->tkx;
-    a.fac = x->fac;
-    if ((rc = ensure_lr(x, 0, a))) return rc;
+    if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);
       HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
@@ -357,8 +404,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.kk = x->tkx;
   }
   if (use_cn) {
-    if ((rc = ensure_fac(x, x->nb_diff[dir])) || (rc = ensure_lr(x, dir, a))) return rc;
-    a.fac = x->fac;
+    if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
   } else {
@@ -542,7 +588,8 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   double *ptrs[] = {x->c, x->vx, x->vy, x->vz, x->vh, x->vf, x->kap, x->sx, x->sy, x->sz, x->kx, x->ky, x->kz,
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
                     x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx, x->lam, x->snap_b,
-                    x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col, x->lr[0], x->lr[1], x->lr[2]};
+                    x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col, x->lr[0], x->lr[1], x->lr[2],
+                    x->lu[0], x->lu[1], x->lu[2], x->junk};
   for (double *p : ptrs)
     if (p) hipFree(p);
   for (double *p : x->ckpt)
@@ -1083,8 +1130,8 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t us
   if (x->ts1_state != TS1_MIRROR)
     return fail(ASH3D_TR_EINVAL, "capture after an eager step (the glue must be the ghost-zeroing kernel)");
   if (x->prof_on) return fail(ASH3D_TR_EINVAL, "kernel timing is on; switch it off before capturing");
-  if (use_diffusion && use_cn && !x->fac)
-    return fail(ASH3D_TR_EINVAL, "capture after an eager step (line-solve scratch not allocated)");
+  if (use_diffusion && use_cn && !(x->lu[0] && x->lu[1] && x->lu[2] && x->junk))
+    return fail(ASH3D_TR_EINVAL, "capture after an eager step (line-solve factors not allocated)");
   if (use_diffusion && !x->tgeom_valid)
     return fail(ASH3D_TR_EINVAL, "capture after an eager step (transposed geometry not built)");
   HIPCHK(hipStreamSynchronize(x->stream));
@@ -1093,6 +1140,7 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t us
     HIPCHK(hipGraphExecDestroy(x->graph[slot]));
     x->graph[slot] = nullptr;
   }
+  for (int k = 0; k < 3; ++k) x->lu_args_set[k] = false;
   HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
   int rc2 = ash3d_tr_step_glue(x);
   if (!rc2) rc2 = ash3d_tr_step(x, itime, dt, use_diffusion, use_cn, imp_fac, is_aloft);
@@ -1106,6 +1154,11 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t us
   e = hipGraphInstantiate(&x->graph[slot], g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "graph instantiation: %s", hipGetErrorString(e));
+  for (int k = 0; k < 3; ++k) {
+    x->graph_lu_on[slot][k] = x->lu_args_set[k];
+    x->graph_lu[slot][k] = x->lu_args[k];
+  }
+  x->graph_tgeom[slot] = use_diffusion != 0;
   return 0;
 }
 
@@ -1115,6 +1168,15 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *x, int32_t itime) {
   const int slot = itime % 2;
   if (!x->graph[slot]) return fail(ASH3D_TR_EINVAL, "no graph captured for step parity %d", slot);
   if (x->ts1_state != TS1_MIRROR) return fail(ASH3D_TR_EINVAL, "the state changed since the capture (sync or load)");
+  // the graph's sweeps read cached geometry and line factors: rebuild
+  // (eagerly, ahead of the graph on the stream) what a geometry/K upload
+  // or an eager step with another dt or range replaced since the capture
+  if (x->graph_tgeom[slot] && (rc = ensure_tgeom(x))) return rc;
+  for (int k = 0; k < 3; ++k) {
+    if (!x->graph_lu_on[slot][k]) continue;
+    DiffArgs a = x->graph_lu[slot][k];
+    if ((rc = ensure_lr(x, k, a)) || (rc = ensure_lu(x, k, a))) return rc;
+  }
   HIPCHK(hipGraphLaunch(x->graph[slot], x->stream));
   x->ts1_state = TS1_MIRROR;
   return 0;

@@ -125,10 +125,19 @@ struct DiffArgs {
   // theta-scheme row factors of each cell c (src/Diffusion.F90:807-816):
   // lr[c] = L = dt*kds_c*sig_c/kap_c, lr[c + ncell] = R = dt*kds_{c+1}*sig_{c+1}/kap_c
   const double *lr;
+  // cached LU factors of the lines (bin-independent; see k_line_lu): per
+  // row i, lu[c] = fact_i, lu[c+ncell] = D_i, lu[c+2*ncell] = DU_i,
+  // lu[c+3*ncell] = DL_i, piv[c] = row interchange at elimination step i;
+  // sing[line] = first zero pivot row of the line (0: none)
+  const double *lu;
+  const uint8_t *piv;
+  const int *sing;
+  double *junk;        // kJunk doubles that masked-off lanes store to
   double dt, theta;
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
   AloftMask aloft;
 };
+constexpr int kJunk = 4096;
 
 // launchers (defined next to their kernels)
 hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double *vx, double *vy,
@@ -147,6 +156,8 @@ hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
 // the row factors L, R of every cell of the sweep's lines (bin-independent)
 hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr);
+// the LU factors of every line of the sweep (reads a.lr)
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);

@@ -53,6 +53,107 @@ __global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
   lr[o + d.ncell] = dt * kds_at(r + 1) * sigat(r + 1) / kap_c;
 }
 
+// LU factorisation of the theta-scheme lines: dgtsv's elimination
+// (partial pivoting) of the matrix alone, one thread per line, in dgtsv's
+// order and arithmetic.  The matrix depends on dt, theta, the active range
+// [rmin, rmax] and the row factors only, so the context keeps the factors
+// between steps and the line solves apply them to each bin's right-hand
+// side.  Per row i (cell rmin-1+i), at that cell's offset: fact_i and the
+// interchange flag piv_i of elimination step i (rows i and i+1), and the U
+// entries D_i, DU_i, DL_i (DL_i: second super-diagonal); D_n on the last
+// row.  sing[line] is dgtsv's INFO for the line (first zero pivot row).
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int rmin = g.rmin, n = g.rmax - g.rmin + 1;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const double th = A.theta;
+  const double *Lf = A.lr, *Rf = A.lr + d.ncell;
+  double *fF = lu, *fD = lu + d.ncell, *fDU = fD + d.ncell, *fDL = fDU + d.ncell;
+  int singular = 0;
+  double Dc = 0.0, DUc = 0.0;
+  // row factors PF rows ahead
+  constexpr int PF = 8;
+  double rL[PF], rR[PF];
+#pragma unroll
+  for (int p = 0; p < PF; ++p) {
+    const long long o = base + sL * (long long)(rmin + min(1 + p, n));
+    rL[p] = Lf[o];
+    rR[p] = Rf[o];
+  }
+  for (int l0 = 1; l0 <= n; l0 += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int l = l0 + p;
+      if (l > n) break;
+      const double LeftFac = rL[p], RightFac = rR[p];
+      {
+        const long long o = base + sL * (long long)(rmin + min(l + PF, n));
+        rL[p] = Lf[o];
+        rR[p] = Rf[o];
+      }
+      const double CenterFac = LeftFac + RightFac;
+      double Dn, DUn = 0.0, DLn = 0.0;
+      if (l == 1) {
+        Dn = 1.0 + (th)*CenterFac - (th)*LeftFac;
+        DUn = -((th)*RightFac);
+      } else if (l < n) {
+        DLn = -((th)*LeftFac);
+        Dn = 1.0 + (th)*CenterFac;
+        DUn = -((th)*RightFac);
+      } else {
+        DLn = -((th)*LeftFac);
+        Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
+      }
+      if (l == 1) {
+        Dc = Dn;
+        DUc = DUn;
+        continue;
+      }
+      const int i = l - 1;
+      const long long fo = base + sL * (long long)(rmin + i);  // cell of row i
+      double DLo, fact;
+      uint8_t p_i;
+      if (fabs(Dc) >= fabs(DLn)) {
+        if (Dc == 0.0) { if (!singular) singular = i; }
+        fact = DLn / Dc;
+        Dn = Dn - fact * DUc;
+        DLo = (i <= n - 2) ? 0.0 : DLn;
+        p_i = 0;
+      } else {
+        fact = Dc / DLn;
+        const double newDi = DLn;
+        const double temp = Dn;
+        Dn = DUc - fact * temp;
+        if (i <= n - 2) {
+          DLo = DUn;
+          DUn = -fact * DLo;
+        } else {
+          DLo = DLn;
+        }
+        DUc = temp;
+        Dc = newDi;
+        p_i = 1;
+      }
+      fF[fo] = fact;
+      piv[fo] = p_i;
+      fD[fo] = Dc;
+      fDU[fo] = DUc;
+      fDL[fo] = DLo;
+      Dc = Dn;
+      DUc = DUn;
+    }
+  }
+  if (Dc == 0.0 && !singular) singular = n;
+  fD[base + sL * (long long)(rmin + n)] = Dc;
+  sing[gid] = singular;
+}
+
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
   const Dims d = A.d;
@@ -143,7 +244,16 @@ __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
 //   interior:  DL = -(th*L), D = 1+th*C, DU = -(th*R)
 //   last row:  DL = -(th*L), D = (1+th*C) - th*R
 //   B = ((1-th)*L*q_left + (1-(1-th)*C)*q_c) + (1-th)*R*q_right
-// with q_left = q_c on the first row and q_right = q_c on the last.
+// with q_left = q_c on the first row and q_right = q_c on the last.  The
+// matrix part is factored once by k_line_lu; here each bin's right-hand
+// side goes through dgtsv's elimination step i, which with rows (i, i+1)
+// holding (Bc, Bn) is
+//   no interchange: row i keeps Bc,  row i+1 gets Bn - fact_i*Bc
+//   interchange:    row i takes Bn,  row i+1 gets Bc - fact_i*Bn
+// i.e. row i keeps y and row i+1 gets x - fact_i*y with (x, y) = (Bn, Bc)
+// or (Bc, Bn), then the back substitution.  Every row issues the same
+// memory operations (masked-off lanes store to A.junk), so the loads kept
+// in flight are waited for exactly.
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
   constexpr int PF = 4;  // rows in flight per stream
@@ -158,17 +268,12 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;  // nlineq
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
-  const long long kS = kline_stride(d, DIR), kB = kline_base(d, DIR, a, b);
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *c = A.c0;
   double *sb = A.scratch_b;
-  // per-line U factors: D, DU, DL (2nd super-diagonal).  They depend on the
-  // line only, so every bin group writes the same values to the same slab
-  // (identical bits whatever the interleaving) and the groups of a tile,
-  // which run together on one XCD, share it through L2.
-  double *fD = A.fac;
-  double *fDU = fD + d.ncell;
-  double *fDL = fDU + d.ncell;
+  double *junk = A.junk + (threadIdx.x + kBlock * (blockIdx.x % (kJunk / kBlock)));
+  const double *fF = A.lu, *fD = A.lu + d.ncell, *fDU = fD + d.ncell, *fDL = fDU + d.ncell;
+  const uint8_t *fP = A.piv;
   bool exists[NB], on[NB];
   bool any_on = false;
 #pragma unroll
@@ -178,57 +283,48 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     any_on = any_on || on[u];
   }
   if (any_on) {
+    const int singular = A.sing[gid];
+    if (singular) atomicCAS(A.flag, 0, singular);
     const double th = A.theta;
-    auto qat = [&](int u, int r) { return c[base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u)]; };
+    auto qofs = [&](int u, int r) { return base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u); };
     const double *Lf = A.lr, *Rf = A.lr + d.ncell;
     double q_l[NB], q_c[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      q_c[u] = exists[u] ? qat(u, rmin) : 0.0;
+      q_c[u] = exists[u] ? c[qofs(u, rmin)] : 0.0;
       q_l[u] = q_c[u];  // BC_left_t0 = q(rmin) (:822)
     }
-    // elimination state: current row i (modified D, DU, B) and the
-    // original sub-diagonal entry that row i+1 carries
-    double Dc = 0.0, DUc = 0.0, Bc[NB];
-    int singular = 0;
-    // Row l needs its factors L, R (cell c = rmin-1+l) and the q of its
-    // right neighbour (cell c+1, while it is active).  A ring of PF rows is
-    // kept in flight: slot p holds row l0+p of the current block of PF rows
-    // and is refilled with row l+PF when row l consumes it.
-    double rL[PF], rR[PF], rq[PF][NB];
+    double Bc[NB];  // right-hand side of the row being eliminated
+    // Row l needs its factors L, R (cell c = rmin-1+l), fact and piv of
+    // elimination step l-1 (cell c-1) and the q of its right neighbour
+    // (cell c+1, while it is active).  A ring of PF rows is kept in
+    // flight: slot p holds row l0+p of the current block of PF rows and is
+    // refilled with row l+PF (the last row again past the end).
+    double rL[PF], rR[PF], rF[PF], rq[PF][NB];
+    uint8_t rP[PF];
     auto fetch_row_in = [&](int slot, int l) {
       const long long o = base + sL * (long long)(rmin + l);  // cell rmin-1+l
       rL[slot] = Lf[o];
       rR[slot] = Rf[o];
+      rF[slot] = fF[o - sL];
+      rP[slot] = fP[o - sL];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) rq[slot][u] = (l < n && exists[u]) ? qat(u, rmin + l) : 0.0;
+      for (int u = 0; u < NB; ++u) rq[slot][u] = c[(l < n && exists[u]) ? qofs(u, rmin + l) : 0];
     };
 #pragma unroll
-    for (int p = 0; p < PF; ++p)
-      if (1 + p <= n) fetch_row_in(p, 1 + p);
+    for (int p = 0; p < PF; ++p) fetch_row_in(p, min(1 + p, n));
     for (int l0 = 1; l0 <= n; l0 += PF) {
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
       const int l = l0 + p;
       if (l > n) break;
-      const double LeftFac = rL[p], RightFac = rR[p];
+      const double LeftFac = rL[p], RightFac = rR[p], fact = rF[p];
+      const bool pv = rP[p] != 0;
       double q_next[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) q_next[u] = rq[p][u];
-      if (l + PF <= n) fetch_row_in(p, l + PF);
+      fetch_row_in(p, min(l + PF, n));
       const double CenterFac = LeftFac + RightFac;
-      double Dn, DUn = 0.0, DLn = 0.0;
-      if (l == 1) {
-        Dn = 1.0 + (th)*CenterFac - (th)*LeftFac;
-        DUn = -((th)*RightFac);
-      } else if (l < n) {
-        DLn = -((th)*LeftFac);
-        Dn = 1.0 + (th)*CenterFac;
-        DUn = -((th)*RightFac);
-      } else {
-        DLn = -((th)*LeftFac);
-        Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
-      }
       double Bn[NB], q_r[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
@@ -237,53 +333,18 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
                 (1.0 - th) * RightFac * q_r[u];
       }
       if (l == 1) {
-        Dc = Dn;
-        DUc = DUn;
 #pragma unroll
         for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
       } else {
-        // dgtsv elimination step i = l-1 with row i+1 = l (NRHS = 1 path)
-        const int i = l - 1;
-        const long long fo = base + sL * (long long)(rmin - 1 + i + 1);  // cell of row i
-        double DLo;  // 2nd super-diagonal of U in row i
-        if (fabs(Dc) >= fabs(DLn)) {
-          if (Dc == 0.0) { if (!singular) singular = i; }
-          const double fact = DLn / Dc;
-          Dn = Dn - fact * DUc;
+        // elimination step i = l-1: row i is final
+        const long long fo = base + sL * (long long)(rmin + l - 1);  // cell of row i
 #pragma unroll
-          for (int u = 0; u < NB; ++u) Bn[u] = Bn[u] - fact * Bc[u];
-          DLo = (i <= n - 2) ? 0.0 : DLn;
-        } else {
-          const double fact = Dc / DLn;
-          const double newDi = DLn;
-          const double temp = Dn;
-          Dn = DUc - fact * temp;
-          if (i <= n - 2) {
-            DLo = DUn;
-            DUn = -fact * DLo;
-          } else {
-            DLo = DLn;
-          }
-          DUc = temp;
-          Dc = newDi;
-#pragma unroll
-          for (int u = 0; u < NB; ++u) {
-            const double tb = Bc[u];
-            Bc[u] = Bn[u];
-            Bn[u] = tb - fact * Bn[u];
-          }
+        for (int u = 0; u < NB; ++u) {
+          const double y = pv ? Bn[u] : Bc[u];
+          const double x = pv ? Bc[u] : Bn[u];
+          *(on[u] ? sb + fo + d.ncell * (long long)(n0 + u) : junk) = y;
+          Bc[u] = x - fact * y;
         }
-        // row i is final: store its U entries and eliminated RHS
-        fD[fo] = Dc;
-        fDU[fo] = DUc;
-        fDL[fo] = DLo;
-#pragma unroll
-        for (int u = 0; u < NB; ++u)
-          if (on[u]) sb[fo + d.ncell * (long long)(n0 + u)] = Bc[u];
-        Dc = Dn;
-        DUc = DUn;
-#pragma unroll
-        for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
       }
       // advance the window
 #pragma unroll
@@ -293,44 +354,41 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
       }
     }
     }
-    if (Dc == 0.0 && !singular) singular = n;
-    if (singular) atomicCAS(A.flag, 0, singular);
     // ---- back substitution (dgtsv, NRHS <= 2 path), row n first
     double x1[NB], x2[NB];
     {
-      const double Dn_ = Dc;
+      const double Dn_ = fD[base + sL * (long long)(rmin + n)];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         x1[u] = Bc[u] / Dn_;
         x2[u] = 0.0;
-        if (on[u]) c[base + sL * (long long)(rmax + 1) + d.ncell * (long long)(n0 + u)] = x1[u];
+        *(on[u] ? c + qofs(u, rmax) : junk) = x1[u];
       }
     }
     // rows' factors and eliminated right-hand sides, PF rows in flight
-    // (slot p holds row i0-p of the current block)
+    // (slot p holds row i0-p of the current block; row 1 again past it)
     double bD[PF], bDU[PF], bDL[PF], bB[PF][NB];
     auto fetch_row = [&](int slot, int i) {
-      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      const long long fo = base + sL * (long long)(rmin + i);
       bD[slot] = fD[fo];
       bDU[slot] = fDU[fo];
       bDL[slot] = fDL[fo];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) bB[slot][u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+      for (int u = 0; u < NB; ++u) bB[slot][u] = sb[on[u] ? fo + d.ncell * (long long)(n0 + u) : 0];
     };
 #pragma unroll
-    for (int p = 0; p < PF; ++p)
-      if (n - 1 - p >= 1) fetch_row(p, n - 1 - p);
+    for (int p = 0; p < PF; ++p) fetch_row(p, max(n - 1 - p, 1));
     for (int i0 = n - 1; i0 >= 1; i0 -= PF) {
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
       const int i = i0 - p;
       if (i < 1) break;
-      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      const long long fo = base + sL * (long long)(rmin + i);
       const double Di = bD[p], DUi = bDU[p], DLi = bDL[p];
       double Bcur[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) Bcur[u] = bB[p][u];
-      if (i - PF >= 1) fetch_row(p, i - PF);
+      fetch_row(p, max(i - PF, 1));
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const double Bi = Bcur[u];
@@ -339,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
         else xi = (Bi - DUi * x1[u] - DLi * x2[u]) / Di;
         x2[u] = x1[u];
         x1[u] = xi;
-        if (on[u]) c[fo + d.ncell * (long long)(n0 + u)] = xi;
+        *(on[u] ? c + fo + d.ncell * (long long)(n0 + u) : junk) = xi;
       }
     }
     }
@@ -374,7 +432,7 @@ struct XState {
 
 template <int NB, int TX>
 __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
-  constexpr int PF = 4;  // row factors in flight; TX cells per LDS tile
+  constexpr int PF = 4;  // rows in flight; TX cells per LDS tile
   __shared__ double s_t[NB][64][TX + 1];
   __shared__ long long s_row[64];          // original offset of cell -1 of each lane's line (-1: none)
   __shared__ unsigned char s_on[64];       // line_on of each lane
@@ -394,23 +452,21 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;
   const long long sL = line_stride(d, 1), base = line_base(d, 1, a, b);
-  const long long kS = kline_stride(d, 1), kB = kline_base(d, 1, a, b);
   const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   const long long obase = o.p3(-1, a, b);
   double *c = X.c;
   double *sb = A.scratch_b;
-  double *fD = A.fac;
-  double *fDU = fD + d.ncell;
-  double *fDL = fDU + d.ncell;
+  double *junk = A.junk + (t + 64 * (blockIdx.x % (kJunk / 64)));
+  const double *fF = A.lu, *fD = A.lu + d.ncell, *fDU = fD + d.ncell, *fDL = fDU + d.ncell;
+  const uint8_t *fP = A.piv;
   bool exists[NB], bin_on[NB], on[NB];
-  bool any_on = false, any_bin = false;
+  bool any_on = false;
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     exists[u] = (n0 + u) < d.ns;
     bin_on[u] = exists[u] && A.aloft.on(n0 + u);
     on[u] = bin_on[u] && line_on;
     any_on = any_on || on[u];
-    any_bin = any_bin || bin_on[u];
   }
   s_row[t] = lane_ok ? obase : -1;
   s_on[t] = line_on ? 1 : 0;
@@ -418,73 +474,99 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
   // the block works if one of its lines is active for one of its bins
   const bool block_on = __syncthreads_or(any_on ? 1 : 0) != 0;
 
-  // q of cells t0 .. t0+TX-1 (active cells only) of all 64 lines
-  auto load_tile = [&](int t0) {
+  // q of cells t0 .. t0+TX-1 of all 64 lines, loaded into registers one
+  // tile ahead of its use: all loads of a tile are issued together (cells
+  // outside [rmin, rmax] and missing lines read element 0 instead, and are
+  // never used), and land in LDS when the sweep enters the tile
+  double pf[NB][TX];
+  auto issue_tile = [&](int t0) {
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       if (!bin_on[u]) continue;  // uniform
-#pragma unroll 4
+#pragma unroll
       for (int m = 0; m < TX; ++m) {
         const int f = m * 64 + t, row = f / TX, col = f % TX, cell = t0 + col;
         const long long rb = s_row[row];
-        s_t[u][row][col] = (rb >= 0 && cell >= rmin && cell <= rmax)
-                               ? c[rb + (cell + 1) + o.ncell * (long long)(n0 + u)]
-                               : 0.0;
+        const bool ok = rb >= 0 && cell >= rmin && cell <= rmax;
+        pf[u][m] = c[ok ? rb + (cell + 1) + o.ncell * (long long)(n0 + u) : 0];
       }
     }
   };
-  // store the solution of cells lo .. lo+TX-1 (active cells of active lines)
+  auto commit_tile = [&]() {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!bin_on[u]) continue;  // uniform
+#pragma unroll
+      for (int m = 0; m < TX; ++m) {
+        const int f = m * 64 + t, row = f / TX, col = f % TX;
+        s_t[u][row][col] = pf[u][m];
+      }
+    }
+  };
+  // store the solution of cells lo .. lo+TX-1 (active cells of active
+  // lines; the other lanes store to junk)
   auto store_tile = [&](int lo) {
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       if (!bin_on[u]) continue;  // uniform
-#pragma unroll 4
+#pragma unroll
       for (int m = 0; m < TX; ++m) {
         const int f = m * 64 + t, row = f / TX, col = f % TX, cell = lo + col;
         const long long rb = s_row[row];
-        if (rb >= 0 && s_on[row] && cell >= rmin && cell <= rmax)
-          c[rb + (cell + 1) + o.ncell * (long long)(n0 + u)] = s_t[u][row][col];
+        const bool ok = rb >= 0 && s_on[row] && cell >= rmin && cell <= rmax;
+        *(ok ? c + rb + (cell + 1) + o.ncell * (long long)(n0 + u) : junk) = s_t[u][row][col];
       }
     }
   };
 
   if (block_on && n > 1) {
+    if (any_on) {
+      const int singular = A.sing[gid];
+      if (singular) atomicCAS(A.flag, 0, singular);
+    }
     const double th = A.theta;
     const double *Lf = A.lr, *Rf = A.lr + d.ncell;
-    load_tile(rmin);
+    issue_tile(rmin);
+    commit_tile();
     __syncthreads();
     int tlo = rmin;  // first cell of the tile in LDS
+    if (rmin + TX <= rmax) issue_tile(rmin + TX);
     double q_l[NB], q_c[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       q_c[u] = s_t[u][t][0];
       q_l[u] = q_c[u];  // BC_left_t0 = q(rmin) (:822)
     }
-    double Dc = 0.0, DUc = 0.0, Bc[NB];
-    int singular = 0;
-    double rL[PF], rR[PF];
+    double Bc[NB];
+    // ring of row factors (L, R of row l; fact, piv of step l-1), PF rows
+    // ahead, the last row again past the end
+    double rL[PF], rR[PF], rF[PF];
+    uint8_t rP[PF];
     auto fetch_lr = [&](int slot, int l) {
-      const long long o = base + sL * (long long)(rmin + l);  // cell rmin-1+l
-      rL[slot] = Lf[o];
-      rR[slot] = Rf[o];
+      const long long oo = base + sL * (long long)(rmin + l);  // cell rmin-1+l
+      rL[slot] = Lf[oo];
+      rR[slot] = Rf[oo];
+      rF[slot] = fF[oo - sL];
+      rP[slot] = fP[oo - sL];
     };
 #pragma unroll
-    for (int p = 0; p < PF; ++p)
-      if (1 + p <= n) fetch_lr(p, 1 + p);
+    for (int p = 0; p < PF; ++p) fetch_lr(p, min(1 + p, n));
     for (int l0 = 1; l0 <= n; l0 += PF) {
 #pragma unroll
       for (int p = 0; p < PF; ++p) {
         const int l = l0 + p;
         if (l > n) break;
-        const double LeftFac = rL[p], RightFac = rR[p];
-        if (l + PF <= n) fetch_lr(p, l + PF);
+        const double LeftFac = rL[p], RightFac = rR[p], fact = rF[p];
+        const bool pv = rP[p] != 0;
+        fetch_lr(p, min(l + PF, n));
         double q_next[NB];
         if (l < n) {
           if (rmin + l >= tlo + TX) {  // next tile (uniform: l and n are)
             __syncthreads();
             tlo += TX;
-            load_tile(tlo);
+            commit_tile();
             __syncthreads();
+            if (tlo + TX <= rmax) issue_tile(tlo + TX);
           }
 #pragma unroll
           for (int u = 0; u < NB; ++u) q_next[u] = s_t[u][t][rmin + l - tlo];
@@ -493,18 +575,6 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
           for (int u = 0; u < NB; ++u) q_next[u] = 0.0;
         }
         const double CenterFac = LeftFac + RightFac;
-        double Dn, DUn = 0.0, DLn = 0.0;
-        if (l == 1) {
-          Dn = 1.0 + (th)*CenterFac - (th)*LeftFac;
-          DUn = -((th)*RightFac);
-        } else if (l < n) {
-          DLn = -((th)*LeftFac);
-          Dn = 1.0 + (th)*CenterFac;
-          DUn = -((th)*RightFac);
-        } else {
-          DLn = -((th)*LeftFac);
-          Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
-        }
         double Bn[NB], q_r[NB];
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -513,53 +583,18 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
                   (1.0 - th) * RightFac * q_r[u];
         }
         if (l == 1) {
-          Dc = Dn;
-          DUc = DUn;
 #pragma unroll
           for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
         } else {
-          const int i = l - 1;
-          const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
-          double DLo;
-          if (fabs(Dc) >= fabs(DLn)) {
-            if (Dc == 0.0) { if (!singular) singular = i; }
-            const double fact = DLn / Dc;
-            Dn = Dn - fact * DUc;
+          // elimination step i = l-1 (see k_diff_cn): row i is final
+          const long long fo = base + sL * (long long)(rmin + l - 1);
 #pragma unroll
-            for (int u = 0; u < NB; ++u) Bn[u] = Bn[u] - fact * Bc[u];
-            DLo = (i <= n - 2) ? 0.0 : DLn;
-          } else {
-            const double fact = Dc / DLn;
-            const double newDi = DLn;
-            const double temp = Dn;
-            Dn = DUc - fact * temp;
-            if (i <= n - 2) {
-              DLo = DUn;
-              DUn = -fact * DLo;
-            } else {
-              DLo = DLn;
-            }
-            DUc = temp;
-            Dc = newDi;
-#pragma unroll
-            for (int u = 0; u < NB; ++u) {
-              const double tb = Bc[u];
-              Bc[u] = Bn[u];
-              Bn[u] = tb - fact * Bn[u];
-            }
+          for (int u = 0; u < NB; ++u) {
+            const double y = pv ? Bn[u] : Bc[u];
+            const double x = pv ? Bc[u] : Bn[u];
+            *(on[u] ? sb + fo + d.ncell * (long long)(n0 + u) : junk) = y;
+            Bc[u] = x - fact * y;
           }
-          if (any_on) {
-            fD[fo] = Dc;
-            fDU[fo] = DUc;
-            fDL[fo] = DLo;
-          }
-#pragma unroll
-          for (int u = 0; u < NB; ++u)
-            if (on[u]) sb[fo + d.ncell * (long long)(n0 + u)] = Bc[u];
-          Dc = Dn;
-          DUc = DUn;
-#pragma unroll
-          for (int u = 0; u < NB; ++u) Bc[u] = Bn[u];
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -568,31 +603,31 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
         }
       }
     }
-    if (Dc == 0.0 && !singular) singular = n;
-    if (singular && any_on) atomicCAS(A.flag, 0, singular);
     // ---- back substitution; solutions collect in an LDS tile covering
     // cells [thi-TX+1, thi], stored when the sweep leaves it
     __syncthreads();  // the q tile is no longer read
     int thi = rmax;
     double x1[NB], x2[NB];
+    {
+      const double Dn_ = fD[base + sL * (long long)(rmin + n)];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      x1[u] = Bc[u] / Dc;
-      x2[u] = 0.0;
-      s_t[u][t][TX - 1] = x1[u];  // cell rmax at column TX-1 of tile [rmax-TX+1, rmax]
+      for (int u = 0; u < NB; ++u) {
+        x1[u] = Bc[u] / Dn_;
+        x2[u] = 0.0;
+        s_t[u][t][TX - 1] = x1[u];  // cell rmax at column TX-1 of tile [rmax-TX+1, rmax]
+      }
     }
     double bD[PF], bDU[PF], bDL[PF], bB[PF][NB];
     auto fetch_row = [&](int slot, int i) {
-      const long long fo = base + sL * (long long)(rmin - 1 + i + 1);
+      const long long fo = base + sL * (long long)(rmin + i);
       bD[slot] = fD[fo];
       bDU[slot] = fDU[fo];
       bDL[slot] = fDL[fo];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) bB[slot][u] = on[u] ? sb[fo + d.ncell * (long long)(n0 + u)] : 0.0;
+      for (int u = 0; u < NB; ++u) bB[slot][u] = sb[on[u] ? fo + d.ncell * (long long)(n0 + u) : 0];
     };
 #pragma unroll
-    for (int p = 0; p < PF; ++p)
-      if (n - 1 - p >= 1) fetch_row(p, n - 1 - p);
+    for (int p = 0; p < PF; ++p) fetch_row(p, max(n - 1 - p, 1));
     for (int i0 = n - 1; i0 >= 1; i0 -= PF) {
 #pragma unroll
       for (int p = 0; p < PF; ++p) {
@@ -609,7 +644,7 @@ __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
         double Bcur[NB];
 #pragma unroll
         for (int u = 0; u < NB; ++u) Bcur[u] = bB[p][u];
-        if (i - PF >= 1) fetch_row(p, i - PF);
+        fetch_row(p, max(i - PF, 1));
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
           const double Bi = Bcur[u];
@@ -654,9 +689,14 @@ template <int NB>
 hipError_t launch_cn_xrows_tx(hipStream_t s, const DiffArgs &a, const XState &x, int tx) {
   // LDS per workgroup NB*64*(TX+1) doubles: at most 34 KiB for NB <= 2 and
   // TX = 32, 35 KiB for NB = 4 and TX = 16, 37 KiB for NB = 8 and TX = 8
-  if (tx <= 8 || NB >= 8) return launch_cn_xrows_nb<NB, 8>(s, a, x);
-  if (tx <= 16 || NB >= 4) return launch_cn_xrows_nb<NB, 16>(s, a, x);
-  return launch_cn_xrows_nb<NB, 32>(s, a, x);
+  if constexpr (NB >= 8) {
+    return launch_cn_xrows_nb<NB, 8>(s, a, x);
+  } else if constexpr (NB >= 4) {
+    return tx <= 8 ? launch_cn_xrows_nb<NB, 8>(s, a, x) : launch_cn_xrows_nb<NB, 16>(s, a, x);
+  } else {
+    if (tx <= 8) return launch_cn_xrows_nb<NB, 8>(s, a, x);
+    return tx <= 16 ? launch_cn_xrows_nb<NB, 16>(s, a, x) : launch_cn_xrows_nb<NB, 32>(s, a, x);
+  }
 }
 
 template <template <int, int> class K, int NB>
@@ -704,6 +744,17 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
     case 0: hipLaunchKernelGGL((k_line_lr<0>), grid, dim3(kBlock), 0, s, a, lr); break;
     case 1: hipLaunchKernelGGL((k_line_lr<1>), grid, dim3(kBlock), 0, s, a, lr); break;
     default: hipLaunchKernelGGL((k_line_lr<2>), grid, dim3(kBlock), 0, s, a, lr); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)((lines + kBlock - 1) / kBlock));
+  switch (a.g.dir) {
+    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
+    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
+    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
   }
   return hipGetLastError();
 }
