@@ -160,7 +160,7 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
             "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
 
 
-def ensemble_leg(pkg, tr, members, steps, size_scale):
+def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
     """Ensemble throughput: one context (own stream) per member on the one
     GPU, all members stepped together; eager launches against replays of
     the captured glue + step graphs."""
@@ -169,6 +169,8 @@ def ensemble_leg(pkg, tr, members, steps, size_scale):
     states = [pkg.workloads.ensemble_member(m, size_scale=size_scale) for m in range(members)]
     log(f"[bench] ensemble: {members} members built in {time.perf_counter() - t0:.1f} s")
     Ts = [tr.Transport(st) for st in states]
+    for T in Ts:
+        T.tune(line_solver=line_solver)
     work = sum(st.ns * st.nx * st.ny * st.nz for st in states)
     for T in Ts:
         T.st.concen_pd[..., 1] = T.st.concen_pd[..., 0]
@@ -224,6 +226,9 @@ def main():
                     help="members of the ensemble leg (0: skip); each on its own context and stream")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-scale", type=float, default=1.0, help="horizontal size factor of the members")
+    ap.add_argument("--line-solver", choices=("exact", "partitioned"), default="partitioned",
+                    help="theta-scheme line solves: dgtsv order (bitwise the reference) or partitioned "
+                         "(lanes of a wave per line; agrees to rounding)")
     ap.add_argument("--adj-periods", type=int, default=2)
     ap.add_argument("--adj-steps-per-period", type=int, default=10)
     args = ap.parse_args()
@@ -253,6 +258,7 @@ def main():
         f"{st.nbytes() / 1e9:.2f} GB host arrays")
 
     T = tr.Transport(st)
+    T.tune(line_solver=args.line_solver)
     itime = 0
     for _ in range(args.warmup):
         itime += 1
@@ -329,7 +335,8 @@ def main():
     ensemble = None
     if args.ensemble_members > 0 and not args.no_profile:
         log("[bench] ensemble leg ...")
-        ensemble = ensemble_leg(pkg, tr, args.ensemble_members, args.ensemble_steps, args.ensemble_scale)
+        ensemble = ensemble_leg(pkg, tr, args.ensemble_members, args.ensemble_steps, args.ensemble_scale,
+                                args.line_solver)
         log("[bench] ensemble: " + json.dumps(ensemble))
     adjoint = None
     if adj_state is not None:
@@ -348,6 +355,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "nx": st.nx, "ny": st.ny, "nz": st.nz, "bins": st.ns,
                    "dt_h": st.dt, "diffusion": "crank-nicolson" if use_diffusion else "off",
+                   "line_solver": args.line_solver if use_diffusion else None,
                    "parallelism": "single gpu",
                    "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults"},
         "step_hbm": {"bytes_per_bincell": round(bpb, 2), "frac_of_peak": round(step_frac, 4)},

@@ -188,8 +188,17 @@ enum {
   ASH3D_TUNE_NB_DIFFUSE_X = 3,
   ASH3D_TUNE_NB_DIFFUSE_Y = 4,
   ASH3D_TUNE_NB_DIFFUSE_Z = 5,
-  ASH3D_TUNE_X_TILE = 6 /* cells per LDS tile of the x-line solves: 8, 16 or 32 */
+  ASH3D_TUNE_X_TILE = 6, /* cells per LDS tile of the x-line solves: 8, 16 or 32 */
+  ASH3D_TUNE_LINE_SOLVER = 7, /* ASH3D_LINE_SOLVER_* below */
+  ASH3D_TUNE_PART_BINS = 8 /* bins per workgroup of the partitioned solves (0: all) */
 };
+/* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
+ * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits
+ * every line over the lanes of a wave (segment elimination, then parallel
+ * cyclic reduction of the segment ends) for lines of 3..512 active cells
+ * (other lines use EXACT); the matrices are strictly diagonally dominant,
+ * so the results agree with dgtsv's to rounding (about 1e-14 relative). */
+enum { ASH3D_LINE_SOLVER_EXACT = 0, ASH3D_LINE_SOLVER_PARTITIONED = 1 };
 int ash3d_tr_tune(ash3d_tr_ctx *ctx, int32_t param, int32_t value);
 
 /* Kernel timing.  No counterpart in the reference, whose only timers are

@@ -1,0 +1,193 @@
+"""The partitioned theta-scheme line solver (line_solver="partitioned")
+against the oracle's dgtsv-order solves.
+
+Lines whose matrix is strictly diagonally dominant are solved in a
+different order than dgtsv's, so the bar there is the 1e-12 relative
+L-inf tolerance (max |gpu - oracle| / max |oracle| per array), not bitwise
+equality; lines that are not dominant (negative diffusivities) and lines
+outside 3..512 active cells take dgtsv's order and stay bitwise.  The
+cases cover every tile shape the solver picks (contiguous x lines of up to
+256, 384 and 512 cells; strided y/z lines of up to 64, 128, 256 and 512).
+"""
+import numpy as np
+import pytest
+
+from helpers import interior_mass, outflow_mass
+from test_gpu_parity import assert_same
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12  # relative L-inf, the parity tolerance for floating-point results
+
+
+@pytest.fixture(scope="module")
+def tr(pkg):
+    import importlib
+    t = importlib.import_module("volcano-ash3d_amd.transport")
+    t.load()
+    return t
+
+
+def run_both(pkg, oracle, tr, st, calls, **tune):
+    a, b = st.copy(), st.copy()
+    O = oracle.Oracle(a)
+    T = tr.Transport(b)
+    T.tune(line_solver="partitioned", **tune)
+    for name, *args in calls:
+        getattr(O, name)(*args)
+        getattr(T, name)(*args)
+    T.sync_to_host()
+    T.close()
+    return a, b
+
+
+@pytest.mark.parametrize("dirn", ["x", "y", "z"])
+@pytest.mark.parametrize("ns", [1, 4])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_sweeps(pkg, oracle, tr, dirn, ns, periodic):
+    st = pkg.workloads.random_state(12, 10, 9, ns, seed=300 + ns, latlon=True, periodic=periodic, use_cn=True)
+    st.dt *= 5.0
+    a, b = run_both(pkg, oracle, tr, st, [("Set_BC", 2), (f"diffCN_{dirn}",)])
+    assert_same(a, b, tol=TOL)
+
+
+@pytest.mark.parametrize("shape", [(400, 130, 70), (300, 300, 20), (200, 60, 150), (520, 40, 3)])
+def test_tile_shapes(pkg, oracle, tr, shape):
+    """x lines of 400 / 300 / 200 / 520 cells (the last beyond 512: dgtsv
+    order), y lines of 130 / 300 / 60 / 40, z lines of 70 / 20 / 150 / 3."""
+    nx, ny, nz = shape
+    st = pkg.workloads.random_state(nx, ny, nz, 2, seed=sum(shape), latlon=True, use_cn=True)
+    st.dt *= 20.0
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)])
+    assert_same(a, b, tol=TOL)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 1.0])
+def test_theta(pkg, oracle, tr, theta):
+    st = pkg.workloads.random_state(11, 9, 12, 3, seed=31, latlon=True, imp_fac=theta)
+    st.dt *= 10.0
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_z",), ("diffCN_y",), ("diffCN_x",)])
+    assert_same(a, b, tol=TOL)
+
+
+def test_subgrid_bounds_and_short_lines(pkg, oracle, tr):
+    """Active ranges inside the grid, bins not aloft, one-cell y lines
+    (dgtsv order: no solve, no copy) and two-cell lines."""
+    st = pkg.workloads.random_state(10, 9, 8, 3, seed=41, latlon=True)
+    st.is_aloft = np.array([1, 0, 1], dtype=np.int32)
+    st.bounds = (2, 8, 3, 3, 2, 7)
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)])
+    assert_same(a, b, tol=TOL)
+    st.bounds = (4, 5, 2, 8, 3, 4)
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)])
+    assert_same(a, b, tol=TOL)
+
+
+def test_non_dominant_lines_bitwise(pkg, oracle, tr):
+    """Negative diffusivities: the matrices are not diagonally dominant and
+    dgtsv takes row interchanges; those lines are solved in dgtsv's order
+    from the cached factors, so they match bit for bit."""
+    st = pkg.workloads.random_state(9, 8, 10, 2, seed=51)
+    rng = np.random.default_rng(0)
+    st.kz[...] = -3.0 * (1.0 + rng.random(st.kz.shape))
+    st.dt = 0.7
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_z",)])
+    assert_same(a, b)
+
+
+def test_mixed_dominant_and_pivoting_lines(pkg, oracle, tr):
+    """Some z columns with negative diffusivity in a sweep that is otherwise
+    solved partitioned: those columns match bit for bit, the rest to the
+    tolerance."""
+    st = pkg.workloads.random_state(40, 30, 24, 3, seed=52, latlon=True)
+    st.kz[5:9, 3:20, :] = -2.0  # kz(i, j, k), i = 5..8, j = 3..19
+    st.dt = 0.5
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_z",)])
+    assert_same(a, b, tol=TOL)
+    cols = (slice(6, 10), slice(4, 21))  # concen_pd(i, j, ...) at python index i+1, j+1
+    assert np.array_equal(a.concen_pd[cols], b.concen_pd[cols])
+
+
+@pytest.mark.parametrize("bins_wg", [1, 2, 0])
+def test_bins_per_workgroup(pkg, oracle, tr, bins_wg):
+    st = pkg.workloads.random_state(30, 20, 16, 5, seed=53, latlon=True, use_cn=True)
+    st.dt *= 5.0
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)], part_bins=bins_wg)
+    assert_same(a, b, tol=TOL)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_multistep_with_glue_and_source(pkg, oracle, tr, periodic):
+    st = pkg.workloads.random_state(16, 12, 10, 3, seed=71, latlon=True, periodic=periodic, use_cn=True)
+    flux = np.zeros((st.nz + 1, st.ns), order="F")
+    flux[3:7, :] = 50.0 + np.arange(st.ns)[None, :]
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    T.tune(line_solver="partitioned")
+    for itime in range(1, 6):
+        for X in (O, T):
+            X.step_glue()
+            X.source_column_add(8, 6, flux)
+            X.transport_step(itime, True)
+    T.sync_to_host()
+    T.close()
+    assert_same(a, b, tol=TOL)
+
+
+def test_large_lonlat_step(pkg, oracle, tr):
+    """One full step (advection + CN) of the 360x240x50, 15-bin lon-lat
+    forecast grid."""
+    st = pkg.workloads.config2_state()
+    a, b = run_both(pkg, oracle, tr, st, [("transport_step", 1, True)])
+    assert_same(a, b, names=("concen_pd", "outflow_xy1_pd", "outflow_xz1_pd", "outflow_xz2_pd",
+                             "deposit_granularity"), tol=TOL)
+
+
+def test_large_lonlat_mass_conservation(pkg, tr):
+    """10 steps of the forecast grid conserve total mass to 1e-13 and keep
+    concentrations >= 0."""
+    st = pkg.workloads.config2_state()
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    m0 = interior_mass(st).sum()
+    with tr.Transport(st) as T:
+        T.tune(line_solver="partitioned")
+        for itime in range(1, 11):
+            T.step_glue()
+            T.transport_step(itime, True)
+        T.sync_to_host()
+    m1 = (interior_mass(st) + outflow_mass(st)).sum()
+    assert abs(m1 - m0) <= 1e-13 * m0
+    assert st.interior().min() >= -1e-9 * st.interior().max()
+
+
+def test_graph_replays_match_eager(pkg, tr):
+    """Captured partitioned steps replay bitwise like eager ones."""
+    st = pkg.workloads.random_state(21, 17, 10, 4, seed=31, latlon=True, periodic=True, use_cn=True)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    a, b = st.copy(), st.copy()
+    E, G = tr.Transport(a), tr.Transport(b)
+    for T in (E, G):
+        T.tune(line_solver="partitioned")
+        T.step_glue()
+        T.transport_step(1, True)
+    G.graph_capture(2, True)
+    G.graph_capture(3, True)
+    for itime in range(2, 7):
+        E.step_glue()
+        E.transport_step(itime, True)
+        G.graph_launch(itime)
+        E.sync_to_host()
+        G.sync_to_host()
+        assert np.array_equal(a.concen_pd, b.concen_pd), itime
+    E.close()
+    G.close()
+
+
+def test_bad_line_solver_rejected(pkg, tr):
+    st = pkg.workloads.random_state(6, 5, 4, 1, seed=1)
+    with tr.Transport(st) as T:
+        with pytest.raises(tr.Ash3dError):
+            T.tune(line_solver=2)
+        with pytest.raises(tr.Ash3dError):
+            T.tune(part_bins=-1)

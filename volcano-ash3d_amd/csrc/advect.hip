@@ -280,8 +280,8 @@ hipError_t launch_nb(hipStream_t s, const AdvArgs &a) {
 // more bins share the per-cell terms, fewer keep more waves resident.
 int pick_bins_per_thread(int ns, int kind, int dir) {
   // measured on the 360x240x50x15 forecast grid (bench autotuner): y
-  // advection 2, z advection 1, x line solves (row tiles) 1, y/z line solves 8
-  const int cap = (kind == 0) ? (dir == 2 ? 1 : 2) : (dir == 0 ? 1 : 8);
+  // advection 2, z advection 1, x line solves (row tiles) 2, y/z line solves 8
+  const int cap = (kind == 0) ? (dir == 2 ? 1 : 2) : (dir == 0 ? 2 : 8);
   int nb = 1;
   while (nb * 2 <= cap && nb * 2 <= ns) nb *= 2;
   return nb;

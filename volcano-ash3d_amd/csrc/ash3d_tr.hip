@@ -75,11 +75,13 @@ struct ash3d_tr_ctx {
   double lr_dt[3] = {0.0, 0.0, 0.0};
   bool lr_ok[3] = {false, false, false};
   // LU factors of the theta-scheme lines per direction (one allocation:
-  // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags and
-  // one INFO per line), valid for the key (dt, theta, rmin, rmax)
+  // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags, one
+  // INFO per line and one diagonal-dominance flag per line), valid for the
+  // key (dt, theta, rmin, rmax)
   double *lu[3] = {nullptr, nullptr, nullptr};
   uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
   int *lu_sing[3] = {nullptr, nullptr, nullptr};
+  uint8_t *lu_ddom[3] = {nullptr, nullptr, nullptr};
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
@@ -125,6 +127,8 @@ struct ash3d_tr_ctx {
   int ts1_state = TS1_EXPLICIT;
   int nb_adv[3] = {4, 4, 4}, nb_diff[3] = {4, 4, 4};  // bins per thread, by direction
   int x_tile = 16;  // cells per LDS tile of the x-line solves
+  int line_solver = ASH3D_LINE_SOLVER_EXACT;  // theta-scheme line solves
+  int part_bins = 0;  // bins per workgroup of the partitioned solves (0: all)
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -306,14 +310,15 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   int rc = 0;
   const long long nc = x->d.ncell, lines = (long long)a.g.Na * a.g.Nb;
   if (!x->lu[dir]) {
-    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2))) return rc;
+    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + (lines + 7) / 8))) return rc;
     x->lu_piv[dir] = (uint8_t *)(x->lu[dir] + 4 * nc);
     x->lu_sing[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8);
+    x->lu_ddom[dir] = (uint8_t *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2);
   }
   if (!x->junk && (rc = dev_alloc(&x->junk, kJunk))) return rc;
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
-    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir]));
+    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ddom[dir]));
     x->lu_ok[dir] = true;
     x->lu_dt[dir] = a.dt;
     x->lu_th[dir] = a.theta;
@@ -323,6 +328,7 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   a.lu = x->lu[dir];
   a.piv = x->lu_piv[dir];
   a.sing = x->lu_sing[dir];
+  a.ddom = x->lu_ddom[dir];
   a.junk = x->junk;
   x->lu_args[dir] = a;
   x->lu_args_set[dir] = true;
@@ -337,6 +343,34 @@ int ensure_fac(ash3d_tr_ctx *x, int nb) {
   if (rc) return rc;
   x->fac_groups = 1;
   return 0;
+}
+
+// Partitioned line solves of one direction when selected and the active
+// line length allows (3..512 cells); returns 1 when they ran.  a holds the
+// sweep's geometry (for x: the x<->y transposed one, which the row factor
+// cache uses) and the original state in a.c0.
+int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
+  *rc = 0;
+  const int n = a.g.rmax - a.g.rmin + 1;
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > 512) return 0;
+  const Dims &o = x->d;
+  PartLayout lay;
+  lay.st0 = 1 + o.sy + o.sz;
+  if (dir == 0) {
+    lay.sta = o.sy; lay.stb = o.sz; lay.str = 1;
+    const Dims &t = a.d;  // transposed: cell (j, r, k) at (j+1) + t.sy*(r+1) + t.sz*(k+1)
+    lay.lr0 = 1 + t.sy + t.sz; lay.lra = 1; lay.lrb = t.sz; lay.lrr = t.sy;
+  } else {
+    lay.sta = 1;
+    lay.stb = (dir == 1) ? o.sz : o.sy;
+    lay.str = (dir == 1) ? o.sy : o.sz;
+    lay.lr0 = lay.st0; lay.lra = lay.sta; lay.lrb = lay.stb; lay.lrr = lay.str;
+  }
+  lay.lr_ncell = a.d.ncell;
+  ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
+  hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins);
+  if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
+  return 1;
 }
 
 int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fac, const int32_t *is_aloft,
@@ -378,6 +412,11 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.sig = x->tsx;
     a.kk = x->tkx;
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
+    if (try_cn_partitioned(x, 0, a, &rc)) {
+      if (rc) return rc;
+      x->ts1_state = TS1_MIRROR;
+      return 0;
+    }
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);
       HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
@@ -405,8 +444,12 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
   }
   if (use_cn) {
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
-    ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
-    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
+    if (!try_cn_partitioned(x, dir, a, &rc)) {
+      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
+      HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
+    } else if (rc) {
+      return rc;
+    }
   } else {
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff[dir]));
@@ -863,6 +906,18 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, do
 int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
   int rc = check_ctx(x);
   if (rc) return rc;
+  if (param == ASH3D_TUNE_LINE_SOLVER) {
+    if (value != ASH3D_LINE_SOLVER_EXACT && value != ASH3D_LINE_SOLVER_PARTITIONED)
+      return fail(ASH3D_TR_EINVAL, "line solver must be %d (dgtsv order) or %d (partitioned) (got %d)",
+                  ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED, value);
+    x->line_solver = value;
+    return 0;
+  }
+  if (param == ASH3D_TUNE_PART_BINS) {
+    if (value < 0) return fail(ASH3D_TR_EINVAL, "bins per workgroup must be >= 0 (got %d)", value);
+    x->part_bins = value;
+    return 0;
+  }
   if (param == ASH3D_TUNE_X_TILE) {
     if (value != 8 && value != 16 && value != 32)
       return fail(ASH3D_TR_EINVAL, "x-line tile must be 8, 16 or 32 cells (got %d)", value);

@@ -132,6 +132,7 @@ struct DiffArgs {
   const double *lu;
   const uint8_t *piv;
   const int *sing;
+  const uint8_t *ddom;  // per line: 1 when every row is strictly diagonally dominant
   double *junk;        // kJunk doubles that masked-off lanes store to
   double dt, theta;
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
@@ -156,8 +157,17 @@ hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
 // the row factors L, R of every cell of the sweep's lines (bin-independent)
 hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr);
+// partitioned line solves (linesolve.hip): offsets of cell r of line (a, b)
+// are st0 + sta*a + stb*b + str*r in the state and lr0 + lra*a + lrb*b +
+// lrr*r in the row factor cache (R at + lr_ncell); hipErrorInvalidValue
+// when the active line length is outside [3, 512]
+struct PartLayout {
+  long long st0, sta, stb, str;
+  long long lr0, lra, lrb, lrr, lr_ncell;
+};
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg);
 // the LU factors of every line of the sweep (reads a.lr)
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing);
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, uint8_t *ddom);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);

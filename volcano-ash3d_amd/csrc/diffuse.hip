@@ -61,9 +61,11 @@ __global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
 // side.  Per row i (cell rmin-1+i), at that cell's offset: fact_i and the
 // interchange flag piv_i of elimination step i (rows i and i+1), and the U
 // entries D_i, DU_i, DL_i (DL_i: second super-diagonal); D_n on the last
-// row.  sing[line] is dgtsv's INFO for the line (first zero pivot row).
+// row.  sing[line] is dgtsv's INFO for the line (first zero pivot row);
+// ddom[line] = 1 when every row satisfies |D| > |DL| + |DU| (strict row
+// diagonal dominance, which the partitioned solves rely on).
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing) {
+__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, uint8_t *ddom) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -76,6 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   const double *Lf = A.lr, *Rf = A.lr + d.ncell;
   double *fF = lu, *fD = lu + d.ncell, *fDU = fD + d.ncell, *fDL = fDU + d.ncell;
   int singular = 0;
+  bool dom = true;
   double Dc = 0.0, DUc = 0.0;
   // row factors PF rows ahead
   constexpr int PF = 8;
@@ -110,6 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
         DLn = -((th)*LeftFac);
         Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
       }
+      dom = dom && (fabs(Dn) > fabs(DLn) + fabs(DUn));  // false for NaN too
       if (l == 1) {
         Dc = Dn;
         DUc = DUn;
@@ -152,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   if (Dc == 0.0 && !singular) singular = n;
   fD[base + sL * (long long)(rmin + n)] = Dc;
   sing[gid] = singular;
+  ddom[gid] = dom ? 1 : 0;
 }
 
 template <int DIR, int NB>
@@ -748,13 +753,13 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
   return hipGetLastError();
 }
 
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing) {
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, uint8_t *ddom) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)((lines + kBlock - 1) / kBlock));
   switch (a.g.dir) {
-    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
-    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
-    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing); break;
+    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
+    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
+    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
   }
   return hipGetLastError();
 }

@@ -1,0 +1,377 @@
+// Partitioned theta-scheme line solves (the fast alternative to the
+// dgtsv-order solves in diffuse.hip).  Same systems as diffCN_{x,y,z}
+// (src/Diffusion.F90:671-1428): per line the tridiagonal matrix of rows
+// l = 1..n (cells rmin..rmax)
+//   a_l = -(th*L_l) (l > 1),  c_l = -(th*R_l) (l < n),
+//   b_l = 1 + th*C_l, minus th*L_1 on the first row and th*R_n on the last,
+//   d_l = (1-th)*L_l*q_{l-1} + (1-(1-th)*C_l)*q_l + (1-th)*R_l*q_{l+1}
+// (q_0 = q_1, q_{n+1} = q_n).  These matrices are strictly row diagonally
+// dominant (|b| >= 1 + |a| + |c| for L, R >= 0), so elimination needs no
+// pivoting and may be reordered: the result agrees with dgtsv's to
+// rounding (about 1e-14 relative), not bit for bit.
+//
+// Each line is cut into LP segments of M rows, one lane per segment (rows
+// past n are identity rows).  A lane eliminates its segment so that every
+// row reads  ah_k*x_first + x_k + ch_k*x_last = dh_k  (the "modified
+// Thomas" step of Laszlo, Giles & Appleyard, ACM TOMS 42(4), 2016); the
+// first and last rows of all segments form a tridiagonal system of 2*LP
+// unknowns, solved by parallel cyclic reduction across the LP lanes
+// (Hockney's PCR, log2(2*LP) levels of lane shuffles); the interior rows
+// then follow from their segment's two end values.  Everything that
+// depends on the matrix only (segment factors, PCR multipliers) is formed
+// once per line and kept in registers while the workgroup runs the bins
+// of its lines one after another; per bin a line costs one read of q and
+// one write of the result.  A bin's tile of lines is staged through LDS
+// (coalesced both ways) while the next bin's tile is already in flight.
+#include "common.hpp"
+
+namespace ash3d {
+
+namespace {
+
+// Offsets of cell r of line (a, b): st0 + sta*a + stb*b + str*r in the
+// state (bin stride ncell) and lr0 + lra*a + lrb*b + lrr*r in the row
+// factor cache (L at lr[o], R at lr[o + lr_ncell]).
+struct PartArgs {
+  long long st0, sta, stb, str;
+  long long lr0, lra, lrb, lrr, lr_ncell;
+  int bins_wg;  // bins per workgroup
+};
+
+// Workgroup b runs on XCD b mod 8.  Each XCD takes a contiguous range of
+// the (tile, bin group) items, in order, so that neighbouring tiles (whose
+// rows can share cache lines) go through the same L2.
+struct Item {
+  long long tile;
+  int group;
+};
+__device__ inline Item xcd_block_item(long long b, long long nwg, int G) {
+  const long long per = nwg / kXcds;  // nwg is a multiple of kXcds
+  const long long it = (b % kXcds) * per + b / kXcds;
+  return Item{it / G, (int)(it % G)};
+}
+
+template <int LP>
+struct Lev {
+  static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
+};
+
+template <int LP, int M, int T, bool CONTIG>
+__global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
+  constexpr int NT = T * LP;
+  constexpr int NPAD = LP * M;
+  constexpr int PITCH = NPAD + 1;
+  constexpr int LEV = Lev<LP>::value;
+  extern __shared__ double s_tile[];  // T lines x PITCH
+  const LineGeom g = A.g;
+  const int ns = A.d.ns;
+  const long long ncell = A.d.ncell;
+  const int tiles_a = (g.Na + T - 1) / T;
+  const int G = (ns + Q.bins_wg - 1) / Q.bins_wg;
+  const Item tg = xcd_block_item(blockIdx.x, gridDim.x, G);
+  if (tg.tile >= (long long)tiles_a * g.Nb) return;  // padding workgroup (uniform)
+  const int b = (int)(tg.tile / tiles_a) + 1;
+  const int a_first = (int)(tg.tile % tiles_a) * T + 1;
+  const int tid = threadIdx.x;
+  const int ln = tid / LP;  // line of this lane within the tile
+  const int p = tid % LP;   // segment of this lane
+  const int a = a_first + ln;
+  const bool has_line = a <= g.Na;
+  const int ac = has_line ? a : g.Na;  // an existing line for masked lanes' reads
+  const int rmin = g.rmin, rmax = g.rmax, n = rmax - rmin + 1, N = g.N;
+  const bool line_on = has_line && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const int u0 = tg.group * Q.bins_wg;
+  const int u1 = min(ns, u0 + Q.bins_wg);
+  bool any_bin = false;
+  for (int u = u0; u < u1; ++u) any_bin = any_bin || A.aloft.on(u);
+  if (line_on && any_bin) {
+    const int singular = A.sing[(long long)(a - 1) + (long long)g.Na * (b - 1)];
+    if (singular) atomicCAS(A.flag, 0, singular);
+  }
+  const double th = A.theta;
+
+  const double omt = 1.0 - th;
+  // ---- per-line factors of this lane's segment (rows i0 = p*M + k)
+  double rL[M], rR[M];          // row factors (zero on padding rows)
+  double al[M], be[M];          // forward: dh_k = al_k*d_k - be_k*dh_{k-1}
+  double ga[M];                 // backward: dh_k -= ga_k*dh_{k+1}
+  double ah[M], ch[M];          // final rows: ah_k*x_first + x_k + ch_k*x_last = dh_k
+  double rho;
+  {
+    double av[M], bv[M], cv[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int i0 = p * M + k;
+      const bool real = i0 < n;
+      const long long o = Q.lr0 + Q.lra * ac + Q.lrb * b + Q.lrr * (long long)(rmin + (real ? i0 : 0));
+      const double L = A.lr[o], R = A.lr[o + Q.lr_ncell];
+      const double C = L + R;
+      const int l = i0 + 1;
+      if (real) {
+        av[k] = (l == 1) ? 0.0 : -(th * L);
+        cv[k] = (l == n) ? 0.0 : -(th * R);
+        double D = 1.0 + th * C;
+        if (l == 1) D = D - th * L;
+        if (l == n) D = D - th * R;
+        bv[k] = D;
+        rL[k] = L;
+        rR[k] = R;
+      } else {
+        av[k] = 0.0; bv[k] = 1.0; cv[k] = 0.0;
+        rL[k] = rR[k] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      if (k < 2) {
+        const double inv = 1.0 / bv[k];
+        al[k] = inv;
+        be[k] = 0.0;
+        ah[k] = av[k] * inv;
+        ch[k] = cv[k] * inv;
+      } else {
+        const int km = k > 0 ? k - 1 : 0;
+        const double r = 1.0 / (bv[k] - av[k] * ch[km]);
+        al[k] = r;
+        be[k] = r * av[k];
+        ah[k] = -be[k] * ah[km];
+        ch[k] = r * cv[k];
+      }
+      ga[k] = ch[k];
+    }
+#pragma unroll
+    for (int k = M - 3; k >= 1; --k) {
+      ah[k] = ah[k] - ga[k] * ah[k + 1];
+      ch[k] = -ga[k] * ch[k + 1];
+    }
+    rho = 1.0 / (1.0 - ga[0] * ah[1]);
+    ah[0] = rho * ah[0];
+    ch[0] = -rho * ga[0] * ch[1];
+  }
+
+  // ---- PCR multipliers of the reduced system: lane p holds equations
+  // 2p (first row: ah_0, 1, ch_0) and 2p+1 (last row: ah_{M-1}, 1, ch_{M-1})
+  double k1[LEV][2], k2[LEV][2], invB[2];
+  {
+    double RA[2] = {ah[0], ah[M - 1]}, RB[2] = {1.0, 1.0}, RC[2] = {ch[0], ch[M - 1]};
+#pragma unroll
+    for (int lev = 0; lev < LEV; ++lev) {
+      double mA[2], mB[2], mC[2], pA[2], pB[2], pC[2];
+      if (lev == 0) {
+        // equation 2p: neighbours 2p-1 (lane p-1, eq 1) and 2p+1 (own eq 1);
+        // equation 2p+1: 2p (own eq 0) and 2p+2 (lane p+1, eq 0)
+        const double uA = __shfl_up(RA[1], 1, LP), uB = __shfl_up(RB[1], 1, LP), uC = __shfl_up(RC[1], 1, LP);
+        const double dA = __shfl_down(RA[0], 1, LP), dB = __shfl_down(RB[0], 1, LP), dC = __shfl_down(RC[0], 1, LP);
+        const bool lo = p >= 1, hi = p + 1 < LP;
+        mA[0] = lo ? uA : 0.0; mB[0] = lo ? uB : 1.0; mC[0] = lo ? uC : 0.0;
+        pA[0] = RA[1]; pB[0] = RB[1]; pC[0] = RC[1];
+        mA[1] = RA[0]; mB[1] = RB[0]; mC[1] = RC[0];
+        pA[1] = hi ? dA : 0.0; pB[1] = hi ? dB : 1.0; pC[1] = hi ? dC : 0.0;
+      } else {
+        const int h = 1 << (lev - 1);
+        const bool lo = p >= h, hi = p + h < LP;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const double uA = __shfl_up(RA[j], h, LP), uB = __shfl_up(RB[j], h, LP), uC = __shfl_up(RC[j], h, LP);
+          const double dA = __shfl_down(RA[j], h, LP), dB = __shfl_down(RB[j], h, LP), dC = __shfl_down(RC[j], h, LP);
+          mA[j] = lo ? uA : 0.0; mB[j] = lo ? uB : 1.0; mC[j] = lo ? uC : 0.0;
+          pA[j] = hi ? dA : 0.0; pB[j] = hi ? dB : 1.0; pC[j] = hi ? dC : 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        k1[lev][j] = RA[j] / mB[j];
+        k2[lev][j] = RC[j] / pB[j];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const double nB = RB[j] - k1[lev][j] * mC[j] - k2[lev][j] * pA[j];
+        RA[j] = -k1[lev][j] * mA[j];
+        RC[j] = -k2[lev][j] * pC[j];
+        RB[j] = nB;
+      }
+    }
+    invB[0] = 1.0 / RB[0];
+    invB[1] = 1.0 / RB[1];
+  }
+
+  // ---- tile staging: elements f of the tile's T lines x n active cells
+  constexpr int PF = M;  // ceil(T * NPAD / NT)
+  // element f: (line l_, cell index i_) with consecutive f along the
+  // direction that is contiguous in memory
+  auto elem_l = [&](int f) { return CONTIG ? f / n : f % T; };
+  auto elem_i = [&](int f) { return CONTIG ? f % n : f / T; };
+  const int nel = T * n;
+  double pf[PF];
+  auto issue = [&](int u) {
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+      const int f = tid + NT * m;
+      const int l_ = elem_l(f), i_ = elem_i(f);
+      const bool ok = f < nel && a_first + l_ <= g.Na;
+      const long long o = Q.st0 + Q.sta * (a_first + l_) + Q.stb * b + Q.str * (long long)(rmin + i_) +
+                          ncell * (long long)u;
+      pf[m] = A.c0[ok ? o : 0];
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+      const int f = tid + NT * m;
+      if (f < nel) s_tile[elem_l(f) * PITCH + elem_i(f)] = pf[m];
+    }
+  };
+
+  double *trow = s_tile + ln * PITCH;
+  // lines whose matrix is not strictly diagonally dominant (e.g. negative
+  // diffusivities) are solved by their first lane in dgtsv's order, from
+  // the cached LU factors, bitwise as the sequential kernels
+  const bool dom = !line_on || A.ddom[(long long)(a - 1) + (long long)g.Na * (b - 1)] != 0;
+  const long long lbase = Q.lr0 + Q.lra * ac + Q.lrb * b;  // row factor / LU offset of cell 0
+  const double *fF = A.lu, *fD = A.lu + Q.lr_ncell, *fDU = fD + Q.lr_ncell, *fDL = fDU + Q.lr_ncell;
+  int u = u0;
+  while (u < u1 && !A.aloft.on(u)) ++u;
+  if (u < u1) issue(u);
+  for (int w = u0; w < u1; ++w) {
+    const bool on = A.aloft.on(w);
+    __syncthreads();  // the previous bin's tile has been stored
+    if (on) {
+      commit();
+      __syncthreads();
+      int un = w + 1;
+      while (un < u1 && !A.aloft.on(un)) ++un;
+      if (un < u1) issue(un);
+      if (!dom) {
+        if (p == 0) {
+          double ql = trow[0], qc = trow[0], Bc = 0.0;
+          for (int l = 1; l <= n; ++l) {
+            const long long o = lbase + Q.lrr * (long long)(rmin + l - 1);
+            const double LeftFac = A.lr[o], RightFac = A.lr[o + Q.lr_ncell];
+            const double CenterFac = LeftFac + RightFac;
+            const double qr = (l < n) ? trow[l] : qc;  // BC_right_t0 (:845)
+            const double Bn = (1.0 - th) * LeftFac * ql + (1.0 - (1.0 - th) * CenterFac) * qc +
+                              (1.0 - th) * RightFac * qr;
+            if (l == 1) {
+              Bc = Bn;
+            } else {
+              const long long fo = o - Q.lrr;  // row l-1
+              const double fact = fF[fo];
+              const bool pv = A.piv[fo] != 0;
+              const double y = pv ? Bn : Bc;
+              const double xx = pv ? Bc : Bn;
+              trow[l - 2] = y;
+              Bc = xx - fact * y;
+            }
+            ql = qc;
+            qc = qr;
+          }
+          double x1 = Bc / fD[lbase + Q.lrr * (long long)(rmin + n - 1)], x2 = 0.0;
+          trow[n - 1] = x1;
+          for (int i = n - 1; i >= 1; --i) {
+            const long long fo = lbase + Q.lrr * (long long)(rmin + i - 1);
+            const double Bi = trow[i - 1];
+            const double xi = (i == n - 1) ? (Bi - fDU[fo] * x1) / fD[fo]
+                                           : (Bi - fDU[fo] * x1 - fDL[fo] * x2) / fD[fo];
+            x2 = x1;
+            x1 = xi;
+            trow[i - 1] = xi;
+          }
+        }
+      } else {
+      // ---- solve this lane's segment of its line
+      double q[M + 2];
+#pragma unroll
+      for (int k = 0; k < M + 2; ++k) {
+        const int i = p * M + k - 1;
+        q[k] = trow[min(max(i, 0), n - 1)];
+      }
+      double dh[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const double d = (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
+        dh[k] = al[k] * d;
+        if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
+      }
+#pragma unroll
+      for (int k = M - 3; k >= 1; --k) dh[k] = dh[k] - ga[k] * dh[k + 1];
+      dh[0] = rho * (dh[0] - ga[0] * dh[1]);
+      double D[2] = {dh[0], dh[M - 1]};
+#pragma unroll
+      for (int lev = 0; lev < LEV; ++lev) {
+        double mD[2], pD[2];
+        if (lev == 0) {
+          const double uD = __shfl_up(D[1], 1, LP), dD = __shfl_down(D[0], 1, LP);
+          mD[0] = p >= 1 ? uD : 0.0;
+          pD[0] = D[1];
+          mD[1] = D[0];
+          pD[1] = p + 1 < LP ? dD : 0.0;
+        } else {
+          const int h = 1 << (lev - 1);
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const double uD = __shfl_up(D[j], h, LP), dD = __shfl_down(D[j], h, LP);
+            mD[j] = p >= h ? uD : 0.0;
+            pD[j] = p + h < LP ? dD : 0.0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) D[j] = D[j] - k1[lev][j] * mD[j] - k2[lev][j] * pD[j];
+      }
+      const double xf = D[0] * invB[0], xl = D[1] * invB[1];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int i = p * M + k;
+        const double x = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
+        if (i < n) trow[i] = x;
+      }
+      }
+      __syncthreads();
+    }
+    // ---- store cells 1..N of the tile's lines: the solution on active
+    // cells of active lines, zero elsewhere (as the reference leaves them)
+    const int nst = T * N;
+    for (int f = tid; f < nst; f += NT) {
+      const int l_ = CONTIG ? f / N : f % T;
+      const int r_ = (CONTIG ? f % N : f / T) + 1;
+      const int al_ = a_first + l_;
+      if (al_ > g.Na) continue;
+      const bool lon = al_ >= g.a0 && al_ <= g.a1 && b >= g.b0 && b <= g.b1;
+      const bool act = on && lon && r_ >= rmin && r_ <= rmax;
+      const double v = act ? s_tile[l_ * PITCH + (r_ - rmin)] : 0.0;
+      A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)r_ + ncell * (long long)w] = v;
+    }
+  }
+}
+
+template <int LP, int M, int T, bool CONTIG>
+hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
+  const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
+  const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
+  const size_t lds = sizeof(double) * (size_t)T * (LP * M + 1);
+  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
+                     q);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg) {
+  const int n = a.g.rmax - a.g.rmin + 1;
+  PartArgs q;
+  q.st0 = lay.st0; q.sta = lay.sta; q.stb = lay.stb; q.str = lay.str;
+  q.lr0 = lay.lr0; q.lra = lay.lra; q.lrb = lay.lrb; q.lrr = lay.lrr; q.lr_ncell = lay.lr_ncell;
+  q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
+  if (n < 3 || n > 512) return hipErrorInvalidValue;
+  // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T
+  if (lay.str == 1) {
+    if (n <= 256) return launch_part<64, 4, 4, true>(s, a, q);
+    if (n <= 384) return launch_part<64, 6, 4, true>(s, a, q);
+    return launch_part<64, 8, 4, true>(s, a, q);
+  }
+  if (n <= 64) return launch_part<16, 4, 16, false>(s, a, q);
+  if (n <= 128) return launch_part<32, 4, 16, false>(s, a, q);
+  if (n <= 256) return launch_part<64, 4, 8, false>(s, a, q);
+  return launch_part<64, 8, 4, false>(s, a, q);
+}
+
+}  // namespace ash3d

@@ -421,11 +421,15 @@ class Transport:
 
     # -- launch tuning ----------------------------------------------------
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z",
-                   "x_tile")
+                   "x_tile", "line_solver", "part_bins")
+    LINE_SOLVERS = ("exact", "partitioned")
 
     def tune(self, **kw):
-        """Set bins-per-thread launch parameters, e.g. tune(nb_advect_z=2)."""
+        """Set launch parameters, e.g. tune(nb_advect_z=2); line_solver takes
+        "exact" (dgtsv order, bitwise the reference) or "partitioned"."""
         for name, value in kw.items():
+            if name == "line_solver" and isinstance(value, str):
+                value = self.LINE_SOLVERS.index(value)
             self._check(self.lib.ash3d_tr_tune(self._ctx, self.TUNE_PARAMS.index(name), int(value)))
 
     # -- kernel timing ----------------------------------------------------
