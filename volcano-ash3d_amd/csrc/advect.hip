@@ -31,6 +31,7 @@ constexpr int kBlock = 256;
 // never re-read from memory.
 template <int DIR, int NB, bool TOPO>
 __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
+  constexpr int PF = (NB >= 2) ? 2 : 4;  // cells in flight
   const Dims d = A.d;
   const LineGeom g = A.g;
   const int ntile = (g.Na + 63) / 64;
@@ -48,8 +49,10 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const long long base = lane_ok ? line_base(d, DIR, a, b) : 0;
   const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *c = A.c0;
+  double *junk = A.junk + (threadIdx.x + 64 * threadIdx.y + 256 * (blockIdx.x % (kJunk / 256)));
 
   bool exists[NB], on[NB];
+  long long bofs[NB];  // bin offset (an existing bin for the others)
   bool any_on = false;
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -57,6 +60,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
     exists[u] = lane_ok && n < d.ns;
     on[u] = exists[u] && line_on && A.aloft.on(n);
     any_on = any_on || on[u];
+    bofs[u] = d.ncell * (long long)min(n, d.ns - 1);
   }
   // outflow entry of this line: outflow(a,b,n) with the transverse extents
   const long long ofs0 = (long long)(a + 1) + (long long)(g.Na + 4) * (long long)(b + 1);
@@ -67,9 +71,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   // that they stay in registers)
   double hq0[NB], hq1[NB], hq2[NB], hq3[NB];
   auto halo = [&](int r, int u) {
-    return (any_on && exists[u] && r >= ldlo && r <= ldhi)
-               ? c[base + sL * (long long)(r + 1) + d.ncell * (long long)(n0 + u)]
-               : 0.0;
+    return (any_on && exists[u] && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
   };
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -79,6 +81,12 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
     hq3[u] = halo(hi + 2, u);
   }
   if (W > 1) __syncthreads();
+
+  // outflow increments of the line ends, applied after the sweep
+  double upd_lo[NB], upd_hi[NB];
+  bool has_lo = false, has_hi = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) upd_lo[u] = upd_hi[u] = 0.0;
 
   if (any_on) {
     const int ncells = rmax - rmin + 1;
@@ -93,139 +101,144 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
       q0[u] = q1[u] = q2[u] = dq0[u] = dq1[u] = dq2[u] = fsr2[u] = fss2[u] = tp[u] = 0.0;
       vb1[u] = ub0[u] = ub1[u] = ub2[u] = 0.0;
     }
-    // ---- gather of cell r (:143-146 / :171-176), issued one cell ahead so
-    // the loads of cell r+1 are in flight while cell r is computed
-    double pf_sig = 0.0, pf_kap = 1.0, pf_vel = 0.0, pf_D = 0.0, pf_q[NB], pf_vf[NB];
-    auto gather = [&](int r) {
-      const bool ld = (r >= ldlo && r <= ldhi);
-      const long long cell = base + sL * (long long)(r + 1);
-      pf_sig = 0.0; pf_kap = 1.0; pf_vel = 0.0; pf_D = 0.0;
-      if (ld) {
-        pf_sig = A.sig[cell];
-        pf_kap = A.kap[cell];
-        if (DIR == 2) pf_vel = A.vel[cell] + A.vh[cell];
-        else pf_vel = A.vel[cell];
-        // DelDxonD_cc(r,j) for x lines, DelDyonD_cc(i,r) for y lines (:163-166, :414-417)
-        if (TOPO && r >= rmin - 1 && r <= rmax + 1)
-          pf_D = (DIR == 0) ? A.deld[(long long)(r + 1) + (long long)(d.nx + 4) * (a + 1)]
-                            : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (r + 1)];
+    // ---- gather of cell r (:143-146 / :171-176): a ring of PF cells in
+    // flight, raw loads from the cell clamped into the gathered range (the
+    // values outside it are replaced when the cell is consumed)
+    double rs_sig[PF], rs_kap[PF], rs_vel[PF], rs_vh[PF], rs_D[PF], rs_q[PF][NB], rs_vf[PF][NB];
+    auto fetch = [&](int slot, int r) {
+      const int rr = min(max(r, ldlo), ldhi);
+      const long long cell = base + sL * (long long)(rr + 1);
+      rs_sig[slot] = A.sig[cell];
+      rs_kap[slot] = A.kap[cell];
+      rs_vel[slot] = A.vel[cell];
+      if (DIR == 2) rs_vh[slot] = A.vh[cell];
+      // DelDxonD_cc(r,j) for x lines, DelDyonD_cc(i,r) for y lines (:163-166, :414-417)
+      if (TOPO) {
+        const int rd = min(max(r, rmin - 1), rmax + 1);
+        rs_D[slot] = (DIR == 0) ? A.deld[(long long)(rd + 1) + (long long)(d.nx + 4) * (a + 1)]
+                                : A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (rd + 1)];
       }
-      const bool halo_cell = (r == lo - 2) || (r == lo - 1) || (r == hi + 1) || (r == hi + 2);
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        pf_q[u] = 0.0;
-        pf_vf[u] = 0.0;
-        if (!halo_cell && ld && exists[u]) pf_q[u] = c[cell + d.ncell * (long long)(n0 + u)];
-        if (DIR == 2 && ld && exists[u]) pf_vf[u] = A.vf[cell + d.ncell * (long long)(n0 + u)];
+        rs_q[slot][u] = c[cell + bofs[u]];
+        if (DIR == 2) rs_vf[slot][u] = A.vf[cell + bofs[u]];
       }
     };
-    gather(lo - 2);
-    for (int r = lo - 2; r <= hi + 2; ++r) {
-      const bool ld = (r >= ldlo && r <= ldhi);
-      const double sig0 = pf_sig, kap0 = pf_kap, D0 = pf_D;
-      const double vel0 = (DIR == 2) ? 0.0 : pf_vel, vzh0 = (DIR == 2) ? pf_vel : 0.0;
-      double vfc[NB];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        // the halo cells come from the registers filled before the barrier
-        double v = pf_q[u];
-        v = (r == lo - 2) ? hq0[u] : v;
-        v = (r == lo - 1) ? hq1[u] : v;
-        v = (r == hi + 1) ? hq2[u] : v;
-        v = (r == hi + 2) ? hq3[u] : v;
-        q0[u] = v;
-        vfc[u] = pf_vf[u];
-      }
-      if (r + 1 <= hi + 2) gather(r + 1);
-      const bool uin = (r >= rmin - 1 && r <= rmax + 1);
-      // dt/kappa (:155-156) -- bin independent
-      const double dtv0 = uin ? dt / kap0 : 0.0;
-      // usig (:159-162)
-      if (DIR != 2) u0 = uin ? 0.5 * (vel1 + vel0) * sig0 : 0.0;
-      double vb0[NB];
+    for (int p = 0; p < PF; ++p) fetch(p, lo - 2 + p);
+    // whole turns of the ring: cells past hi+2 update nothing (their
+    // stores go to junk and no outflow is recorded)
+    for (int r0 = lo - 2; r0 <= hi + 2; r0 += PF) {
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        vb0[u] = 0.0;
-        if (DIR == 2) {
-          if (ld && exists[u]) vb0[u] = vzh0 + vfc[u];
-          ub0[u] = uin ? 0.5 * (vb1[u] + vb0[u]) * sig0 : 0.0;
+      for (int p = 0; p < PF; ++p) {
+        const int r = r0 + p;
+        const bool ld = (r >= ldlo && r <= ldhi);
+        const double sig0 = ld ? rs_sig[p] : 0.0, kap0 = ld ? rs_kap[p] : 1.0;
+        const double vel0 = (DIR == 2 || !ld) ? 0.0 : rs_vel[p];
+        const double vzh0 = (DIR == 2 && ld) ? rs_vel[p] + rs_vh[p] : 0.0;
+        const double D0 = (TOPO && ld && r >= rmin - 1 && r <= rmax + 1) ? rs_D[p] : 0.0;
+        double vfc[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          // the halo cells come from the registers filled before the barrier
+          double v = (ld && exists[u]) ? rs_q[p][u] : 0.0;
+          v = (r == lo - 2) ? hq0[u] : v;
+          v = (r == lo - 1) ? hq1[u] : v;
+          v = (r == hi + 1) ? hq2[u] : v;
+          v = (r == hi + 2) ? hq3[u] : v;
+          q0[u] = v;
+          vfc[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
         }
-        dq0[u] = q0[u] - q1[u];  // (:168-169), used only for r >= rmin-1
-      }
-      // ---- interface I = r-1
-      const int I = r - 1;
-      const bool fss_in = (lim != 0) && I >= rmin && I <= rmax + 1;
-      const bool fs_in = I >= rmin - 1 && I <= rmax + 1;
-      // ---- cell c = r-2
-      const int cc = r - 2;
-      const bool upd_in = cc >= lo && cc <= hi && cc >= rmin - 1 && cc <= rmax + 1;
+        fetch(p, r + PF);
+        const bool uin = (r >= rmin - 1 && r <= rmax + 1);
+        // dt/kappa (:155-156) -- bin independent
+        const double dtv0 = uin ? dt / kap0 : 0.0;
+        // usig (:159-162)
+        if (DIR != 2) u0 = uin ? 0.5 * (vel1 + vel0) * sig0 : 0.0;
+        double vb0[NB];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) {
-        const double uI = (DIR == 2) ? ub1[u] : u1;
-        const double uIm = (DIR == 2) ? ub2[u] : u2;
-        const double uIp = (DIR == 2) ? ub0[u] : u0;
-        double fssI = 0.0, fsrI = 0.0, fslI = 0.0, TI = 0.0;
-        if (fss_in) fssI = limited_flux(lim, dq2[u], dq1[u], dq0[u], uI, dtv1);
-        if (fs_in) {
-          // (:241-249)
-          fsrI = fmax(0.0, uI) * dq1[u];
-          fslI = fmin(0.0, uI) * dq1[u];
-          const double divu_p = (fmax(0.0, uI) - fmax(0.0, uIm));
-          const double divu_m = (fmin(0.0, uIp) - fmin(0.0, uI));
-          fsrI = fsrI + q1[u] * divu_m;
-          fslI = fslI + q2[u] * divu_p;
-          if (TOPO) {
-            // topographic term (:252-257): T_I on the right fluctuation of
-            // interface I, T_{I-1} on the left one
-            TI = 0.5 * vel1 * (sig1 + sig0) * q1[u] * D1;
-            fsrI = fsrI + TI;
-            fslI = fslI - tp[u];
+        for (int u = 0; u < NB; ++u) {
+          vb0[u] = 0.0;
+          if (DIR == 2) {
+            if (ld && exists[u]) vb0[u] = vzh0 + vfc[u];
+            ub0[u] = uin ? 0.5 * (vb1[u] + vb0[u]) * sig0 : 0.0;
           }
+          dq0[u] = q0[u] - q1[u];  // (:168-169), used only for r >= rmin-1
         }
-        if (upd_in && on[u]) {
-          // (:264-281)
+        // ---- interface I = r-1
+        const int I = r - 1;
+        const bool fss_in = (lim != 0) && I >= rmin && I <= rmax + 1;
+        const bool fs_in = I >= rmin - 1 && I <= rmax + 1;
+        // ---- cell c = r-2
+        const int cc = r - 2;
+        const bool upd_in = cc >= lo && cc <= hi && cc >= rmin - 1 && cc <= rmax + 1;
+        const bool st_in = upd_in && cc >= rmin && cc <= rmax;
+        const bool at_lo = upd_in && cc == 0 && rmin == 1 && (DIR != 0 || !A.periodic);
+        const bool at_hi = upd_in && rmax == N && (DIR != 0 || !A.periodic) &&
+                           cc == ((DIR == 2) ? ncells + 1 : N + 1);
+        has_lo = has_lo || at_lo;
+        has_hi = has_hi || at_hi;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const double uI = (DIR == 2) ? ub1[u] : u1;
+          const double uIm = (DIR == 2) ? ub2[u] : u2;
+          const double uIp = (DIR == 2) ? ub0[u] : u0;
+          double fssI = 0.0, fsrI = 0.0, fslI = 0.0, TI = 0.0;
+          if (fss_in) fssI = limited_flux(lim, dq2[u], dq1[u], dq0[u], uI, dtv1);
+          if (fs_in) {
+            // (:241-249)
+            fsrI = fmax(0.0, uI) * dq1[u];
+            fslI = fmin(0.0, uI) * dq1[u];
+            const double divu_p = (fmax(0.0, uI) - fmax(0.0, uIm));
+            const double divu_m = (fmin(0.0, uIp) - fmin(0.0, uI));
+            fsrI = fsrI + q1[u] * divu_m;
+            fslI = fslI + q2[u] * divu_p;
+            if (TOPO) {
+              // topographic term (:252-257): T_I on the right fluctuation of
+              // interface I, T_{I-1} on the left one
+              TI = 0.5 * vel1 * (sig1 + sig0) * q1[u] * D1;
+              fsrI = fsrI + TI;
+              fslI = fslI - tp[u];
+            }
+          }
+          // (:264-281); every cell issues one store (masked-off ones to junk)
           const double upd = -dtv2 * (fslI + fsr2[u] + fssI - fss2[u]);
-          const long long cellc = base + sL * (long long)(cc + 1) + d.ncell * (long long)(n0 + u);
-          if (cc >= rmin && cc <= rmax) c[cellc] = q2[u] + upd;
-          const long long of = ofs0 + ofs_bin * (long long)(n0 + u);
-          if (DIR == 0) {
-            // (:289-294) x outflow, skipped when periodic
-            if (cc == 0 && rmin == 1 && !A.periodic) A.of1[of] = A.of1[of] + upd;
-            if (cc == N + 1 && rmax == N && !A.periodic) A.of2[of] = A.of2[of] + upd;
-          } else if (DIR == 1) {
-            // (:540-545)
-            if (cc == 0 && rmin == 1) A.of1[of] = A.of1[of] + upd;
-            if (cc == N + 1 && rmax == N) A.of2[of] = A.of2[of] + upd;
-          } else {
-            // deposit and top outflow (src/AdvectionVert_DCU.f90:306-311);
-            // the top uses update_cc(ncells+1)
-            if (cc == 0 && rmin == 1) A.of1[of] = A.of1[of] + upd;
-            if (cc == ncells + 1 && rmax == N) A.of2[of] = A.of2[of] + upd;
+          *((st_in && on[u]) ? c + base + sL * (long long)(cc + 1) + bofs[u] : junk) = q2[u] + upd;
+          // x outflow skipped when periodic (:289-294); y (:540-545); deposit
+          // and top outflow of z, the top from update_cc(ncells+1)
+          // (src/AdvectionVert_DCU.f90:306-311)
+          if (at_lo) upd_lo[u] = upd;
+          if (at_hi) upd_hi[u] = upd;
+          // shift the per-bin window
+          fsr2[u] = fsrI;
+          fss2[u] = fssI;
+          tp[u] = TI;
+          q2[u] = q1[u];
+          q1[u] = q0[u];
+          dq2[u] = dq1[u];
+          dq1[u] = dq0[u];
+          if (DIR == 2) {
+            ub2[u] = ub1[u];
+            ub1[u] = ub0[u];
+            vb1[u] = vb0[u];
           }
         }
-        // shift the per-bin window
-        fsr2[u] = fsrI;
-        fss2[u] = fssI;
-        tp[u] = TI;
-        q2[u] = q1[u];
-        q1[u] = q0[u];
-        dq2[u] = dq1[u];
-        dq1[u] = dq0[u];
-        if (DIR == 2) {
-          ub2[u] = ub1[u];
-          ub1[u] = ub0[u];
-          vb1[u] = vb0[u];
-        }
+        // shift the bin-independent window
+        u2 = u1;
+        u1 = u0;
+        dtv2 = dtv1;
+        dtv1 = dtv0;
+        vel1 = vel0;
+        sig1 = sig0;
+        D1 = D0;
       }
-      // shift the bin-independent window
-      u2 = u1;
-      u1 = u0;
-      dtv2 = dtv1;
-      dtv1 = dtv0;
-      vel1 = vel0;
-      sig1 = sig0;
-      D1 = D0;
     }
+  }
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const long long of = ofs0 + ofs_bin * (long long)(n0 + u);
+    if (has_lo && on[u]) A.of1[of] = A.of1[of] + upd_lo[u];
+    if (has_hi && on[u]) A.of2[of] = A.of2[of] + upd_hi[u];
   }
   // cells the reference leaves at zero: inactive bins and lines, and cells
   // outside [rmin, rmax] (zero fill of ts1 then copy, :122, :302-303)
@@ -233,9 +246,8 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (!exists[u]) continue;
-    const long long bofs = d.ncell * (long long)(n0 + u);
     for (int r = zlo; r <= zhi; ++r)
-      if (!on[u] || r < rmin || r > rmax) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+      if (!on[u] || r < rmin || r > rmax) c[base + sL * (long long)(r + 1) + bofs[u]] = 0.0;
     if (DIR == 2 && w == 0) {
       // DepositGranularity = outflow_xy1 (src/AdvectionVert_DCU.f90:321);
       // segment 0 owns the bottom ghost update
@@ -279,9 +291,9 @@ hipError_t launch_nb(hipStream_t s, const AdvArgs &a) {
 // Default bins per thread (kind 0 advection, 1 diffusion; dir 0..2):
 // more bins share the per-cell terms, fewer keep more waves resident.
 int pick_bins_per_thread(int ns, int kind, int dir) {
-  // measured on the 360x240x50x15 forecast grid (bench autotuner): y
-  // advection 2, z advection 1, x line solves (row tiles) 2, y/z line solves 8
-  const int cap = (kind == 0) ? (dir == 2 ? 1 : 2) : (dir == 0 ? 2 : 8);
+  // measured on the 360x240x50x15 forecast grid (bench autotuner): y and z
+  // advection 2, x line solves (row tiles) 2, y/z line solves 8
+  const int cap = (kind == 0) ? 2 : (dir == 0 ? 2 : 8);
   int nb = 1;
   while (nb * 2 <= cap && nb * 2 <= ns) nb *= 2;
   return nb;

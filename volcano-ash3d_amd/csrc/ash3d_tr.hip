@@ -237,6 +237,7 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
   a.limiter = x->limiter;
   make_mask(x, is_aloft, &a.aloft);
   a.kap = x->kap;
+  a.junk = x->junk;
   if (dir == 0) {
     a.vel = x->vx; a.sig = x->sx; a.of1 = x->ofyz1; a.of2 = x->ofyz2;
     a.deld = (x->zscaling == 2) ? x->ddx : nullptr;
@@ -605,7 +606,8 @@ int ash3d_tr_create(const ash3d_tr_desc *desc, ash3d_tr_ctx **out) {
     g_err = msg;
     return rc;
   }
-  if (hipMalloc((void **)&x->flag, sizeof(int)) != hipSuccess || hipMemset(x->flag, 0, sizeof(int)) != hipSuccess) {
+  if (hipMalloc((void **)&x->flag, sizeof(int)) != hipSuccess || hipMemset(x->flag, 0, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&x->junk, sizeof(double) * kJunk) != hipSuccess) {
     ash3d_tr_destroy(x);
     return fail(ASH3D_TR_ENOMEM, "flag allocation failed");
   }

@@ -95,6 +95,8 @@ __device__ inline TileGroup xcd_tile_group(long long b, int G) {
   return TileGroup{(m / G) * kXcds + x, (int)(m % G)};
 }
 
+constexpr int kJunk = 4096;
+
 struct AdvArgs {
   Dims d;
   LineGeom g;
@@ -107,6 +109,7 @@ struct AdvArgs {
   const double *deld;  // DelD{x,y}onD_cc (x,y with ZScaling_ID == 2)
   double *of1, *of2;   // outflow on the low / high side of the line
   double *dep;         // DepositGranularity (z)
+  double *junk;        // kJunk doubles that masked-off lanes store to
   double dt;
   int periodic;
   int limiter;
@@ -138,7 +141,6 @@ struct DiffArgs {
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
   AloftMask aloft;
 };
-constexpr int kJunk = 4096;
 
 // launchers (defined next to their kernels)
 hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double *vx, double *vy,
