@@ -56,10 +56,12 @@ def step_bytes_per_bincell(st, use_diffusion):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_forecast_360x240x50x15.md")
 
 
-def measured_traffic(kind, st, tuning):
+def measured_traffic(kind, st, tuning, line_solver, defaults):
     """HBM bytes per launch of a kernel kind from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
-    on this grid with the same bins per thread; else None."""
+    on this grid with the same kernel: the same line solver for the
+    diffusion sweeps, the same bins per thread for the streamed advection
+    (tuned, else the library default); else None."""
     if (st.nx, st.ny, st.nz, st.ns) != (360, 240, 50, 15) or not os.path.exists(TRAFFIC_FILE):
         return None
     text = open(TRAFFIC_FILE).read()
@@ -67,9 +69,15 @@ def measured_traffic(kind, st, tuning):
     rec = json.loads(block).get(kind)
     if rec is None:
         return None
-    tuned = (tuning or {}).get("nb_" + kind, [None])[0]
-    if rec["bins_per_thread"] is not None and tuned not in (None, rec["bins_per_thread"]):
+    if kind.startswith("diffuse") and (rec.get("line_solver") != line_solver or
+                                       (tuning or {}).get("part_bins", [0])[0] not in (0, None)):
         return None
+    if rec["bins_per_thread"] is not None:
+        nb = (tuning or {}).get("nb_" + kind, [None])[0]
+        if nb is None:
+            nb = defaults.get(kind)
+        if nb != rec["bins_per_thread"]:
+            return None
     return rec["traffic_bytes"]
 
 
@@ -269,7 +277,7 @@ def main():
     tuning = None
     if not args.no_tune:
         at = importlib.import_module("volcano-ash3d_amd.autotune")
-        tuning = at.autotune(T, use_diffusion)
+        tuning = at.autotune(T, use_diffusion, line_solver=args.line_solver)
         log("[bench] autotune: " + json.dumps({k: [v[0], {str(a): round(b, 4) for a, b in (v[1] or {}).items()}]
                                                 for k, v in tuning.items()}))
         itime += 1000
@@ -320,13 +328,14 @@ def main():
                    key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dominant]
     achieved = dk["gbps"]
-    traffic = measured_traffic(dominant, st, tuning)
+    defaults = {"advect_y": 2, "advect_z": 2}  # the library's bins per thread on this grid
+    traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "avg_ms": dk["avg_ms"], "algorithmic_bytes": algorithmic_bytes(dominant, st),
                 "traffic_source": "profiles/hbm_traffic_forecast_360x240x50x15.md" if traffic else None}
     for k, v in kernels.items():
-        t = measured_traffic(k, st, tuning)
+        t = measured_traffic(k, st, tuning, args.line_solver, defaults)
         if t:
             v["traffic_gbps"] = round(t / (v["avg_ms"] * 1e-3) / 1e9, 1)
     log("[bench] kernels: " + json.dumps(kernels))

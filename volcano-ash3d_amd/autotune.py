@@ -15,6 +15,9 @@ import os
 PARAMS = (("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("nb_diffuse_x", "diffuse_x"),
           ("x_tile", "diffuse_x"), ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
 X_TILES = (8, 16, 32)
+# the partitioned line solves have one setting for the three directions:
+# bins per workgroup (0: all bins of the tile's lines in one workgroup)
+PART_BINS = (0, 8, 4)
 
 
 def _key(st):
@@ -22,7 +25,10 @@ def _key(st):
 
 
 def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), itime0: int = 1000,
-             cache_path: str | None = None) -> dict:
+             cache_path: str | None = None, line_solver: str = "exact") -> dict:
+    """line_solver: the solver the context runs ("exact" tunes the
+    dgtsv-order kernels' bins per thread and x tile, "partitioned" the
+    bins per workgroup of the partitioned solves)."""
     st = T.st
     if cache_path and os.path.exists(cache_path):
         cached = json.load(open(cache_path)).get(_key(st))
@@ -31,12 +37,17 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), it
             return {p: (v, None) for p, v in cached.items()}
     chosen = {}
     itime = itime0
-    for param, kind in PARAMS:
-        if kind.startswith("diffuse") and not use_diffusion:
+    params = [p for p in PARAMS if not (line_solver == "partitioned" and p[1].startswith("diffuse"))]
+    if line_solver == "partitioned":
+        params.append(("part_bins", ("diffuse_x", "diffuse_y", "diffuse_z")))
+    for param, kind in params:
+        kinds = kind if isinstance(kind, tuple) else (kind,)
+        if kinds[0].startswith("diffuse") and not use_diffusion:
             continue
+        values = X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else candidates
         times = {}
-        for nb in (X_TILES if param == "x_tile" else candidates):
-            if param != "x_tile" and nb > 1 and nb // 2 >= st.ns:
+        for nb in values:
+            if param not in ("x_tile", "part_bins") and nb > 1 and nb // 2 >= st.ns:
                 continue
             T.tune(**{param: nb})
             T.profile(True)
@@ -46,8 +57,7 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), it
                 T.transport_step(itime, use_diffusion)
             prof = T.profile_read()
             T.profile(False)
-            ms, n = prof[kind]
-            times[nb] = ms / max(n, 1)
+            times[nb] = sum(prof[k][0] / max(prof[k][1], 1) for k in kinds)
         best = min(times, key=times.get)
         T.tune(**{param: best})
         chosen[param] = (best, times)

@@ -76,12 +76,12 @@ struct ash3d_tr_ctx {
   bool lr_ok[3] = {false, false, false};
   // LU factors of the theta-scheme lines per direction (one allocation:
   // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags, one
-  // INFO per line and one diagonal-dominance flag per line), valid for the
-  // key (dt, theta, rmin, rmax)
+  // INFO per line and the count of lines that are not diagonally
+  // dominant), valid for the key (dt, theta, rmin, rmax)
   double *lu[3] = {nullptr, nullptr, nullptr};
   uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
   int *lu_sing[3] = {nullptr, nullptr, nullptr};
-  uint8_t *lu_ddom[3] = {nullptr, nullptr, nullptr};
+  int *lu_ndom[3] = {nullptr, nullptr, nullptr};
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
@@ -311,15 +311,15 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   int rc = 0;
   const long long nc = x->d.ncell, lines = (long long)a.g.Na * a.g.Nb;
   if (!x->lu[dir]) {
-    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + (lines + 7) / 8))) return rc;
+    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 1))) return rc;
     x->lu_piv[dir] = (uint8_t *)(x->lu[dir] + 4 * nc);
     x->lu_sing[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8);
-    x->lu_ddom[dir] = (uint8_t *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2);
+    x->lu_ndom[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2);
   }
   if (!x->junk && (rc = dev_alloc(&x->junk, kJunk))) return rc;
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
-    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ddom[dir]));
+    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir]));
     x->lu_ok[dir] = true;
     x->lu_dt[dir] = a.dt;
     x->lu_th[dir] = a.theta;
@@ -329,7 +329,8 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   a.lu = x->lu[dir];
   a.piv = x->lu_piv[dir];
   a.sing = x->lu_sing[dir];
-  a.ddom = x->lu_ddom[dir];
+  a.ndom = x->lu_ndom[dir];
+  a.sel = 0;
   a.junk = x->junk;
   x->lu_args[dir] = a;
   x->lu_args_set[dir] = true;
@@ -347,12 +348,16 @@ int ensure_fac(ash3d_tr_ctx *x, int nb) {
 }
 
 // Partitioned line solves of one direction when selected and the active
-// line length allows (3..512 cells); returns 1 when they ran.  a holds the
-// sweep's geometry (for x: the x<->y transposed one, which the row factor
-// cache uses) and the original state in a.c0.
+// line length allows (3..512 cells); returns 1 when they were launched, and
+// then sets a.sel = 2 so that the dgtsv-order kernel the caller launches
+// next runs only if the sweep has lines that are not diagonally dominant
+// (the partitioned kernel then returns at once).  a holds the sweep's
+// geometry (for x: the x<->y transposed one, which the row factor cache
+// uses) and the original state in a.c0.
 int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
   *rc = 0;
   const int n = a.g.rmax - a.g.rmin + 1;
+  a.sel = 0;
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > 512) return 0;
   const Dims &o = x->d;
   PartLayout lay;
@@ -368,9 +373,10 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
     lay.lr0 = lay.st0; lay.lra = lay.sta; lay.lrb = lay.stb; lay.lrr = lay.str;
   }
   lay.lr_ncell = a.d.ncell;
-  ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
+  a.sel = 1;
   hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins);
   if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
+  a.sel = 2;
   return 1;
 }
 
@@ -413,13 +419,9 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     a.sig = x->tsx;
     a.kk = x->tkx;
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
-    if (try_cn_partitioned(x, 0, a, &rc)) {
-      if (rc) return rc;
-      x->ts1_state = TS1_MIRROR;
-      return 0;
-    }
     {
-      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);
+      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for both kernels
+      if (try_cn_partitioned(x, 0, a, &rc) && rc) return rc;
       HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
     }
     x->ts1_state = TS1_MIRROR;
@@ -445,12 +447,9 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
   }
   if (use_cn) {
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
-    if (!try_cn_partitioned(x, dir, a, &rc)) {
-      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
-      HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
-    } else if (rc) {
-      return rc;
-    }
+    ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);  // one record for both kernels
+    if (try_cn_partitioned(x, dir, a, &rc) && rc) return rc;
+    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
   } else {
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff[dir]));

@@ -135,7 +135,12 @@ struct DiffArgs {
   const double *lu;
   const uint8_t *piv;
   const int *sing;
-  const uint8_t *ddom;  // per line: 1 when every row is strictly diagonally dominant
+  // number of lines of the sweep whose matrix is not diagonally dominant
+  // (from k_line_lu); sel = 1: the kernel runs only when it is zero (the
+  // partitioned solves), sel = 2: only when it is not (their dgtsv-order
+  // alternative), sel = 0: always
+  const int *ndom;
+  int sel;
   double *junk;        // kJunk doubles that masked-off lanes store to
   double dt, theta;
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
@@ -169,7 +174,7 @@ struct PartLayout {
 };
 hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg);
 // the LU factors of every line of the sweep (reads a.lr)
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, uint8_t *ddom);
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);

@@ -11,6 +11,8 @@
 // right-hand sides to the ts1 slot (which the reference zero-fills anyway),
 // and the back substitution streams the line in reverse writing ts0.
 // No contraction (-ffp-contract=off): bitwise the reference arithmetic.
+#include <float.h>
+
 #include "common.hpp"
 
 namespace ash3d {
@@ -61,11 +63,12 @@ __global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
 // side.  Per row i (cell rmin-1+i), at that cell's offset: fact_i and the
 // interchange flag piv_i of elimination step i (rows i and i+1), and the U
 // entries D_i, DU_i, DL_i (DL_i: second super-diagonal); D_n on the last
-// row.  sing[line] is dgtsv's INFO for the line (first zero pivot row);
-// ddom[line] = 1 when every row satisfies |D| > |DL| + |DU| (strict row
-// diagonal dominance, which the partitioned solves rely on).
+// row.  sing[line] is dgtsv's INFO for the line (first zero pivot row).
+// *ndom counts the lines that are not diagonally dominant M-matrix rows
+// (some L or R negative or not finite, or theta < 0): the partitioned
+// solves, which eliminate without pivoting, are used only when it is zero.
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, uint8_t *ddom) {
+__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, int *ndom) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -113,7 +116,9 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
         DLn = -((th)*LeftFac);
         Dn = 1.0 + (th)*CenterFac - (th)*RightFac;
       }
-      dom = dom && (fabs(Dn) > fabs(DLn) + fabs(DUn));  // false for NaN too
+      // L, R >= 0 and finite, theta >= 0: every row has D >= 1 + |DL| + |DU|
+      // in exact arithmetic (false for NaN too)
+      dom = dom && LeftFac >= 0.0 && RightFac >= 0.0 && LeftFac <= DBL_MAX && RightFac <= DBL_MAX && th >= 0.0;
       if (l == 1) {
         Dc = Dn;
         DUc = DUn;
@@ -156,8 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   if (Dc == 0.0 && !singular) singular = n;
   fD[base + sL * (long long)(rmin + n)] = Dc;
   sing[gid] = singular;
-  ddom[gid] = dom ? 1 : 0;
+  if (!dom) atomicAdd(ndom, 1);
 }
+
+__global__ void k_zero_i32(int *p) { *p = 0; }
 
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
@@ -262,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
   constexpr int PF = 4;  // rows in flight per stream
+  if (A.sel == 2 && *A.ndom == 0) return;  // the partitioned solves ran (uniform)
   const Dims d = A.d;
   const LineGeom g = A.g;
   const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
@@ -438,6 +446,7 @@ struct XState {
 template <int NB, int TX>
 __global__ __launch_bounds__(64) void k_diff_cn_xrows(DiffArgs A, XState X) {
   constexpr int PF = 4;  // rows in flight; TX cells per LDS tile
+  if (A.sel == 2 && *A.ndom == 0) return;  // the partitioned solves ran (uniform)
   __shared__ double s_t[NB][64][TX + 1];
   __shared__ long long s_row[64];          // original offset of cell -1 of each lane's line (-1: none)
   __shared__ unsigned char s_on[64];       // line_on of each lane
@@ -753,13 +762,14 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
   return hipGetLastError();
 }
 
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, uint8_t *ddom) {
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)((lines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, ndom);
   switch (a.g.dir) {
-    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
-    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
-    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ddom); break;
+    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
+    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
+    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
   }
   return hipGetLastError();
 }

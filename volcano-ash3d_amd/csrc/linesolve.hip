@@ -23,6 +23,8 @@
 // of its lines one after another; per bin a line costs one read of q and
 // one write of the result.  A bin's tile of lines is staged through LDS
 // (coalesced both ways) while the next bin's tile is already in flight.
+// Sweeps with any line that is not a diagonally dominant M-matrix (negative
+// diffusivities, non-finite factors) are left to the dgtsv-order kernels.
 #include "common.hpp"
 
 namespace ash3d {
@@ -63,6 +65,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   constexpr int PITCH = NPAD + 1;
   constexpr int LEV = Lev<LP>::value;
   extern __shared__ double s_tile[];  // T lines x PITCH
+  // a sweep with lines that are not diagonally dominant M-matrix rows runs
+  // the dgtsv-order kernel instead (launched after this one; uniform exit)
+  if (*A.ndom != 0) return;
   const LineGeom g = A.g;
   const int ns = A.d.ns;
   const long long ncell = A.d.ncell;
@@ -223,12 +228,6 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   };
 
   double *trow = s_tile + ln * PITCH;
-  // lines whose matrix is not strictly diagonally dominant (e.g. negative
-  // diffusivities) are solved by their first lane in dgtsv's order, from
-  // the cached LU factors, bitwise as the sequential kernels
-  const bool dom = !line_on || A.ddom[(long long)(a - 1) + (long long)g.Na * (b - 1)] != 0;
-  const long long lbase = Q.lr0 + Q.lra * ac + Q.lrb * b;  // row factor / LU offset of cell 0
-  const double *fF = A.lu, *fD = A.lu + Q.lr_ncell, *fDU = fD + Q.lr_ncell, *fDL = fDU + Q.lr_ncell;
   int u = u0;
   while (u < u1 && !A.aloft.on(u)) ++u;
   if (u < u1) issue(u);
@@ -241,43 +240,6 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       int un = w + 1;
       while (un < u1 && !A.aloft.on(un)) ++un;
       if (un < u1) issue(un);
-      if (!dom) {
-        if (p == 0) {
-          double ql = trow[0], qc = trow[0], Bc = 0.0;
-          for (int l = 1; l <= n; ++l) {
-            const long long o = lbase + Q.lrr * (long long)(rmin + l - 1);
-            const double LeftFac = A.lr[o], RightFac = A.lr[o + Q.lr_ncell];
-            const double CenterFac = LeftFac + RightFac;
-            const double qr = (l < n) ? trow[l] : qc;  // BC_right_t0 (:845)
-            const double Bn = (1.0 - th) * LeftFac * ql + (1.0 - (1.0 - th) * CenterFac) * qc +
-                              (1.0 - th) * RightFac * qr;
-            if (l == 1) {
-              Bc = Bn;
-            } else {
-              const long long fo = o - Q.lrr;  // row l-1
-              const double fact = fF[fo];
-              const bool pv = A.piv[fo] != 0;
-              const double y = pv ? Bn : Bc;
-              const double xx = pv ? Bc : Bn;
-              trow[l - 2] = y;
-              Bc = xx - fact * y;
-            }
-            ql = qc;
-            qc = qr;
-          }
-          double x1 = Bc / fD[lbase + Q.lrr * (long long)(rmin + n - 1)], x2 = 0.0;
-          trow[n - 1] = x1;
-          for (int i = n - 1; i >= 1; --i) {
-            const long long fo = lbase + Q.lrr * (long long)(rmin + i - 1);
-            const double Bi = trow[i - 1];
-            const double xi = (i == n - 1) ? (Bi - fDU[fo] * x1) / fD[fo]
-                                           : (Bi - fDU[fo] * x1 - fDL[fo] * x2) / fD[fo];
-            x2 = x1;
-            x1 = xi;
-            trow[i - 1] = xi;
-          }
-        }
-      } else {
       // ---- solve this lane's segment of its line
       double q[M + 2];
 #pragma unroll
@@ -323,7 +285,6 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
         const int i = p * M + k;
         const double x = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
         if (i < n) trow[i] = x;
-      }
       }
       __syncthreads();
     }

@@ -250,7 +250,8 @@ def config2_state(seed: int = 2, ns: int = 15, nx: int = 360, ny: int = 240, nz:
 def ensemble_member(member: int, ns: int = 10, size_scale: float = 1.0, seed: int = 3) -> TransportState:
     """One member of the ensemble workload: a lon-lat 0.25 deg grid whose
     size is drawn uniformly in [120,400] x [80,300] x [40,60] cells (times
-    size_scale horizontally), dz 0.5 km, ns bins with |vf| log-spaced
+    size_scale horizontally), centred at 20 N (every row within 57.5 S..57.5
+    N), dz 0.5 km, ns bins with |vf| log-spaced
     0.02-3 m/s, the zonal jet with the member's own perturbations (sigma
     2 m/s), Kh = Kz = 1000 m2/s, Crank-Nicolson, and a cloud aloft around a
     plume height drawn in 8-20 km."""
@@ -259,7 +260,7 @@ def ensemble_member(member: int, ns: int = 10, size_scale: float = 1.0, seed: in
     ny = max(8, int(rng.integers(80, 301) * size_scale))
     nz = int(rng.integers(40, 61))
     st = TransportState(nx, ny, nz, ns, limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5)
-    lonlat_mesh(st, 180.0, 25.0, 0.25, 0.25, np.full(nz, 0.5))
+    lonlat_mesh(st, 180.0, 20.0 - 0.125 * ny, 0.25, 0.25, np.full(nz, 0.5))
     zonal_jet_winds(st, rng, sigma_mps=2.0)
     set_uniform_fall(st, fall_velocities(ns, 0.02, 3.0))
     set_diffusivity(st, 1000.0, 1000.0)
