@@ -1,8 +1,9 @@
 """Build the shim test driver: the drop-in Fortran modules of
 volcano-ash3d_amd/fortran compiled against the test host modules, linked to
-libash3d_tr.so.  Two variants, as the reference's DIFFMETH switch:
-_build/cn (-DCRANKNIC), _build/ex (explicit diffusion) and _build/sg
-(-DCRANKNIC -DFAST_SUBGRID)."""
+libash3d_tr.so.  Variants, as the reference's DIFFMETH switch:
+_build/cn (-DCRANKNIC), _build/ex (explicit diffusion), _build/sg
+(-DCRANKNIC -DFAST_SUBGRID) and _build/cnp (-DCRANKNIC with the driver
+selecting the partitioned line solver)."""
 import os
 import subprocess
 import sys
@@ -14,9 +15,10 @@ SHIMS = ["ash3d_tr_c.f90", "Diffusion.F90", "ash3d_tr_attach.F90", "AdvectionHor
          "AdvectionHorz.F90", "AdvectionVert_DCU.F90", "Set_BC.F90", "Ash3d_Adjoint.F90"]
 # test programs: the reference driver's hot-path sequence, and a source
 # inversion through the adjoint entry points (Crank-Nicolson build only)
-PROGRAMS = {"shim_driver": ("cn", "ex", "sg"), "inversion_driver": ("cn",)}
+PROGRAMS = {"shim_driver": ("cn", "ex", "sg", "cnp"), "inversion_driver": ("cn",)}
 VARIANTS = (("cn", ["-DCRANKNIC", "-DLIM_SUPERBEE"]), ("ex", ["-DLIM_SUPERBEE"]),
-            ("sg", ["-DCRANKNIC", "-DLIM_SUPERBEE", "-DFAST_SUBGRID"]))
+            ("sg", ["-DCRANKNIC", "-DLIM_SUPERBEE", "-DFAST_SUBGRID"]),
+            ("cnp", ["-DCRANKNIC", "-DLIM_SUPERBEE", "-DLINE_SOLVER_PARTITIONED"]))
 FC = os.environ.get("ASH3D_FC", "/opt/rocm/llvm/bin/amdflang")
 
 

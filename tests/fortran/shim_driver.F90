@@ -23,6 +23,10 @@
       use AdvectionHorz,     only : AdvectHorz
       use AdvectionVert_DCU, only : AdvectVert
       use ash3d_tr_c,        only : ash3d_tr_resident
+#ifdef LINE_SOLVER_PARTITIONED
+      use ash3d_tr_c,        only : ash3d_tr_ctx, ash3d_tr_check, c_ash3d_tr_tune, &
+                                    ASH3D_TUNE_LINE_SOLVER, ASH3D_LINE_SOLVER_PARTITIONED
+#endif
 
       implicit none
 
@@ -37,6 +41,8 @@
         end subroutine ash3d_tr_download
         subroutine ash3d_tr_detach()
         end subroutine ash3d_tr_detach
+        subroutine ash3d_tr_attach()
+        end subroutine ash3d_tr_attach
       end interface
 
       character(len=512) :: fin, fout
@@ -82,6 +88,12 @@
       close(10)
 
       ash3d_tr_resident = resident /= 0
+#ifdef LINE_SOLVER_PARTITIONED
+      ! the one added call that selects the partitioned line solver
+      call ash3d_tr_attach()
+      call ash3d_tr_check(c_ash3d_tr_tune(ash3d_tr_ctx, ASH3D_TUNE_LINE_SOLVER, &
+                                          ASH3D_LINE_SOLVER_PARTITIONED), 'tune')
+#endif
       do itime = 1, nsteps
         if (ash3d_tr_resident) then
           call ash3d_tr_glue()

@@ -68,6 +68,37 @@ def test_shim_driver_matches_oracle(pkg, oracle, tmp_path, resident, use_cn):
         assert np.array_equal(a, b), f"{name}: max abs diff {np.max(np.abs(a - b)):.3e}"
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("resident", [False, True])
+def test_shim_driver_partitioned_solver(pkg, oracle, tmp_path, resident):
+    """The same driver with one added call selecting the partitioned line
+    solver: every module array within 1e-12 relative L-inf of the oracle's
+    dgtsv-order run (bitwise where no line solve is involved)."""
+    exe = os.path.join(HERE, "fortran", "_build", "cnp", "shim_driver")
+    assert os.path.exists(exe), "build the shim driver (tests/fortran/build_shim.py)"
+    st = pkg.workloads.random_state(14, 11, 9, 3, seed=5, latlon=True, periodic=True, use_cn=True)
+    st.is_aloft = np.array([1, 1, 0], dtype=np.int32)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    nsteps = 3
+    ref = st.copy()
+    O = oracle.Oracle(ref)
+    for itime in range(1, nsteps + 1):
+        O.step_glue()
+        O.transport_step(itime, True)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_case(fin, st, True, nsteps, resident)
+    r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = st.copy()
+    read_case(fout, got)
+    for name in ORDER:
+        a, b = getattr(got, name), getattr(ref, name)
+        scale = np.max(np.abs(b))
+        err = np.max(np.abs(a - b)) / scale if scale > 0 else np.max(np.abs(a - b))
+        assert err <= 1e-12, f"{name}: rel L-inf {err:.3e}"
+    assert not np.array_equal(got.concen_pd, ref.concen_pd)  # the other solver did run
+
+
 def test_shim_sources_present():
     """CPU check: the drop-in sources carry the reference's module names."""
     fdir = os.path.join(os.path.dirname(HERE), "volcano-ash3d_amd", "fortran")

@@ -34,10 +34,15 @@
              c_ash3d_tr_checkpoints, c_ash3d_tr_checkpoint, c_ash3d_tr_adj_init, &
              c_ash3d_tr_adj_step, c_ash3d_tr_adj_source, c_ash3d_tr_adj_source_read, &
              c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load, c_ash3d_tr_minmax_index, &
-             c_ash3d_tr_met_push, c_ash3d_tr_met_interp
+             c_ash3d_tr_met_push, c_ash3d_tr_met_interp, c_ash3d_tr_tune, &
+             ASH3D_TUNE_LINE_SOLVER, ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED
 
       integer(c_int32_t), parameter :: ASH3D_TR_ABI_VERSION = 1
       integer(c_int32_t), parameter :: ASH3D_SYNC_ALL = 255
+      ! launch tuning (include/ash3d_tr.h): theta-scheme line solver
+      integer(c_int32_t), parameter :: ASH3D_TUNE_LINE_SOLVER = 7
+      integer(c_int32_t), parameter :: ASH3D_LINE_SOLVER_EXACT = 0
+      integer(c_int32_t), parameter :: ASH3D_LINE_SOLVER_PARTITIONED = 1
 
       ! struct ash3d_tr_desc of include/ash3d_tr.h
       type, bind(C) :: ash3d_tr_desc
@@ -195,6 +200,11 @@
           type(c_ptr), value :: ctx
           real(c_double), value :: frac
           integer(c_int32_t), value :: use_vf
+        end function
+        integer(c_int) function c_ash3d_tr_tune(ctx, param, val) bind(C, name='ash3d_tr_tune')
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: param, val
         end function
         type(c_ptr) function c_ash3d_tr_last_error() bind(C, name='ash3d_tr_last_error')
           import :: c_ptr
