@@ -289,6 +289,47 @@ int ash3d_tr_adj_source_read(ash3d_tr_ctx *ctx, double *out);
 int ash3d_tr_column_load(ash3d_tr_ctx *ctx, const double *dz, double *out);
 int ash3d_tr_adj_column_load(ash3d_tr_ctx *ctx, const double *dz, const double *w);
 
+/* ---- Output products and their ESRI ASCII files -----------------------
+ * ash3d_tr_products replaces ConcentrationCalculator
+ * (src/Output_Vars.f90:1026-1137) and AshThicknessCalculator (:855-893) on
+ * the device copy of concen_pd(...,ts1) and DepositGranularity.  Inputs:
+ * dz_vec_pd(1:nz), z_cc_pd(1:nz) (km); bounds = imin,imax,jmin,jmax of the
+ * columns ConcentrationCalculator visits (NULL: the whole grid); the
+ * thresholds and the deposit density.  Outputs (host, nx*ny, x fastest; any
+ * may be NULL): CloudLoad (t/km2), MaxConcentration (mg/m3), MaxHeight and
+ * MinHeight (km), DepositThickness (mm), Mask_Cloud and Mask_Deposit (0/1),
+ * and CloudArea / DepositAreaCovered (sums of sigma_nz_pd(i,j,1) over the
+ * masked columns, in the reference's loop order, from the descriptor's
+ * host geometry). */
+typedef struct {
+  double cloudcon_thresh;  /* CLOUDCON_THRESH (kg/km3) */
+  double cloudload_thresh; /* CLOUDLOAD_THRESH (t/km2) */
+  double depo_thresh;      /* DEPO_THRESH (mm) */
+  double deposit_density;  /* DepositDensity (kg/m3) */
+} ash3d_tr_product_params;
+typedef struct {
+  double *cloud_load, *max_concentration, *max_height, *min_height, *deposit_thickness;
+  int32_t *mask_cloud, *mask_deposit;
+  double cloud_area, deposit_area; /* set by the call */
+} ash3d_tr_products_out;
+int ash3d_tr_products(ash3d_tr_ctx *ctx, const double *dz, const double *zcc, const int32_t *bounds,
+                      const ash3d_tr_product_params *params, ash3d_tr_products_out *out);
+/* write_2D_ASCII (src/write_ASCII.f90:264-382): field(nx,ny) x fastest,
+ * mask (0/1, NULL = all) replaces unmasked cells by fill_value (up to six
+ * characters, e.g. "-9999."); is_latlon selects the header (lonLL, latLL,
+ * de, dn in degrees, lonLL > 180 shifted by -360) or the projected one
+ * (xLL, yLL, dx, dy in km, written in m).  Values are written as real(4)
+ * in 10f18.6 records from the northern row down.  Host only: no device or
+ * context is needed. */
+int ash3d_tr_write_esri_ascii(const char *path, int32_t nx, int32_t ny, const double *field,
+                              const int32_t *mask, const char *fill_value, int32_t is_latlon, double xll,
+                              double yll, double dx, double dy);
+/* The file name write_2D_ASCII gives a product: the trimmed root, then
+ * "____final" or the output time as 00h.hhhrs / 0hh.hhhrs / hhh.hhhrs, then
+ * ".dat" (roots containing "outvar" or "ArrivalTime" get no time).  Writes
+ * at most outlen bytes including the terminating NUL. */
+int ash3d_tr_esri_ascii_name(const char *root, double time_h, int32_t is_final, char *out, int32_t outlen);
+
 /* ---- Driver work between the hot-path calls ---------------------------
  * Met steps: the two bracketing wind fields of the wind_grid module
  * (v*_meso_{last,next}_step_sp, m/s, FP32, unpadded (1:nx,1:ny,1:nz) and

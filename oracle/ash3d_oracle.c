@@ -1117,3 +1117,82 @@ int orc_transport_step_subgrid(orc_state *s, int itime, int use_diffusion, int i
   }
   return 0;
 }
+
+/* ConcentrationCalculator (src/Output_Vars.f90:1026-1137), on ts1, and
+ * AshThicknessCalculator (:855-893), loop for loop. */
+void orc_products(const orc_state *s, const double *dz, const double *zcc, const double prm[4], double *out,
+                  int32_t *masks, double areas[2]) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz, ns = s->ns;
+  const long long nxy = (long long)nx * ny;
+  const double CLOUDCON_THRESH = prm[0], CLOUDLOAD_THRESH = prm[1], DEPO_THRESH = prm[2], DepositDensity = prm[3];
+  const double KG_2_MG = 1.0e6, KM3_2_M3 = 1.0e9, KM2_2_M2 = 1.0e6, M_2_MM = 1.0e3;
+  double *CloudLoad = out, *MaxConcentration = out + nxy, *MaxHeight = out + 2 * nxy, *MinHeight = out + 3 * nxy;
+  double *DepositThickness = out + 4 * nxy;
+  int32_t *Mask_Cloud = masks, *Mask_Deposit = masks + nxy;
+  double *TotalConcentration = (double *)malloc(sizeof(double) * (size_t)nz);
+  double CloudArea = 0.0, DepositAreaCovered = 0.0;
+  long long t;
+  int i, j, k, n;
+  for (t = 0; t < nxy; t++) {
+    CloudLoad[t] = 0.0;
+    MaxConcentration[t] = 0.0;
+    MaxHeight[t] = 0.0;
+    MinHeight[t] = 100.0;
+    Mask_Cloud[t] = 0;
+  }
+  if (ns > 0) {
+    for (i = s->imin; i <= s->imax; i++) {
+      for (j = s->jmin; j <= s->jmax; j++) {
+        const long long c = (long long)(i - 1) + (long long)nx * (j - 1);
+        const double CellArea = s->sigma_nz_pd[P3(s, i, j, 1)];
+        double MaxTotalConcentration;
+        for (k = 1; k <= nz; k++) TotalConcentration[k - 1] = 0.0;
+        for (k = 1; k <= nz; k++) {
+          double sum = 0.0;
+          for (n = 1; n <= ns; n++) sum += s->concen_pd[PC(s, i, j, k, n, TS1)];
+          CloudLoad[c] = CloudLoad[c] + sum * dz[k - 1] / 1.0e3;
+          TotalConcentration[k - 1] = sum;
+          if (TotalConcentration[k - 1] >= CLOUDCON_THRESH) MaxHeight[c] = zcc[k - 1] + 0.5 * dz[k - 1];
+        }
+        for (k = nz; k >= 1; k--)
+          if (TotalConcentration[k - 1] <= CLOUDCON_THRESH) MinHeight[c] = zcc[k - 1] - 0.5 * dz[k - 1];
+        MaxTotalConcentration = TotalConcentration[0];
+        for (k = 2; k <= nz; k++)
+          if (TotalConcentration[k - 1] > MaxTotalConcentration) MaxTotalConcentration = TotalConcentration[k - 1];
+        Mask_Cloud[c] = CloudLoad[c] >= CLOUDLOAD_THRESH;
+        if (Mask_Cloud[c]) {
+          if (MinHeight[c] < 0.0 && MaxHeight[c] > 0.0) MinHeight[c] = 0.0;
+          MinHeight[c] = MaxHeight[c] < MinHeight[c] ? MaxHeight[c] : MinHeight[c];
+          CloudArea = CloudArea + CellArea;
+          MaxConcentration[c] = MaxTotalConcentration * KG_2_MG / KM3_2_M3;
+        } else {
+          CloudLoad[c] = 0.0;
+          MaxConcentration[c] = 0.0;
+          MaxHeight[c] = 0.0;
+          MinHeight[c] = 0.0;
+        }
+      }
+    }
+  }
+  free(TotalConcentration);
+  for (t = 0; t < nxy; t++) {
+    Mask_Deposit[t] = 0;
+    DepositThickness[t] = 0.0;
+  }
+  if (ns > 0) {
+    for (i = 1; i <= nx; i++) {
+      for (j = 1; j <= ny; j++) {
+        const long long c = (long long)(i - 1) + (long long)nx * (j - 1);
+        double sum = 0.0;
+        for (n = 1; n <= ns; n++) sum += s->deposit_granularity[c + nxy * (n - 1)];
+        DepositThickness[c] = sum * dz[0] / KM2_2_M2 / DepositDensity * M_2_MM;
+        if (DepositThickness[c] > DEPO_THRESH) {
+          Mask_Deposit[c] = 1;
+          DepositAreaCovered = DepositAreaCovered + s->sigma_nz_pd[P3(s, i, j, 1)];
+        }
+      }
+    }
+  }
+  areas[0] = CloudArea;
+  areas[1] = DepositAreaCovered;
+}

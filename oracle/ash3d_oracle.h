@@ -24,6 +24,8 @@
  *   wind time interpolation src/MesoInterpolater.f90:203-230
  *   Adjust_DT              src/Adjust_dt.f90:89-333
  *   get_minmax_index       src/Calc_Mesh.f90:432-542 (and the FAST_SUBGRID step)
+ *   ConcentrationCalculator src/Output_Vars.f90:1026-1137
+ *   AshThicknessCalculator src/Output_Vars.f90:855-893
  *
  * Arrays use the reference's Fortran column-major layouts and lower bounds
  * (src/Ash3d_VariableModules.f90:918-946, src/Diffusion.F90:95-101):
@@ -132,6 +134,15 @@ int orc_minmax_index(const orc_state *s, int ivent, int jvent, int kvent, double
  * src/Diffusion.F90:168-175); returns 2 if get_minmax_index stops. */
 int orc_transport_step_subgrid(orc_state *s, int itime, int use_diffusion, int ivent, int jvent, int kvent,
                                double thresh, int32_t bounds[6]);
+
+/* ConcentrationCalculator and AshThicknessCalculator.  dz, zcc: dz_vec_pd,
+ * z_cc_pd (1:nz); prm = {CLOUDCON_THRESH, CLOUDLOAD_THRESH, DEPO_THRESH,
+ * DepositDensity}; columns imin..imax x jmin..jmax of the state.  out:
+ * CloudLoad, MaxConcentration, MaxHeight, MinHeight, DepositThickness
+ * (5 x nx*ny, x fastest); masks: Mask_Cloud, Mask_Deposit (2 x nx*ny);
+ * areas: CloudArea, DepositAreaCovered. */
+void orc_products(const orc_state *s, const double *dz, const double *zcc, const double prm[4], double *out,
+                  int32_t *masks, double areas[2]);
 
 void orc_dgtsv(int n, int nrhs, double *dl, double *d, double *du, double *b,
                int ldb, int *info);

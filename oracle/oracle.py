@@ -134,6 +134,8 @@ def load():
         lib.orc_transport_step_subgrid.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_double, _ip]
         lib.orc_transport_step_subgrid.restype = ctypes.c_int
+        lib.orc_products.argtypes = [P, _dp, _dp, _dp, _dp, _ip, _dp]
+        lib.orc_products.restype = None
         _lib = lib
     return _lib
 
@@ -255,6 +257,27 @@ class Oracle:
         rc = self._call("orc_transport_step_subgrid", itime, int(bool(use_diffusion)), ivent, jvent, kvent,
                         float(thresh), b.ctypes.data_as(_ip))
         return rc, tuple(int(v) for v in b)
+
+    def products(self, dz, zcc, cloudcon_thresh=1.0e-3, cloudload_thresh=0.2, depo_thresh=1.0e-2,
+                 deposit_density=1000.0) -> dict:
+        """ConcentrationCalculator + AshThicknessCalculator over the state's
+        imin..imax x jmin..jmax columns (same keys as the library's)."""
+        nx, ny = self.st.nx, self.st.ny
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        zcc = np.ascontiguousarray(zcc, dtype=np.float64)
+        prm = np.array([cloudcon_thresh, cloudload_thresh, depo_thresh, deposit_density])
+        out = np.zeros(5 * nx * ny)
+        masks = np.zeros(2 * nx * ny, dtype=np.int32)
+        areas = np.zeros(2)
+        self._call("orc_products", _ptr(dz), _ptr(zcc), _ptr(prm), _ptr(out), masks.ctypes.data_as(_ip), _ptr(areas))
+        planes = out.reshape(5, ny, nx)
+        res = {k: np.asfortranarray(planes[m].T) for m, k in enumerate(
+            ("cloud_load", "max_concentration", "max_height", "min_height", "deposit_thickness"))}
+        mk = masks.reshape(2, ny, nx)
+        res["mask_cloud"] = np.asfortranarray(mk[0].T)
+        res["mask_deposit"] = np.asfortranarray(mk[1].T)
+        res["cloud_area"], res["deposit_area"] = float(areas[0]), float(areas[1])
+        return res
 
     @property
     def last_info(self) -> int:

@@ -115,6 +115,7 @@ struct ash3d_tr_ctx {
   double *src_grad = nullptr;
   int n_src_slots = 0;
   double *col = nullptr;  // column-load staging: nz dz values, then nx*ny loads / weights
+  double *prod = nullptr; // output products: 5 planes, 2 mask planes, dz and z_cc
   std::vector<double *> ckpt;  // checkpoint slots
   // met steps: two slots of vx, vy, vz (n3) and vf (n3*ns) in FP32
   float *met[2][4] = {};
@@ -633,7 +634,7 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
                     x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx, x->lam, x->snap_b,
                     x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col, x->lr[0], x->lr[1], x->lr[2],
-                    x->lu[0], x->lu[1], x->lu[2], x->junk};
+                    x->lu[0], x->lu[1], x->lu[2], x->junk, x->prod};
   for (double *p : ptrs)
     if (p) hipFree(p);
   for (double *p : x->ckpt)
@@ -1163,6 +1164,88 @@ int ash3d_tr_column_load(ash3d_tr_ctx *x, const double *dz, double *out) {
   HIPCHK(launch_column_load(x->stream, d, src, x->col, x->col + d.nz));
   HIPCHK(hipMemcpyAsync(out, x->col + d.nz, sizeof(double) * nxy, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  return 0;
+}
+
+int ash3d_tr_products(ash3d_tr_ctx *x, const double *dz, const double *zcc, const int32_t *bounds,
+                      const ash3d_tr_product_params *prm, ash3d_tr_products_out *out) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  if (!dz || !zcc || !prm || !out) return fail(ASH3D_TR_EINVAL, "null dz, z_cc, params or output");
+  const Dims &d = x->d;
+  const long long nxy = (long long)d.nx * d.ny;
+  int32_t b[4] = {1, d.nx, 1, d.ny};
+  if (bounds) {
+    for (int k = 0; k < 4; ++k) b[k] = bounds[k];
+    if (b[0] < 1 || b[1] > d.nx || b[2] < 1 || b[3] > d.ny)
+      return fail(ASH3D_TR_EINVAL, "product bounds outside the grid");
+  }
+  if (!x->prod && (rc = dev_alloc(&x->prod, 6 * nxy + 2 * (long long)d.nz))) return rc;
+  double *dzd = x->prod + 6 * nxy, *zcd = dzd + d.nz;
+  HIPCHK(hipStreamSynchronize(x->stream));  // the staging buffer may still be read
+  HIPCHK(hipMemcpyAsync(dzd, dz, sizeof(double) * (size_t)d.nz, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(zcd, zcc, sizeof(double) * (size_t)d.nz, hipMemcpyHostToDevice, x->stream));
+  ProdArgs p{};
+  p.d = d;
+  // ConcentrationCalculator reads ts1
+  p.c = (x->ts1_state == TS1_ZERO) ? nullptr : (x->ts1_state == TS1_EXPLICIT) ? x->ts1() : x->ts0();
+  p.dep = x->dep;
+  p.dz = dzd;
+  p.zcc = zcd;
+  p.cloudcon = prm->cloudcon_thresh;
+  p.cloudload = prm->cloudload_thresh;
+  p.density = prm->deposit_density;
+  p.depo = prm->depo_thresh;
+  p.i0 = b[0]; p.i1 = b[1]; p.j0 = b[2]; p.j1 = b[3];
+  p.out = x->prod;
+  HIPCHK(launch_products(x->stream, p));
+  double *planes[5] = {out->cloud_load, out->max_concentration, out->max_height, out->min_height,
+                       out->deposit_thickness};
+  for (int k = 0; k < 5; ++k)
+    if (planes[k])
+      HIPCHK(hipMemcpyAsync(planes[k], x->prod + k * nxy, sizeof(double) * (size_t)nxy, hipMemcpyDeviceToHost,
+                            x->stream));
+  std::vector<int32_t> masks((size_t)(2 * nxy));
+  HIPCHK(hipMemcpyAsync(masks.data(), x->prod + 5 * nxy, sizeof(int32_t) * (size_t)(2 * nxy),
+                        hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (out->mask_cloud) std::memcpy(out->mask_cloud, masks.data(), sizeof(int32_t) * (size_t)nxy);
+  if (out->mask_deposit) std::memcpy(out->mask_deposit, masks.data() + nxy, sizeof(int32_t) * (size_t)nxy);
+  // CloudArea (i outer, j inner over the box) and DepositAreaCovered (whole
+  // grid), sigma_nz_pd(i,j,1) from the host geometry
+  const double *sz = x->host.sigma_nz_pd;
+  double ca = 0.0, da = 0.0;
+  for (int i = 1; i <= d.nx; ++i)
+    for (int j = 1; j <= d.ny; ++j) {
+      const long long t = (long long)(i - 1) + (long long)d.nx * (j - 1);
+      const double area = sz ? sz[d.p3(i, j, 1)] : 0.0;
+      if (i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && masks[(size_t)t]) ca = ca + area;
+      if (masks[(size_t)(nxy + t)]) da = da + area;
+    }
+  out->cloud_area = ca;
+  out->deposit_area = da;
+  return 0;
+}
+
+int ash3d_tr_write_esri_ascii(const char *path, int32_t nx, int32_t ny, const double *field, const int32_t *mask,
+                              const char *fill_value, int32_t is_latlon, double xll, double yll, double dx,
+                              double dy) {
+  if (!path || !field || !fill_value) return fail(ASH3D_TR_EINVAL, "null path, field or fill value");
+  if (nx < 1 || ny < 1) return fail(ASH3D_TR_EINVAL, "bad field size %d x %d", nx, ny);
+  const std::string text = esri_ascii_text(nx, ny, field, mask, fill_value, is_latlon, xll, yll, dx, dy);
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(ASH3D_TR_EINVAL, "cannot open %s for writing", path);
+  const size_t w = fwrite(text.data(), 1, text.size(), f);
+  const int c = fclose(f);
+  if (w != text.size() || c != 0) return fail(ASH3D_TR_EINVAL, "writing %s failed", path);
+  return 0;
+}
+
+int ash3d_tr_esri_ascii_name(const char *root, double time_h, int32_t is_final, char *out, int32_t outlen) {
+  if (!root || !out || outlen < 1) return fail(ASH3D_TR_EINVAL, "null root or output");
+  const std::string n = esri_ascii_name(root, time_h, is_final);
+  if ((int32_t)n.size() + 1 > outlen) return fail(ASH3D_TR_EINVAL, "name needs %d bytes", (int)n.size() + 1);
+  std::memcpy(out, n.c_str(), n.size() + 1);
   return 0;
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace ash3d {
 
 // concen_pd(-1:nx+2,-1:ny+2,-1:nz+2,1:ns,0:1) and the other padded arrays
@@ -184,6 +186,20 @@ hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, in
 hipError_t launch_mass_totals(hipStream_t s, const Dims &d, const double *c0, const double *kap,
                               const double *dep, const double *yz1, const double *yz2, const double *xz1,
                               const double *xz2, const double *xy2, double *partial, double *out);
+// output products (products.hip): c = the ts1 view of the state (null:
+// all zero), dep = DepositGranularity, dz/zcc = dz_vec_pd, z_cc_pd (nz);
+// out = 5 planes of nx*ny doubles then 2 planes of int32 masks
+struct ProdArgs {
+  Dims d;
+  const double *c, *dep, *dz, *zcc;
+  double cloudcon, cloudload, density, depo;
+  int i0, i1, j0, j1;
+  double *out;
+};
+hipError_t launch_products(hipStream_t s, const ProdArgs &p);
+std::string esri_ascii_text(int nx, int ny, const double *field, const int32_t *mask, const char *fill,
+                            int latlon, double xll, double yll, double dx, double dy);
+std::string esri_ascii_name(const char *root, double time_h, int is_final);
 // driver work on the device (meteo.hip)
 hipError_t launch_met_interp(hipStream_t s, const Dims &d, const float *const last[4], const float *const next[4],
                              double frac, bool with_vf, double *vx, double *vy, double *vz, double *vf);

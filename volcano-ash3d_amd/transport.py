@@ -69,6 +69,23 @@ class DtParams(ctypes.Structure):
                 ("next_write_time", ctypes.c_double), ("simtime", ctypes.c_double)]
 
 
+class ProductParams(ctypes.Structure):
+    """ash3d_tr_product_params: thresholds of ConcentrationCalculator and
+    AshThicknessCalculator and the deposit density."""
+    _fields_ = [("cloudcon_thresh", ctypes.c_double), ("cloudload_thresh", ctypes.c_double),
+                ("depo_thresh", ctypes.c_double), ("deposit_density", ctypes.c_double)]
+
+
+class ProductsOut(ctypes.Structure):
+    """ash3d_tr_products_out."""
+    _fields_ = [("cloud_load", ctypes.POINTER(ctypes.c_double)),
+                ("max_concentration", ctypes.POINTER(ctypes.c_double)),
+                ("max_height", ctypes.POINTER(ctypes.c_double)), ("min_height", ctypes.POINTER(ctypes.c_double)),
+                ("deposit_thickness", ctypes.POINTER(ctypes.c_double)),
+                ("mask_cloud", ctypes.POINTER(ctypes.c_int32)), ("mask_deposit", ctypes.POINTER(ctypes.c_int32)),
+                ("cloud_area", ctypes.c_double), ("deposit_area", ctypes.c_double)]
+
+
 _fp = ctypes.POINTER(ctypes.c_float)
 
 # every exported entry point and its argument types
@@ -112,6 +129,11 @@ _SIGS = {
     "ash3d_tr_adj_source_read": [ctypes.c_void_p, _dp],
     "ash3d_tr_column_load": [ctypes.c_void_p, _dp, _dp],
     "ash3d_tr_adj_column_load": [ctypes.c_void_p, _dp, _dp],
+    "ash3d_tr_products": [ctypes.c_void_p, _dp, _dp, _ip, ctypes.POINTER(ProductParams), ctypes.POINTER(ProductsOut)],
+    "ash3d_tr_write_esri_ascii": [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, _dp, _ip, ctypes.c_char_p,
+                                  ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double],
+    "ash3d_tr_esri_ascii_name": [ctypes.c_char_p, ctypes.c_double, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32],
     "ash3d_tr_graph_capture": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_double, _ip],
     "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
@@ -320,6 +342,34 @@ class Transport:
         self._check(self.lib.ash3d_tr_checkpoint(self._ctx, int(slot), CKPT_LOAD))
 
     # -- observation operator ----------------------------------------------
+    def products(self, dz, zcc, bounds=None, cloudcon_thresh=1.0e-3, cloudload_thresh=0.2, depo_thresh=1.0e-2,
+                 deposit_density=1000.0) -> dict:
+        """ConcentrationCalculator + AshThicknessCalculator on the device
+        state: a dict of (nx, ny) Fortran-order arrays (cloud_load,
+        max_concentration, max_height, min_height, deposit_thickness,
+        mask_cloud, mask_deposit) and the cloud / deposit areas."""
+        nx, ny = self.st.nx, self.st.ny
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        zcc = np.ascontiguousarray(zcc, dtype=np.float64)
+        assert dz.shape == (self.st.nz,) and zcc.shape == (self.st.nz,)
+        res = {k: np.zeros((nx, ny), order="F") for k in ("cloud_load", "max_concentration", "max_height",
+                                                            "min_height", "deposit_thickness")}
+        res["mask_cloud"] = np.zeros((nx, ny), dtype=np.int32, order="F")
+        res["mask_deposit"] = np.zeros((nx, ny), dtype=np.int32, order="F")
+        o = ProductsOut()
+        for k in ("cloud_load", "max_concentration", "max_height", "min_height", "deposit_thickness"):
+            setattr(o, k, res[k].ctypes.data_as(_dp))
+        o.mask_cloud = res["mask_cloud"].ctypes.data_as(_ip)
+        o.mask_deposit = res["mask_deposit"].ctypes.data_as(_ip)
+        prm = ProductParams(cloudcon_thresh, cloudload_thresh, depo_thresh, deposit_density)
+        b = None if bounds is None else np.ascontiguousarray(bounds[:4], dtype=np.int32)
+        self._check(self.lib.ash3d_tr_products(self._ctx, dz.ctypes.data_as(_dp), zcc.ctypes.data_as(_dp),
+                                               None if b is None else b.ctypes.data_as(_ip), ctypes.byref(prm),
+                                               ctypes.byref(o)))
+        res["cloud_area"] = o.cloud_area
+        res["deposit_area"] = o.deposit_area
+        return res
+
     def column_load(self, dz: np.ndarray) -> np.ndarray:
         """Cloud load (t/km2) of every column, (nx, ny) Fortran order."""
         dz = np.ascontiguousarray(dz, dtype=np.float64)
@@ -443,6 +493,30 @@ class Transport:
             self._check(self.lib.ash3d_tr_profile_read(self._ctx, k, ctypes.byref(ms), ctypes.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+
+def write_esri_ascii(path: str, field: np.ndarray, mask=None, fill_value: str = "-9999.", is_latlon=False,
+                     xll=0.0, yll=0.0, dx=1.0, dy=1.0):
+    """write_2D_ASCII of an (nx, ny) field (src/write_ASCII.f90:264-382)."""
+    lib = load()
+    f = np.asfortranarray(field, dtype=np.float64)
+    nx, ny = f.shape
+    m = None if mask is None else np.asfortranarray(mask, dtype=np.int32)
+    rc = lib.ash3d_tr_write_esri_ascii(path.encode(), nx, ny, f.ctypes.data_as(_dp),
+                                       None if m is None else m.ctypes.data_as(_ip), fill_value.encode(),
+                                       int(bool(is_latlon)), float(xll), float(yll), float(dx), float(dy))
+    if rc:
+        raise Ash3dError(rc, lib.ash3d_tr_last_error().decode())
+
+
+def esri_ascii_name(root: str, time_h: float = 0.0, is_final: bool = False) -> str:
+    """The file name write_2D_ASCII gives a product root at an output time."""
+    lib = load()
+    buf = ctypes.create_string_buffer(256)
+    rc = lib.ash3d_tr_esri_ascii_name(root.encode(), float(time_h), int(bool(is_final)), buf, 256)
+    if rc:
+        raise Ash3dError(rc, lib.ash3d_tr_last_error().decode())
+    return buf.value.decode()
 
 
 def build_info() -> str:
