@@ -289,6 +289,72 @@ int ash3d_tr_adj_source_read(ash3d_tr_ctx *ctx, double *out);
 int ash3d_tr_column_load(ash3d_tr_ctx *ctx, const double *dz, double *out);
 int ash3d_tr_adj_column_load(ash3d_tr_ctx *ctx, const double *dz, const double *w);
 
+/* ---- Source term of the standard single-column sources (host) ---------
+ * The routines of src/Source.f90 that build what ash3d_tr_source_column_add
+ * inserts each step; no device or context is needed.  Umbrella sources
+ * (3x3 insertion, src/Source_Umbrella.f90) are not provided: their
+ * normalized column is the Suzuki one, but their insertion is not.
+ *   ash3d_tr_source_column    Calc_Normalized_SourceCol (:287-490) and
+ *                             EruptivePulse_MassFluxRate (:508-589):
+ *                             norm(i,k) at norm[i-1 + neruptions*(k-1)],
+ *                             k = 1..nz; mass_flux_rate (kg/hr) and
+ *                             end_time (h) per pulse (either may be NULL)
+ *   ash3d_tr_eruptive_pulses  CheckEruptivePulses (:613-685) for the step
+ *                             [time, time+dt]: the pulse indices
+ *                             ieruption/jeruption (1-based, kept by the
+ *                             caller between steps; both start at 1), the
+ *                             fraction of dt of each pulse, and whether a
+ *                             source is inserted in this step.  As in the
+ *                             reference, a pulse that starts inside dt is
+ *                             not counted in that step.
+ *   ash3d_tr_source_nodes     TephraSourceNodes (:708-783): flux(k,n) =
+ *                             SourceNodeFlux (kg/km3/hr), k = 1..nz+1 (row
+ *                             nz+1 zero), n = 1..ns (bins past n_gs_max
+ *                             zero), the layout of ash3d_tr_source_column_add;
+ *                             kappa_col = kappa_pd(ivent,jvent,1:nz); checks
+ *                             that the fluxes carry the mass flux rate
+ *                             (EINVAL otherwise, as the reference stops) and
+ *                             sets ieruption = jeruption. */
+enum {
+  ASH3D_SRC_SUZUKI = 0,
+  ASH3D_SRC_UMBRELLA = 1,     /* normalized column only */
+  ASH3D_SRC_UMBRELLA_AIR = 2, /* normalized column only */
+  ASH3D_SRC_LINE = 3,
+  ASH3D_SRC_POINT = 4,
+  ASH3D_SRC_PROFILE = 5,
+  ASH3D_SRC_OTHER = 6 /* zero column and mass flux rate */
+};
+typedef struct {
+  int32_t source_type;        /* ASH3D_SRC_* (SourceType) */
+  int32_t neruptions;
+  const double *start_time;   /* e_StartTime(1:neruptions), h, on the driver's clock */
+  const double *duration;     /* e_Duration, h */
+  const double *volume;       /* e_Volume, km3 DRE */
+  const double *plume_height; /* e_PlumeHeight, km */
+  double suzuki_a;            /* Suzuki_A */
+  double magma_density;       /* MagmaDensity, kg/m3 */
+  double z_volcano;           /* vent elevation, km */
+  int32_t zscaling_id;        /* ZScaling_ID: 0 s = z, 1 shifted, 2 scaled */
+  double zsurf_vent;          /* Zsurf(ivent,jvent), km */
+  double ztop;                /* Ztop, km */
+  const double *s_lb;         /* s_lb_pd(1:nz+1), cell lower bounds in s */
+  const double *s_cc;         /* s_cc_pd(1:nz) */
+  const double *ds;           /* ds_vec_pd(1:nz) */
+  const double *bin_mass;     /* Tephra_bin_mass(1:n_gs_max) */
+  /* profile sources */
+  const double *prof_dz;      /* e_prof_dz(1:neruptions), km */
+  const int32_t *prof_nz;     /* e_prof_nzpoints(1:neruptions) */
+  const double *prof_volume;  /* e_prof_Volume(i,kk) at prof_volume[i-1 + prof_ld*(kk-1)], km3 */
+  int32_t prof_ld;            /* leading dimension of prof_volume (neruptions) */
+} ash3d_tr_source_desc;
+int ash3d_tr_source_column(const ash3d_tr_source_desc *src, int32_t nz, double *norm, double *mass_flux_rate,
+                           double *end_time);
+int ash3d_tr_eruptive_pulses(const ash3d_tr_source_desc *src, const double *end_time, double time, double dt,
+                             int32_t *ieruption, int32_t *jeruption, double *frac, int32_t *in_dt);
+int ash3d_tr_source_nodes(const ash3d_tr_source_desc *src, int32_t nz, int32_t ns, int32_t n_gs_max,
+                          const double *norm, const double *mass_flux_rate, const double *frac,
+                          const double *kappa_col, int32_t *ieruption, int32_t jeruption, double *flux);
+
 /* ---- Output products and their ESRI ASCII files -----------------------
  * ash3d_tr_products replaces ConcentrationCalculator
  * (src/Output_Vars.f90:1026-1137) and AshThicknessCalculator (:855-893) on

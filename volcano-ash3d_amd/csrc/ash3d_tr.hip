@@ -1493,3 +1493,7 @@ int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
 }
 
 }  // extern "C"
+
+namespace ash3d {
+int report_error(int code, const char *msg) { return fail(code, "%s", msg); }
+}  // namespace ash3d

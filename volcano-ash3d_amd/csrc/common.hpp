@@ -200,6 +200,8 @@ hipError_t launch_products(hipStream_t s, const ProdArgs &p);
 std::string esri_ascii_text(int nx, int ny, const double *field, const int32_t *mask, const char *fill,
                             int latlon, double xll, double yll, double dx, double dy);
 std::string esri_ascii_name(const char *root, double time_h, int is_final);
+// record msg as the library's last error (ash3d_tr_last_error); returns code
+int report_error(int code, const char *msg);
 // driver work on the device (meteo.hip)
 hipError_t launch_met_interp(hipStream_t s, const Dims &d, const float *const last[4], const float *const next[4],
                              double frac, bool with_vf, double *vx, double *vy, double *vz, double *vf);

@@ -86,6 +86,20 @@ class ProductsOut(ctypes.Structure):
                 ("cloud_area", ctypes.c_double), ("deposit_area", ctypes.c_double)]
 
 
+class SourceDesc(ctypes.Structure):
+    """ash3d_tr_source_desc: eruptive pulses and vertical grid of the source
+    builders (src/Source.f90)."""
+    _fields_ = [("source_type", ctypes.c_int32), ("neruptions", ctypes.c_int32),
+                ("start_time", ctypes.POINTER(ctypes.c_double)), ("duration", ctypes.POINTER(ctypes.c_double)),
+                ("volume", ctypes.POINTER(ctypes.c_double)), ("plume_height", ctypes.POINTER(ctypes.c_double)),
+                ("suzuki_a", ctypes.c_double), ("magma_density", ctypes.c_double), ("z_volcano", ctypes.c_double),
+                ("zscaling_id", ctypes.c_int32), ("zsurf_vent", ctypes.c_double), ("ztop", ctypes.c_double),
+                ("s_lb", ctypes.POINTER(ctypes.c_double)), ("s_cc", ctypes.POINTER(ctypes.c_double)),
+                ("ds", ctypes.POINTER(ctypes.c_double)), ("bin_mass", ctypes.POINTER(ctypes.c_double)),
+                ("prof_dz", ctypes.POINTER(ctypes.c_double)), ("prof_nz", ctypes.POINTER(ctypes.c_int32)),
+                ("prof_volume", ctypes.POINTER(ctypes.c_double)), ("prof_ld", ctypes.c_int32)]
+
+
 _fp = ctypes.POINTER(ctypes.c_float)
 
 # every exported entry point and its argument types
@@ -134,6 +148,11 @@ _SIGS = {
                                   ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double],
     "ash3d_tr_esri_ascii_name": [ctypes.c_char_p, ctypes.c_double, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32],
+    "ash3d_tr_source_column": [ctypes.POINTER(SourceDesc), ctypes.c_int32, _dp, _dp, _dp],
+    "ash3d_tr_eruptive_pulses": [ctypes.POINTER(SourceDesc), _dp, ctypes.c_double, ctypes.c_double, _ip, _ip, _dp,
+                                 _ip],
+    "ash3d_tr_source_nodes": [ctypes.POINTER(SourceDesc), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                              _dp, _dp, _ip, ctypes.c_int32, _dp],
     "ash3d_tr_graph_capture": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_double, _ip],
     "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
