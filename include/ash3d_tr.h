@@ -190,7 +190,10 @@ enum {
   ASH3D_TUNE_NB_DIFFUSE_Z = 5,
   ASH3D_TUNE_X_TILE = 6, /* cells per LDS tile of the x-line solves: 8, 16 or 32 */
   ASH3D_TUNE_LINE_SOLVER = 7, /* ASH3D_LINE_SOLVER_* below */
-  ASH3D_TUNE_PART_BINS = 8 /* bins per workgroup of the partitioned solves (0: all) */
+  ASH3D_TUNE_PART_BINS = 8, /* bins per workgroup of the partitioned solves (0: all) */
+  ASH3D_TUNE_PART_SHAPE_X = 9, /* tile shape of the partitioned solves (0 or 1; defaults 1, 1, 0), x lines */
+  ASH3D_TUNE_PART_SHAPE_Y = 10,
+  ASH3D_TUNE_PART_SHAPE_Z = 11
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

@@ -51,14 +51,17 @@ def test_sweeps(pkg, oracle, tr, dirn, ns, periodic):
     assert_same(a, b, tol=TOL)
 
 
+@pytest.mark.parametrize("alt", [0, 1])
 @pytest.mark.parametrize("shape", [(400, 130, 70), (300, 300, 20), (200, 60, 150), (520, 40, 3)])
-def test_tile_shapes(pkg, oracle, tr, shape):
+def test_tile_shapes(pkg, oracle, tr, shape, alt):
     """x lines of 400 / 300 / 200 / 520 cells (the last beyond 512: dgtsv
-    order), y lines of 130 / 300 / 60 / 40, z lines of 70 / 20 / 150 / 3."""
+    order), y lines of 130 / 300 / 60 / 40, z lines of 70 / 20 / 150 / 3,
+    in both tile shapes of every line-length class."""
     nx, ny, nz = shape
     st = pkg.workloads.random_state(nx, ny, nz, 2, seed=sum(shape), latlon=True, use_cn=True)
     st.dt *= 20.0
-    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)])
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)],
+                    part_shape_x=alt, part_shape_y=alt, part_shape_z=alt)
     assert_same(a, b, tol=TOL)
 
 

@@ -39,15 +39,17 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), it
     itime = itime0
     params = [p for p in PARAMS if not (line_solver == "partitioned" and p[1].startswith("diffuse"))]
     if line_solver == "partitioned":
-        params.append(("part_bins", ("diffuse_x", "diffuse_y", "diffuse_z")))
+        params += [("part_shape_x", "diffuse_x"), ("part_shape_y", "diffuse_y"), ("part_shape_z", "diffuse_z"),
+                   ("part_bins", ("diffuse_x", "diffuse_y", "diffuse_z"))]
     for param, kind in params:
         kinds = kind if isinstance(kind, tuple) else (kind,)
         if kinds[0].startswith("diffuse") and not use_diffusion:
             continue
-        values = X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else candidates
+        values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
+                  (0, 1) if param.startswith("part_shape") else candidates)
         times = {}
         for nb in values:
-            if param not in ("x_tile", "part_bins") and nb > 1 and nb // 2 >= st.ns:
+            if param.startswith("nb_") and nb > 1 and nb // 2 >= st.ns:
                 continue
             T.tune(**{param: nb})
             T.profile(True)

@@ -130,6 +130,9 @@ struct ash3d_tr_ctx {
   int x_tile = 16;  // cells per LDS tile of the x-line solves
   int line_solver = ASH3D_LINE_SOLVER_EXACT;  // theta-scheme line solves
   int part_bins = 0;  // bins per workgroup of the partitioned solves (0: all)
+  // tile shape of the partitioned solves, by direction (measured on the
+  // 360x240x50 forecast grid: 2-line x workgroups, 16-line y tiles)
+  int part_shape[3] = {1, 1, 0};
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -375,7 +378,7 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
   }
   lay.lr_ncell = a.d.ncell;
   a.sel = 1;
-  hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins);
+  hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins, x->part_shape[dir]);
   if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
   a.sel = 2;
   return 1;
@@ -913,6 +916,11 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
       return fail(ASH3D_TR_EINVAL, "line solver must be %d (dgtsv order) or %d (partitioned) (got %d)",
                   ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED, value);
     x->line_solver = value;
+    return 0;
+  }
+  if (param >= ASH3D_TUNE_PART_SHAPE_X && param <= ASH3D_TUNE_PART_SHAPE_Z) {
+    if (value != 0 && value != 1) return fail(ASH3D_TR_EINVAL, "tile shape must be 0 or 1 (got %d)", value);
+    x->part_shape[param - ASH3D_TUNE_PART_SHAPE_X] = value;
     return 0;
   }
   if (param == ASH3D_TUNE_PART_BINS) {

@@ -174,7 +174,7 @@ struct PartLayout {
   long long st0, sta, stb, str;
   long long lr0, lra, lrb, lrr, lr_ncell;
 };
-hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg);
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape);
 // the LU factors of every line of the sweep (reads a.lr)
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom);
 // x-line solves on the original state layout (geometry, K and scratch in

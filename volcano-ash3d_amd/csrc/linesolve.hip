@@ -316,18 +316,31 @@ hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
 
 }  // namespace
 
-hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg) {
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape) {
   const int n = a.g.rmax - a.g.rmin + 1;
   PartArgs q;
   q.st0 = lay.st0; q.sta = lay.sta; q.stb = lay.stb; q.str = lay.str;
   q.lr0 = lay.lr0; q.lra = lay.lra; q.lrb = lay.lrb; q.lrr = lay.lrr; q.lr_ncell = lay.lr_ncell;
   q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
   if (n < 3 || n > 512) return hipErrorInvalidValue;
-  // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T
+  // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
+  // shape 1: contiguous lines two per workgroup (fewer waves held at each
+  // barrier), strided lines in wider tiles (longer coalesced row segments)
   if (lay.str == 1) {
+    if (shape == 1) {
+      if (n <= 256) return launch_part<64, 4, 2, true>(s, a, q);
+      if (n <= 384) return launch_part<64, 6, 2, true>(s, a, q);
+      return launch_part<64, 8, 2, true>(s, a, q);
+    }
     if (n <= 256) return launch_part<64, 4, 4, true>(s, a, q);
     if (n <= 384) return launch_part<64, 6, 4, true>(s, a, q);
     return launch_part<64, 8, 4, true>(s, a, q);
+  }
+  if (shape == 1) {
+    if (n <= 64) return launch_part<16, 4, 32, false>(s, a, q);
+    if (n <= 128) return launch_part<32, 4, 16, false>(s, a, q);
+    if (n <= 256) return launch_part<32, 8, 16, false>(s, a, q);
+    return launch_part<64, 8, 4, false>(s, a, q);
   }
   if (n <= 64) return launch_part<16, 4, 16, false>(s, a, q);
   if (n <= 128) return launch_part<32, 4, 16, false>(s, a, q);
