@@ -25,6 +25,8 @@
 // (coalesced both ways) while the next bin's tile is already in flight.
 // Sweeps with any line that is not a diagonally dominant M-matrix (negative
 // diffusivities, non-finite factors) are left to the dgtsv-order kernels.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace ash3d {
@@ -58,12 +60,41 @@ struct Lev {
   static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
 };
 
-template <int LP, int M, int T, bool CONTIG>
+// DPP lane moves of a double (two 32-bit halves); lanes whose source lies
+// outside the row read 0.  dpp_ctrl 0x110+h: row_shr:h (lane l reads l-h of
+// its 16-lane row); 0x100+h: row_shl:h; 0x138: wave_shr:1; 0x130: wave_shl:1.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// The value of segment p-H (up) / p+H (down) of the same line, for lanes
+// where that segment exists (others get an unspecified value and are masked
+// by the caller).  A line is LP consecutive lanes: with LP = 16 it is one
+// DPP row and every shift stays in the row; shifts by one use the
+// whole-wave DPP moves; the others go through the LDS crossbar.
+template <int LP, int H>
+__device__ inline double seg_up(double v) {
+  if constexpr (LP == 16) return dpp_mov<0x110 + H>(v);
+  else if constexpr (H == 1) return dpp_mov<0x138>(v);
+  else return __shfl_up(v, H, LP);
+}
+template <int LP, int H>
+__device__ inline double seg_down(double v) {
+  if constexpr (LP == 16) return dpp_mov<0x100 + H>(v);
+  else if constexpr (H == 1) return dpp_mov<0x130>(v);
+  else return __shfl_down(v, H, LP);
+}
+
+template <int LP, int M, int T, bool CONTIG, int PD>
 __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   constexpr int NT = T * LP;
   constexpr int NPAD = LP * M;
   constexpr int PITCH = NPAD + 1;
   constexpr int LEV = Lev<LP>::value;
+  static_assert(PD == 1 || PD == 2, "one or two bins in flight");
   extern __shared__ double s_tile[];  // T lines x PITCH
   // a sweep with lines that are not diagonally dominant M-matrix rows runs
   // the dgtsv-order kernel instead (launched after this one; uniform exit)
@@ -207,8 +238,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   auto elem_l = [&](int f) { return CONTIG ? f / n : f % T; };
   auto elem_i = [&](int f) { return CONTIG ? f % n : f / T; };
   const int nel = T * n;
-  double pf[PF];
-  auto issue = [&](int u) {
+  double pf[PD][PF];  // bins in flight, one register slot each
+  auto issue = [&](auto slot_tag, int u) {
+    constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
       const int f = tid + NT * m;
@@ -216,100 +248,195 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       const bool ok = f < nel && a_first + l_ <= g.Na;
       const long long o = Q.st0 + Q.sta * (a_first + l_) + Q.stb * b + Q.str * (long long)(rmin + i_) +
                           ncell * (long long)u;
-      pf[m] = A.c0[ok ? o : 0];
+      pf[sl][m] = A.c0[ok ? o : 0];
     }
   };
-  auto commit = [&]() {
+  auto commit = [&](auto slot_tag) {
+    constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
       const int f = tid + NT * m;
-      if (f < nel) s_tile[elem_l(f) * PITCH + elem_i(f)] = pf[m];
+      if (f < nel) s_tile[elem_l(f) * PITCH + elem_i(f)] = pf[sl][m];
+    }
+  };
+  auto next_on = [&](int u) {
+    while (u < u1 && !A.aloft.on(u)) ++u;
+    return u;
+  };
+  double *trow = s_tile + ln * PITCH;
+
+  // ---- solve this lane's segment of its line from the tile in LDS and
+  // leave the solution there
+  auto solve = [&]() {
+    double q[M + 2];
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      const int i = p * M + k - 1;
+      q[k] = trow[min(max(i, 0), n - 1)];
+    }
+    double dh[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double d = (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
+      dh[k] = al[k] * d;
+      if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
+    }
+#pragma unroll
+    for (int k = M - 3; k >= 1; --k) dh[k] = dh[k] - ga[k] * dh[k + 1];
+    dh[0] = rho * (dh[0] - ga[0] * dh[1]);
+    double D[2] = {dh[0], dh[M - 1]};
+    auto level = [&](auto lev_tag) {
+      constexpr int lev = decltype(lev_tag)::value;
+      constexpr int h = (lev == 0) ? 1 : 1 << (lev - 1);
+      double mD[2], pD[2];
+      if constexpr (lev == 0) {
+        const double uD = seg_up<LP, 1>(D[1]), dD = seg_down<LP, 1>(D[0]);
+        mD[0] = p >= 1 ? uD : 0.0;
+        pD[0] = D[1];
+        mD[1] = D[0];
+        pD[1] = p + 1 < LP ? dD : 0.0;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const double uD = seg_up<LP, h>(D[j]), dD = seg_down<LP, h>(D[j]);
+          mD[j] = p >= h ? uD : 0.0;
+          pD[j] = p + h < LP ? dD : 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) D[j] = D[j] - k1[lev][j] * mD[j] - k2[lev][j] * pD[j];
+    };
+    level(std::integral_constant<int, 0>{});
+    level(std::integral_constant<int, 1>{});
+    level(std::integral_constant<int, 2>{});
+    level(std::integral_constant<int, 3>{});
+    level(std::integral_constant<int, 4>{});
+    if constexpr (LEV > 5) level(std::integral_constant<int, 5>{});
+    if constexpr (LEV > 6) level(std::integral_constant<int, 6>{});
+    const double xf = D[0] * invB[0], xl = D[1] * invB[1];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int i = p * M + k;
+      const double x = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
+      if (i < n) trow[i] = x;
     }
   };
 
-  double *trow = s_tile + ln * PITCH;
-  int u = u0;
-  while (u < u1 && !A.aloft.on(u)) ++u;
-  if (u < u1) issue(u);
-  for (int w = u0; w < u1; ++w) {
-    const bool on = A.aloft.on(w);
-    __syncthreads();  // the previous bin's tile has been stored
-    if (on) {
-      commit();
-      __syncthreads();
-      int un = w + 1;
-      while (un < u1 && !A.aloft.on(un)) ++un;
-      if (un < u1) issue(un);
-      // ---- solve this lane's segment of its line
-      double q[M + 2];
-#pragma unroll
-      for (int k = 0; k < M + 2; ++k) {
-        const int i = p * M + k - 1;
-        q[k] = trow[min(max(i, 0), n - 1)];
-      }
-      double dh[M];
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        const double d = (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
-        dh[k] = al[k] * d;
-        if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
-      }
-#pragma unroll
-      for (int k = M - 3; k >= 1; --k) dh[k] = dh[k] - ga[k] * dh[k + 1];
-      dh[0] = rho * (dh[0] - ga[0] * dh[1]);
-      double D[2] = {dh[0], dh[M - 1]};
-#pragma unroll
-      for (int lev = 0; lev < LEV; ++lev) {
-        double mD[2], pD[2];
-        if (lev == 0) {
-          const double uD = __shfl_up(D[1], 1, LP), dD = __shfl_down(D[0], 1, LP);
-          mD[0] = p >= 1 ? uD : 0.0;
-          pD[0] = D[1];
-          mD[1] = D[0];
-          pD[1] = p + 1 < LP ? dD : 0.0;
-        } else {
-          const int h = 1 << (lev - 1);
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            const double uD = __shfl_up(D[j], h, LP), dD = __shfl_down(D[j], h, LP);
-            mD[j] = p >= h ? uD : 0.0;
-            pD[j] = p + h < LP ? dD : 0.0;
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) D[j] = D[j] - k1[lev][j] * mD[j] - k2[lev][j] * pD[j];
-      }
-      const double xf = D[0] * invB[0], xl = D[1] * invB[1];
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        const int i = p * M + k;
-        const double x = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
-        if (i < n) trow[i] = x;
-      }
-      __syncthreads();
-    }
-    // ---- store cells 1..N of the tile's lines: the solution on active
-    // cells of active lines, zero elsewhere (as the reference leaves them)
-    const int nst = T * N;
-    for (int f = tid; f < nst; f += NT) {
-      const int l_ = CONTIG ? f / N : f % T;
-      const int r_ = (CONTIG ? f % N : f / T) + 1;
+  // ---- stores.  Cells 1..N of the tile's lines hold, after the sweep, the
+  // solution on active cells of active lines of aloft bins and zero
+  // everywhere else (as the reference leaves them).  The zeros outside the
+  // active cells, and whole bins that are not aloft, are written first;
+  // the per-bin store of the active cells then has the fixed shape of the
+  // tile load (PF stores per lane, masked-off ones to the junk buffer), so
+  // that the compiler can count the memory operations in flight exactly
+  // and wait for one bin's loads without waiting for the next one's.
+  double *junk = A.junk + (tid + NT * (blockIdx.x % (kJunk / NT)));
+  auto zero_cells = [&](int w, int r_lo, int r_hi) {
+    const int len = r_hi - r_lo + 1;
+    if (len <= 0) return;
+    for (int f = tid; f < T * len; f += NT) {
+      const int l_ = CONTIG ? f / len : f % T;
+      const int r_ = (CONTIG ? f % len : f / T) + r_lo;
       const int al_ = a_first + l_;
-      if (al_ > g.Na) continue;
+      if (al_ <= g.Na) A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)r_ + ncell * (long long)w] = 0.0;
+    }
+  };
+  for (int w = u0; w < u1; ++w) {
+    if (!A.aloft.on(w)) {
+      zero_cells(w, 1, N);
+    } else {
+      zero_cells(w, 1, rmin - 1);
+      zero_cells(w, rmax + 1, N);
+    }
+  }
+  auto store = [&](int w) {
+    auto one = [&](int f, bool in) {
+      const int l_ = elem_l(f), i_ = elem_i(f);
+      const int al_ = a_first + l_;
+      const bool ok = in && al_ <= g.Na;
       const bool lon = al_ >= g.a0 && al_ <= g.a1 && b >= g.b0 && b <= g.b1;
-      const bool act = on && lon && r_ >= rmin && r_ <= rmax;
-      const double v = act ? s_tile[l_ * PITCH + (r_ - rmin)] : 0.0;
-      A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)r_ + ncell * (long long)w] = v;
+      const double v = (ok && lon) ? s_tile[l_ * PITCH + i_] : 0.0;
+      const long long o = Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)(rmin + i_) + ncell * (long long)w;
+      *(ok ? A.c0 + o : junk) = v;
+    };
+    if constexpr (PD == 2) {
+#pragma unroll
+      for (int m = 0; m < PF; ++m) one(tid + NT * m, tid + NT * m < nel);
+    } else {
+      // one bin in flight: a plain loop (fewer registers)
+      for (int f = tid; f < nel; f += NT) {
+        const int l_ = elem_l(f), i_ = elem_i(f);
+        const int al_ = a_first + l_;
+        if (al_ > g.Na) continue;
+        const bool lon = al_ >= g.a0 && al_ <= g.a1 && b >= g.b0 && b <= g.b1;
+        A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)(rmin + i_) + ncell * (long long)w] =
+            lon ? s_tile[l_ * PITCH + i_] : 0.0;
+      }
+    }
+  };
+
+  if constexpr (PD == 1) {
+    int u = next_on(u0);
+    if (u < u1) issue(std::integral_constant<int, 0>{}, u);
+    while (u < u1) {
+      __syncthreads();  // the previous bin's tile has been stored
+      commit(std::integral_constant<int, 0>{});
+      __syncthreads();
+      const int un = next_on(u + 1);
+      if (un < u1) issue(std::integral_constant<int, 0>{}, un);
+      solve();
+      __syncthreads();
+      store(u);
+      u = un;
+    }
+  } else {
+    // the aloft bins two at a time, slot 0 then slot 1, so that each slot
+    // stays in its registers; a load past the last aloft bin re-reads the
+    // last one (its tile was just read: a cache hit) instead of being
+    // skipped, which would make the slot's contents depend on the path
+    int nact = 0, last = u0;
+    for (int w = u0; w < u1; ++w)
+      if (A.aloft.on(w)) { ++nact; last = w; }
+    auto nxt = [&](int u) {
+      const int v = next_on(u);
+      return v < u1 ? v : last;
+    };
+    int ua = next_on(u0);
+    int ub = nxt(ua + 1);
+    if (nact > 0) {
+      issue(std::integral_constant<int, 0>{}, ua);
+      issue(std::integral_constant<int, 1>{}, ub);
+    }
+    for (int k = 0; k < nact; k += 2) {
+      __syncthreads();  // the previous bin's tile has been stored
+      commit(std::integral_constant<int, 0>{});
+      __syncthreads();
+      const int uc = nxt(ub + 1);
+      issue(std::integral_constant<int, 0>{}, uc);
+      solve();
+      __syncthreads();
+      store(ua);
+      ua = uc;
+      if (k + 1 >= nact) break;
+      __syncthreads();
+      commit(std::integral_constant<int, 1>{});
+      __syncthreads();
+      const int ud = nxt(uc + 1);
+      issue(std::integral_constant<int, 1>{}, ud);
+      solve();
+      __syncthreads();
+      store(ub);
+      ub = ud;
     }
   }
 }
 
-template <int LP, int M, int T, bool CONTIG>
+template <int LP, int M, int T, bool CONTIG, int PD = (M <= 6 ? 2 : 1)>
 hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
   const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
   const size_t lds = sizeof(double) * (size_t)T * (LP * M + 1);
-  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
+  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
                      q);
   return hipGetLastError();
 }
