@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "lanes.hpp"
 
 namespace ash3d {
 
@@ -59,34 +60,6 @@ template <int LP>
 struct Lev {
   static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
 };
-
-// DPP lane moves of a double (two 32-bit halves); lanes whose source lies
-// outside the row read 0.  dpp_ctrl 0x110+h: row_shr:h (lane l reads l-h of
-// its 16-lane row); 0x100+h: row_shl:h; 0x138: wave_shr:1; 0x130: wave_shl:1.
-template <int CTRL>
-__device__ inline double dpp_mov(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
-
-// The value of segment p-H (up) / p+H (down) of the same line, for lanes
-// where that segment exists (others get an unspecified value and are masked
-// by the caller).  A line is LP consecutive lanes: with LP = 16 it is one
-// DPP row and every shift stays in the row; shifts by one use the
-// whole-wave DPP moves; the others go through the LDS crossbar.
-template <int LP, int H>
-__device__ inline double seg_up(double v) {
-  if constexpr (LP == 16) return dpp_mov<0x110 + H>(v);
-  else if constexpr (H == 1) return dpp_mov<0x138>(v);
-  else return __shfl_up(v, H, LP);
-}
-template <int LP, int H>
-__device__ inline double seg_down(double v) {
-  if constexpr (LP == 16) return dpp_mov<0x100 + H>(v);
-  else if constexpr (H == 1) return dpp_mov<0x130>(v);
-  else return __shfl_down(v, H, LP);
-}
 
 template <int LP, int M, int T, bool CONTIG, int PD>
 __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
@@ -290,7 +263,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       constexpr int h = (lev == 0) ? 1 : 1 << (lev - 1);
       double mD[2], pD[2];
       if constexpr (lev == 0) {
-        const double uD = seg_up<LP, 1>(D[1]), dD = seg_down<LP, 1>(D[0]);
+        double uD, dD, t;
+        lane_updown<LP, 1>(D[1], uD, t);
+        lane_updown<LP, 1>(D[0], t, dD);
         mD[0] = p >= 1 ? uD : 0.0;
         pD[0] = D[1];
         mD[1] = D[0];
@@ -298,7 +273,8 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       } else {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const double uD = seg_up<LP, h>(D[j]), dD = seg_down<LP, h>(D[j]);
+          double uD, dD;
+          lane_updown<LP, h>(D[j], uD, dD);
           mD[j] = p >= h ? uD : 0.0;
           pD[j] = p + h < LP ? dD : 0.0;
         }
