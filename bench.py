@@ -100,7 +100,7 @@ def cpu_baseline(st, use_diffusion, steps):
                       f"{st.nx}x{st.ny}x{st.nz}x{st.ns} workload, {dt:.1f} s"}
 
 
-def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
+def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="exact", tuning=None):
     """Time a source-inversion gradient on the same grid: forward run with
     checkpoints, then the checkpointed reverse sweep.  Returns the
     adjoint/forward time ratio and the per-step times of ash3d_tr_adj_step
@@ -119,6 +119,8 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
         inv = adj.SourceInversion(st, st.nx // 4, st.ny // 2, n_heights=H, n_periods=periods,
                                   steps_per_period=steps_per_period, frac=frac, dz=dz,
                                   obs_every=steps_per_period // 2, use_diffusion=use_diffusion, segment=segment)
+        # the forward leg's line solver and launch settings
+        inv.T.tune(line_solver=line_solver, **{k: v[0] for k, v in (tuning or {}).items()})
         w = np.random.default_rng(1).random((len(inv.obs_steps), st.nx, st.ny))
         inv.forward(s)  # warm-up (allocations, first touch)
         inv.gradient(w)
@@ -161,6 +163,7 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period):
     inv.close()
     return {"workload": f"source inversion, {periods} periods x {steps_per_period} steps, {H} source heights, "
                         f"column loads every {steps_per_period // 2} steps, every step input kept on the device",
+            "line_solver": line_solver,
             "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
             "ratio_sqrt_checkpointing": round(g_rep / f_rep, 3), "sqrt_segment": K_sqrt,
             "adj_step_ms": round(1e3 * t_adj, 3), "step_ms": round(1e3 * t_step, 3),
@@ -350,7 +353,8 @@ def main():
     adjoint = None
     if adj_state is not None:
         log("[bench] adjoint leg ...")
-        adjoint = adjoint_leg(pkg, adj_state, use_diffusion, args.adj_periods, args.adj_steps_per_period)
+        adjoint = adjoint_leg(pkg, adj_state, use_diffusion, args.adj_periods, args.adj_steps_per_period,
+                              args.line_solver, tuning)
         log("[bench] adjoint: " + json.dumps(adjoint))
     cpu = None
     if cpu_state is not None:

@@ -51,8 +51,9 @@ def forward(oracle, st, calls):
     return interior(a, a.concen_pd[..., 0]).copy()
 
 
-def adjoint(tr, st, lam, method, *args):
+def adjoint(tr, st, lam, method, *args, line_solver="exact", **tune):
     T = tr.Transport(st.copy())
+    T.tune(line_solver=line_solver, **tune)
     T.adj_init(0)
     T.adj_set(lam)
     getattr(T, method)(*args)
@@ -116,29 +117,37 @@ def test_adjoint_long_x_rows(pkg, oracle, tr):
     check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_advect", 0))
 
 
+SOLVERS = ["exact", "partitioned"]
+
+
 @pytest.mark.parametrize("case", [0, 2])
 @pytest.mark.parametrize("use_cn", [True, False])
 @pytest.mark.parametrize("dirn", [0, 1, 2])
-def test_adjoint_diffusion_sweeps(pkg, oracle, tr, case, use_cn, dirn):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_adjoint_diffusion_sweeps(pkg, oracle, tr, case, use_cn, dirn, solver):
+    if solver == "partitioned" and not use_cn:
+        pytest.skip("the line solver only applies to theta-scheme lines")
     st = pkg.workloads.random_state(12, 10, 9, 3, seed=500 + case + dirn, **CASES[case])
     name = ("diffCN_" if use_cn else "diff_") + "xyz"[dirn]
     Fq = forward(oracle, st, [(name,)])
     lam = rand_lambda(st, 23)
-    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, use_cn))
+    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, use_cn, line_solver=solver))
 
 
 @pytest.mark.parametrize("theta", [0.0, 0.5, 1.0])
-def test_adjoint_cn_theta_and_subgrid(pkg, oracle, tr, theta):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_adjoint_cn_theta_and_subgrid(pkg, oracle, tr, theta, solver):
     st = pkg.workloads.random_state(12, 11, 10, 4, seed=77, latlon=True, imp_fac=theta)
     st.bounds = (2, 10, 3, 11, 2, 9)
     st.is_aloft = np.array([1, 1, 0, 1], dtype=np.int32)
     for dirn in range(3):
         Fq = forward(oracle, st, [("diffCN_" + "xyz"[dirn],)])
         lam = rand_lambda(st, 31 + dirn)
-        check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True))
+        check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True, line_solver=solver))
 
 
-def test_adjoint_cn_pivoting(pkg, oracle, tr):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_adjoint_cn_pivoting(pkg, oracle, tr, solver):
     # strong implicit coupling with tiny cells: rows where dgtsv swaps
     st = pkg.workloads.random_state(10, 9, 12, 2, seed=3, latlon=False, imp_fac=1.0)
     st.kz[...] *= 4000.0
@@ -146,19 +155,49 @@ def test_adjoint_cn_pivoting(pkg, oracle, tr):
     for dirn in (2,):
         Fq = forward(oracle, st, [("diffCN_z",)])
         lam = rand_lambda(st, 2)
-        check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True))
+        check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True, line_solver=solver))
+
+
+# line lengths that select each tile shape of the partitioned solves
+# (lanes per line x rows per lane: 16x4, 32x4, 64x4, 64x6, 64x8)
+@pytest.mark.parametrize("n", [9, 70, 200, 300, 420])
+@pytest.mark.parametrize("dirn", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1])
+def test_adjoint_partitioned_line_lengths(pkg, oracle, tr, n, dirn, shape):
+    dims = [7, 6, 5]
+    dims[dirn] = n
+    st = pkg.workloads.random_state(*dims, 3, seed=40 + n + dirn, latlon=True, imp_fac=0.5)
+    st.is_aloft = np.array([1, 0, 1], dtype=np.int32)
+    Fq = forward(oracle, st, [("diffCN_" + "xyz"[dirn],)])
+    lam = rand_lambda(st, 61 + dirn)
+    knob = "part_shape_" + "xyz"[dirn]
+    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True, line_solver="partitioned",
+                                   **{knob: shape}))
+
+
+def test_adjoint_partitioned_falls_back_on_negative_diffusivity(pkg, oracle, tr):
+    """A line with a negative diffusivity is not an M-matrix row: the sweep
+    runs the dgtsv-order adjoint (the device-side choice), whose result is
+    the exact-solver adjoint bit for bit."""
+    st = pkg.workloads.random_state(11, 9, 8, 2, seed=5, latlon=False, imp_fac=0.5)
+    st.ky[4, :, 3] = -0.3 * np.abs(st.ky[4, :, 3])
+    lam = rand_lambda(st, 8)
+    g_exact = adjoint(tr, st, lam, "adj_diffuse", 1, True)
+    g_part = adjoint(tr, st, lam, "adj_diffuse", 1, True, line_solver="partitioned")
+    assert np.array_equal(g_exact, g_part)
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("itime", [1, 2])
-@pytest.mark.parametrize("mode", ["cn", "explicit", "none"])
+@pytest.mark.parametrize("mode", ["cn", "cn_partitioned", "explicit", "none"])
 def test_adjoint_step(pkg, oracle, tr, case, itime, mode):
     st = pkg.workloads.random_state(12, 10, 8, 3, seed=900 + case + 3 * itime, **CASES[case])
-    st.use_cn = mode == "cn"
+    st.use_cn = mode.startswith("cn")
     use_diff = mode != "none"
     Fq = forward(oracle, st, [("transport_step", itime, use_diff)])
     lam = rand_lambda(st, 99)
-    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_step", itime, use_diff))
+    solver = "partitioned" if mode == "cn_partitioned" else "exact"
+    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_step", itime, use_diff, line_solver=solver))
 
 
 def test_adjoint_step_finite_differences(pkg, oracle, tr):

@@ -347,6 +347,9 @@ struct RowC {
 
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
+  // the partitioned adjoint solves ran (launched before this kernel) when
+  // every line of the sweep is a diagonally dominant M-matrix (uniform exit)
+  if (A.sel == 2 && *A.ndom == 0) return;
   const Dims d = A.d;
   const LineGeom g = A.g;
   const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);

@@ -358,7 +358,7 @@ int ensure_fac(ash3d_tr_ctx *x, int nb) {
 // (the partitioned kernel then returns at once).  a holds the sweep's
 // geometry (for x: the x<->y transposed one, which the row factor cache
 // uses) and the original state in a.c0.
-int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
+int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjoint = false) {
   *rc = 0;
   const int n = a.g.rmax - a.g.rmin + 1;
   a.sel = 0;
@@ -377,8 +377,9 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc) {
     lay.lr0 = lay.st0; lay.lra = lay.sta; lay.lrb = lay.stb; lay.lrr = lay.str;
   }
   lay.lr_ncell = a.d.ncell;
+  lay.kap = x->kap;
   a.sel = 1;
-  hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins, x->part_shape[dir]);
+  hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins, x->part_shape[dir], adjoint);
   if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
   a.sel = 2;
   return 1;
@@ -509,19 +510,34 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
   double *lam = x->lam;
   const bool transposed = (dir == 0);
   if (transposed) {
+    // x lines as the y lines of the x<->y transposed grid
     const Dims &d = x->d;
     if ((rc = ensure_tgeom(x))) return rc;
     if (!x->tq && (rc = dev_alloc(&x->tq, x->n_conc()))) return rc;
-    {
-      ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
-      HIPCHK(launch_transpose_xy(x->stream, x->lam, x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns));
-    }
     const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
     a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
     a.g = make_line_geom(a.d, 1, tb);
     a.kap = x->tkap;
     a.sig = x->tsx;
     a.kk = x->tkx;
+  }
+  // partitioned adjoint solves in place on lambda (x lines read along the
+  // rows of the original layout); the dgtsv-order kernels below return at
+  // once on the device when those ran
+  const int *skip = nullptr;
+  if (use_cn && x->line_solver == ASH3D_LINE_SOLVER_PARTITIONED) {
+    if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
+    a.c0 = x->lam;
+    ProfScope ps(x, ASH3D_PROF_ADJ_DIFFUSE);
+    if (try_cn_partitioned(x, dir, a, &rc, true)) {
+      if (rc) return rc;
+      skip = a.ndom;
+    }
+  }
+  if (transposed) {
+    const Dims &d = x->d;
+    ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
+    HIPCHK(launch_transpose_xy(x->stream, x->lam, x->tq, d.nx + 4, d.ny + 4, (long long)(d.nz + 4) * d.ns, skip));
     lam = x->tq;
   }
   {
@@ -538,7 +554,7 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
   if (transposed) {
     ProfScope ps(x, ASH3D_PROF_TRANSPOSE);
     HIPCHK(launch_transpose_xy(x->stream, x->tq, x->lam, x->d.ny + 4, x->d.nx + 4,
-                               (long long)(x->d.nz + 4) * x->d.ns));
+                               (long long)(x->d.nz + 4) * x->d.ns, skip));
   }
   return 0;
 }

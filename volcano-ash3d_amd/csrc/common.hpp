@@ -173,8 +173,12 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr);
 struct PartLayout {
   long long st0, sta, stb, str;
   long long lr0, lra, lrb, lrr, lr_ncell;
+  const double *kap = nullptr;  // cell volumes in the state's layout (adjoint only)
 };
-hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape);
+// adjoint: lambda <- K B A^{-1} K^{-1} lambda (= B^T A^{-T} lambda, since
+// kappa_l R_l = kappa_{l+1} L_{l+1} makes K^{-1} A and K^{-1} B symmetric)
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
+                               bool adjoint = false);
 // the LU factors of every line of the sweep (reads a.lr)
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom);
 // x-line solves on the original state layout (geometry, K and scratch in
@@ -182,7 +186,9 @@ hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t 
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);
 int pick_bins_per_thread(int ns, int kind, int dir);
 long long mass_partials_size(const Dims &d);
-hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes);
+// skip_if_zero: when given, the launch does nothing if *skip_if_zero == 0
+hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes,
+                               const int *skip_if_zero = nullptr);
 hipError_t launch_mass_totals(hipStream_t s, const Dims &d, const double *c0, const double *kap,
                               const double *dep, const double *yz1, const double *yz2, const double *xz1,
                               const double *xz2, const double *xy2, double *partial, double *out);

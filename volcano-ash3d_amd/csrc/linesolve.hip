@@ -40,7 +40,8 @@ namespace {
 struct PartArgs {
   long long st0, sta, stb, str;
   long long lr0, lra, lrb, lrr, lr_ncell;
-  int bins_wg;  // bins per workgroup
+  const double *kap;  // cell volumes at the state's cell offsets (adjoint)
+  int bins_wg;        // bins per workgroup
 };
 
 // Workgroup b runs on XCD b mod 8.  Each XCD takes a contiguous range of
@@ -61,7 +62,7 @@ struct Lev {
   static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
 };
 
-template <int LP, int M, int T, bool CONTIG, int PD>
+template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ>
 __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   constexpr int NT = T * LP;
   constexpr int NPAD = LP * M;
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   double ga[M];                 // backward: dh_k -= ga_k*dh_{k+1}
   double ah[M], ch[M];          // final rows: ah_k*x_first + x_k + ch_k*x_last = dh_k
   double rho;
+  double kr[M], rk[M];  // adjoint: kappa of each row and its inverse
   {
     double av[M], bv[M], cv[M];
 #pragma unroll
@@ -128,6 +130,10 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       } else {
         av[k] = 0.0; bv[k] = 1.0; cv[k] = 0.0;
         rL[k] = rR[k] = 0.0;
+      }
+      if (ADJ) {
+        kr[k] = Q.kap[Q.st0 + Q.sta * ac + Q.stb * b + Q.str * (long long)(rmin + (real ? i0 : 0))];
+        rk[k] = 1.0 / kr[k];
       }
     }
 #pragma unroll
@@ -250,7 +256,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
     double dh[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-      const double d = (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
+      // forward: the explicit half-step B q; adjoint: K^{-1} lambda
+      const double d = ADJ ? q[k + 1] * rk[k]
+                           : (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
       dh[k] = al[k] * d;
       if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
     }
@@ -290,11 +298,32 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
     if constexpr (LEV > 5) level(std::integral_constant<int, 5>{});
     if constexpr (LEV > 6) level(std::integral_constant<int, 6>{});
     const double xf = D[0] * invB[0], xl = D[1] * invB[1];
+    double xs[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const int i = p * M + k;
-      const double x = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
-      if (i < n) trow[i] = x;
+    for (int k = 0; k < M; ++k) xs[k] = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
+    if constexpr (ADJ) {
+      // lambda' = K B u with u = A^{-1} K^{-1} lambda: B's stencil needs the
+      // last row of the segment before and the first row of the one after;
+      // the line ends take u_0 = u_1 and u_{n+1} = u_n as B does
+      double xprev, xnext, t;
+      lane_updown<LP, 1>(xs[M - 1], xprev, t);
+      lane_updown<LP, 1>(xs[0], t, xnext);
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int i = p * M + k;
+        double xm = (k == 0) ? xprev : xs[k > 0 ? k - 1 : 0];
+        double xp = (k == M - 1) ? xnext : xs[k < M - 1 ? k + 1 : 0];
+        if (i == 0) xm = xs[k];
+        if (i == n - 1) xp = xs[k];
+        const double v = (omt * rL[k]) * xm + (1.0 - omt * (rL[k] + rR[k])) * xs[k] + (omt * rR[k]) * xp;
+        if (i < n) trow[i] = kr[k] * v;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int i = p * M + k;
+        if (i < n) trow[i] = xs[k];
+      }
     }
   };
 
@@ -407,48 +436,61 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   }
 }
 
-template <int LP, int M, int T, bool CONTIG, int PD = (M <= 6 ? 2 : 1)>
+template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = ((M <= 4 || (M <= 6 && !ADJ)) ? 2 : 1)>
 hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
   const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
   const size_t lds = sizeof(double) * (size_t)T * (LP * M + 1);
-  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
+  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
                      q);
   return hipGetLastError();
 }
 
+template <bool ADJ>
+hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArgs &q, bool contig, int n, int shape) {
+  // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
+  // shape 1: contiguous lines two per workgroup (fewer waves held at each
+  // barrier), strided lines in wider tiles (longer coalesced row segments)
+  if (contig) {
+    if (shape == 1) {
+      if (n <= 256) return launch_part<64, 4, 2, true, ADJ>(s, a, q);
+      if (n <= 384) return launch_part<64, 6, 2, true, ADJ>(s, a, q);
+      return launch_part<64, 8, 2, true, ADJ>(s, a, q);
+    }
+    if (n <= 256) return launch_part<64, 4, 4, true, ADJ>(s, a, q);
+    if (n <= 384) return launch_part<64, 6, 4, true, ADJ>(s, a, q);
+    return launch_part<64, 8, 4, true, ADJ>(s, a, q);
+  }
+  // (the adjoint keeps two more values per row: 4 rows per lane up to 256)
+  if (shape == 1) {
+    if (n <= 64) return launch_part<16, 4, 32, false, ADJ>(s, a, q);
+    if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
+    if (n <= 256) {
+      if constexpr (ADJ) return launch_part<64, 4, 8, false, ADJ>(s, a, q);
+      else return launch_part<32, 8, 16, false, ADJ>(s, a, q);
+    }
+    return launch_part<64, 8, 4, false, ADJ>(s, a, q);
+  }
+  if (n <= 64) return launch_part<16, 4, 16, false, ADJ>(s, a, q);
+  if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
+  if (n <= 256) return launch_part<64, 4, 8, false, ADJ>(s, a, q);
+  return launch_part<64, 8, 4, false, ADJ>(s, a, q);
+}
+
 }  // namespace
 
-hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape) {
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
+                               bool adjoint) {
   const int n = a.g.rmax - a.g.rmin + 1;
   PartArgs q;
   q.st0 = lay.st0; q.sta = lay.sta; q.stb = lay.stb; q.str = lay.str;
   q.lr0 = lay.lr0; q.lra = lay.lra; q.lrb = lay.lrb; q.lrr = lay.lrr; q.lr_ncell = lay.lr_ncell;
+  q.kap = lay.kap;
   q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
   if (n < 3 || n > 512) return hipErrorInvalidValue;
-  // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
-  // shape 1: contiguous lines two per workgroup (fewer waves held at each
-  // barrier), strided lines in wider tiles (longer coalesced row segments)
-  if (lay.str == 1) {
-    if (shape == 1) {
-      if (n <= 256) return launch_part<64, 4, 2, true>(s, a, q);
-      if (n <= 384) return launch_part<64, 6, 2, true>(s, a, q);
-      return launch_part<64, 8, 2, true>(s, a, q);
-    }
-    if (n <= 256) return launch_part<64, 4, 4, true>(s, a, q);
-    if (n <= 384) return launch_part<64, 6, 4, true>(s, a, q);
-    return launch_part<64, 8, 4, true>(s, a, q);
-  }
-  if (shape == 1) {
-    if (n <= 64) return launch_part<16, 4, 32, false>(s, a, q);
-    if (n <= 128) return launch_part<32, 4, 16, false>(s, a, q);
-    if (n <= 256) return launch_part<32, 8, 16, false>(s, a, q);
-    return launch_part<64, 8, 4, false>(s, a, q);
-  }
-  if (n <= 64) return launch_part<16, 4, 16, false>(s, a, q);
-  if (n <= 128) return launch_part<32, 4, 16, false>(s, a, q);
-  if (n <= 256) return launch_part<64, 4, 8, false>(s, a, q);
-  return launch_part<64, 8, 4, false>(s, a, q);
+  if (adjoint && !q.kap) return hipErrorInvalidValue;
+  return adjoint ? launch_cn_part_shapes<true>(s, a, q, lay.str == 1, n, shape)
+                 : launch_cn_part_shapes<false>(s, a, q, lay.str == 1, n, shape);
 }
 
 }  // namespace ash3d

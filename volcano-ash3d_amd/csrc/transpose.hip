@@ -13,8 +13,10 @@ constexpr int kT = 32;
 
 // dst[p*dps + b + B*a] = src[p*sps + a + A*b] for a < A, b < B, plane p.
 __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ src, double *__restrict__ dst,
-                                                   int A, int B, long long sps, long long dps) {
+                                                   int A, int B, long long sps, long long dps,
+                                                   const int *skip_if_zero) {
   __shared__ double tile[kT][kT + 1];
+  if (skip_if_zero && *skip_if_zero == 0) return;  // uniform
   const long long p = blockIdx.z;
   const int a0 = blockIdx.x * kT, b0 = blockIdx.y * kT;
   const double *s = src + p * sps;
@@ -32,10 +34,11 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ sr
 
 }  // namespace
 
-hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes) {
+hipError_t launch_transpose_xy(hipStream_t s, const double *src, double *dst, int A, int B, long long nplanes,
+                               const int *skip_if_zero) {
   const long long ps = (long long)A * B;
   dim3 grid((unsigned)((A + kT - 1) / kT), (unsigned)((B + kT - 1) / kT), (unsigned)nplanes);
-  hipLaunchKernelGGL(k_transpose, grid, dim3(kT, 8), 0, s, src, dst, A, B, ps, ps);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(kT, 8), 0, s, src, dst, A, B, ps, ps, skip_if_zero);
   return hipGetLastError();
 }
 
