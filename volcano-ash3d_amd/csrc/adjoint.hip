@@ -189,6 +189,190 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_lines(AdvArgs A, const do
   }
 }
 
+// ---- y and z: segmented lines, NB bins per lane -----------------------
+// The layout of the forward streamed sweep: a workgroup is 64 lanes
+// (consecutive transverse index a, coalesced) x W segments of the same
+// lines; segment w emits the adjoint of cells [lo, hi] (a partition of
+// 1..N) and streams cells lo-3 .. hi+3, since cell c collects interfaces
+// c-1 .. c+2 and interface I reads cells I-2 .. I+1.  lambda is updated in
+// place: the halo cells lo-2, lo-1, hi+1, hi+2 (emitted by the
+// neighbouring segments) are read into registers before a barrier.  The
+// pending sums of each cell are formed in the same order as the one-lane
+// kernel above (e3, e2, e1, e0, then the direct term).
+template <int DIR, int NB, bool TOPO>
+__global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double *__restrict__ qpre, double *lam) {
+  constexpr int PF = 2;  // cells in flight
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int ntile = (g.Na + 63) / 64;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
+  const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
+  const int b = (int)(tg.tile / ntile) + 1;
+  const bool lane_ok = a <= g.Na;
+  const int W = blockDim.y, w = threadIdx.y;
+  const int n0 = tg.group * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int seg = (N + W - 1) / W;
+  const int lo = 1 + w * seg, hi = min(N, lo + seg - 1);
+  const long long sL = line_stride(d, DIR);
+  const long long base = lane_ok ? line_base(d, DIR, a, b) : 0;
+  const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  double *junk = A.junk + (threadIdx.x + 64 * threadIdx.y + 256 * (blockIdx.x % (kJunk / 256)));
+
+  bool exists[NB], on[NB];
+  long long bofs[NB];  // bin offset (an existing bin for the others)
+  bool any_on = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int n = n0 + u;
+    exists[u] = lane_ok && n < d.ns;
+    on[u] = exists[u] && line_on && A.aloft.on(n);
+    any_on = any_on || on[u];
+    bofs[u] = d.ncell * (long long)min(n, d.ns - 1);
+  }
+  const int ldlo = rmin - 2, ldhi = rmax + 2;  // cells the forward gathers
+  auto act = [&](int r) { return r >= rmin && r <= rmax; };
+  // lambda of the halo cells (zero where mu and the direct term vanish)
+  double hl0[NB], hl1[NB], hl2[NB], hl3[NB];
+  auto halo = [&](int r, int u) {
+    return (any_on && exists[u] && act(r)) ? lam[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
+  };
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    hl0[u] = halo(lo - 2, u);
+    hl1[u] = halo(lo - 1, u);
+    hl2[u] = halo(hi + 1, u);
+    hl3[u] = halo(hi + 2, u);
+  }
+  if (W > 1) __syncthreads();
+
+  if (any_on && lo <= hi) {
+    const double dt = A.dt;
+    const int lim = A.limiter;
+    // ring of PF cells in flight: raw loads from the cell clamped into the
+    // gathered range (values outside it are replaced when consumed)
+    double rs_vel[PF], rs_vh[PF], rs_sig[PF], rs_kap[PF], rs_D[PF], rs_q[PF][NB], rs_l[PF][NB], rs_vf[PF][NB];
+    auto fetch = [&](int slot, int r) {
+      const int rr = min(max(r, ldlo), ldhi);
+      const long long cell = base + sL * (long long)(rr + 1);
+      rs_vel[slot] = A.vel[cell];
+      if (DIR == 2) rs_vh[slot] = A.vh[cell];
+      rs_sig[slot] = A.sig[cell];
+      rs_kap[slot] = A.kap[cell];
+      if (TOPO) {
+        const int rd = min(max(r, rmin - 1), rmax + 1);
+        rs_D[slot] = A.deld[(long long)(a + 1) + (long long)(d.nx + 4) * (rd + 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        rs_q[slot][u] = qpre[cell + bofs[u]];
+        rs_l[slot][u] = lam[cell + bofs[u]];
+        if (DIR == 2) rs_vf[slot][u] = A.vf[cell + bofs[u]];
+      }
+    };
+    // windows (suffix 1 = r-1, 2 = r-2, 3 = r-3)
+    double q1[NB], q2[NB], q3[NB], mu1[NB], mu2[NB], dr1[NB], dr2[NB], dr3[NB];
+    double acc0[NB], acc1[NB], acc2[NB];
+    double vb1[NB], ub1[NB], ub2[NB];  // z: per-bin velocity and usig
+    double vel1 = 0.0, sig1 = 0.0, u1 = 0.0, u2 = 0.0, dtk1 = 0.0, D1 = 0.0, tf2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      q1[u] = q2[u] = q3[u] = mu1[u] = mu2[u] = dr1[u] = dr2[u] = dr3[u] = 0.0;
+      acc0[u] = acc1[u] = acc2[u] = 0.0;
+      vb1[u] = ub1[u] = ub2[u] = 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < PF; ++p) fetch(p, lo - 3 + p);
+    for (int r0 = lo - 3; r0 <= hi + 3; r0 += PF) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const int r = r0 + p;
+        const bool ld = r >= ldlo && r <= ldhi;
+        const bool uin = r >= rmin - 1 && r <= rmax + 1;
+        const double sig0 = ld ? rs_sig[p] : 0.0;
+        const double vel0 = ld ? rs_vel[p] : 0.0;
+        const double vzh0 = (DIR == 2 && ld) ? rs_vel[p] + rs_vh[p] : 0.0;
+        const double dtk0 = dt / rs_kap[p];
+        const double D0 = (TOPO && uin) ? rs_D[p] : 0.0;
+        double q0[NB], l0[NB], vf0[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          q0[u] = (ld && exists[u]) ? rs_q[p][u] : 0.0;
+          double v = rs_l[p][u];
+          v = (r == lo - 2) ? hl0[u] : v;
+          v = (r == lo - 1) ? hl1[u] : v;
+          v = (r == hi + 1) ? hl2[u] : v;
+          v = (r == hi + 2) ? hl3[u] : v;
+          l0[u] = v;
+          vf0[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
+        }
+        fetch(p, r + PF);
+        // usig of interface r (:159-162), tf*D of cell r-1 (:252-257)
+        const double u0 = (DIR != 2 && uin) ? 0.5 * (vel1 + vel0) * sig0 : 0.0;
+        const bool uin1 = r - 1 >= rmin - 1 && r - 1 <= rmax + 1;
+        const double tf1 = (TOPO && uin1) ? 0.5 * vel1 * (sig1 + sig0) * D1 : 0.0;
+        // interface I = r-1
+        const int I = r - 1;
+        const bool ifc = I >= rmin && I <= rmax + 1;
+        const int cc = r - 3;  // the cell whose sum is complete
+        const bool st = cc >= lo && cc <= hi;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          double vb0 = 0.0, ub0 = 0.0;
+          if (DIR == 2) {
+            if (ld && exists[u]) vb0 = vzh0 + vf0[u];
+            ub0 = uin ? 0.5 * (vb1[u] + vb0) * sig0 : 0.0;
+          }
+          const double uI = (DIR == 2) ? ub1[u] : u1;    // usig_I
+          const double uIm = (DIR == 2) ? ub2[u] : u2;   // usig_{I-1}
+          const double uIp = (DIR == 2) ? ub0 : u0;      // usig_{I+1}
+          double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+          if (ifc) {
+            const double divp = fmax(0.0, uI) - fmax(0.0, uIm);
+            const double divm = fmin(0.0, uIp) - fmin(0.0, uI);
+            const double aus = fabs(uI);
+            const double gI = 0.5 * aus * (1.0 - dtk1 * aus);
+            interface_adjoint(lim, q3[u], q2[u], q1[u], q0[u], uI, divp, divm, gI, tf1, tf2, mu2[u], mu1[u], e0,
+                              e1, e2, e3);
+          }
+          const double v = acc0[u] + e0;
+          *((st && on[u]) ? lam + base + sL * (long long)(cc + 1) + bofs[u] : junk) = dr3[u] + v;
+          acc0[u] = acc1[u] + e1;
+          acc1[u] = acc2[u] + e2;
+          acc2[u] = e3;
+          // mu_r = -dt/kappa_r lambda_r and the direct term of cell r
+          const double mu0 = act(r) ? -dtk0 * l0[u] : 0.0;
+          const double dr0 = act(r) ? l0[u] : 0.0;
+          q3[u] = q2[u]; q2[u] = q1[u]; q1[u] = q0[u];
+          mu2[u] = mu1[u]; mu1[u] = mu0;
+          dr3[u] = dr2[u]; dr2[u] = dr1[u]; dr1[u] = dr0;
+          if (DIR == 2) {
+            ub2[u] = ub1[u];
+            ub1[u] = ub0;
+            vb1[u] = vb0;
+          }
+        }
+        u2 = u1;
+        u1 = u0;
+        tf2 = tf1;
+        vel1 = vel0;
+        sig1 = sig0;
+        dtk1 = dtk0;
+        D1 = D0;
+      }
+    }
+  }
+  // inactive bins and lines leave the sweep as zero
+  if (lo <= hi) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      if (!exists[u] || on[u]) continue;
+      for (int r = lo; r <= hi; ++r) lam[base + sL * (long long)(r + 1) + bofs[u]] = 0.0;
+    }
+  }
+}
+
 // ---- x: one workgroup per row, gather through LDS ---------------------
 template <int CPT, bool TOPO>
 __global__ __launch_bounds__(256) void k_adj_advect_x_rows(AdvArgs A, const double *__restrict__ qpre,
@@ -603,6 +787,34 @@ hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double
 }
 
 template <int DIR, int NB>
+hipError_t adj_seg_dir(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam) {
+  const int ntile = (a.g.Na + 63) / 64;
+  const long long groups = (a.d.ns + NB - 1) / NB;
+  // segments per line as for the forward sweep: enough waves to fill the
+  // chip, segments of at least 16 cells, at most 4
+  const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
+  int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
+  W = W < 1 ? 1 : (W > 4 ? 4 : W);
+  while (W > 1 && a.g.N / W < 16) --W;
+  dim3 grid((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
+  dim3 block(64, W);
+  if (DIR == 1 && a.deld != nullptr)
+    hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true>), grid, block, 0, s, a, qpre, lam);
+  else
+    hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false>), grid, block, 0, s, a, qpre, lam);
+  return hipGetLastError();
+}
+
+template <int DIR>
+hipError_t adj_seg(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
+  switch (nb) {
+    case 1: return adj_seg_dir<DIR, 1>(s, a, qpre, lam);
+    case 2: return adj_seg_dir<DIR, 2>(s, a, qpre, lam);
+    default: return adj_seg_dir<DIR, 4>(s, a, qpre, lam);
+  }
+}
+
+template <int DIR, int NB>
 hipError_t adj_cn_dir(hipStream_t s, const DiffArgs &a, double *lam) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
@@ -623,7 +835,8 @@ hipError_t adj_cn_nb(hipStream_t s, const DiffArgs &a, double *lam, int nb) {
 }  // namespace
 
 // x rows through LDS (rows of up to 8*256-4 cells, longer rows streamed
-// per lane), y and z lines streamed; a.periodic selects the periodic x fold.
+// per lane), y and z lines in segments; a.periodic selects the periodic x
+// fold.
 hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
   if (a.g.dir == 0) {
     const int cells = a.d.nx + 4;
@@ -640,8 +853,8 @@ hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre
       default: return adj_x_rows_cpt<8>(s, a, qpre, lam, bt);
     }
   }
-  if (a.g.dir == 1) return adj_lines<1>(s, a, qpre, lam, nb);
-  return adj_lines<2>(s, a, qpre, lam, nb);
+  if (a.g.dir == 1) return adj_seg<1>(s, a, qpre, lam, nb);
+  return adj_seg<2>(s, a, qpre, lam, nb);
 }
 
 // a.g.dir is 1 or 2 (x lines run on the transposed grid as dir 1)
