@@ -60,8 +60,9 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
     """HBM bytes per launch of a kernel kind from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
     on this grid with the same kernel: the same line solver for the
-    diffusion sweeps, the same bins per thread for the streamed advection
-    (tuned, else the library default); else None."""
+    diffusion sweeps (and tile shape for the partitioned solves), the same
+    bins per thread for the streamed advection (tuned, else the library
+    default); else None."""
     if (st.nx, st.ny, st.nz, st.ns) != (360, 240, 50, 15) or not os.path.exists(TRAFFIC_FILE):
         return None
     text = open(TRAFFIC_FILE).read()
@@ -69,9 +70,15 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
     rec = json.loads(block).get(kind)
     if rec is None:
         return None
-    if kind.startswith("diffuse") and (rec.get("line_solver") != line_solver or
-                                       (tuning or {}).get("part_bins", [0])[0] not in (0, None)):
-        return None
+    if kind.startswith("diffuse"):
+        if rec.get("line_solver") != line_solver or (tuning or {}).get("part_bins", [0])[0] not in (0, None):
+            return None
+        knob = "part_shape_" + kind[-1]
+        shape = (tuning or {}).get(knob, [None])[0]
+        if shape is None:
+            shape = defaults.get(knob)
+        if line_solver == "partitioned" and rec.get("part_shape") != shape:
+            return None
     if rec["bins_per_thread"] is not None:
         nb = (tuning or {}).get("nb_" + kind, [None])[0]
         if nb is None:
@@ -331,7 +338,8 @@ def main():
                    key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dominant]
     achieved = dk["gbps"]
-    defaults = {"advect_y": 2, "advect_z": 2}  # the library's bins per thread on this grid
+    # the library's bins per thread and tile shapes on this grid
+    defaults = {"advect_y": 2, "advect_z": 2, "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
     traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
