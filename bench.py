@@ -339,7 +339,7 @@ def main():
     dk = kernels[dominant]
     achieved = dk["gbps"]
     # the library's bins per thread and tile shapes on this grid
-    defaults = {"advect_y": 2, "advect_z": 2, "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
+    defaults = {"advect_y": 3, "advect_z": 2, "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
     traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

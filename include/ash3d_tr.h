@@ -178,7 +178,8 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, 
 int ash3d_tr_mass_totals(ash3d_tr_ctx *ctx, double *aloft, double *deposit, double *outflow);
 
 /* Launch tuning (no counterpart in the reference): the number of grain
- * bins one thread carries through a sweep (1, 2, 4 or 8).  More bins share
+ * bins one thread carries through a sweep (1, 2, 4 or 8; 3 also for the
+ * advection sweeps).  More bins share
  * the per-cell terms (dt/kappa, interface speeds, line factorisations);
  * fewer keep more waves resident.  Results do not depend on it. */
 enum {

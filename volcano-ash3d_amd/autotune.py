@@ -24,7 +24,7 @@ def _key(st):
     return f"{st.nx}x{st.ny}x{st.nz}x{st.ns}"
 
 
-def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), itime0: int = 1000,
+def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8), itime0: int = 1000,
              cache_path: str | None = None, line_solver: str = "exact") -> dict:
     """line_solver: the solver the context runs ("exact" tunes the
     dgtsv-order kernels' bins per thread and x tile, "partitioned" the
@@ -51,6 +51,8 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 4, 8), it
         for nb in values:
             if param.startswith("nb_") and nb > 1 and nb // 2 >= st.ns:
                 continue
+            if param.startswith("nb_diffuse") and nb == 3:
+                continue  # 3 bins per thread: the advection sweeps only
             T.tune(**{param: nb})
             T.profile(True)
             for _ in range(steps):

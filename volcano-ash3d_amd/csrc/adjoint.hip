@@ -810,6 +810,7 @@ hipError_t adj_seg(hipStream_t s, const AdvArgs &a, const double *qpre, double *
   switch (nb) {
     case 1: return adj_seg_dir<DIR, 1>(s, a, qpre, lam);
     case 2: return adj_seg_dir<DIR, 2>(s, a, qpre, lam);
+    case 3: return adj_seg_dir<DIR, 3>(s, a, qpre, lam);
     default: return adj_seg_dir<DIR, 4>(s, a, qpre, lam);
   }
 }

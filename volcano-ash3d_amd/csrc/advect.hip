@@ -291,8 +291,10 @@ hipError_t launch_nb(hipStream_t s, const AdvArgs &a) {
 // Default bins per thread (kind 0 advection, 1 diffusion; dir 0..2):
 // more bins share the per-cell terms, fewer keep more waves resident.
 int pick_bins_per_thread(int ns, int kind, int dir) {
-  // measured on the 360x240x50x15 forecast grid (bench autotuner): y and z
-  // advection 2, x line solves (row tiles) 2, y/z line solves 8
+  // measured on the 360x240x50x15 forecast grid (bench autotuner): y
+  // advection 3, z advection 2, x line solves (row tiles) 2, y/z line
+  // solves 8
+  if (kind == 0 && dir == 1 && ns >= 3) return 3;
   const int cap = (kind == 0) ? 2 : (dir == 0 ? 2 : 8);
   int nb = 1;
   while (nb * 2 <= cap && nb * 2 <= ns) nb *= 2;
@@ -308,6 +310,7 @@ hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb) {
   switch (nb) {
     case 1: return launch_nb<1>(s, a);
     case 2: return launch_nb<2>(s, a);
+    case 3: return launch_nb<3>(s, a);
     case 4: return launch_nb<4>(s, a);
     default: return launch_nb<8>(s, a);
   }

@@ -950,9 +950,10 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     x->x_tile = value;
     return 0;
   }
-  if (value != 1 && value != 2 && value != 4 && value != 8)
-    return fail(ASH3D_TR_EINVAL, "bins per thread must be 1, 2, 4 or 8 (got %d)", value);
-  if (param >= ASH3D_TUNE_NB_ADVECT_X && param <= ASH3D_TUNE_NB_ADVECT_Z) {
+  const bool adv = param >= ASH3D_TUNE_NB_ADVECT_X && param <= ASH3D_TUNE_NB_ADVECT_Z;
+  if (value != 1 && value != 2 && !(adv && value == 3) && value != 4 && value != 8)
+    return fail(ASH3D_TR_EINVAL, "bins per thread must be 1, 2, 4 or 8, or 3 for the advection (got %d)", value);
+  if (adv) {
     x->nb_adv[param - ASH3D_TUNE_NB_ADVECT_X] = value;
   } else if (param >= ASH3D_TUNE_NB_DIFFUSE_X && param <= ASH3D_TUNE_NB_DIFFUSE_Z) {
     HIPCHK(hipStreamSynchronize(x->stream));  // the CN scratch is sized by it
