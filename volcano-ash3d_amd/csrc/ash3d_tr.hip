@@ -385,13 +385,16 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   return 1;
 }
 
+// neumann = false: the caller has just run Set_BC(2), whose ghosts are the
+// ones the routine's own Neumann pass would write (every ghost equals the
+// clamped, x-wrapped interior cell, and no interior cell has changed since)
 int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fac, const int32_t *is_aloft,
-                  const int32_t *bounds) {
+                  const int32_t *bounds, bool neumann = true) {
   int rc = check_bounds(x, bounds);
   if (rc) return rc;
   if (dir < 0 || dir > 2) return fail(ASH3D_TR_EINVAL, "dir must be 0, 1 or 2 (got %d)", dir);
   // the routine's own Neumann ghosts of its direction (incl. periodic x)
-  {
+  if (neumann) {
     ProfScope ps(x, ASH3D_PROF_BC);
     HIPCHK(launch_neumann_dir(x->stream, x->d, dir, x->periodic, x->ts0()));
   }
@@ -879,7 +882,7 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
       HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
     }
     const int dfirst = (itime % 2 == 0) ? 0 : 1;
-    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b, false))) return rc;
     if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
     if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b))) return rc;
   }
@@ -1510,7 +1513,7 @@ int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
     const int32_t g[6] = {std::max(1, b[0] - 1), std::min(x->d.nx, b[1] + 1), std::max(1, b[2] - 1),
                           std::min(x->d.ny, b[3] + 1), std::max(1, b[4] - 1), std::min(x->d.nz, b[5] + 1)};
     const int dfirst = (itime % 2 == 0) ? 0 : 1;
-    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, g))) return rc;
+    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, g, false))) return rc;
     if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, g))) return rc;
     if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, g))) return rc;
   }

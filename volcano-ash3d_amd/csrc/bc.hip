@@ -4,6 +4,8 @@
 // glue (src/Ash3d.F90:376-378) and the vent-column source add (:428-431).
 // All of these touch O(surface) cells; every thread writes one cell and
 // reads only interior cells, so no ordering between threads is needed.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ash3d {
@@ -50,16 +52,28 @@ __device__ inline int xmap(int i, int nx, int periodic) {
 
 enum { GH_ZERO = 0, GH_BC1 = 1, GH_BC2 = 2 };
 
-// One thread per ghost cell of plane blockIdx.y-1 (k) of bin blockIdx.z.
-// mode GH_ZERO: 0; GH_BC1: Set_BC(1) concentration rule (:77-91); GH_BC2:
-// Set_BC(2) (:98-118) -- the sequential x (periodic override), y, z mirrors
-// leave every ghost equal to c(xmap(i), clamp(j), clamp(k)).
-__global__ __launch_bounds__(kBlock) void k_ghosts(Dims d, int periodic, int mode, double *c0,
-                                                  int kplane0) {
-  const int k = (int)blockIdx.y + kplane0;
-  const int n = (int)blockIdx.z;
-  int i, j;
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+// Every ghost cell of every bin in one launch (bin blockIdx.y): per bin
+// the four ghost planes k = -1, 0, nz+1, nz+2, then the rings of the
+// interior planes k = 1..nz.  mode GH_ZERO: 0; GH_BC1: Set_BC(1)
+// concentration rule (:77-91); GH_BC2: Set_BC(2) (:98-118) -- the
+// sequential x (periodic override), y, z mirrors leave every ghost equal to
+// c(xmap(i), clamp(j), clamp(k)).
+__global__ __launch_bounds__(kBlock) void k_ghosts_all(Dims d, int periodic, int mode, double *c0) {
+  const int n = (int)blockIdx.y;
+  long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long plane = (long long)(d.nx + 4) * (d.ny + 4);
+  const long long ring = 4LL * (d.nx + 4) + 4LL * d.ny;
+  int k, i, j;
+  if (t < 4 * plane) {
+    const int p = (int)(t / plane);
+    k = (p < 2) ? p - 1 : d.nz + p - 1;  // -1, 0, nz+1, nz+2
+    t -= p * plane;
+  } else {
+    t -= 4 * plane;
+    if (t >= (long long)d.nz * ring) return;
+    k = 1 + (int)(t / ring);
+    t %= ring;
+  }
   if (!ghost_cell(d, k, t, i, j)) return;
   double *c = c0 + d.ncell * n;
   double v = 0.0;
@@ -73,21 +87,17 @@ __global__ __launch_bounds__(kBlock) void k_ghosts(Dims d, int periodic, int mod
 }
 
 hipError_t launch_ghosts(hipStream_t s, const Dims &d, int periodic, int mode, double *c0) {
-  // ghost planes k = -1,0 and nz+1,nz+2: full planes
-  const long long full = (long long)(d.nx + 4) * (d.ny + 4);
+  const long long plane = (long long)(d.nx + 4) * (d.ny + 4);
   const long long ring = 4LL * (d.nx + 4) + 4LL * d.ny;
-  dim3 gfull((unsigned)((full + kBlock - 1) / kBlock), 2, d.ns);
-  hipLaunchKernelGGL(k_ghosts, gfull, dim3(kBlock), 0, s, d, periodic, mode, c0, -1);
-  hipLaunchKernelGGL(k_ghosts, gfull, dim3(kBlock), 0, s, d, periodic, mode, c0, d.nz + 1);
-  dim3 gring((unsigned)((ring + kBlock - 1) / kBlock), d.nz, d.ns);
-  hipLaunchKernelGGL(k_ghosts, gring, dim3(kBlock), 0, s, d, periodic, mode, c0, 1);
+  const long long per_bin = 4 * plane + (long long)d.nz * ring;
+  dim3 grid((unsigned)((per_bin + kBlock - 1) / kBlock), d.ns);
+  hipLaunchKernelGGL(k_ghosts_all, grid, dim3(kBlock), 0, s, d, periodic, mode, c0);
   return hipGetLastError();
 }
 
 // Set_BC(1) velocities (:44-70): zero-order extrapolation into the ghost
 // layers of vx (x, periodic wrap), vy (y), vz and every vf bin (z).
-__global__ __launch_bounds__(kBlock) void k_vel_x(Dims d, int periodic, double *vx) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+__device__ inline void vel_x(Dims d, int periodic, double *vx, long long t) {
   const long long nl = (long long)(d.ny + 4) * (d.nz + 4);
   if (t >= nl) return;
   const int j = (int)(t % (d.ny + 4)) - 1, k = (int)(t / (d.ny + 4)) - 1;
@@ -105,8 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_vel_x(Dims d, int periodic, double *
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_vel_y(Dims d, double *vy) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+__device__ inline void vel_y(Dims d, double *vy, long long t) {
   const long long nl = (long long)(d.nx + 4) * (d.nz + 4);
   if (t >= nl) return;
   const int i = (int)(t % (d.nx + 4)) - 1, k = (int)(t / (d.nx + 4)) - 1;
@@ -117,18 +126,27 @@ __global__ __launch_bounds__(kBlock) void k_vel_y(Dims d, double *vy) {
   vy[d.p3(i, ny + 2, k)] = vy[d.p3(i, ny, k)];
 }
 
-// vz when blockIdx.y == 0, vf bin blockIdx.y-1 otherwise
-__global__ __launch_bounds__(kBlock) void k_vel_z(Dims d, double *vz, double *vf) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+// vz or one bin of vf
+__device__ inline void vel_z(Dims d, double *v, long long t) {
   const long long nl = (long long)(d.nx + 4) * (d.ny + 4);
   if (t >= nl) return;
   const int i = (int)(t % (d.nx + 4)) - 1, j = (int)(t / (d.nx + 4)) - 1;
-  double *v = blockIdx.y == 0 ? vz : vf + d.ncell * (long long)(blockIdx.y - 1);
   const int nz = d.nz;
   v[d.p3(i, j, -1)] = v[d.p3(i, j, 1)];
   v[d.p3(i, j, 0)] = v[d.p3(i, j, 1)];
   v[d.p3(i, j, nz + 1)] = v[d.p3(i, j, nz)];
   v[d.p3(i, j, nz + 2)] = v[d.p3(i, j, nz)];
+}
+
+// All velocity ghosts in one launch: blockIdx.y 0 = vx (x lines), 1 = vy
+// (y lines), 2 = vz, 3 + n = vf of bin n (z lines).
+__global__ __launch_bounds__(kBlock) void k_vel_all(Dims d, int periodic, double *vx, double *vy, double *vz,
+                                                   double *vf) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int y = (int)blockIdx.y;
+  if (y == 0) vel_x(d, periodic, vx, t);
+  else if (y == 1) vel_y(d, vy, t);
+  else vel_z(d, y == 2 ? vz : vf + d.ncell * (long long)(y - 3), t);
 }
 
 // Neumann ghosts along one direction for all bins of ts0, all transverse
@@ -201,9 +219,8 @@ hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double
   const long long nx_l = (long long)(d.ny + 4) * (d.nz + 4);
   const long long ny_l = (long long)(d.nx + 4) * (d.nz + 4);
   const long long nz_l = (long long)(d.nx + 4) * (d.ny + 4);
-  hipLaunchKernelGGL(k_vel_x, dim3(blocks_for(nx_l)), dim3(kBlock), 0, s, d, periodic, vx);
-  hipLaunchKernelGGL(k_vel_y, dim3(blocks_for(ny_l)), dim3(kBlock), 0, s, d, vy);
-  hipLaunchKernelGGL(k_vel_z, dim3(blocks_for(nz_l), 1 + d.ns), dim3(kBlock), 0, s, d, vz, vf);
+  const long long nl = std::max(nx_l, std::max(ny_l, nz_l));
+  hipLaunchKernelGGL(k_vel_all, dim3(blocks_for(nl), 3 + d.ns), dim3(kBlock), 0, s, d, periodic, vx, vy, vz, vf);
   return hipGetLastError();
 }
 
