@@ -85,6 +85,11 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
             nb = defaults.get(kind)
         if nb != rec["bins_per_thread"]:
             return None
+        gy = (tuning or {}).get("adv_groups_" + kind[-1], [None])[0]
+        if gy is None:
+            gy = defaults.get("adv_groups_" + kind[-1])
+        if gy != rec.get("adv_groups", 0):
+            return None
     return rec["traffic_bytes"]
 
 
@@ -339,7 +344,8 @@ def main():
     dk = kernels[dominant]
     achieved = dk["gbps"]
     # the library's bins per thread and tile shapes on this grid
-    defaults = {"advect_y": 3, "advect_z": 2, "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
+    defaults = {"advect_y": 3, "advect_z": 2, "adv_groups_y": 0, "adv_groups_z": 4, "part_shape_x": 1,
+                "part_shape_y": 1, "part_shape_z": 0}
     traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

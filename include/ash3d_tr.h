@@ -194,7 +194,12 @@ enum {
   ASH3D_TUNE_PART_BINS = 8, /* bins per workgroup of the partitioned solves (0: all) */
   ASH3D_TUNE_PART_SHAPE_X = 9, /* tile shape of the partitioned solves (0 or 1; defaults 1, 1, 0), x lines */
   ASH3D_TUNE_PART_SHAPE_Y = 10,
-  ASH3D_TUNE_PART_SHAPE_Z = 11
+  ASH3D_TUNE_PART_SHAPE_Z = 11,
+  /* streamed y/z advection: 0 cuts the lines of a workgroup into segments;
+   * 1..4 gives each wave of a workgroup one bin group of the same whole
+   * lines, so the per-cell fields are read once for all of them */
+  ASH3D_TUNE_ADV_GROUPS_Y = 12,
+  ASH3D_TUNE_ADV_GROUPS_Z = 13
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

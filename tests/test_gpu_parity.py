@@ -263,6 +263,26 @@ def test_mass_totals_on_device(pkg, tr, periodic):
     assert np.array_equal(a1, a2) and np.array_equal(d1, d2) and np.array_equal(o1, o2)
 
 
+@pytest.mark.parametrize("nb,gy", [(1, 4), (2, 3), (2, 4), (3, 4), (4, 1)])
+def test_bin_groups_per_workgroup(pkg, oracle, tr, nb, gy):
+    """Streamed y/z advection with the bin groups of a tile in one workgroup
+    (whole lines per wave) gives the oracle's bits, including partial last
+    groups (7 bins) and workgroups with fewer groups than waves."""
+    st = pkg.workloads.random_state(70, 33, 21, 7, seed=17 + nb + gy, latlon=True, zscaling_id=2)
+    st.is_aloft = np.array([1, 1, 0, 1, 1, 1, 1], dtype=np.int32)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    T.tune(nb_advect_y=nb, nb_advect_z=nb, adv_groups_y=gy, adv_groups_z=gy)
+    for itime in (1, 2):
+        for X in (O, T):
+            X.step_glue()
+            X.transport_step(itime, True)
+    T.sync_to_host()
+    T.close()
+    assert_same(a, b)
+
+
 @pytest.mark.parametrize("nb,x_tile", [(1, 8), (1, 32), (2, 16), (2, 32), (3, 16), (4, 16), (8, 8)])
 def test_tuning_does_not_change_results(pkg, oracle, tr, nb, x_tile):
     """Every bins-per-thread and x-line tile setting gives the oracle's bits

@@ -13,6 +13,8 @@
 // The arithmetic reproduces the reference expression by expression (no
 // contraction: the library is built with -ffp-contract=off), so results are
 // bitwise those of the Fortran routines.
+#include <algorithm>
+
 #include "common.hpp"
 #include "dcu.hpp"
 
@@ -35,13 +37,17 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const int ntile = (g.Na + 63) / 64;
-  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  // A.gy > 0: the waves of a workgroup take gy bin groups of the same lines
+  // (whole lines, so the per-cell fields are read by all of them at once)
+  const int G = (d.ns + NB - 1) / NB;
+  const bool gmode = A.gy > 0;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, gmode ? (G + A.gy - 1) / A.gy : G);
   if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
   const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
   const int b = (int)(tg.tile / ntile) + 1;
   const bool lane_ok = a <= g.Na;
-  const int W = blockDim.y, w = threadIdx.y;
-  const int n0 = tg.group * NB;
+  const int W = gmode ? 1 : blockDim.y, w = gmode ? 0 : threadIdx.y;
+  const int n0 = (gmode ? tg.group * A.gy + (int)threadIdx.y : tg.group) * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int seg = (N + 2 + W - 1) / W;
   const int lo = w * seg, hi = min(N + 1, lo + seg - 1);
@@ -259,17 +265,27 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 }
 
 template <int DIR, int NB>
-hipError_t launch_dir(hipStream_t s, const AdvArgs &a) {
+hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in) {
+  AdvArgs a = a_in;
   const int ntile = (a.g.Na + 63) / 64;
   const long long groups = (a.d.ns + NB - 1) / NB;
-  // segments per line: enough waves to fill the chip, segments >= 16 cells,
-  // at most 4 (one wave per SIMD of the CU holding the workgroup)
-  const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
-  int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
-  W = W < 1 ? 1 : (W > 4 ? 4 : W);
-  while (W > 1 && (a.g.N + 2) / W < 16) --W;
-  dim3 grid((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
-  dim3 block(64, W);
+  dim3 grid, block;
+  if (a.gy > 0) {
+    // bin groups along y: gy of them per workgroup (at most 4 waves: the
+    // kernel's launch bounds)
+    a.gy = (int)std::min<long long>(std::min(a.gy, 4), groups);
+    grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * ((groups + a.gy - 1) / a.gy)));
+    block = dim3(64, a.gy);
+  } else {
+    // segments per line: enough waves to fill the chip, segments >= 16
+    // cells, at most 4 (one wave per SIMD of the CU holding the workgroup)
+    const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
+    int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
+    W = W < 1 ? 1 : (W > 4 ? 4 : W);
+    while (W > 1 && (a.g.N + 2) / W < 16) --W;
+    grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
+    block = dim3(64, W);
+  }
   if (DIR != 2 && a.deld != nullptr)
     hipLaunchKernelGGL((k_advect<DIR, NB, true>), grid, block, 0, s, a);
   else

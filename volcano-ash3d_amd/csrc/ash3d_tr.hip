@@ -133,6 +133,9 @@ struct ash3d_tr_ctx {
   // tile shape of the partitioned solves, by direction (measured on the
   // 360x240x50 forecast grid: 2-line x workgroups, 16-line y tiles)
   int part_shape[3] = {1, 1, 0};
+  // streamed advection: bin groups per workgroup (0: line segments); the
+  // z default is the autotuner's choice on the 360x240x50x15 forecast grid
+  int adv_gy[3] = {0, 0, 4};
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -242,6 +245,7 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
   make_mask(x, is_aloft, &a.aloft);
   a.kap = x->kap;
   a.junk = x->junk;
+  a.gy = x->adv_gy[dir];
   if (dir == 0) {
     a.vel = x->vx; a.sig = x->sx; a.of1 = x->ofyz1; a.of2 = x->ofyz2;
     a.deld = (x->zscaling == 2) ? x->ddx : nullptr;
@@ -935,6 +939,12 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
       return fail(ASH3D_TR_EINVAL, "line solver must be %d (dgtsv order) or %d (partitioned) (got %d)",
                   ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED, value);
     x->line_solver = value;
+    return 0;
+  }
+  if (param == ASH3D_TUNE_ADV_GROUPS_Y || param == ASH3D_TUNE_ADV_GROUPS_Z) {
+    if (value < 0 || value > 4)
+      return fail(ASH3D_TR_EINVAL, "bin groups per workgroup must be 0..4 (got %d)", value);
+    x->adv_gy[param == ASH3D_TUNE_ADV_GROUPS_Y ? 1 : 2] = value;
     return 0;
   }
   if (param >= ASH3D_TUNE_PART_SHAPE_X && param <= ASH3D_TUNE_PART_SHAPE_Z) {

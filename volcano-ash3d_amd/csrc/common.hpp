@@ -115,6 +115,7 @@ struct AdvArgs {
   double dt;
   int periodic;
   int limiter;
+  int gy;              // streamed sweeps: bin groups per workgroup along y (0: line segments along y)
   AloftMask aloft;
 };
 
