@@ -54,7 +54,8 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const long long sL = line_stride(d, DIR);
   const long long base = lane_ok ? line_base(d, DIR, a, b) : 0;
   const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
-  double *c = A.c0;
+  const double *c = A.c0;
+  double *co = A.cout;  // == c0 for the in-place sweep
   double *junk = A.junk + (threadIdx.x + 64 * threadIdx.y + 256 * (blockIdx.x % (kJunk / 256)));
 
   bool exists[NB], on[NB];
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
           }
           // (:264-281); every cell issues one store (masked-off ones to junk)
           const double upd = -dtv2 * (fslI + fsr2[u] + fssI - fss2[u]);
-          *((st_in && on[u]) ? c + base + sL * (long long)(cc + 1) + bofs[u] : junk) = q2[u] + upd;
+          *((st_in && on[u]) ? co + base + sL * (long long)(cc + 1) + bofs[u] : junk) = q2[u] + upd;
           // x outflow skipped when periodic (:289-294); y (:540-545); deposit
           // and top outflow of z, the top from update_cc(ncells+1)
           // (src/AdvectionVert_DCU.f90:306-311)
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   for (int u = 0; u < NB; ++u) {
     if (!exists[u]) continue;
     for (int r = zlo; r <= zhi; ++r)
-      if (!on[u] || r < rmin || r > rmax) c[base + sL * (long long)(r + 1) + bofs[u]] = 0.0;
+      if (!on[u] || r < rmin || r > rmax) co[base + sL * (long long)(r + 1) + bofs[u]] = 0.0;
     if (DIR == 2 && w == 0) {
       // DepositGranularity = outflow_xy1 (src/AdvectionVert_DCU.f90:321);
       // segment 0 owns the bottom ghost update

@@ -39,7 +39,8 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
   const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
   const long long base = d.p3(-1, j, k);  // cell -1 of the row
   const int t = threadIdx.x, BT = blockDim.x;
-  double *c0 = A.c0;
+  const double *c0 = A.c0;
+  double *co = A.cout;  // == c0 for the in-place sweep
 
   int cc[CPT];
   bool valid[CPT];
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
       // zero fill of ts1 then copy (:122, :302-303)
 #pragma unroll
       for (int m = 0; m < CPT; ++m)
-        if (cc[m] >= 1 && cc[m] <= nx) c0[base + (cc[m] + 1) + bofs] = 0.0;
+        if (cc[m] >= 1 && cc[m] <= nx) co[base + (cc[m] + 1) + bofs] = 0.0;
       continue;
     }
     double q[CPT], qm1[CPT], dq[CPT], fss[CPT], fsr[CPT];
@@ -174,12 +175,12 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
       if (!valid[m]) continue;
       if (c >= rmin - 1 && c <= rmax + 1) {
         const double upd = -dtv[m] * (s_fsl[p + 1] + fsr[m] + s_fss[p + 1] - fss[m]);
-        if (c >= rmin && c <= rmax) c0[base + p + bofs] = q[m] + upd;
+        if (c >= rmin && c <= rmax) co[base + p + bofs] = q[m] + upd;
         const long long of = (long long)(j + 1) + (long long)(d.ny + 4) * ((long long)(k + 1) + (long long)(d.nz + 4) * n);
         if (c == 0 && rmin == 1 && !A.periodic) A.of1[of] = A.of1[of] + upd;
         if (c == nx + 1 && rmax == nx && !A.periodic) A.of2[of] = A.of2[of] + upd;
       }
-      if (c >= 1 && c <= nx && (c < rmin || c > rmax)) c0[base + p + bofs] = 0.0;
+      if (c >= 1 && c <= nx && (c < rmin || c > rmax)) co[base + p + bofs] = 0.0;
     }
   }
 }

@@ -239,6 +239,7 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
   a.d = x->d;
   a.g = make_line_geom(x->d, dir, bounds);
   a.c0 = c0;
+  a.cout = c0;
   a.dt = dt;
   a.periodic = x->periodic;
   a.limiter = x->limiter;
@@ -260,14 +261,18 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
 }
 
 // One advection sweep on ts0, or on another state buffer whose outflow and
-// deposit go to the sink (recomputation for the adjoint).
+// deposit go to the sink (recomputation for the adjoint).  With out, the
+// sweep reads state and writes its result to out (every interior cell of
+// the sweep's lines, then the ghost cells copied from state): out ends as
+// state would after the in-place sweep, and state is left as it was.
 int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
-                 bool with_bc, bool bc_vel, double *state = nullptr) {
+                 bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr) {
   int rc = check_bounds(x, bounds);
   if (rc) return rc;
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if (with_bc && (rc = set_bc1(x, bc_vel, state))) return rc;
   AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, state ? state : x->ts0());
+  if (state && out) a.cout = out;
   if (state) {
     const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
     a.of1 = x->sink;
@@ -277,6 +282,10 @@ int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, c
   {
     ProfScope ps(x, ASH3D_PROF_ADVECT_X + dir);
     HIPCHK(launch_advect(x->stream, a, x->nb_adv[dir]));
+  }
+  if (state && out) {
+    ProfScope ps(x, ASH3D_PROF_COPY);
+    HIPCHK(launch_copy_ghosts(x->stream, x->d, state, out));
   }
   if (!state) x->ts1_state = TS1_MIRROR;
   return 0;
@@ -1141,11 +1150,9 @@ int ash3d_tr_adj_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_dif
   // sweep from the step input (ts0), as ash3d_tr_step computes them
   if ((rc = set_bc1(x, true))) return rc;
   const int first = (itime % 2 == 0) ? 1 : 0;
-  if ((rc = copy_state(x, x->snap_b, x->ts0()))) return rc;
-  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->snap_b))) return rc;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->ts0(), x->snap_b))) return rc;
   if (x->periodic && (rc = set_bc1(x, false, x->snap_b))) return rc;
-  if ((rc = copy_state(x, x->snap_z, x->snap_b))) return rc;
-  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, false, false, x->snap_z))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, false, false, x->snap_b, x->snap_z))) return rc;
   if (x->periodic && (rc = set_bc1(x, false, x->snap_z))) return rc;
   // adjoint sweeps in reverse order (Set_BC(2) ghosts are not read by the
   // diffusion sweeps, which set their own)

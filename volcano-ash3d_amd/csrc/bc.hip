@@ -50,15 +50,16 @@ __device__ inline int xmap(int i, int nx, int periodic) {
   return i < 1 ? 1 : nx;
 }
 
-enum { GH_ZERO = 0, GH_BC1 = 1, GH_BC2 = 2 };
+enum { GH_ZERO = 0, GH_BC1 = 1, GH_BC2 = 2, GH_COPY = 3 };
 
 // Every ghost cell of every bin in one launch (bin blockIdx.y): per bin
 // the four ghost planes k = -1, 0, nz+1, nz+2, then the rings of the
-// interior planes k = 1..nz.  mode GH_ZERO: 0; GH_BC1: Set_BC(1)
+// interior planes k = 1..nz.  mode GH_COPY: src's value; GH_ZERO: 0; GH_BC1: Set_BC(1)
 // concentration rule (:77-91); GH_BC2: Set_BC(2) (:98-118) -- the
 // sequential x (periodic override), y, z mirrors leave every ghost equal to
 // c(xmap(i), clamp(j), clamp(k)).
-__global__ __launch_bounds__(kBlock) void k_ghosts_all(Dims d, int periodic, int mode, double *c0) {
+__global__ __launch_bounds__(kBlock) void k_ghosts_all(Dims d, int periodic, int mode, double *c0,
+                                                      const double *src) {
   const int n = (int)blockIdx.y;
   long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   const long long plane = (long long)(d.nx + 4) * (d.ny + 4);
@@ -82,16 +83,19 @@ __global__ __launch_bounds__(kBlock) void k_ghosts_all(Dims d, int periodic, int
       v = c[d.p3(xmap(i, d.nx, 1), j, k)];
   } else if (mode == GH_BC2) {
     v = c[d.p3(xmap(i, d.nx, periodic), clampi(j, 1, d.ny), clampi(k, 1, d.nz))];
+  } else if (mode == GH_COPY) {
+    v = src[d.p3(i, j, k) + d.ncell * n];
   }
   c[d.p3(i, j, k)] = v;
 }
 
-hipError_t launch_ghosts(hipStream_t s, const Dims &d, int periodic, int mode, double *c0) {
+hipError_t launch_ghosts(hipStream_t s, const Dims &d, int periodic, int mode, double *c0,
+                         const double *src = nullptr) {
   const long long plane = (long long)(d.nx + 4) * (d.ny + 4);
   const long long ring = 4LL * (d.nx + 4) + 4LL * d.ny;
   const long long per_bin = 4 * plane + (long long)d.nz * ring;
   dim3 grid((unsigned)((per_bin + kBlock - 1) / kBlock), d.ns);
-  hipLaunchKernelGGL(k_ghosts_all, grid, dim3(kBlock), 0, s, d, periodic, mode, c0);
+  hipLaunchKernelGGL(k_ghosts_all, grid, dim3(kBlock), 0, s, d, periodic, mode, c0, src);
   return hipGetLastError();
 }
 
@@ -234,6 +238,10 @@ hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0
 
 hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0) {
   return launch_ghosts(s, d, 0, GH_ZERO, c0);
+}
+
+hipError_t launch_copy_ghosts(hipStream_t s, const Dims &d, const double *src, double *dst) {
+  return launch_ghosts(s, d, 0, GH_COPY, dst, src);
 }
 
 hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0) {

@@ -102,7 +102,8 @@ constexpr int kJunk = 4096;
 struct AdvArgs {
   Dims d;
   LineGeom g;
-  double *c0;          // concen ts0
+  double *c0;          // concen ts0 (read)
+  double *cout;        // where the interior cells of the sweep's lines are written (c0: in place)
   const double *vel;   // vx_pd / vy_pd / vz_pd
   const double *vh;    // vh_pd (z)
   const double *vf;    // vf_pd (z)
@@ -157,6 +158,8 @@ hipError_t launch_set_bc1_conc(hipStream_t s, const Dims &d, int periodic, doubl
 hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0);
 hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0);
 hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0);
+// dst's ghost cells (every bin) = src's
+hipError_t launch_copy_ghosts(hipStream_t s, const Dims &d, const double *src, double *dst);
 hipError_t launch_fill(hipStream_t s, double *p, long long n, double v);
 hipError_t launch_mirror_interior(hipStream_t s, const Dims &d, const double *c0, double *c1);
 hipError_t launch_source_add(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt,
