@@ -85,6 +85,14 @@ struct ash3d_tr_ctx {
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
+  // host copy of each direction's non-dominant line count (pinned, read
+  // back asynchronously after a factorisation): 0 unknown, 1 in flight
+  // (ndom_ev), 2 known.  When known, a sweep launches only the solver it
+  // needs instead of both with the choice made on the device.
+  int *ndom_h = nullptr;
+  hipEvent_t ndom_ev[3] = {};
+  int ndom_state[3] = {0, 0, 0};
+  bool capturing = false;
   double *junk = nullptr;  // kJunk doubles that masked-off lanes store to
   // the sweep arguments the factors of each direction were last requested
   // with, and those of the sweeps in each captured graph (graph_launch
@@ -337,6 +345,7 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
     HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir]));
+    x->ndom_state[dir] = 0;
     x->lu_ok[dir] = true;
     x->lu_dt[dir] = a.dt;
     x->lu_th[dir] = a.theta;
@@ -371,11 +380,47 @@ int ensure_fac(ash3d_tr_ctx *x, int nb) {
 // (the partitioned kernel then returns at once).  a holds the sweep's
 // geometry (for x: the x<->y transposed one, which the row factor cache
 // uses) and the original state in a.c0.
+// The non-dominant line count of direction dir's current factors as the
+// host knows it, or -1 while unknown (the first sweeps after a
+// factorisation, and always inside a graph capture, whose sweeps keep the
+// device-side choice so that a replay after a refactorisation is right).
+int host_ndom(ash3d_tr_ctx *x, int dir) {
+  if (x->capturing || !x->lu_ok[dir]) return -1;
+  if (x->ndom_state[dir] == 0) {
+    if (!x->ndom_h && hipHostMalloc((void **)&x->ndom_h, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      x->ndom_h = nullptr;
+      return -1;
+    }
+    if (!x->ndom_ev[dir] && hipEventCreateWithFlags(&x->ndom_ev[dir], hipEventDisableTiming) != hipSuccess) {
+      x->ndom_ev[dir] = nullptr;
+      return -1;
+    }
+    if (hipMemcpyAsync(x->ndom_h + dir, x->lu_ndom[dir], sizeof(int), hipMemcpyDeviceToHost, x->stream) !=
+            hipSuccess ||
+        hipEventRecord(x->ndom_ev[dir], x->stream) != hipSuccess)
+      return -1;
+    x->ndom_state[dir] = 1;
+    return -1;
+  }
+  if (x->ndom_state[dir] == 1) {
+    if (hipEventQuery(x->ndom_ev[dir]) != hipSuccess) return -1;
+    x->ndom_state[dir] = 2;
+  }
+  return x->ndom_h[dir];
+}
+
+// Launches the partitioned solves of a sweep when they apply.  Returns 0
+// when they were not launched (the caller runs the dgtsv-order kernel as
+// usual), 1 when launched with the choice left to the device (the caller
+// launches the dgtsv-order kernel too, a.sel = 2), 2 when launched for a
+// sweep known to be dominant (the caller launches nothing else).
 int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjoint = false) {
   *rc = 0;
   const int n = a.g.rmax - a.g.rmin + 1;
   a.sel = 0;
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > 512) return 0;
+  const int hn = host_ndom(x, dir);
+  if (hn > 0) return 0;
   const Dims &o = x->d;
   PartLayout lay;
   lay.st0 = 1 + o.sy + o.sz;
@@ -395,7 +440,7 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins, x->part_shape[dir], adjoint);
   if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
   a.sel = 2;
-  return 1;
+  return hn == 0 ? 2 : 1;
 }
 
 // neumann = false: the caller has just run Set_BC(2), whose ghosts are the
@@ -442,8 +487,9 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for both kernels
-      if (try_cn_partitioned(x, 0, a, &rc) && rc) return rc;
-      HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
+      const int tp = try_cn_partitioned(x, 0, a, &rc);
+      if (rc) return rc;
+      if (tp != 2) HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
     }
     x->ts1_state = TS1_MIRROR;
     return 0;
@@ -469,8 +515,9 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
   if (use_cn) {
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);  // one record for both kernels
-    if (try_cn_partitioned(x, dir, a, &rc) && rc) return rc;
-    HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
+    const int tp = try_cn_partitioned(x, dir, a, &rc);
+    if (rc) return rc;
+    if (tp != 2) HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
   } else {
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff[dir]));
@@ -545,10 +592,10 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
     a.c0 = x->lam;
     ProfScope ps(x, ASH3D_PROF_ADJ_DIFFUSE);
-    if (try_cn_partitioned(x, dir, a, &rc, true)) {
-      if (rc) return rc;
-      skip = a.ndom;
-    }
+    const int tp = try_cn_partitioned(x, dir, a, &rc, true);
+    if (rc) return rc;
+    if (tp == 2) return 0;  // known dominant: nothing else to run
+    if (tp == 1) skip = a.ndom;
   }
   if (transposed) {
     const Dims &d = x->d;
@@ -686,6 +733,9 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
     if (x->flux_ev[k]) hipEventDestroy(x->flux_ev[k]);
   }
   if (x->flag) hipFree(x->flag);
+  if (x->ndom_h) hipHostFree(x->ndom_h);
+  for (int k = 0; k < 3; ++k)
+    if (x->ndom_ev[k]) hipEventDestroy(x->ndom_ev[k]);
   for (auto &e : x->ev_pool) {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
@@ -1326,8 +1376,10 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t us
   }
   for (int k = 0; k < 3; ++k) x->lu_args_set[k] = false;
   HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  x->capturing = true;
   int rc2 = ash3d_tr_step_glue(x);
   if (!rc2) rc2 = ash3d_tr_step(x, itime, dt, use_diffusion, use_cn, imp_fac, is_aloft);
+  x->capturing = false;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(x->stream, &g);
   if (rc2) {
