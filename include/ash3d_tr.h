@@ -364,6 +364,74 @@ int ash3d_tr_source_nodes(const ash3d_tr_source_desc *src, int32_t nz, int32_t n
                           const double *norm, const double *mass_flux_rate, const double *frac,
                           const double *kappa_col, int32_t *ieruption, int32_t jeruption, double *flux);
 
+/* ---- Umbrella sources (src/Source_Umbrella.f90) -------------------------
+ * Host work (a few hundred operations per call):
+ *   ash3d_tr_umbrella_stencil  the weights of Allocate_Source_Umbrella
+ *                              (:85-153): avg_stencil[(i-1)+3(j-1)+9(k-1)] =
+ *                              kappa(ivent-2+i, jvent-2+j, k) / (sum of the
+ *                              3x3 kappa of level k), scale_fac[k-1] =
+ *                              kappa(ivent, jvent, k) / that sum, k = 1..nz;
+ *                              kappa_pd is the padded (nx+4, ny+4, nz+4) array.
+ *   ash3d_tr_umbrella_levels   ibase/itop (:139-150): the levels k = 1..nz
+ *                              with z(k-1) < 0.75 H <= z(k) and
+ *                              z(k-1) < H <= z(k), z = z_vec_init(0:nz) (km),
+ *                              H the first pulse's plume height; 0 when no
+ *                              level brackets them.
+ *   ash3d_tr_umbrella_winds    umbrella_winds (:206-394): the radial spreading
+ *                              wind (km/hr) of the umbrella cloud at every
+ *                              cell centre (nx*ny, x fastest), the same on
+ *                              every level ibase..itop; zero at the vent, off
+ *                              the cloud and while the cloud is no wider than
+ *                              the 3x3 source patch.  Lon-lat grids only.
+ *   ash3d_tr_umbrella_nodes    TephraSourceNodes_Umbrella (:409-448): the
+ *                              3x3 patch flux flux_umb[(i-1)+3(j-1)+9(k-1)+
+ *                              9(nz+1)n] from the column flux of
+ *                              ash3d_tr_source_nodes: the column below ibase,
+ *                              flux*scale_fac on all nine cells of levels
+ *                              ibase..itop, nothing above.
+ *   ash3d_tr_umbrella_vol_inc  SourceVolInc_Umbrella (:464-522), km3 DRE.
+ * Device work:
+ *   ash3d_tr_umbrella_source_add  src/Ash3d.F90:447-460: on levels
+ *                              ibase..itop each bin's 3x3 patch around the
+ *                              vent is replaced by its stencil-weighted
+ *                              average (AvgCon_Umbrella, :539-560), then
+ *                              dt*flux_umb is added to the patch on levels
+ *                              1..nz+1; bins 1..n_gs_max.
+ *   ash3d_tr_umbrella_winds_add   src/MesoInterpolater.f90:238-243: vx_pd and
+ *                              vy_pd += the umbrella winds on levels
+ *                              ibase..itop (call after the met interpolation). */
+typedef struct {
+  int32_t vel_model;          /* VelMod_umb: 1 (Eq. 3 with the legacy r^3 term) or 2 (Eq. 4) */
+  double k_entrainment;       /* k_entrainment_umb (0.1) */
+  double lambda;              /* lambda_umb, cloud shape factor (0.2) */
+  double n_bv;                /* N_BV_umb, Brunt-Vaisala frequency, 1/s (0.02) */
+  double mass_flux_rate;      /* MassFluxRate(1), kg/hr */
+  int32_t airborne_one_bin;   /* SourceType 'umbrella_air' with n_gs_max == 1: flux x 20 */
+  double lat_volcano, lon_volcano;  /* degrees */
+  double time, dt;            /* hours since the eruption start; the step */
+  int32_t first_time;         /* 1: use min(simtime, end_time) instead of max(time, dt) */
+  double simtime, end_time;   /* Simtime_in_hours, e_EndTime(1) */
+  double de_km, dn_km;        /* cell width and height, km */
+  int32_t ivent, jvent;
+  int32_t is_latlon;          /* projected grids are an error, as in the reference */
+} ash3d_tr_umbrella_wind_params;
+int ash3d_tr_umbrella_stencil(const double *kappa_pd, int32_t nx, int32_t ny, int32_t nz, int32_t ivent,
+                              int32_t jvent, double *avg_stencil, double *scale_fac);
+int ash3d_tr_umbrella_levels(const double *z_vec_init, int32_t nz, double plume_height, int32_t *ibase,
+                             int32_t *itop);
+int ash3d_tr_umbrella_winds(const ash3d_tr_umbrella_wind_params *p, int32_t nx, int32_t ny, const double *lon_cc,
+                            const double *lat_cc, double *uvx, double *uvy);
+int ash3d_tr_umbrella_nodes(const double *flux, int32_t nz, int32_t ns, int32_t n_gs_max, int32_t ibase,
+                            int32_t itop, const double *scale_fac, double *flux_umb);
+int ash3d_tr_umbrella_vol_inc(const double *flux, int32_t nz, int32_t ns, int32_t n_gs_max, int32_t ibase,
+                              int32_t itop, const double *avg_stencil, const double *kappa_vent_col, double dt,
+                              double magma_density, double *vol);
+int ash3d_tr_umbrella_source_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, double dt, int32_t ibase,
+                                 int32_t itop, int32_t n_gs_max, const double *avg_stencil,
+                                 const double *flux_umb);
+int ash3d_tr_umbrella_winds_add(ash3d_tr_ctx *ctx, int32_t ibase, int32_t itop, const double *uvx,
+                                const double *uvy);
+
 /* ---- Output products and their ESRI ASCII files -----------------------
  * ash3d_tr_products replaces ConcentrationCalculator
  * (src/Output_Vars.f90:1026-1137) and AshThicknessCalculator (:855-893) on

@@ -889,6 +889,49 @@ void orc_source_column_add(orc_state *s, int iv, int jv, const double *flux) {
     }
 }
 
+/* src/Ash3d.F90:447-460: umbrella source.  On levels ibase..itop each
+   bin's 3x3 patch around the vent takes AvgCon_Umbrella
+   (src/Source_Umbrella.f90:539-560) of itself; then dt*flux_umb is added to
+   the patch on levels 1..nz+1.  avg_stencil(3,3,nz), flux_umb(3,3,nz+1,ns),
+   Fortran order; bins 1..n_gs_max. */
+void orc_umbrella_source_add(orc_state *s, int iv, int jv, int ibase, int itop, int n_gs_max,
+                             const double *avg_stencil, const double *flux_umb) {
+  int i, j, k, n;
+  for (k = ibase; k <= itop; k++)
+    for (n = 1; n <= n_gs_max; n++) {
+      double avg = 0.0;
+      for (i = 1; i <= 3; i++)
+        for (j = 1; j <= 3; j++)
+          avg = avg + s->concen_pd[PC(s, iv - 2 + i, jv - 2 + j, k, n, TS0)] *
+                          avg_stencil[(size_t)(i - 1) + 3 * (size_t)(j - 1) + 9 * (size_t)(k - 1)];
+      for (i = 1; i <= 3; i++)
+        for (j = 1; j <= 3; j++) s->concen_pd[PC(s, iv - 2 + i, jv - 2 + j, k, n, TS0)] = avg;
+    }
+  for (n = 1; n <= n_gs_max; n++)
+    for (k = 1; k <= s->nz + 1; k++)
+      for (j = 1; j <= 3; j++)
+        for (i = 1; i <= 3; i++) {
+          size_t o = PC(s, iv - 2 + i, jv - 2 + j, k, n, TS0);
+          s->concen_pd[o] = s->concen_pd[o] + s->dt * flux_umb[(size_t)(i - 1) + 3 * (size_t)(j - 1) +
+                                                               9 * (size_t)(k - 1) +
+                                                               9 * (size_t)(s->nz + 1) * (size_t)(n - 1)];
+        }
+}
+
+/* src/MesoInterpolater.f90:238-243: vx_pd, vy_pd += uvx_pd, uvy_pd on
+   levels ibase..itop (one value per column: umbrella_winds sets every
+   level alike, src/Source_Umbrella.f90:381-385) */
+void orc_umbrella_winds_add(orc_state *s, int ibase, int itop, const double *uvx, const double *uvy) {
+  int i, j, k;
+  for (k = ibase; k <= itop; k++)
+    for (j = 1; j <= s->ny; j++)
+      for (i = 1; i <= s->nx; i++) {
+        size_t o = P3(s, i, j, k), q = (size_t)(i - 1) + (size_t)s->nx * (size_t)(j - 1);
+        s->vx_pd[o] = s->vx_pd[o] + uvx[q];
+        s->vy_pd[o] = s->vy_pd[o] + uvy[q];
+      }
+}
+
 /* src/Ash3d.F90:491-510: the hot-path call sequence of one time step */
 int orc_transport_step(orc_state *s, int itime, int use_diffusion) {
   if (orc_set_bc(s, 1)) return 1;

@@ -98,6 +98,9 @@ void orc_diffuse_vert(orc_state *s);
 void orc_step_glue(orc_state *s);
 void orc_source_column_add(orc_state *s, int ivent, int jvent,
                            const double *flux /* (1:nz+1,1:ns) */);
+void orc_umbrella_source_add(orc_state *s, int iv, int jv, int ibase, int itop, int n_gs_max,
+                             const double *avg_stencil, const double *flux_umb);
+void orc_umbrella_winds_add(orc_state *s, int ibase, int itop, const double *uvx, const double *uvy);
 int orc_transport_step(orc_state *s, int itime, int use_diffusion);
 
 /* Met-step winds in m/s, FP32, unpadded (1:nx,1:ny,1:nz) and vf

@@ -118,6 +118,11 @@ def load():
         lib.orc_transport_step.restype = ctypes.c_int
         lib.orc_source_column_add.argtypes = [P, ctypes.c_int, ctypes.c_int, _dp]
         lib.orc_source_column_add.restype = None
+        lib.orc_umbrella_source_add.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, _dp, _dp]
+        lib.orc_umbrella_source_add.restype = None
+        lib.orc_umbrella_winds_add.argtypes = [P, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        lib.orc_umbrella_winds_add.restype = None
         lib.orc_dgtsv.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int)]
         lib.orc_dgtsv.restype = None
@@ -228,6 +233,17 @@ class Oracle:
     def source_column_add(self, ivent: int, jvent: int, flux: np.ndarray):
         flux = np.asfortranarray(flux, dtype=np.float64)
         self._call("orc_source_column_add", ivent, jvent, _ptr(flux))
+
+    def umbrella_source_add(self, ivent: int, jvent: int, ibase: int, itop: int, n_gs_max: int, avg_stencil,
+                            flux_umb):
+        st = np.asfortranarray(avg_stencil, dtype=np.float64)
+        fu = np.asfortranarray(flux_umb, dtype=np.float64)
+        self._call("orc_umbrella_source_add", ivent, jvent, ibase, itop, n_gs_max, _ptr(st), _ptr(fu))
+
+    def umbrella_winds_add(self, ibase: int, itop: int, uvx, uvy):
+        ux = np.asfortranarray(uvx, dtype=np.float64)
+        uy = np.asfortranarray(uvy, dtype=np.float64)
+        self._call("orc_umbrella_winds_add", ibase, itop, _ptr(ux), _ptr(uy))
 
     def transport_step(self, itime: int, use_diffusion: bool) -> int:
         return self._call("orc_transport_step", itime, int(bool(use_diffusion)))

@@ -93,6 +93,13 @@ struct ash3d_tr_ctx {
   hipEvent_t ndom_ev[3] = {};
   int ndom_state[3] = {0, 0, 0};
   bool capturing = false;
+  // umbrella source: device copies of the stencil + patch flux and of the
+  // two wind planes, uploaded through one pinned staging buffer each
+  double *umb_src = nullptr, *umb_wind = nullptr;
+  double *umb_pin[2] = {nullptr, nullptr};
+  size_t umb_pin_bytes[2] = {0, 0};
+  hipEvent_t umb_ev[2] = {};
+  bool umb_used[2] = {false, false};
   double *junk = nullptr;  // kJunk doubles that masked-off lanes store to
   // the sweep arguments the factors of each direction were last requested
   // with, and those of the sweeps in each captured graph (graph_launch
@@ -734,6 +741,12 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
   }
   if (x->flag) hipFree(x->flag);
   if (x->ndom_h) hipHostFree(x->ndom_h);
+  if (x->umb_src) hipFree(x->umb_src);
+  if (x->umb_wind) hipFree(x->umb_wind);
+  for (int k = 0; k < 2; ++k) {
+    if (x->umb_pin[k]) hipHostFree(x->umb_pin[k]);
+    if (x->umb_ev[k]) hipEventDestroy(x->umb_ev[k]);
+  }
   for (int k = 0; k < 3; ++k)
     if (x->ndom_ev[k]) hipEventDestroy(x->ndom_ev[k]);
   for (auto &e : x->ev_pool) {
@@ -987,6 +1000,73 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, do
   x->flux_used[k] = true;
   x->flux_slot = (k + 1) % ash3d_tr_ctx::kFluxRing;
   HIPCHK(launch_source_add(x->stream, x->d, x->ts0(), ivent, jvent, dt, x->flux));
+  return 0;
+}
+
+// Copy n doubles from host memory to dev on the context's stream through
+// pinned staging buffer `which` (waits for that buffer's previous upload).
+static int upload_staged(ash3d_tr_ctx *x, int which, double *dev, const double *const *parts, const size_t *counts,
+                         int nparts) {
+  size_t n = 0;
+  for (int q = 0; q < nparts; ++q) n += counts[q];
+  const size_t bytes = sizeof(double) * n;
+  if (x->umb_pin_bytes[which] < bytes) {
+    if (x->umb_used[which]) HIPCHK(hipEventSynchronize(x->umb_ev[which]));
+    if (x->umb_pin[which]) HIPCHK(hipHostFree(x->umb_pin[which]));
+    x->umb_pin[which] = nullptr;
+    HIPCHK(hipHostMalloc((void **)&x->umb_pin[which], bytes, hipHostMallocDefault));
+    x->umb_pin_bytes[which] = bytes;
+    x->umb_used[which] = false;
+  }
+  if (!x->umb_ev[which]) HIPCHK(hipEventCreateWithFlags(&x->umb_ev[which], hipEventDisableTiming));
+  if (x->umb_used[which]) HIPCHK(hipEventSynchronize(x->umb_ev[which]));
+  size_t at = 0;
+  for (int q = 0; q < nparts; ++q) {
+    std::memcpy(x->umb_pin[which] + at, parts[q], sizeof(double) * counts[q]);
+    at += counts[q];
+  }
+  HIPCHK(hipMemcpyAsync(dev, x->umb_pin[which], bytes, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipEventRecord(x->umb_ev[which], x->stream));
+  x->umb_used[which] = true;
+  return 0;
+}
+
+int ash3d_tr_umbrella_source_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, double dt, int32_t ibase,
+                                 int32_t itop, int32_t n_gs_max, const double *avg_stencil,
+                                 const double *flux_umb) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const Dims &d = x->d;
+  if (!avg_stencil || !flux_umb) return fail(ASH3D_TR_EINVAL, "null stencil or flux");
+  if (ivent < 1 || ivent > d.nx || jvent < 1 || jvent > d.ny)
+    return fail(ASH3D_TR_EINVAL, "vent (%d,%d) outside the grid", ivent, jvent);
+  if (itop > d.nz || (itop >= ibase && ibase < 1))
+    return fail(ASH3D_TR_EINVAL, "umbrella levels %d..%d outside 1..nz", ibase, itop);
+  if (n_gs_max < 0 || n_gs_max > d.ns) return fail(ASH3D_TR_EINVAL, "n_gs_max %d outside 0..ns", n_gs_max);
+  const size_t nst = 9 * (size_t)d.nz, nfu = 9 * (size_t)(d.nz + 1) * d.ns;
+  if (!x->umb_src && (rc = dev_alloc(&x->umb_src, (long long)(nst + nfu)))) return rc;
+  const double *parts[2] = {avg_stencil, flux_umb};
+  const size_t counts[2] = {nst, nfu};
+  if ((rc = upload_staged(x, 0, x->umb_src, parts, counts, 2))) return rc;
+  HIPCHK(launch_umbrella_source(x->stream, d, x->ts0(), ivent, jvent, dt, ibase, itop, n_gs_max, x->umb_src,
+                                x->umb_src + nst));
+  return 0;
+}
+
+int ash3d_tr_umbrella_winds_add(ash3d_tr_ctx *x, int32_t ibase, int32_t itop, const double *uvx,
+                                const double *uvy) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const Dims &d = x->d;
+  if (!uvx || !uvy) return fail(ASH3D_TR_EINVAL, "null umbrella wind");
+  if (itop < ibase) return 0;  // no level brackets the cloud
+  if (ibase < 1 || itop > d.nz) return fail(ASH3D_TR_EINVAL, "umbrella levels %d..%d outside 1..nz", ibase, itop);
+  const size_t plane = (size_t)d.nx * d.ny;
+  if (!x->umb_wind && (rc = dev_alloc(&x->umb_wind, 2 * (long long)plane))) return rc;
+  const double *parts[2] = {uvx, uvy};
+  const size_t counts[2] = {plane, plane};
+  if ((rc = upload_staged(x, 1, x->umb_wind, parts, counts, 2))) return rc;
+  HIPCHK(launch_umbrella_winds(x->stream, d, x->vx, x->vy, ibase, itop, x->umb_wind, x->umb_wind + plane));
   return 0;
 }
 

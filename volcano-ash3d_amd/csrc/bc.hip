@@ -210,6 +210,46 @@ __global__ __launch_bounds__(kBlock) void k_source(Dims d, double *c0, int iv, i
   c0[o] = c0[o] + dt * flux[t];
 }
 
+// Umbrella source (src/Ash3d.F90:447-460), one thread per (level k =
+// 1..nz+1, bin n < n_gs_max): on levels ibase..itop the 3x3 patch is set to
+// its stencil-weighted average (AvgCon_Umbrella, src/Source_Umbrella.f90:
+// 539-560, summed i-outer, j-inner), then dt*flux_umb is added to it.
+__global__ __launch_bounds__(kBlock) void k_umbrella_source(Dims d, double *c0, int iv, int jv, double dt,
+                                                           int ibase, int itop, int ngs, const double *st,
+                                                           const double *fu) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)(d.nz + 1) * ngs) return;
+  const int k = (int)(t % (d.nz + 1)) + 1, n = (int)(t / (d.nz + 1));
+  double *c = c0 + d.ncell * n;
+  if (k >= ibase && k <= itop) {
+    double a = 0.0;
+    for (int i = 1; i <= 3; ++i)
+      for (int j = 1; j <= 3; ++j)
+        a = a + c[d.p3(iv - 2 + i, jv - 2 + j, k)] * st[(i - 1) + 3 * (j - 1) + 9 * (k - 1)];
+    for (int i = 1; i <= 3; ++i)
+      for (int j = 1; j <= 3; ++j) c[d.p3(iv - 2 + i, jv - 2 + j, k)] = a;
+  }
+  const double *f = fu + 9 * (long long)(k - 1) + 9LL * (d.nz + 1) * n;
+  for (int i = 1; i <= 3; ++i)
+    for (int j = 1; j <= 3; ++j) {
+      const long long o = d.p3(iv - 2 + i, jv - 2 + j, k);
+      c[o] = c[o] + dt * f[(i - 1) + 3 * (j - 1)];
+    }
+}
+
+// vx_pd, vy_pd += the umbrella winds (one value per column) on levels
+// ibase..itop (src/MesoInterpolater.f90:238-243)
+__global__ __launch_bounds__(kBlock) void k_umbrella_winds(Dims d, double *vx, double *vy, int ibase, int itop,
+                                                          const double *uvx, const double *uvy) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long plane = (long long)d.nx * d.ny;
+  if (t >= plane * (itop - ibase + 1)) return;
+  const int i = (int)(t % d.nx) + 1, j = (int)((t / d.nx) % d.ny) + 1, k = ibase + (int)(t / plane);
+  const long long o = d.p3(i, j, k), q = (long long)(i - 1) + (long long)d.nx * (j - 1);
+  vx[o] = vx[o] + uvx[q];
+  vy[o] = vy[o] + uvy[q];
+}
+
 unsigned blocks_for(long long n) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b > 65535LL * 16) b = 65535LL * 16;
@@ -265,6 +305,23 @@ hipError_t launch_mirror_interior(hipStream_t s, const Dims &d, const double *c0
   unsigned b = blocks_for(d.ncell * d.ns);
   if (b > 8192) b = 8192;
   hipLaunchKernelGGL(k_mirror, dim3(b), dim3(kBlock), 0, s, d, c0, c1);
+  return hipGetLastError();
+}
+
+hipError_t launch_umbrella_source(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt, int ibase,
+                                  int itop, int ngs, const double *st, const double *fu) {
+  const long long tot = (long long)(d.nz + 1) * ngs;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_umbrella_source, dim3(blocks_for(tot)), dim3(kBlock), 0, s, d, c0, iv, jv, dt, ibase, itop,
+                     ngs, st, fu);
+  return hipGetLastError();
+}
+
+hipError_t launch_umbrella_winds(hipStream_t s, const Dims &d, double *vx, double *vy, int ibase, int itop,
+                                 const double *uvx, const double *uvy) {
+  const long long tot = (long long)d.nx * d.ny * (itop - ibase + 1);
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_umbrella_winds, dim3(blocks_for(tot)), dim3(kBlock), 0, s, d, vx, vy, ibase, itop, uvx, uvy);
   return hipGetLastError();
 }
 

@@ -162,6 +162,12 @@ hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0);
 hipError_t launch_copy_ghosts(hipStream_t s, const Dims &d, const double *src, double *dst);
 hipError_t launch_fill(hipStream_t s, double *p, long long n, double v);
 hipError_t launch_mirror_interior(hipStream_t s, const Dims &d, const double *c0, double *c1);
+// umbrella source: 3x3 patch averaging (levels ibase..itop) and insertion
+// (levels 1..nz+1) for bins < ngs; umbrella winds added to vx, vy
+hipError_t launch_umbrella_source(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt, int ibase,
+                                  int itop, int ngs, const double *st, const double *fu);
+hipError_t launch_umbrella_winds(hipStream_t s, const Dims &d, double *vx, double *vy, int ibase, int itop,
+                                 const double *uvx, const double *uvy);
 hipError_t launch_source_add(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt,
                              const double *flux);
 hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);

@@ -5,9 +5,13 @@
 //   EruptivePulse_MassFluxRate  src/Source.f90:508-589
 //   CheckEruptivePulses         src/Source.f90:613-685
 //   TephraSourceNodes           src/Source.f90:708-783
+//   Source_Umbrella             src/Source_Umbrella.f90:85-560 (stencil,
+//                               levels, radial winds, 3x3 patch flux, volume)
 // Each is evaluated expression by expression in the reference's order with
 // the C library's exp, so the results are those of the Fortran routines.
 #include <math.h>
+
+#include <algorithm>
 #include <stdio.h>
 #include <string.h>
 
@@ -198,6 +202,139 @@ int ash3d_tr_source_nodes(const ash3d_tr_source_desc *s, int32_t nz, int32_t ns,
     return src_fail(msg);
   }
   *ieruption = jeruption;  // (:774)
+  return 0;
+}
+
+// ---- umbrella sources ----------------------------------------------------
+
+int ash3d_tr_umbrella_stencil(const double *kappa_pd, int32_t nx, int32_t ny, int32_t nz, int32_t ivent,
+                              int32_t jvent, double *avg_stencil, double *scale_fac) {
+  if (!kappa_pd || !avg_stencil || !scale_fac || nx < 1 || ny < 1 || nz < 1)
+    return src_fail("ash3d_tr_umbrella_stencil: null or empty input");
+  if (ivent < 1 || ivent > nx || jvent < 1 || jvent > ny)
+    return src_fail("ash3d_tr_umbrella_stencil: vent outside the grid");
+  const long long sy = nx + 4, sz = (long long)(nx + 4) * (ny + 4);
+  auto kap = [&](int i, int j, int k) { return kappa_pd[(i + 1) + sy * (j + 1) + sz * (k + 1)]; };
+  for (int k = 1; k <= nz; ++k) {  // (:126-137)
+    double tot_vol = 0.0;
+    for (int i = 1; i <= 3; ++i)
+      for (int j = 1; j <= 3; ++j) {
+        const double v = kap(ivent - 2 + i, jvent - 2 + j, k);
+        avg_stencil[(i - 1) + 3 * (j - 1) + 9 * (k - 1)] = v;
+        tot_vol = tot_vol + v;
+      }
+    scale_fac[k - 1] = kap(ivent, jvent, k) / tot_vol;
+    for (int q = 0; q < 9; ++q) avg_stencil[q + 9 * (k - 1)] = avg_stencil[q + 9 * (k - 1)] / tot_vol;
+  }
+  return 0;
+}
+
+int ash3d_tr_umbrella_levels(const double *z_vec_init, int32_t nz, double plume_height, int32_t *ibase,
+                             int32_t *itop) {
+  if (!z_vec_init || !ibase || !itop || nz < 1) return src_fail("ash3d_tr_umbrella_levels: null or empty input");
+  *itop = 0;
+  *ibase = 0;
+  for (int k = 1; k <= nz; ++k) {  // (:139-150)
+    if (z_vec_init[k] >= plume_height && z_vec_init[k - 1] < plume_height) *itop = k;
+    if (z_vec_init[k] >= 0.75 * plume_height && z_vec_init[k - 1] < 0.75 * plume_height) *ibase = k;
+  }
+  return 0;
+}
+
+int ash3d_tr_umbrella_winds(const ash3d_tr_umbrella_wind_params *p, int32_t nx, int32_t ny, const double *lon_cc,
+                            const double *lat_cc, double *uvx, double *uvy) {
+  constexpr double PI = 3.141592653589793;  // src/Ash3d_VariableModules.f90:275-288
+  constexpr double DEG2RAD = 1.7453292519943295e-2, DEG2KMLAT = 111.0, DEG2KMLON = 111.321;
+  constexpr double KM_2_M = 1.0e3, MPS_2_KMPHR = 3.6, HR_2_S = 3.6e+3;
+  if (!p || !lon_cc || !lat_cc || !uvx || !uvy || nx < 1 || ny < 1)
+    return src_fail("ash3d_tr_umbrella_winds: null or empty input");
+  for (long long c = 0; c < (long long)nx * ny; ++c) uvx[c] = uvy[c] = 0.0;  // (:257-258)
+  if (!p->is_latlon)
+    return src_fail("ash3d_tr_umbrella_winds: umbrella_winds is not yet set up to handle projected coordinates");
+  if (p->vel_model != 1 && p->vel_model != 2)
+    return src_fail("ash3d_tr_umbrella_winds: Umb.Vel.Model unknown.  Should be 1 or 2.");
+  double mfr = p->mass_flux_rate / HR_2_S;  // kg/s (:270)
+  if (p->airborne_one_bin) mfr = 20.0 * mfr;  // (:274-276)
+  const double C_Costa = (fabs(p->lat_volcano) < 23.0) ? 0.43e3 : 0.87e3;  // (:279-285)
+  // Eq. 2 of Mastin and Van Eaton (2020), m3/s (:288-289)
+  const double qnow = C_Costa * sqrt(p->k_entrainment) * pow(mfr, 3.0 / 4.0) / pow(p->n_bv, 5.0 / 4.0);
+  const double etime_s = p->first_time ? fmin(p->simtime, p->end_time) * HR_2_S  // (:318-324)
+                                       : fmax(p->time, p->dt) * HR_2_S;
+  // Eq. 1 and its time derivative (:327-336)
+  const double base = 3.0 * p->lambda * p->n_bv * qnow / (2.0 * PI);
+  const double cloud_radius = pow(base, 1.0 / 3.0) * pow(etime_s, 2.0 / 3.0) / KM_2_M;
+  const double edge_speed = (2.0 / 3.0) * pow(base, 1.0 / 3.0) * pow(etime_s, -1.0 / 3.0);
+  const double node_w = (3.0 / 2.0) * p->de_km, node_h = (3.0 / 2.0) * p->dn_km;  // (:122-123)
+  if (cloud_radius <= fmax(node_w, node_h)) return 0;  // (:338-340)
+  const int ew_nodes = (int)(cloud_radius / p->de_km) + 1, ns_nodes = (int)(cloud_radius / p->dn_km) + 1;
+  const int west = std::max(1, p->ivent - ew_nodes), east = std::min(nx, p->ivent + ew_nodes);
+  const int south = std::max(1, p->jvent - ns_nodes), north = std::min(ny, p->jvent + ns_nodes);
+  for (int ii = west; ii <= east; ++ii) {  // (:351-389)
+    for (int jj = south; jj <= north; ++jj) {
+      if (ii == p->ivent && jj == p->jvent) continue;
+      const double avg_lat = (lat_cc[jj - 1] + p->lat_volcano) / 2.0;
+      const double ns_km = (lat_cc[jj - 1] - p->lat_volcano) * DEG2KMLAT;
+      const double ew_km = (lon_cc[ii - 1] - p->lon_volcano) * cos(avg_lat * DEG2RAD) * DEG2KMLON;
+      const double radnow = sqrt(ns_km * ns_km + ew_km * ew_km);
+      if (radnow < cloud_radius) {
+        double w;
+        if (p->vel_model == 1) {
+          const double rexp = 3.0;  // the legacy exponent (:366-367)
+          w = edge_speed * MPS_2_KMPHR * (3.0 / 4.0) * (cloud_radius / radnow) *
+              (1.0 + (1.0 / 3.0) * pow(radnow / cloud_radius, rexp));
+        } else {
+          w = edge_speed * MPS_2_KMPHR * pow(cloud_radius / radnow, 0.5);
+        }
+        const double thetanow = atan2(ns_km, ew_km);
+        uvx[(ii - 1) + (long long)nx * (jj - 1)] = w * cos(thetanow);
+        uvy[(ii - 1) + (long long)nx * (jj - 1)] = w * sin(thetanow);
+      }
+    }
+  }
+  return 0;
+}
+
+int ash3d_tr_umbrella_nodes(const double *flux, int32_t nz, int32_t ns, int32_t n_gs_max, int32_t ibase,
+                            int32_t itop, const double *scale_fac, double *flux_umb) {
+  if (!flux || !scale_fac || !flux_umb || nz < 1 || ns < 1)
+    return src_fail("ash3d_tr_umbrella_nodes: null or empty input");
+  if (n_gs_max < 0 || n_gs_max > ns) return src_fail("ash3d_tr_umbrella_nodes: n_gs_max outside 0..ns");
+  if (itop > nz) return src_fail("ash3d_tr_umbrella_nodes: itop above the grid");
+  auto fu = [&](int i, int j, int k, int n) -> double & {
+    return flux_umb[(i - 1) + 3 * (j - 1) + 9 * (long long)(k - 1) + 9LL * (nz + 1) * (n - 1)];
+  };
+  for (long long c = 0; c < 9LL * (nz + 1) * ns; ++c) flux_umb[c] = 0.0;
+  for (int k = 1; k <= nz + 1; ++k) {  // (:429-444)
+    for (int n = 1; n <= n_gs_max; ++n) {
+      const double f = flux[(k - 1) + (long long)(nz + 1) * (n - 1)];
+      if (k < ibase) {
+        fu(2, 2, k, n) = f;  // below the cloud: the column
+      } else if (k <= itop) {
+        for (int i = 1; i <= 3; ++i)
+          for (int j = 1; j <= 3; ++j) fu(i, j, k, n) = f * scale_fac[k - 1];
+      }  // above the cloud: nothing
+    }
+  }
+  return 0;
+}
+
+int ash3d_tr_umbrella_vol_inc(const double *flux, int32_t nz, int32_t ns, int32_t n_gs_max, int32_t ibase,
+                              int32_t itop, const double *avg_stencil, const double *kappa_vent_col, double dt,
+                              double magma_density, double *vol) {
+  if (!flux || !avg_stencil || !kappa_vent_col || !vol || nz < 1 || n_gs_max < 0 || n_gs_max > ns)
+    return src_fail("ash3d_tr_umbrella_vol_inc: bad input");
+  double tmp = 0.0;
+  auto f = [&](int k, int n) { return flux[(k - 1) + (long long)(nz + 1) * (n - 1)]; };
+  for (int n = 1; n <= n_gs_max; ++n)  // (:489-499)
+    for (int k = 1; k <= ibase - 1; ++k)
+      tmp = tmp + dt * f(k, n) * kappa_vent_col[k - 1] / magma_density / KM3_2_M3;
+  for (int i = 1; i <= 3; ++i)  // (:501-516), levels below 1 skipped
+    for (int j = 1; j <= 3; ++j)
+      for (int k = std::max(ibase, 1); k <= itop; ++k)
+        for (int n = 1; n <= n_gs_max; ++n)
+          tmp = tmp + dt * f(k, n) * avg_stencil[(i - 1) + 3 * (j - 1) + 9 * (k - 1)] * kappa_vent_col[k - 1] /
+                          magma_density / KM3_2_M3;
+  *vol = tmp;
   return 0;
 }
 

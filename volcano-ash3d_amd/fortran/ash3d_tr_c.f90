@@ -35,6 +35,7 @@
              c_ash3d_tr_adj_step, c_ash3d_tr_adj_source, c_ash3d_tr_adj_source_read, &
              c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load, c_ash3d_tr_minmax_index, &
              c_ash3d_tr_met_push, c_ash3d_tr_met_interp, c_ash3d_tr_tune, &
+             c_ash3d_tr_umbrella_source_add, c_ash3d_tr_umbrella_winds_add, &
              ASH3D_TUNE_LINE_SOLVER, ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED
 
       integer(c_int32_t), parameter :: ASH3D_TR_ABI_VERSION = 1
@@ -136,6 +137,21 @@
           integer(c_int32_t), value :: iv, jv
           real(c_double), value :: dt
           real(c_double), intent(in) :: flux(*)
+        end function
+        integer(c_int) function c_ash3d_tr_umbrella_source_add(ctx, iv, jv, dt, ibase, itop, ngs, stenc, flux) &
+            bind(C, name='ash3d_tr_umbrella_source_add')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: iv, jv, ibase, itop, ngs
+          real(c_double), value :: dt
+          real(c_double), intent(in) :: stenc(*), flux(*)
+        end function
+        integer(c_int) function c_ash3d_tr_umbrella_winds_add(ctx, ibase, itop, uvx, uvy) &
+            bind(C, name='ash3d_tr_umbrella_winds_add')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: ibase, itop
+          real(c_double), intent(in) :: uvx(*), uvy(*)
         end function
         integer(c_int) function c_ash3d_tr_checkpoints(ctx, nslots) bind(C, name='ash3d_tr_checkpoints')
           import :: c_int, c_ptr, c_int32_t

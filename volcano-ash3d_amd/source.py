@@ -4,7 +4,10 @@ C ABI.  `Source` holds the eruptive pulses, builds the normalized source
 column and the mass flux rates once (Calc_Normalized_SourceCol,
 EruptivePulse_MassFluxRate), and per time step finds the active pulses
 (CheckEruptivePulses), forms SourceNodeFlux (TephraSourceNodes) and inserts
-it with the device column add (src/Ash3d.F90:414-431).
+it with the device column add (src/Ash3d.F90:414-431).  The umbrella
+helpers at the end are the host side of src/Source_Umbrella.f90 (stencil,
+levels, radial winds, 3x3 patch flux, inserted volume); the device side is
+Transport.umbrella_source_add / umbrella_winds_add.
 """
 from __future__ import annotations
 
@@ -111,3 +114,77 @@ class Source:
         flux = self.nodes(kap, st.ns)
         T.source_column_add(ivent, jvent, flux)
         return flux
+
+
+# ---- umbrella sources (src/Source_Umbrella.f90) -----------------------------
+
+UmbrellaWindParams = _tr.UmbrellaWindParams
+
+
+def _check(lib, rc):
+    if rc:
+        raise _tr.Ash3dError(rc, lib.ash3d_tr_last_error().decode())
+
+
+def umbrella_stencil(kappa_pd, ivent: int, jvent: int):
+    """Allocate_Source_Umbrella's weights: (avg_stencil (3,3,nz), scale_fac
+    (nz)) from the padded kappa_pd (nx+4, ny+4, nz+4)."""
+    lib = _tr.load()
+    kap = np.asfortranarray(kappa_pd, dtype=np.float64)
+    nx, ny, nz = (n - 4 for n in kap.shape)
+    st = np.zeros((3, 3, nz), order="F")
+    sf = np.zeros(nz)
+    _check(lib, lib.ash3d_tr_umbrella_stencil(kap.ctypes.data_as(_dp), nx, ny, nz, ivent, jvent,
+                                              st.ctypes.data_as(_dp), sf.ctypes.data_as(_dp)))
+    return st, sf
+
+
+def umbrella_levels(z_vec_init, plume_height: float):
+    """(ibase, itop): the levels holding 0.75 and 1.0 of the plume height;
+    z_vec_init = z(0:nz) in km."""
+    lib = _tr.load()
+    z = _arr(z_vec_init)
+    ib, it = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib, lib.ash3d_tr_umbrella_levels(z.ctypes.data_as(_dp), len(z) - 1, float(plume_height),
+                                             ctypes.byref(ib), ctypes.byref(it)))
+    return ib.value, it.value
+
+
+def umbrella_winds(params: UmbrellaWindParams, lon_cc, lat_cc):
+    """The radial umbrella-cloud winds (uvx, uvy), (nx, ny) in km/hr."""
+    lib = _tr.load()
+    lon, lat = _arr(lon_cc), _arr(lat_cc)
+    nx, ny = len(lon), len(lat)
+    ux, uy = np.zeros((nx, ny), order="F"), np.zeros((nx, ny), order="F")
+    _check(lib, lib.ash3d_tr_umbrella_winds(ctypes.byref(params), nx, ny, lon.ctypes.data_as(_dp),
+                                            lat.ctypes.data_as(_dp), ux.ctypes.data_as(_dp),
+                                            uy.ctypes.data_as(_dp)))
+    return ux, uy
+
+
+def umbrella_nodes(flux, ibase: int, itop: int, scale_fac, n_gs_max: int | None = None):
+    """TephraSourceNodes_Umbrella: the 3x3 patch flux (3,3,nz+1,ns) from the
+    column flux (nz+1, ns)."""
+    lib = _tr.load()
+    f = np.asfortranarray(flux, dtype=np.float64)
+    nz1, ns = f.shape
+    sf = _arr(scale_fac)
+    fu = np.zeros((3, 3, nz1, ns), order="F")
+    _check(lib, lib.ash3d_tr_umbrella_nodes(f.ctypes.data_as(_dp), nz1 - 1, ns, ns if n_gs_max is None else n_gs_max,
+                                            ibase, itop, sf.ctypes.data_as(_dp), fu.ctypes.data_as(_dp)))
+    return fu
+
+
+def umbrella_vol_inc(flux, ibase: int, itop: int, avg_stencil, kappa_vent_col, dt: float, magma_density: float,
+                     n_gs_max: int | None = None) -> float:
+    """SourceVolInc_Umbrella: km3 DRE inserted in one step."""
+    lib = _tr.load()
+    f = np.asfortranarray(flux, dtype=np.float64)
+    nz1, ns = f.shape
+    st = np.asfortranarray(avg_stencil, dtype=np.float64)
+    kap = _arr(kappa_vent_col)
+    vol = ctypes.c_double()
+    _check(lib, lib.ash3d_tr_umbrella_vol_inc(f.ctypes.data_as(_dp), nz1 - 1, ns, ns if n_gs_max is None else n_gs_max,
+                                              ibase, itop, st.ctypes.data_as(_dp), kap.ctypes.data_as(_dp),
+                                              float(dt), float(magma_density), ctypes.byref(vol)))
+    return vol.value

@@ -86,6 +86,18 @@ class ProductsOut(ctypes.Structure):
                 ("cloud_area", ctypes.c_double), ("deposit_area", ctypes.c_double)]
 
 
+class UmbrellaWindParams(ctypes.Structure):
+    """ash3d_tr_umbrella_wind_params: the inputs of umbrella_winds
+    (src/Source_Umbrella.f90:206-394)."""
+    _fields_ = [("vel_model", ctypes.c_int32), ("k_entrainment", ctypes.c_double), ("lambda_", ctypes.c_double),
+                ("n_bv", ctypes.c_double), ("mass_flux_rate", ctypes.c_double),
+                ("airborne_one_bin", ctypes.c_int32), ("lat_volcano", ctypes.c_double),
+                ("lon_volcano", ctypes.c_double), ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("first_time", ctypes.c_int32), ("simtime", ctypes.c_double), ("end_time", ctypes.c_double),
+                ("de_km", ctypes.c_double), ("dn_km", ctypes.c_double), ("ivent", ctypes.c_int32),
+                ("jvent", ctypes.c_int32), ("is_latlon", ctypes.c_int32)]
+
+
 class SourceDesc(ctypes.Structure):
     """ash3d_tr_source_desc: eruptive pulses and vertical grid of the source
     builders (src/Source.f90)."""
@@ -153,6 +165,18 @@ _SIGS = {
                                  _ip],
     "ash3d_tr_source_nodes": [ctypes.POINTER(SourceDesc), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
                               _dp, _dp, _ip, ctypes.c_int32, _dp],
+    "ash3d_tr_umbrella_stencil": [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, _dp, _dp],
+    "ash3d_tr_umbrella_levels": [_dp, ctypes.c_int32, ctypes.c_double, _ip, _ip],
+    "ash3d_tr_umbrella_winds": [ctypes.POINTER(UmbrellaWindParams), ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp,
+                                _dp],
+    "ash3d_tr_umbrella_nodes": [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                ctypes.c_int32, _dp, _dp],
+    "ash3d_tr_umbrella_vol_inc": [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp],
+    "ash3d_tr_umbrella_source_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp],
+    "ash3d_tr_umbrella_winds_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _dp, _dp],
     "ash3d_tr_graph_capture": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_double, _ip],
     "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
@@ -328,6 +352,26 @@ class Transport:
         self._check(self.lib.ash3d_tr_source_column_add(self._ctx, ivent, jvent, self.st.dt,
                                                         flux.ctypes.data_as(_dp)))
 
+
+    def umbrella_source_add(self, ivent: int, jvent: int, ibase: int, itop: int, n_gs_max: int,
+                            avg_stencil: np.ndarray, flux_umb: np.ndarray, dt: float | None = None):
+        """The umbrella source step of src/Ash3d.F90:447-460 on ts0:
+        avg_stencil (3,3,nz), flux_umb (3,3,nz+1,ns) (see source.py)."""
+        st = avg_stencil if avg_stencil.flags.f_contiguous else np.asfortranarray(avg_stencil)
+        fu = np.asfortranarray(flux_umb, dtype=np.float64)
+        assert st.shape == (3, 3, self.st.nz) and fu.shape == (3, 3, self.st.nz + 1, self.st.ns)
+        self._check(self.lib.ash3d_tr_umbrella_source_add(self._ctx, ivent, jvent,
+                                                          self.st.dt if dt is None else float(dt), ibase, itop,
+                                                          n_gs_max, np.asfortranarray(st, dtype=np.float64)
+                                                          .ctypes.data_as(_dp), fu.ctypes.data_as(_dp)))
+
+    def umbrella_winds_add(self, ibase: int, itop: int, uvx: np.ndarray, uvy: np.ndarray):
+        """vx_pd, vy_pd += the umbrella winds (nx, ny) on levels ibase..itop."""
+        ux = np.asfortranarray(uvx, dtype=np.float64)
+        uy = np.asfortranarray(uvy, dtype=np.float64)
+        assert ux.shape == uy.shape == (self.st.nx, self.st.ny)
+        self._check(self.lib.ash3d_tr_umbrella_winds_add(self._ctx, ibase, itop, ux.ctypes.data_as(_dp),
+                                                         uy.ctypes.data_as(_dp)))
 
     def mass_totals(self):
         """Per-bin (aloft, deposit, outflow) masses in kg, computed on the
