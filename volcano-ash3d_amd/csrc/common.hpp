@@ -81,11 +81,14 @@ __host__ __device__ inline long long kline_stride(const Dims &d, int dir) {
 
 // XCD-aware order of a flattened (tile, bin group) grid.  Workgroups are
 // dispatched round-robin over the XCDs (workgroup b to XCD b mod 8), each
-// XCD with its own L2.  The m-th workgroup dispatched to XCD x takes tile
-// (m / G) * 8 + x and bin group m mod G, so the G bin groups of a tile run
-// back to back on one XCD and read the tile's per-cell fields (winds,
-// geometry, K, line factors) through the same L2.  A launch has
-// padded_tiles(T) * G workgroups; those with tile >= T return at once.
+// XCD with its own L2.  XCD x takes the x-th eighth of the (tile, group)
+// items in order: the m-th workgroup dispatched to it takes item
+// x * (items / 8) + m, i.e. tile item / G and bin group item mod G.  So the
+// G bin groups of a tile run back to back on one XCD and read the tile's
+// per-cell fields (winds, geometry, K, line factors) through the same L2,
+// and neighbouring tiles, whose rows share the cache lines at their edges,
+// go through that L2 too.  A launch has padded_tiles(T) * G workgroups
+// (a multiple of 8); those with tile >= T return at once.
 constexpr int kXcds = 8;
 __host__ __device__ inline long long padded_tiles(long long T) { return (T + kXcds - 1) / kXcds * kXcds; }
 struct TileGroup {
@@ -93,8 +96,9 @@ struct TileGroup {
   int group;
 };
 __device__ inline TileGroup xcd_tile_group(long long b, int G) {
-  const long long x = b % kXcds, m = b / kXcds;
-  return TileGroup{(m / G) * kXcds + x, (int)(m % G)};
+  const long long per = (long long)gridDim.x / kXcds;
+  const long long it = (b % kXcds) * per + b / kXcds;
+  return TileGroup{it / G, (int)(it % G)};
 }
 
 constexpr int kJunk = 4096;
