@@ -114,9 +114,10 @@ def cpu_baseline(st, use_diffusion, steps):
 
 def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="exact", tuning=None):
     """Time a source-inversion gradient on the same grid: forward run with
-    checkpoints, then the checkpointed reverse sweep.  Returns the
-    adjoint/forward time ratio and the per-step times of ash3d_tr_adj_step
-    and ash3d_tr_step on resident states."""
+    checkpoints (and the kept advection-sweep inputs), then the checkpointed
+    reverse sweep.  Returns the adjoint/forward time ratio and the per-step
+    times of ash3d_tr_adj_step (sweep inputs recomputed),
+    ash3d_tr_adj_step_kept and ash3d_tr_step on resident states."""
     adj = importlib.import_module("volcano-ash3d_amd.adjoint")
     import numpy as np
     st.concen_pd[...] = 0.0
@@ -165,6 +166,18 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="
     t_adj = (time.perf_counter() - t0) / reps
     prof = T.profile_read()
     T.profile(False)
+    # the adjoint step with the sweep inputs kept by the forward step (what
+    # the inversion's reverse sweep runs for every kept step)
+    sb, sz = inv._snaps(0)
+    T.checkpoint_load(0)
+    T.transport_step_keep(1, use_diffusion, sb, sz)
+    T.checkpoint_load(0)
+    T.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        T.adj_step_kept(1, use_diffusion, sb, sz)
+    T.synchronize()
+    t_adj_kept = (time.perf_counter() - t0) / reps
     T.checkpoint_load(0)
     T.synchronize()
     t0 = time.perf_counter()
@@ -178,8 +191,9 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="
             "line_solver": line_solver,
             "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
             "ratio_sqrt_checkpointing": round(g_rep / f_rep, 3), "sqrt_segment": K_sqrt,
-            "adj_step_ms": round(1e3 * t_adj, 3), "step_ms": round(1e3 * t_step, 3),
-            "step_ratio": round(t_adj / t_step, 3), "grad_norm": float(np.abs(g).max()),
+            "adj_step_ms": round(1e3 * t_adj, 3), "adj_step_kept_ms": round(1e3 * t_adj_kept, 3),
+            "step_ms": round(1e3 * t_step, 3), "step_ratio": round(t_adj / t_step, 3),
+            "step_ratio_kept": round(t_adj_kept / t_step, 3), "grad_norm": float(np.abs(g).max()),
             "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
 
 

@@ -153,6 +153,13 @@ int ash3d_tr_diffuse_vert(ash3d_tr_ctx *ctx, int32_t use_cn, double dt, double i
  * Set_BC(2); DiffuseHorz(itime); DiffuseVert. */
 int ash3d_tr_step(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion,
                   int32_t use_cn, double imp_fac, const int32_t *is_aloft);
+/* ash3d_tr_step whose advection sweeps run out of place through checkpoint
+ * slots slot_b and slot_z (ash3d_tr_checkpoints), which keep the inputs of
+ * the second horizontal and of the vertical sweep for
+ * ash3d_tr_adj_step_kept.  The step's results are ash3d_tr_step's, bit for
+ * bit.  No counterpart in the reference (it has no adjoint). */
+int ash3d_tr_step_keep(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                       double imp_fac, const int32_t *is_aloft, int32_t slot_b, int32_t slot_z);
 
 /* Step glue of the driver, src/Ash3d.F90:376-378:
  * concen_pd(:,:,:,:,ts0) = concen_pd(:,:,:,:,ts1); concen_pd(...,ts1) = 0. */
@@ -286,6 +293,10 @@ int ash3d_tr_adj_diffuse(ash3d_tr_ctx *ctx, int32_t dir, int32_t use_cn, double 
  * adjoint sweeps run in reverse order. */
 int ash3d_tr_adj_step(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
                       double imp_fac, const int32_t *is_aloft);
+/* ash3d_tr_adj_step with the sweep inputs kept by ash3d_tr_step_keep in
+ * slots slot_b, slot_z instead of recomputed (ts0 holds the step input) */
+int ash3d_tr_adj_step_kept(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                           double imp_fac, const int32_t *is_aloft, int32_t slot_b, int32_t slot_z);
 /* Source gradient: slot(k,n) += dt * lambda(ivent,jvent,k,n), k = 1..nz+1,
  * the adjoint of ash3d_tr_source_column_add; read returns all slots
  * (slot-major, then bin, then k). */

@@ -17,17 +17,24 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
+def tr_mod(pkg):
+    t = importlib.import_module("volcano-ash3d_amd.transport")
+    t.load()
+    return t
+
+
+@pytest.fixture(scope="module")
 def adj(pkg):
     importlib.import_module("volcano-ash3d_amd.transport").load()
     return importlib.import_module("volcano-ash3d_amd.adjoint")
 
 
-def make(pkg, adj, segment=None, use_cn=True, periodic=False, seed=7):
+def make(pkg, adj, segment=None, use_cn=True, periodic=False, seed=7, keep=True):
     st = pkg.workloads.random_state(20, 16, 10, 3, seed=seed, latlon=True, periodic=periodic, use_cn=use_cn)
     st.concen_pd[...] = 0.0
     dz = 0.5 + 0.05 * np.arange(10)
     inv = adj.SourceInversion(st, 8, 7, n_heights=6, n_periods=3, steps_per_period=4, frac=[0.5, 0.3, 0.2],
-                              dz=dz, obs_every=2, use_diffusion=True, segment=segment)
+                              dz=dz, obs_every=2, use_diffusion=True, segment=segment, keep_snapshots=keep)
     return inv
 
 
@@ -64,6 +71,57 @@ def test_gradient_independent_of_segment(pkg, adj):
         grads.append(inv.gradient(w))
         inv.close()
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_kept_snapshots_change_nothing(pkg, adj, periodic):
+    """Keeping the advection sweeps' inputs in the forward run (and in the
+    replays) instead of recomputing them in the adjoint steps leaves the
+    loads and the gradient bit for bit as they were."""
+    rng = np.random.default_rng(4)
+    s = 1e4 * rng.random((3, 6))
+    out = []
+    for keep in (False, True):
+        for K in (5, 12):
+            inv = make(pkg, adj, segment=K, periodic=periodic, seed=5, keep=keep)
+            w = np.random.default_rng(6).random((len(inv.obs_steps), 20, 16))
+            out.append((inv.forward(s), inv.gradient(w)))
+            inv.close()
+    for loads, grad in out[1:]:
+        assert np.array_equal(loads, out[0][0]) and np.array_equal(grad, out[0][1])
+
+
+def test_step_keep_and_adj_step_kept(pkg, tr_mod):
+    """ash3d_tr_step_keep gives ash3d_tr_step's state, outflows and deposit;
+    ash3d_tr_adj_step_kept with its slots gives ash3d_tr_adj_step's lambda."""
+    st = pkg.workloads.random_state(17, 13, 9, 3, seed=21, latlon=True, periodic=True, use_cn=True)
+    A, B = tr_mod.Transport(st.copy()), tr_mod.Transport(st.copy())
+    B.checkpoints(3)
+    for X in (A, B):
+        X.step_glue()
+    B.checkpoint_save(0)
+    A.transport_step(2, True)
+    B.transport_step_keep(2, True, 1, 2)
+    A.sync_to_host()
+    B.sync_to_host()
+    for name in ("concen_pd", "outflow_xz1_pd", "outflow_xz2_pd", "outflow_yz1_pd", "outflow_yz2_pd",
+                 "outflow_xy1_pd", "outflow_xy2_pd", "deposit_granularity"):
+        assert np.array_equal(getattr(A.st, name), getattr(B.st, name)), name
+    lam = np.zeros(st.concen_pd.shape[:4], order="F")
+    lam[2:-2, 2:-2, 2:-2] = np.random.default_rng(2).random((17, 13, 9, 3))
+    B.checkpoint_load(0)
+    B.adj_init(0)
+    B.adj_set(lam)
+    B.adj_step_kept(2, True, 1, 2)
+    g_kept = B.adj_get()
+    B.checkpoint_load(0)
+    B.adj_set(lam)
+    B.adj_step(2, True)
+    assert np.array_equal(g_kept, B.adj_get())
+    with pytest.raises(tr_mod.Ash3dError, match="snapshot slots"):
+        B.transport_step_keep(2, True, 1, 1)
+    A.close()
+    B.close()
 
 
 def test_torch_autograd(pkg, adj):

@@ -18,7 +18,10 @@ step.  The steps form segments of K; the forward run stores the input of
 the first step of every segment but the last, and the input of every step
 of the last segment.  The reverse sweep walks the last segment backwards
 directly, and replays every other segment from its checkpoint (keeping
-each step input) before walking it backwards.  Device memory: n_seg + K + 1
+each step input) before walking it backwards.  With keep_snapshots (the
+default) a kept step also keeps the inputs of its second and third
+advection sweeps (ash3d_tr_step_keep), which the adjoint step would
+otherwise recompute.  Device memory: n_seg + K (3K with snapshots) + 1
 states (the +1 holds the initial state); extra work: one forward step per
 step outside the last segment.  A run whose step inputs all fit the device-memory
 budget is a single segment and never replays; otherwise K is the longest
@@ -38,7 +41,8 @@ from .state import TransportState
 class SourceInversion:
     def __init__(self, st: TransportState, ivent: int, jvent: int, n_heights: int, n_periods: int,
                  steps_per_period: int, frac, dz, obs_every: int = 1, use_diffusion: bool = True,
-                 segment: int = None, itime0: int = 1, device: int = 0, mem_budget: float = 64e9):
+                 segment: int = None, itime0: int = 1, device: int = 0, mem_budget: float = 64e9,
+                 keep_snapshots: bool = True):
         if not 1 <= n_heights <= st.nz + 1:
             raise ValueError(f"n_heights must be in 1..nz+1 (got {n_heights})")
         self.st = st
@@ -53,20 +57,27 @@ class SourceInversion:
         self.n_steps = self.P * self.spp
         self.obs_steps = [t for t in range(self.n_steps) if (t + 1) % obs_every == 0]
         self._obs_index = {t: i for i, t in enumerate(self.obs_steps)}
+        # states kept per step of the segment being reversed: its input, and
+        # with keep_snapshots the inputs of its second and third advection
+        # sweeps (the adjoint step then need not recompute them)
+        self.per_step = 3 if keep_snapshots else 1
+        self.keep = bool(keep_snapshots)
         if segment:
             self.K = int(segment)
         else:
             state_bytes = 8.0 * st.ns * (st.nx + 4) * (st.ny + 4) * (st.nz + 4)
-            slots = max(3, int(mem_budget // state_bytes) - 1)
-            if self.n_steps + 1 <= slots:
+            slots = max(1 + 2 * self.per_step, int(mem_budget // state_bytes) - 1)
+            if self.per_step * self.n_steps + 1 <= slots:
                 self.K = self.n_steps  # every step input stored: no replay
             else:  # the longest segments whose states fit
-                self.K = max([K for K in range(1, self.n_steps + 1) if -(-self.n_steps // K) + K <= slots]
+                self.K = max([K for K in range(1, self.n_steps + 1)
+                              if -(-self.n_steps // K) + self.per_step * K <= slots]
                              or [max(1, int(math.ceil(math.sqrt(self.n_steps))))])
         self.n_seg = (self.n_steps + self.K - 1) // self.K
         self.T = _tr.Transport(st, device=device)
         # slot 0: initial state; 1..n_seg: segment inputs; then K step inputs
-        self.T.checkpoints(1 + self.n_seg + self.K)
+        # (and 2K snapshots)
+        self.T.checkpoints(1 + self.n_seg + self.per_step * self.K)
         self.T.step_glue()  # ts0 <- the initial ts1, as the first step's glue would
         self.T.checkpoint_save(0)
         self.T.adj_init(self.P)
@@ -84,6 +95,17 @@ class SourceInversion:
 
     def _inner(self, t):
         return 1 + self.n_seg + t % self.K
+
+    def _snaps(self, t):
+        """The two snapshot slots of step t of the segment being reversed."""
+        base = 1 + self.n_seg + self.K + 2 * (t % self.K)
+        return base, base + 1
+
+    def _step(self, t, keep):
+        if keep and self.keep:
+            self.T.transport_step_keep(self.itime0 + t, self.use_diffusion, *self._snaps(t))
+        else:
+            self.T.transport_step(self.itime0 + t, self.use_diffusion)
 
     def _begin_step(self, s, t):
         """Glue and source insertion of step t (the forward step's input)."""
@@ -104,7 +126,7 @@ class SourceInversion:
                 self.T.checkpoint_save(self._inner(t))
             elif t % self.K == 0:
                 self.T.checkpoint_save(1 + t // self.K)
-            self.T.transport_step(self.itime0 + t, self.use_diffusion)
+            self._step(t, t // self.K == last)
             if t in self._obs_index:
                 loads[self._obs_index[t]] = self.T.column_load(self.dz)
         return loads
@@ -120,20 +142,28 @@ class SourceInversion:
         T.adj_init(self.P)
         for seg in reversed(range(self.n_seg)):
             t0, t1 = seg * self.K, min((seg + 1) * self.K, self.n_steps)
+            kept = set()  # steps whose sweep inputs are in their snapshot slots
             if seg < self.n_seg - 1:
-                # replay the segment from its checkpoint, keeping every step input
+                # replay the segment from its checkpoint, keeping every step
+                # input (and the sweep inputs of all but its last step)
                 T.checkpoint_load(1 + seg)
                 for t in range(t0, t1):
                     if t > t0:
                         self._begin_step(s, t)
                     T.checkpoint_save(self._inner(t))
                     if t < t1 - 1:
-                        T.transport_step(self.itime0 + t, self.use_diffusion)
+                        self._step(t, True)
+                        kept.add(t)
+            elif self.keep:
+                kept = set(range(t0, t1))  # the forward run kept them
             for t in reversed(range(t0, t1)):
                 if t in self._obs_index:
                     T.adj_column_load(self.dz, w[self._obs_index[t]])
                 T.checkpoint_load(self._inner(t))
-                T.adj_step(self.itime0 + t, self.use_diffusion)
+                if self.keep and t in kept:
+                    T.adj_step_kept(self.itime0 + t, self.use_diffusion, *self._snaps(t))
+                else:
+                    T.adj_step(self.itime0 + t, self.use_diffusion)
                 T.adj_source(self.ivent, self.jvent, t // self.spp)
         G = T.adj_source_read(self.P)  # (P, ns, nz+1): sum over steps of dt * lambda(vent)
         grad = np.zeros((self.P, self.H))

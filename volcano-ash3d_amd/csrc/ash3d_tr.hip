@@ -276,19 +276,20 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
 }
 
 // One advection sweep on ts0, or on another state buffer whose outflow and
-// deposit go to the sink (recomputation for the adjoint).  With out, the
-// sweep reads state and writes its result to out (every interior cell of
-// the sweep's lines, then the ghost cells copied from state): out ends as
-// state would after the in-place sweep, and state is left as it was.
+// deposit go to the sink (recomputation for the adjoint; to_sink = false
+// keeps the real outflow and deposit arrays).  With out, the sweep reads
+// state and writes its result to out (every interior cell of the sweep's
+// lines, then the ghost cells copied from state): out ends as state would
+// after the in-place sweep, and state is left as it was.
 int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
-                 bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr) {
+                 bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr, bool to_sink = true) {
   int rc = check_bounds(x, bounds);
   if (rc) return rc;
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if (with_bc && (rc = set_bc1(x, bc_vel, state))) return rc;
   AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, state ? state : x->ts0());
   if (state && out) a.cout = out;
-  if (state) {
+  if (state && to_sink) {
     const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
     a.of1 = x->sink;
     a.of2 = x->sink + face;
@@ -936,6 +937,20 @@ int ash3d_tr_diffuse_vert(ash3d_tr_ctx *x, int32_t use_cn, double dt, double imp
   return sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b);
 }
 
+// DiffuseHorz / DiffuseVert of one step on ts0 (src/Ash3d.F90:496-508)
+static int step_diffusion(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_cn, double imp_fac,
+                          const int32_t *is_aloft, const int32_t *b) {
+  int rc;
+  {
+    ProfScope ps(x, ASH3D_PROF_BC);
+    HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
+  }
+  const int dfirst = (itime % 2 == 0) ? 0 : 1;
+  if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b, false))) return rc;
+  if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
+  return sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b);
+}
+
 int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
                   double imp_fac, const int32_t *is_aloft) {
   int rc = check_ctx(x);
@@ -952,16 +967,30 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false))) return rc;
   if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false))) return rc;
   if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false))) return rc;
-  if (use_diffusion) {
-    {
-      ProfScope ps(x, ASH3D_PROF_BC);
-      HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
-    }
-    const int dfirst = (itime % 2 == 0) ? 0 : 1;
-    if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b, false))) return rc;
-    if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
-    if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b))) return rc;
-  }
+  if (use_diffusion) return step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b);
+  return 0;
+}
+
+int ash3d_tr_step_keep(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                       double imp_fac, const int32_t *is_aloft, int32_t slot_b, int32_t slot_z) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const int ns = (int)x->ckpt.size();
+  if (slot_b < 0 || slot_b >= ns || slot_z < 0 || slot_z >= ns || slot_b == slot_z)
+    return fail(ASH3D_TR_EINVAL, "snapshot slots %d, %d of %d (two different slots)", slot_b, slot_z, ns);
+  int32_t b[6];
+  full_bounds(x, b);
+  // ash3d_tr_step's advection, each sweep out of place: ts0 -> slot_b ->
+  // slot_z -> ts0 (the same kernels and Set_BC(1) passes, so the same bits)
+  if ((rc = set_bc1(x, true))) return rc;
+  const int first = (itime % 2 == 0) ? 1 : 0;
+  const bool again = x->periodic != 0;
+  double *sb = x->ckpt[slot_b], *sz = x->ckpt[slot_z];
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->ts0(), sb, false))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false, sb, sz, false))) return rc;
+  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false, sz, x->ts0(), false))) return rc;
+  x->ts1_state = TS1_MIRROR;
+  if (use_diffusion) return step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b);
   return 0;
 }
 
@@ -1263,8 +1292,26 @@ int ash3d_tr_adj_diffuse(ash3d_tr_ctx *x, int32_t dir, int32_t use_cn, double dt
   return adj_diffuse_sweep(x, dir, use_cn, dt, imp_fac, is_aloft, bounds);
 }
 
+static int adj_step_impl(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                         double imp_fac, const int32_t *is_aloft, double *kept_b, double *kept_z);
+
 int ash3d_tr_adj_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
                       double imp_fac, const int32_t *is_aloft) {
+  return adj_step_impl(x, itime, dt, use_diffusion, use_cn, imp_fac, is_aloft, nullptr, nullptr);
+}
+
+int ash3d_tr_adj_step_kept(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                           double imp_fac, const int32_t *is_aloft, int32_t slot_b, int32_t slot_z) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const int ns = (int)x->ckpt.size();
+  if (slot_b < 0 || slot_b >= ns || slot_z < 0 || slot_z >= ns || slot_b == slot_z)
+    return fail(ASH3D_TR_EINVAL, "snapshot slots %d, %d of %d (two different slots)", slot_b, slot_z, ns);
+  return adj_step_impl(x, itime, dt, use_diffusion, use_cn, imp_fac, is_aloft, x->ckpt[slot_b], x->ckpt[slot_z]);
+}
+
+static int adj_step_impl(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
+                         double imp_fac, const int32_t *is_aloft, double *kept_b, double *kept_z) {
   int rc = check_ctx(x);
   if (rc || (rc = need_adjoint(x))) return rc;
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
@@ -1276,14 +1323,18 @@ int ash3d_tr_adj_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_dif
     HIPCHK(launch_fill(x->stream, x->lam, x->n_conc(), 0.0));
     return 0;
   }
-  // recompute the inputs of the second horizontal sweep and of the vertical
-  // sweep from the step input (ts0), as ash3d_tr_step computes them
+  // the inputs of the second horizontal sweep and of the vertical sweep:
+  // kept by ash3d_tr_step_keep, or recomputed from the step input (ts0) as
+  // ash3d_tr_step computes them
   if ((rc = set_bc1(x, true))) return rc;
   const int first = (itime % 2 == 0) ? 1 : 0;
-  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->ts0(), x->snap_b))) return rc;
-  if (x->periodic && (rc = set_bc1(x, false, x->snap_b))) return rc;
-  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, false, false, x->snap_b, x->snap_z))) return rc;
-  if (x->periodic && (rc = set_bc1(x, false, x->snap_z))) return rc;
+  double *qb = kept_b ? kept_b : x->snap_b, *qz = kept_z ? kept_z : x->snap_z;
+  if (!kept_b) {
+    if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, x->ts0(), x->snap_b))) return rc;
+    if (x->periodic && (rc = set_bc1(x, false, x->snap_b))) return rc;
+    if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, false, false, x->snap_b, x->snap_z))) return rc;
+    if (x->periodic && (rc = set_bc1(x, false, x->snap_z))) return rc;
+  }
   // adjoint sweeps in reverse order (Set_BC(2) ghosts are not read by the
   // diffusion sweeps, which set their own)
   if (use_diffusion) {
@@ -1292,8 +1343,8 @@ int ash3d_tr_adj_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_dif
     if ((rc = adj_diffuse_sweep(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
     if ((rc = adj_diffuse_sweep(x, dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
   }
-  if ((rc = adj_advect_sweep(x, 2, dt, is_aloft, b, x->snap_z))) return rc;
-  if ((rc = adj_advect_sweep(x, 1 - first, dt, is_aloft, b, x->snap_b))) return rc;
+  if ((rc = adj_advect_sweep(x, 2, dt, is_aloft, b, qz))) return rc;
+  if ((rc = adj_advect_sweep(x, 1 - first, dt, is_aloft, b, qb))) return rc;
   return adj_advect_sweep(x, first, dt, is_aloft, b, x->ts0());
 }
 

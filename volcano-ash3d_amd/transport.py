@@ -134,6 +134,8 @@ _SIGS = {
     "ash3d_tr_diffuse_vert": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _ip],
     "ash3d_tr_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                       ctypes.c_double, _ip],
+    "ash3d_tr_step_keep": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                           ctypes.c_double, _ip, ctypes.c_int32, ctypes.c_int32],
     "ash3d_tr_step_glue": [ctypes.c_void_p],
     "ash3d_tr_source_column_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp],
     "ash3d_tr_mass_totals": [ctypes.c_void_p, _dp, _dp, _dp],
@@ -151,6 +153,8 @@ _SIGS = {
                              ctypes.c_double, _ip, _ip],
     "ash3d_tr_adj_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                           ctypes.c_double, _ip],
+    "ash3d_tr_adj_step_kept": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_double, _ip, ctypes.c_int32, ctypes.c_int32],
     "ash3d_tr_adj_source": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32],
     "ash3d_tr_adj_source_read": [ctypes.c_void_p, _dp],
     "ash3d_tr_column_load": [ctypes.c_void_p, _dp, _dp],
@@ -343,6 +347,14 @@ class Transport:
         self._check(self.lib.ash3d_tr_step(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
                                            int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft()))
 
+    def transport_step_keep(self, itime: int, use_diffusion: bool, slot_b: int, slot_z: int):
+        """transport_step, keeping the inputs of the second horizontal and
+        the vertical advection sweep in checkpoint slots slot_b, slot_z."""
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_step_keep(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                                int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft(),
+                                                int(slot_b), int(slot_z)))
+
     def step_glue(self):
         self._check(self.lib.ash3d_tr_step_glue(self._ctx))
 
@@ -472,6 +484,13 @@ class Transport:
         self._full_bounds()
         self._check(self.lib.ash3d_tr_adj_step(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
                                                int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft()))
+
+    def adj_step_kept(self, itime: int, use_diffusion: bool, slot_b: int, slot_z: int):
+        """adj_step with the sweep inputs transport_step_keep kept."""
+        self._full_bounds()
+        self._check(self.lib.ash3d_tr_adj_step_kept(self._ctx, itime, self.st.dt, int(bool(use_diffusion)),
+                                                    int(bool(self.st.use_cn)), self.st.imp_fac, self._aloft(),
+                                                    int(slot_b), int(slot_z)))
 
     def adj_source(self, ivent: int, jvent: int, slot: int):
         self._check(self.lib.ash3d_tr_adj_source(self._ctx, ivent, jvent, self.st.dt, int(slot)))
