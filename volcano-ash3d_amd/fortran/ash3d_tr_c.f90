@@ -36,6 +36,7 @@
              c_ash3d_tr_column_load, c_ash3d_tr_adj_column_load, c_ash3d_tr_minmax_index, &
              c_ash3d_tr_met_push, c_ash3d_tr_met_interp, c_ash3d_tr_tune, &
              c_ash3d_tr_umbrella_source_add, c_ash3d_tr_umbrella_winds_add, &
+             c_ash3d_tr_step_keep, c_ash3d_tr_adj_step_kept, &
              ASH3D_TUNE_LINE_SOLVER, ASH3D_LINE_SOLVER_EXACT, ASH3D_LINE_SOLVER_PARTITIONED
 
       integer(c_int32_t), parameter :: ASH3D_TR_ABI_VERSION = 1
@@ -173,6 +174,22 @@
           import :: c_int, c_ptr, c_int32_t, c_double
           type(c_ptr), value :: ctx
           integer(c_int32_t), value :: itime, use_diff, use_cn
+          real(c_double), value :: dt, imp_fac
+          integer(c_int32_t), intent(in) :: aloft(*)
+        end function
+        integer(c_int) function c_ash3d_tr_adj_step_kept(ctx, itime, dt, use_diff, use_cn, imp_fac, aloft, &
+                                                         slot_b, slot_z) bind(C, name='ash3d_tr_adj_step_kept')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: itime, use_diff, use_cn, slot_b, slot_z
+          real(c_double), value :: dt, imp_fac
+          integer(c_int32_t), intent(in) :: aloft(*)
+        end function
+        integer(c_int) function c_ash3d_tr_step_keep(ctx, itime, dt, use_diff, use_cn, imp_fac, aloft, &
+                                                     slot_b, slot_z) bind(C, name='ash3d_tr_step_keep')
+          import :: c_int, c_ptr, c_int32_t, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int32_t), value :: itime, use_diff, use_cn, slot_b, slot_z
           real(c_double), value :: dt, imp_fac
           integer(c_int32_t), intent(in) :: aloft(*)
         end function
