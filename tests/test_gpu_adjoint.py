@@ -90,19 +90,21 @@ def test_adjoint_advection_sweeps(pkg, oracle, tr, case, ns, dirn):
     check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_advect", dirn))
 
 
+@pytest.mark.parametrize("gy", [0, 3])
 @pytest.mark.parametrize("nb", [1, 3, 4])
 @pytest.mark.parametrize("dirn", [1, 2])
-def test_adjoint_advection_bins_per_thread(pkg, oracle, tr, nb, dirn):
+def test_adjoint_advection_bins_per_thread(pkg, oracle, tr, nb, dirn, gy):
     """y/z adjoint sweeps with 1, 3 and 4 bins per lane (7 bins: partial
-    last groups) and lines long enough for several segments per line."""
+    last groups), lines long enough for several segments per line (gy = 0)
+    or bin groups of whole lines per workgroup (gy = 3)."""
     dims = [9, 8, 7]
     dims[dirn] = 70
     st = pkg.workloads.random_state(*dims, 7, seed=70 + nb + dirn, latlon=True, zscaling_id=2)
     st.is_aloft = np.array([1, 1, 0, 1, 1, 1, 0], dtype=np.int32)
     Fq = forward(oracle, st, [(("advect_x", "advect_y", "advect_z")[dirn],)])
     lam = rand_lambda(st, 13 + nb)
-    knob = "nb_advect_" + "xyz"[dirn]
-    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_advect", dirn, **{knob: nb}))
+    ax = "xyz"[dirn]
+    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_advect", dirn, **{"nb_advect_" + ax: nb, "adv_groups_" + ax: gy}))
 
 
 @pytest.mark.parametrize("lim", ["none", "laxwen", "bw", "fromm", "minmod", "superbee", "mc"])

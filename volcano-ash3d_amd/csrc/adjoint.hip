@@ -24,6 +24,8 @@
 // the forward sweeps cache.  The explicit scheme
 // (src/Diffusion.F90:335-377) is a three-point stencil whose Neumann (or
 // periodic x) ghosts fold back onto the end cells.
+#include <algorithm>
+
 #include "common.hpp"
 #include "dcu.hpp"
 
@@ -205,13 +207,17 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
   const Dims d = A.d;
   const LineGeom g = A.g;
   const int ntile = (g.Na + 63) / 64;
-  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  // A.gy > 0: the waves of a workgroup take gy bin groups of the same whole
+  // lines (as the forward sweep), else W segments of the lines
+  const int G = (d.ns + NB - 1) / NB;
+  const bool gmode = A.gy > 0;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, gmode ? (G + A.gy - 1) / A.gy : G);
   if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
   const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
   const int b = (int)(tg.tile / ntile) + 1;
   const bool lane_ok = a <= g.Na;
-  const int W = blockDim.y, w = threadIdx.y;
-  const int n0 = tg.group * NB;
+  const int W = gmode ? 1 : blockDim.y, w = gmode ? 0 : threadIdx.y;
+  const int n0 = (gmode ? tg.group * A.gy + (int)threadIdx.y : tg.group) * NB;
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int seg = (N + W - 1) / W;
   const int lo = 1 + w * seg, hi = min(N, lo + seg - 1);
@@ -787,17 +793,26 @@ hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double
 }
 
 template <int DIR, int NB>
-hipError_t adj_seg_dir(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam) {
+hipError_t adj_seg_dir(hipStream_t s, const AdvArgs &a_in, const double *qpre, double *lam) {
+  AdvArgs a = a_in;
   const int ntile = (a.g.Na + 63) / 64;
   const long long groups = (a.d.ns + NB - 1) / NB;
-  // segments per line as for the forward sweep: enough waves to fill the
-  // chip, segments of at least 16 cells, at most 4
-  const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
-  int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
-  W = W < 1 ? 1 : (W > 4 ? 4 : W);
-  while (W > 1 && a.g.N / W < 16) --W;
-  dim3 grid((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
-  dim3 block(64, W);
+  dim3 grid, block;
+  if (a.gy > 0) {
+    // bin groups along y, at most 4 per workgroup (the launch bounds)
+    a.gy = (int)std::min<long long>(std::min(a.gy, 4), groups);
+    grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * ((groups + a.gy - 1) / a.gy)));
+    block = dim3(64, a.gy);
+  } else {
+    // segments per line as for the forward sweep: enough waves to fill the
+    // chip, segments of at least 16 cells, at most 4
+    const long long lanes = (long long)ntile * 64 * a.g.Nb * groups;
+    int W = (int)((1LL << 21) / (lanes > 0 ? lanes : 1));
+    W = W < 1 ? 1 : (W > 4 ? 4 : W);
+    while (W > 1 && a.g.N / W < 16) --W;
+    grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
+    block = dim3(64, W);
+  }
   if (DIR == 1 && a.deld != nullptr)
     hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true>), grid, block, 0, s, a, qpre, lam);
   else
