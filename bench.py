@@ -61,8 +61,8 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
     on this grid with the same kernel: the same line solver for the
     diffusion sweeps (and tile shape for the partitioned solves), the same
-    bins per thread for the streamed advection (tuned, else the library
-    default); else None."""
+    bins per thread for the streamed advection and row kernel for the x
+    advection (tuned, else the library default); else None."""
     if (st.nx, st.ny, st.nz, st.ns) != (360, 240, 50, 15) or not os.path.exists(TRAFFIC_FILE):
         return None
     text = open(TRAFFIC_FILE).read()
@@ -78,6 +78,12 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
         if shape is None:
             shape = defaults.get(knob)
         if line_solver == "partitioned" and rec.get("part_shape") != shape:
+            return None
+    if kind == "advect_x":
+        xr = (tuning or {}).get("adv_x_rows", [None])[0]
+        if xr is None:
+            xr = defaults.get("adv_x_rows")
+        if rec.get("x_rows", 0) != xr:
             return None
     if rec["bins_per_thread"] is not None:
         nb = (tuning or {}).get("nb_" + kind, [None])[0]
@@ -358,8 +364,8 @@ def main():
     dk = kernels[dominant]
     achieved = dk["gbps"]
     # the library's bins per thread and tile shapes on this grid
-    defaults = {"advect_y": 3, "advect_z": 2, "adv_groups_y": 0, "adv_groups_z": 4, "part_shape_x": 1,
-                "part_shape_y": 1, "part_shape_z": 0}
+    defaults = {"advect_y": 3, "advect_z": 2, "adv_groups_y": 0, "adv_groups_z": 4, "adv_x_rows": 1,
+                "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
     traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

@@ -206,7 +206,11 @@ enum {
    * 1..4 gives each wave of a workgroup one bin group of the same whole
    * lines, so the per-cell fields are read once for all of them */
   ASH3D_TUNE_ADV_GROUPS_Y = 12,
-  ASH3D_TUNE_ADV_GROUPS_Z = 13
+  ASH3D_TUNE_ADV_GROUPS_Z = 13,
+  /* x advection rows: 1 (default) one wavefront per row, the row in
+   * registers (nx even, nx <= 508, at most 64 bins; other grids use 0),
+   * 0 one workgroup per row staged through LDS */
+  ASH3D_TUNE_ADV_X_ROWS = 14
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

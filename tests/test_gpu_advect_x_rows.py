@@ -1,0 +1,77 @@
+"""x advection rows against the oracle, bitwise, with both row kernels:
+one wavefront per row (the row in registers as 16-byte cell pairs, the
+neighbours by DPP rotations across chunks of 128 cells) and one workgroup
+per row (the row staged through LDS).  Row lengths sit at and around the
+chunk edges (nx + 4 = 128, 256, 384, 512); odd nx, nx > 508 and more than
+64 bins take the workgroup kernel whatever the setting."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import assert_same, run_both
+
+pytestmark = pytest.mark.gpu
+
+NX = [2, 12, 124, 126, 252, 254, 380, 382, 508, 13, 510]
+VARIANTS = {
+    "cartesian": dict(latlon=False, periodic=False, zscaling_id=0),
+    "lonlat_periodic": dict(latlon=True, periodic=True, zscaling_id=0),
+    "topography": dict(latlon=False, periodic=False, zscaling_id=2),
+    "subgrid_aloft": dict(latlon=True, periodic=False, zscaling_id=0),
+}
+
+
+@pytest.fixture(scope="module")
+def tr(pkg):
+    import importlib
+    t = importlib.import_module("volcano-ash3d_amd.transport")
+    t.load()
+    return t
+
+
+def advect_x_both(pkg, oracle, tr, st, xrows):
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    T.tune(adv_x_rows=xrows)
+    O.advect_x()
+    T.advect_x()
+    T.sync_to_host()
+    T.close()
+    return a, b
+
+
+@pytest.mark.parametrize("xrows", [1, 0])
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("nx", NX)
+def test_x_rows_kernels(pkg, oracle, tr, nx, variant, xrows):
+    ns = 5
+    st = pkg.workloads.random_state(nx, 5, 4, ns, seed=nx + 31 * len(variant), **VARIANTS[variant])
+    if variant == "subgrid_aloft":
+        lo = min(3, nx)
+        st.bounds = (lo, max(lo, nx - 2), 2, 4, 2, 3)
+        st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
+    a, b = advect_x_both(pkg, oracle, tr, st, xrows)
+    assert_same(a, b)
+
+
+@pytest.mark.parametrize("ns", [1, 2, 15, 64, 65])
+def test_x_rows_bin_counts(pkg, oracle, tr, ns):
+    """Odd and even bin counts (the wavefront kernel takes bins in pairs),
+    the last bin count it takes (64: one outflow lane per bin) and the
+    first it leaves to the workgroup kernel."""
+    st = pkg.workloads.random_state(126, 3, 3, ns, seed=ns, latlon=True)
+    st.is_aloft = np.ones(ns, dtype=np.int32)
+    st.is_aloft[::3] = 0 if ns > 2 else 1
+    a, b = advect_x_both(pkg, oracle, tr, st, 1)
+    assert_same(a, b)
+
+
+def test_x_rows_in_a_step_sequence(pkg, oracle, tr):
+    """Two full steps with diffusion on an even lon-lat grid (x advection
+    first in one step, second in the other), bitwise."""
+    st = pkg.workloads.random_state(60, 22, 9, 7, seed=77, latlon=True, zscaling_id=2)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    calls = []
+    for itime in (1, 2):
+        calls += [("step_glue",), ("transport_step", itime, True)]
+    a, b = run_both(pkg, oracle, tr, st, calls)
+    assert_same(a, b)

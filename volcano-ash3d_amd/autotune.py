@@ -12,13 +12,15 @@ from __future__ import annotations
 import json
 import os
 
-PARAMS = (("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("adv_groups_y", "advect_y"),
+PARAMS = (("adv_x_rows", "advect_x"), ("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("adv_groups_y", "advect_y"),
           ("adv_groups_z", "advect_z"), ("nb_diffuse_x", "diffuse_x"), ("x_tile", "diffuse_x"),
           ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
 X_TILES = (8, 16, 32)
 # streamed advection: line segments per workgroup (0) or up to 4 bin groups
 # of the tile's lines per workgroup
 ADV_GROUPS = (0, 4)
+# x advection rows: one wavefront per row (1) or one workgroup per row (0)
+ADV_X_ROWS = (1, 0)
 # the partitioned line solves have one setting for the three directions:
 # bins per workgroup (0: all bins of the tile's lines in one workgroup)
 PART_BINS = (0, 8, 4)
@@ -51,6 +53,7 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             continue
         values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
                   ADV_GROUPS if param.startswith("adv_groups") else
+                  ADV_X_ROWS if param == "adv_x_rows" else
                   (0, 1) if param.startswith("part_shape") else candidates)
         times = {}
         for nb in values:

@@ -14,6 +14,7 @@
 // contraction -- bitwise equal to the streaming kernel and the oracle.
 #include "common.hpp"
 #include "dcu.hpp"
+#include "lanes.hpp"
 
 namespace ash3d {
 
@@ -185,6 +186,267 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
   }
 }
 
+
+// ---------------------------------------------------------------------
+// One wavefront per x row, the row in registers.  Lane l of chunk m holds
+// the cell pair c = -1 + 128 m + 2 l + e (e = 0, 1), loaded and stored as
+// one 16-byte access (nx even: every pair is two ghost or two interior
+// cells).  The neighbours c-1 and c+1 come from the same lane (the other
+// element of the pair) or from the next/previous lane by a whole-wave DPP
+// rotation, with the rotated value of the neighbouring chunk at lanes 0
+// and 63 and zero beyond the row (what the LDS version reads there).  No
+// LDS and no barriers: the row of the next bin is loaded while the current
+// one is updated.  Every load and store of the bin loop is issued on every
+// iteration (clamped addresses, masked-off pairs stored to junk), so the
+// wait before a row is used counts exactly the memory operations issued
+// after its loads.  The outflow increments of the row ends are gathered in
+// lane n for bin n and added after the loop.
+//
+// The arithmetic is that of k_advect_x_rows expression by expression.
+
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)), dpp_i32<CTRL>(__double2loint(v)));
+}
+constexpr int kWaveRor1 = 0x13C;  // lane l <- lane l-1, lane 0 <- lane 63
+constexpr int kWaveRol1 = 0x134;  // lane l <- lane l+1, lane 63 <- lane 0
+
+// o = v at cell c-1 (0 before the row)
+template <int NC>
+__device__ inline void row_prev(const double (&v)[NC][2], double (&o)[NC][2], bool lane0) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRor1>(v[m][1]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    o[m][0] = lane0 ? (m > 0 ? r[m > 0 ? m - 1 : 0] : 0.0) : r[m];
+    o[m][1] = v[m][0];
+  }
+}
+// o = v at cell c+1 (0 after the chunks)
+template <int NC>
+__device__ inline void row_next(const double (&v)[NC][2], double (&o)[NC][2], bool lane63) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRol1>(v[m][0]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    o[m][0] = v[m][1];
+    o[m][1] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m] : 0.0) : r[m];
+  }
+}
+
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int nx = d.nx, W = nx + 4;
+  const int l = (int)(threadIdx.x & 63);
+  const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= (long long)d.ny * d.nz) return;  // whole wavefront
+  const int j = (int)(row % d.ny) + 1;
+  const int k = (int)(row / d.ny) + 1;
+  const int rmin = g.rmin, rmax = g.rmax;
+  const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
+  const long long base = d.p3(-1, j, k);  // cell -1 of the row
+  const double *c0 = A.c0;
+  double *co = A.cout;  // == c0 for the in-place sweep
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 512)));
+  const bool lane0 = l == 0, lane63 = l == 63;
+
+  // per element: cell, row position, masks (bin-independent)
+  int cc[NC][2];
+  bool valid[NC][2], ld[NC][2], uin[NC][2];
+  int pp[NC];       // row position of the pair, clamped into the row
+  bool store[NC];   // an interior pair (both cells in 1..nx)
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int p0 = 128 * m + 2 * l;
+    pp[m] = min(p0, W - 2);
+    store[m] = p0 >= 2 && p0 + 1 <= nx + 1;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = p0 + e - 1;
+      cc[m][e] = c;
+      valid[m][e] = p0 + e < W;
+      ld[m][e] = valid[m][e] && c >= rmin - 2 && c <= rmax + 2;
+      uin[m][e] = valid[m][e] && c >= rmin - 1 && c <= rmax + 1;
+    }
+  }
+
+  if (!row_on) {
+    for (int n = 0; n < d.ns; ++n) {
+      const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (store[m]) *(double2 *)(co + base + pp[m] + bofs) = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+
+  // ---- bin-independent terms of the row
+  double u[NC][2], dtv[NC][2], divp[NC][2], divm[NC][2];
+  double tf[NC][2], D[NC][2], tfm1[NC][2], Dm1[NC][2];
+  {
+    double vel[NC][2], sig[NC][2], kap[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const long long o = base + pp[m];
+      const double2 v2 = *(const double2 *)(A.vel + o);
+      const double2 s2 = *(const double2 *)(A.sig + o);
+      const double2 k2 = *(const double2 *)(A.kap + o);
+      double2 d2 = make_double2(0.0, 0.0);
+      if (TOPO) d2 = *(const double2 *)(A.deld + (long long)pp[m] + (long long)(nx + 4) * (j + 1));
+      const double vv[2] = {v2.x, v2.y}, ss[2] = {s2.x, s2.y}, kk[2] = {k2.x, k2.y}, dd[2] = {d2.x, d2.y};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        vel[m][e] = ld[m][e] ? vv[e] : 0.0;
+        sig[m][e] = ld[m][e] ? ss[e] : 0.0;
+        kap[m][e] = ld[m][e] ? kk[e] : 1.0;
+        D[m][e] = (TOPO && uin[m][e]) ? dd[e] : 0.0;
+      }
+    }
+    double velm1[NC][2], sigp1[NC][2];
+    row_prev<NC>(vel, velm1, lane0);
+    if (TOPO) row_next<NC>(sig, sigp1, lane63);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        // usig (:159-162), dt/kappa (:155-156)
+        u[m][e] = uin[m][e] ? 0.5 * (velm1[m][e] + vel[m][e]) * sig[m][e] : 0.0;
+        dtv[m][e] = uin[m][e] ? A.dt / kap[m][e] : 0.0;
+        // topographic factor 0.5*vel_I*(sig_I+sig_{I+1}) (:252-254)
+        tf[m][e] = (TOPO && valid[m][e]) ? 0.5 * vel[m][e] * (sig[m][e] + sigp1[m][e]) : 0.0;
+      }
+    double um1[NC][2], up1[NC][2];
+    row_prev<NC>(u, um1, lane0);
+    row_next<NC>(u, up1, lane63);
+    if (TOPO) {
+      row_prev<NC>(tf, tfm1, lane0);
+      row_prev<NC>(D, Dm1, lane0);
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        // divergence corrections (:245-246)
+        divp[m][e] = (fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]));
+        divm[m][e] = (fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]));
+        if (!TOPO) tfm1[m][e] = Dm1[m][e] = 0.0;
+      }
+  }
+
+  // outflow increments: lane n keeps those of bin n (at most 64 bins)
+  const bool of_lo = rmin == 1 && !A.periodic;
+  const bool of_hi = rmax == nx && !A.periodic;
+  const int m_hi = (nx + 2) >> 7, l_hi = ((nx + 2) & 127) >> 1;  // cell nx+1: element 0
+  double acc_lo = 0.0, acc_hi = 0.0;
+
+  const int lim = A.limiter;
+  // two bins in flight: the rows of bins n and n+1 are in slots A and B;
+  // slot A is refilled with bin n+2 as soon as bin n is gathered
+  double qa[NC][2], qb[NC][2];
+  auto fetch = [&](double (&qs)[NC][2], int n) {
+    // a bin past the end or one that is off: a cached re-read of bin n-2
+    const int nn = (n < d.ns && A.aloft.on(n)) ? n : max(0, min(n - 2, d.ns - 1));
+    const long long bofs = d.ncell * (long long)nn;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const double2 q2 = *(const double2 *)(c0 + base + pp[m] + bofs);
+      qs[m][0] = q2.x;
+      qs[m][1] = q2.y;
+    }
+  };
+  auto bin = [&](double (&qs)[NC][2], int n) {
+    const bool on = A.aloft.on(n);
+    double q[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) q[m][e] = ld[m][e] ? qs[m][e] : 0.0;  // gather (:143)
+    fetch(qs, n + 2);
+    // dq (:168-169)
+    double qm1[NC][2], dq[NC][2];
+    row_prev<NC>(q, qm1, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) dq[m][e] = valid[m][e] ? q[m][e] - qm1[m][e] : 0.0;
+    double dqm[NC][2], dqp[NC][2];
+    row_prev<NC>(dq, dqm, lane0);
+    row_next<NC>(dq, dqp, lane63);
+    // limited flux and fluctuations at interface I = c (:178-259)
+    double fss[NC][2], fsr[NC][2], fsl[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int c = cc[m][e];
+        fss[m][e] = 0.0; fsr[m][e] = 0.0; fsl[m][e] = 0.0;
+        if (lim != 0 && valid[m][e] && c >= rmin && c <= rmax + 1)
+          fss[m][e] = limited_flux(lim, dqm[m][e], dq[m][e], dqp[m][e], u[m][e], dtv[m][e]);
+        if (uin[m][e]) {
+          double r = fmax(0.0, u[m][e]) * dq[m][e];
+          double t = fmin(0.0, u[m][e]) * dq[m][e];
+          r = r + q[m][e] * divm[m][e];
+          t = t + qm1[m][e] * divp[m][e];
+          if (TOPO) {
+            r = r + tf[m][e] * q[m][e] * D[m][e];
+            t = t - tfm1[m][e] * qm1[m][e] * Dm1[m][e];
+          }
+          fsr[m][e] = r;
+          fsl[m][e] = t;
+        }
+      }
+    double fsln[NC][2], fssn[NC][2];
+    row_next<NC>(fsl, fsln, lane63);
+    row_next<NC>(fss, fssn, lane63);
+    // cell update (:264-281), interior write and outflow (:285-294)
+    double upd[NC][2], res[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int c = cc[m][e];
+        upd[m][e] = uin[m][e] ? -dtv[m][e] * (fsln[m][e] + fsr[m][e] + fssn[m][e] - fss[m][e]) : 0.0;
+        res[m][e] = (on && c >= rmin && c <= rmax) ? q[m][e] + upd[m][e] : 0.0;
+      }
+    const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      double *dst = store[m] ? co + base + pp[m] + bofs : junk;
+      *(double2 *)dst = make_double2(res[m][0], res[m][1]);
+    }
+    if (on) {
+      // cell 0 is element 1 of lane 0, chunk 0; cell nx+1 element 0 of lane l_hi, chunk m_hi
+      double uh = upd[0][0];
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (m == m_hi) uh = upd[m][0];
+      const double lo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(upd[0][1]), 0),
+                                         __builtin_amdgcn_readlane(__double2loint(upd[0][1]), 0));
+      const double hi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(uh), l_hi),
+                                         __builtin_amdgcn_readlane(__double2loint(uh), l_hi));
+      if (l == n) { acc_lo = lo; acc_hi = hi; }
+    }
+  };
+  fetch(qa, 0);
+  fetch(qb, 1);
+  int n = 0;
+  for (; n + 1 < d.ns; n += 2) {
+    bin(qa, n);
+    bin(qb, n + 1);
+  }
+  if (n < d.ns) bin(qa, n);
+  // outflow (:285-294): bin n from lane n
+  if (l < d.ns && A.aloft.on(l)) {
+    const long long of = (long long)(j + 1) + (long long)(d.ny + 4) * ((long long)(k + 1) + (long long)(d.nz + 4) * l);
+    if (of_lo) A.of1[of] = A.of1[of] + acc_lo;
+    if (of_hi) A.of2[of] = A.of2[of] + acc_hi;
+  }
+}
+
 template <int CPT>
 hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt) {
   const size_t shmem = sizeof(double) * 6 * (size_t)(a.d.nx + 4);
@@ -202,12 +464,39 @@ hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt) {
   return hipGetLastError();
 }
 
+template <int NC>
+hipError_t launch_wave(hipStream_t s, const AdvArgs &a) {
+  const long long rows = (long long)a.d.ny * a.d.nz;
+  dim3 grid((unsigned)((rows + 3) / 4));
+  if (a.deld != nullptr)
+    hipLaunchKernelGGL((k_advect_x_wave<NC, true>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_advect_x_wave<NC, false>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
 }  // namespace
 
 // Rows of up to 8*256-4 cells; returns hipErrorInvalidValue beyond that so
-// the caller falls back to the streaming kernel.
+// the caller falls back to the streaming kernel.  One wavefront per row
+// where the row fits 4 chunks of 128 cells in pairs (nx even, nx <= 508),
+// the outflow increments fit one lane per bin (ns <= 64) and the arrays
+// allow 16-byte accesses.
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
   const int cells = a.d.nx + 4;
+  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= 512 && a.d.ns <= 64 && aligned16(a.c0) &&
+                    aligned16(a.cout) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
+                    aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
+  if (wave) {
+    switch ((cells + 127) / 128) {
+      case 1: return launch_wave<1>(s, a);
+      case 2: return launch_wave<2>(s, a);
+      case 3: return launch_wave<3>(s, a);
+      default: return launch_wave<4>(s, a);
+    }
+  }
   int cpt = (cells + 255) / 256;
   if (cpt > 8) return hipErrorInvalidValue;
   if (cpt == 3) cpt = 4;

@@ -151,6 +151,7 @@ struct ash3d_tr_ctx {
   // streamed advection: bin groups per workgroup (0: line segments); the
   // z default is the autotuner's choice on the 360x240x50x15 forecast grid
   int adv_gy[3] = {0, 0, 4};
+  int adv_xrows = 1;
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -262,6 +263,7 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
   a.kap = x->kap;
   a.junk = x->junk;
   a.gy = x->adv_gy[dir];
+  a.xrows = x->adv_xrows;
   if (dir == 0) {
     a.vel = x->vx; a.sig = x->sx; a.of1 = x->ofyz1; a.of2 = x->ofyz2;
     a.deld = (x->zscaling == 2) ? x->ddx : nullptr;
@@ -1113,6 +1115,12 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     if (value < 0 || value > 4)
       return fail(ASH3D_TR_EINVAL, "bin groups per workgroup must be 0..4 (got %d)", value);
     x->adv_gy[param == ASH3D_TUNE_ADV_GROUPS_Y ? 1 : 2] = value;
+    return 0;
+  }
+  if (param == ASH3D_TUNE_ADV_X_ROWS) {
+    if (value != 0 && value != 1)
+      return fail(ASH3D_TR_EINVAL, "x rows kernel must be 0 (workgroup rows) or 1 (wavefront rows) (got %d)", value);
+    x->adv_xrows = value;
     return 0;
   }
   if (param >= ASH3D_TUNE_PART_SHAPE_X && param <= ASH3D_TUNE_PART_SHAPE_Z) {

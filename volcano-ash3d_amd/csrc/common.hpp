@@ -121,6 +121,7 @@ struct AdvArgs {
   int periodic;
   int limiter;
   int gy;              // streamed sweeps: bin groups per workgroup along y (0: line segments along y)
+  int xrows;           // x rows: 1 one wavefront per row where it applies, 0 one workgroup per row
   AloftMask aloft;
 };
 
