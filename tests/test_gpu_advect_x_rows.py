@@ -75,3 +75,23 @@ def test_x_rows_in_a_step_sequence(pkg, oracle, tr):
         calls += [("step_glue",), ("transport_step", itime, True)]
     a, b = run_both(pkg, oracle, tr, st, calls)
     assert_same(a, b)
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("nx", NX)
+def test_adjoint_x_rows_kernels(pkg, oracle, tr, nx, variant):
+    """The adjoint x sweep: the wavefront kernel gives the bits of the
+    workgroup kernel, and both satisfy <F(q), w> = <q, J(q)^T w> with the
+    oracle's forward sweep F (the identity the adjoint tests pin)."""
+    import test_gpu_adjoint as ta
+    st = pkg.workloads.random_state(nx, 5, 4, 5, seed=nx + 7 * len(variant), **VARIANTS[variant])
+    if variant == "subgrid_aloft":
+        lo = min(3, nx)
+        st.bounds = (lo, max(lo, nx - 2), 2, 4, 2, 3)
+        st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
+    Fq = ta.forward(oracle, st, [("advect_x",)])
+    lam = ta.rand_lambda(st, nx)
+    g1 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=1)
+    g0 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=0)
+    assert np.array_equal(g1, g0)
+    ta.check_dot(st, Fq, lam, g1)

@@ -28,6 +28,7 @@
 
 #include "common.hpp"
 #include "dcu.hpp"
+#include "lanes.hpp"
 
 namespace ash3d {
 
@@ -529,6 +530,199 @@ __global__ __launch_bounds__(256) void k_adj_advect_x_rows(AdvArgs A, const doub
   }
 }
 
+// ---- x rows, one wavefront per row ------------------------------------
+// The row in registers in the pair layout of lanes.hpp, as the forward
+// wavefront kernel: the pre-sweep row and lambda of a bin are loaded two
+// bins ahead; no LDS and no barriers.  Every load and store of the bin loop
+// is issued on every iteration (clamped addresses, junk stores).  The
+// terms, their order and the periodic fold are those of
+// k_adj_advect_x_rows.
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(256) void k_adj_advect_x_wave(AdvArgs A, const double *__restrict__ qpre,
+                                                           double *lam) {
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int nx = d.nx, W = nx + 4;
+  const int l = (int)(threadIdx.x & 63);
+  const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= (long long)d.ny * d.nz) return;  // whole wavefront
+  const int j = (int)(row % d.ny) + 1;
+  const int k = (int)(row / d.ny) + 1;
+  const int rmin = g.rmin, rmax = g.rmax;
+  const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
+  const long long base = d.p3(-1, j, k);
+  const int lim = A.limiter;
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 512)));
+  const bool lane0 = l == 0, lane63 = l == 63;
+
+  int cc[NC][2];
+  bool valid[NC][2], ld[NC][2], uin[NC][2], act[NC][2];
+  int pp[NC];
+  bool store[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int p0 = 128 * m + 2 * l;
+    pp[m] = min(p0, W - 2);
+    store[m] = p0 >= 2 && p0 + 1 <= nx + 1;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = p0 + e - 1;
+      cc[m][e] = c;
+      valid[m][e] = p0 + e < W;
+      ld[m][e] = valid[m][e] && c >= rmin - 2 && c <= rmax + 2;
+      uin[m][e] = valid[m][e] && c >= rmin - 1 && c <= rmax + 1;
+      act[m][e] = valid[m][e] && c >= rmin && c <= rmax;
+    }
+  }
+  if (!row_on) {
+    for (int n = 0; n < d.ns; ++n) {
+      const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (store[m]) *(double2 *)(lam + base + pp[m] + bofs) = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+
+  // bin-independent terms at interface / cell c (as the forward row kernel);
+  // the divergence corrections and g_I are formed again in each bin from
+  // u and dt/kappa (fewer registers, the same values)
+  double u[NC][2], dtv[NC][2], tfD[NC][2], tfDm1[NC][2];
+  {
+    double vel[NC][2], sig[NC][2], kap[NC][2], D[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const long long o = base + pp[m];
+      const double2 v2 = *(const double2 *)(A.vel + o);
+      const double2 s2 = *(const double2 *)(A.sig + o);
+      const double2 k2 = *(const double2 *)(A.kap + o);
+      double2 d2 = make_double2(0.0, 0.0);
+      if (TOPO) d2 = *(const double2 *)(A.deld + (long long)pp[m] + (long long)(nx + 4) * (j + 1));
+      const double vv[2] = {v2.x, v2.y}, ss[2] = {s2.x, s2.y}, kk[2] = {k2.x, k2.y}, dd[2] = {d2.x, d2.y};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        vel[m][e] = ld[m][e] ? vv[e] : 0.0;
+        sig[m][e] = ld[m][e] ? ss[e] : 0.0;
+        kap[m][e] = ld[m][e] ? kk[e] : 1.0;
+        D[m][e] = (TOPO && uin[m][e]) ? dd[e] : 0.0;
+      }
+    }
+    double velm1[NC][2], sigp1[NC][2];
+    row_prev<NC>(vel, velm1, lane0);
+    if (TOPO) row_next<NC>(sig, sigp1, lane63);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        u[m][e] = uin[m][e] ? 0.5 * (velm1[m][e] + vel[m][e]) * sig[m][e] : 0.0;
+        dtv[m][e] = uin[m][e] ? A.dt / kap[m][e] : 0.0;
+        tfD[m][e] = (TOPO && valid[m][e]) ? 0.5 * vel[m][e] * (sig[m][e] + sigp1[m][e]) * D[m][e] : 0.0;
+      }
+    if (TOPO) row_prev<NC>(tfD, tfDm1, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        if (!TOPO) tfDm1[m][e] = 0.0;
+  }
+
+  // periodic fold: cells nx+1, nx+2 are element 0, 1 of lane l_hi, chunk m_hi
+  const int m_hi = (nx + 2) >> 7, l_hi = ((nx + 2) & 127) >> 1;
+  double sa[NC][2], sq[NC][2], sb[NC][2], sr[NC][2];  // lambda and q of bins n, n+1
+  auto fetch = [&](double (&sl)[NC][2], double (&sqq)[NC][2], int n) {
+    const int nn = (n < d.ns && A.aloft.on(n)) ? n : max(0, min(n - 2, d.ns - 1));
+    const long long bofs = d.ncell * (long long)nn;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const double2 l2 = *(const double2 *)(lam + base + pp[m] + bofs);
+      const double2 q2 = *(const double2 *)(qpre + base + pp[m] + bofs);
+      sl[m][0] = l2.x; sl[m][1] = l2.y;
+      sqq[m][0] = q2.x; sqq[m][1] = q2.y;
+    }
+  };
+  auto bin = [&](double (&sl)[NC][2], double (&sqq)[NC][2], int n) {
+    const bool on = A.aloft.on(n);
+    double lm[NC][2], q[NC][2], mu[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        lm[m][e] = act[m][e] ? sl[m][e] : 0.0;
+        q[m][e] = ld[m][e] ? sqq[m][e] : 0.0;
+      }
+    fetch(sl, sqq, n + 2);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) mu[m][e] = act[m][e] ? -dtv[m][e] * lm[m][e] : 0.0;
+    double um1[NC][2], up1[NC][2];
+    row_prev<NC>(u, um1, lane0);
+    row_next<NC>(u, up1, lane63);
+    double qm2[NC][2], qm1[NC][2], qp1[NC][2], mum1[NC][2];
+    row_prev2<NC>(q, qm2, lane0);
+    row_prev<NC>(q, qm1, lane0);
+    row_next<NC>(q, qp1, lane63);
+    row_prev<NC>(mu, mum1, lane0);
+    double e0[NC][2], e1[NC][2], e2[NC][2], e3[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int c = cc[m][e];
+        e0[m][e] = e1[m][e] = e2[m][e] = e3[m][e] = 0.0;
+        if (valid[m][e] && c >= rmin && c <= rmax + 1) {
+          const double divp = fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]);
+          const double divm = fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]);
+          const double aus = fabs(u[m][e]);
+          const double gI = 0.5 * aus * (1.0 - dtv[m][e] * aus);
+          interface_adjoint(lim, qm2[m][e], qm1[m][e], q[m][e], qp1[m][e], u[m][e], divp, divm, gI, tfD[m][e],
+                            tfDm1[m][e], mum1[m][e], mu[m][e], e0[m][e], e1[m][e], e2[m][e], e3[m][e]);
+        }
+      }
+    // interfaces r+2, r+1, r, r-1 reach cell r as their I-2, I-1, I, I+1
+    double e0n[NC][2], e1n[NC][2], e3p[NC][2], v[NC][2];
+    row_next2<NC>(e0, e0n, lane63);
+    row_next<NC>(e1, e1n, lane63);
+    row_prev<NC>(e3, e3p, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) v[m][e] = e0n[m][e] + e1n[m][e] + e2[m][e] + e3p[m][e];
+    if (A.periodic) {
+      double vh0 = v[0][0], vh1 = v[0][1];
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (m == m_hi) { vh0 = v[m][0]; vh1 = v[m][1]; }
+      const double ga = lane_value(vh0, l_hi), gb = lane_value(vh1, l_hi);  // cells nx+1, nx+2
+      const double gc = lane_value(v[0][0], 0), gd = lane_value(v[0][1], 0);  // cells -1, 0
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int c = cc[m][e];
+          if (c == 1) v[m][e] += ga;
+          if (c == 2) v[m][e] += gb;
+          if (c == nx - 1) v[m][e] += gc;
+          if (c == nx) v[m][e] += gd;
+        }
+    }
+    const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      double *dst = store[m] ? lam + base + pp[m] + bofs : junk;
+      *(double2 *)dst = on ? make_double2(lm[m][0] + v[m][0], lm[m][1] + v[m][1]) : make_double2(0.0, 0.0);
+    }
+  };
+  fetch(sa, sq, 0);
+  fetch(sb, sr, 1);
+  int n = 0;
+  for (; n + 1 < d.ns; n += 2) {
+    bin(sa, sq, n);
+    bin(sb, sr, n + 1);
+  }
+  if (n < d.ns) bin(sa, sq, n);
+}
+
 // ---- theta-scheme lines: lambda' = B^T A^{-T} lambda -------------------
 struct RowC {
   double a, b, c;     // B row: a q_{l-1} + b q_l + c q_{l+1}
@@ -781,6 +975,19 @@ hipError_t adj_x_rows_cpt(hipStream_t s, const AdvArgs &a, const double *qpre, d
   return hipGetLastError();
 }
 
+template <int NC>
+hipError_t adj_x_wave(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam) {
+  const long long rows = (long long)a.d.ny * a.d.nz;
+  dim3 grid((unsigned)((rows + 3) / 4));
+  if (a.deld != nullptr)
+    hipLaunchKernelGGL((k_adj_advect_x_wave<NC, true>), grid, dim3(256), 0, s, a, qpre, lam);
+  else
+    hipLaunchKernelGGL((k_adj_advect_x_wave<NC, false>), grid, dim3(256), 0, s, a, qpre, lam);
+  return hipGetLastError();
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
 template <int DIR>
 hipError_t adj_lines(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
@@ -856,6 +1063,18 @@ hipError_t adj_cn_nb(hipStream_t s, const DiffArgs &a, double *lam, int nb) {
 hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
   if (a.g.dir == 0) {
     const int cells = a.d.nx + 4;
+    // one wavefront per row under the forward's conditions
+    const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 512) && aligned16(lam) &&
+                      aligned16(qpre) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
+                      aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
+    if (wave) {
+      switch ((cells + 127) / 128) {
+        case 1: return adj_x_wave<1>(s, a, qpre, lam);
+        case 2: return adj_x_wave<2>(s, a, qpre, lam);
+        case 3: return adj_x_wave<3>(s, a, qpre, lam);
+        default: return adj_x_wave<4>(s, a, qpre, lam);
+      }
+    }
     int cpt = (cells + 255) / 256;
     if (cpt > 8) return adj_lines<0>(s, a, qpre, lam, nb);
     if (cpt == 3) cpt = 4;

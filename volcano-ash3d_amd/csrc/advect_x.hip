@@ -204,38 +204,6 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
 //
 // The arithmetic is that of k_advect_x_rows expression by expression.
 
-template <int CTRL>
-__device__ inline double dpp_f64(double v) {
-  return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)), dpp_i32<CTRL>(__double2loint(v)));
-}
-constexpr int kWaveRor1 = 0x13C;  // lane l <- lane l-1, lane 0 <- lane 63
-constexpr int kWaveRol1 = 0x134;  // lane l <- lane l+1, lane 63 <- lane 0
-
-// o = v at cell c-1 (0 before the row)
-template <int NC>
-__device__ inline void row_prev(const double (&v)[NC][2], double (&o)[NC][2], bool lane0) {
-  double r[NC];
-#pragma unroll
-  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRor1>(v[m][1]);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) {
-    o[m][0] = lane0 ? (m > 0 ? r[m > 0 ? m - 1 : 0] : 0.0) : r[m];
-    o[m][1] = v[m][0];
-  }
-}
-// o = v at cell c+1 (0 after the chunks)
-template <int NC>
-__device__ inline void row_next(const double (&v)[NC][2], double (&o)[NC][2], bool lane63) {
-  double r[NC];
-#pragma unroll
-  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRol1>(v[m][0]);
-#pragma unroll
-  for (int m = 0; m < NC; ++m) {
-    o[m][0] = v[m][1];
-    o[m][1] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m] : 0.0) : r[m];
-  }
-}
-
 template <int NC, bool TOPO>
 __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   const Dims d = A.d;
@@ -424,10 +392,8 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 #pragma unroll
       for (int m = 0; m < NC; ++m)
         if (m == m_hi) uh = upd[m][0];
-      const double lo = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(upd[0][1]), 0),
-                                         __builtin_amdgcn_readlane(__double2loint(upd[0][1]), 0));
-      const double hi = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(uh), l_hi),
-                                         __builtin_amdgcn_readlane(__double2loint(uh), l_hi));
+      const double lo = lane_value(upd[0][1], 0);
+      const double hi = lane_value(uh, l_hi);
       if (l == n) { acc_lo = lo; acc_hi = hi; }
     }
   };
@@ -481,12 +447,14 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // Rows of up to 8*256-4 cells; returns hipErrorInvalidValue beyond that so
 // the caller falls back to the streaming kernel.  One wavefront per row
-// where the row fits 4 chunks of 128 cells in pairs (nx even, nx <= 508),
-// the outflow increments fit one lane per bin (ns <= 64) and the arrays
-// allow 16-byte accesses.
+// where the row fits 4 chunks of 128 cells in pairs (nx even, nx <= 508;
+// with topography 2 chunks, nx <= 252, as the extra per-cell terms would
+// leave one wave per SIMD), the outflow increments fit one lane per bin
+// (ns <= 64) and the arrays allow 16-byte accesses.
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
   const int cells = a.d.nx + 4;
-  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= 512 && a.d.ns <= 64 && aligned16(a.c0) &&
+  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 512) && a.d.ns <= 64 &&
+                    aligned16(a.c0) &&
                     aligned16(a.cout) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
                     aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
   if (wave) {

@@ -11,7 +11,8 @@
 namespace ash3d {
 
 // dpp_ctrl: 0x100+h row_shl:h, 0x110+h row_shr:h, 0x120+h row_ror:h,
-// 0x130 wave_shl:1, 0x138 wave_shr:1; source lanes outside the row read 0
+// 0x130 wave_shl:1, 0x134 wave_rol:1, 0x138 wave_shr:1, 0x13C wave_ror:1;
+// source lanes outside the row (or wave) read 0
 template <int CTRL>
 __device__ inline int dpp_i32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
@@ -60,6 +61,79 @@ __device__ inline void lane_updown(double v, double &up, double &dn) {
   const UpDown hi = lane_updown<LP, H>(__double2hiint(v));
   up = __hiloint2double(hi.up, lo.up);
   dn = __hiloint2double(hi.dn, lo.dn);
+}
+
+
+// ---- rows held by one wavefront ---------------------------------------
+// A row of up to 128*NC cells as NC chunks: lane l of chunk m holds the
+// cell pair p = 128 m + 2 l + e (e = 0, 1) in v[m][e].  The neighbours of
+// a cell come from the other element of the pair or from the next or
+// previous lane by a whole-wave DPP rotation; lanes 0 and 63 take the
+// rotated value of the neighbouring chunk, and positions before the row or
+// after the last chunk read zero.  Every lane executes these moves (they
+// must not sit under divergent control flow).
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)), dpp_i32<CTRL>(__double2loint(v)));
+}
+constexpr int kWaveRor1 = 0x13C;  // lane l <- lane l-1, lane 0 <- lane 63
+constexpr int kWaveRol1 = 0x134;  // lane l <- lane l+1, lane 63 <- lane 0
+
+// o = v at p-1
+template <int NC>
+__device__ inline void row_prev(const double (&v)[NC][2], double (&o)[NC][2], bool lane0) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRor1>(v[m][1]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    o[m][0] = lane0 ? (m > 0 ? r[m > 0 ? m - 1 : 0] : 0.0) : r[m];
+    o[m][1] = v[m][0];
+  }
+}
+// o = v at p+1
+template <int NC>
+__device__ inline void row_next(const double (&v)[NC][2], double (&o)[NC][2], bool lane63) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRol1>(v[m][0]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    o[m][0] = v[m][1];
+    o[m][1] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m] : 0.0) : r[m];
+  }
+}
+// o = v at p-2 (the previous lane's same element)
+template <int NC>
+__device__ inline void row_prev2(const double (&v)[NC][2], double (&o)[NC][2], bool lane0) {
+  double r[NC][2];
+#pragma unroll
+  for (int m = 0; m < NC; ++m)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) r[m][e] = dpp_f64<kWaveRor1>(v[m][e]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) o[m][e] = lane0 ? (m > 0 ? r[m > 0 ? m - 1 : 0][e] : 0.0) : r[m][e];
+}
+// o = v at p+2 (the next lane's same element)
+template <int NC>
+__device__ inline void row_next2(const double (&v)[NC][2], double (&o)[NC][2], bool lane63) {
+  double r[NC][2];
+#pragma unroll
+  for (int m = 0; m < NC; ++m)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) r[m][e] = dpp_f64<kWaveRol1>(v[m][e]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      o[m][e] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m][e] : 0.0) : r[m][e];
+}
+// the double v of lane `lane` (uniform) in every lane
+__device__ inline double lane_value(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
 }
 
 }  // namespace ash3d
