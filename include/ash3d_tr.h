@@ -208,7 +208,7 @@ enum {
   ASH3D_TUNE_ADV_GROUPS_Y = 12,
   ASH3D_TUNE_ADV_GROUPS_Z = 13,
   /* x advection rows (forward and adjoint): 1 (default) one wavefront
-   * per row, the row in registers (nx even, nx <= 508 or with
+   * per row, the row in registers (nx even, nx <= 380 or with
    * ZScaling_ID 2 nx <= 252, at most 64 bins for the forward; other grids
    * use 0), 0 one workgroup per row staged through LDS */
   ASH3D_TUNE_ADV_X_ROWS = 14

@@ -2,8 +2,9 @@
 one wavefront per row (the row in registers as 16-byte cell pairs, the
 neighbours by DPP rotations across chunks of 128 cells) and one workgroup
 per row (the row staged through LDS).  Row lengths sit at and around the
-chunk edges (nx + 4 = 128, 256, 384, 512); odd nx, nx > 508 and more than
-64 bins take the workgroup kernel whatever the setting."""
+chunk edges (nx + 4 = 128, 256, 384); odd nx, nx > 380 (> 252 with
+topography) and more than 64 bins take the workgroup kernel whatever the
+setting."""
 import numpy as np
 import pytest
 

@@ -56,6 +56,44 @@ def test_met_interp(pkg, oracle, tr, use_vf):
     same(a, b)
 
 
+def test_steps_follow_wind_changes(pkg, oracle, tr):
+    """The library applies Set_BC(1) to the velocities only when they have
+    changed since its last pass; steps between met interpolations and a
+    host upload of new winds and fall velocities match the oracle, which
+    applies it every step, bitwise."""
+    st = pkg.workloads.random_state(14, 10, 7, 3, seed=8, latlon=True)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    steps = met_steps(st, 9)
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    T.met_push(*steps[0])
+    T.met_push(*steps[1])
+    itime = 0
+    for frac in (0.0, 0.5):
+        O.met_interp(steps[0], steps[1], frac, True)
+        T.met_interp(frac, True)
+        for _ in range(2):
+            itime += 1
+            for X in (O, T):
+                X.step_glue()
+                X.transport_step(itime, True)
+    rng = np.random.default_rng(4)
+    new = {n: 20.0 * rng.standard_normal(getattr(a, n).shape) for n in ("vx_pd", "vy_pd", "vz_pd")}
+    new["vf_pd"] = -np.abs(rng.standard_normal(a.vf_pd.shape))
+    for h in (a, b):
+        for n, v in new.items():
+            getattr(h, n)[...] = v
+    T.sync_from_host(tr.SYNC_WINDS | tr.SYNC_VF)
+    for _ in range(2):
+        itime += 1
+        for X in (O, T):
+            X.step_glue()
+            X.transport_step(itime, True)
+    T.sync_to_host()
+    T.close()
+    same(a, b)
+
+
 def params(cls, dz, **kw):
     p = cls()
     base = dict(is_latlon=0, use_met_next=0, n_gs_aloft=3, use_diffusion=0, use_cn=1, dx=4.0, dy=3.0,

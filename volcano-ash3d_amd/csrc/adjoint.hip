@@ -1064,15 +1064,14 @@ hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre
   if (a.g.dir == 0) {
     const int cells = a.d.nx + 4;
     // one wavefront per row under the forward's conditions
-    const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 512) && aligned16(lam) &&
+    const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 384) && aligned16(lam) &&
                       aligned16(qpre) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
                       aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
     if (wave) {
       switch ((cells + 127) / 128) {
         case 1: return adj_x_wave<1>(s, a, qpre, lam);
         case 2: return adj_x_wave<2>(s, a, qpre, lam);
-        case 3: return adj_x_wave<3>(s, a, qpre, lam);
-        default: return adj_x_wave<4>(s, a, qpre, lam);
+        default: return adj_x_wave<3>(s, a, qpre, lam);
       }
     }
     int cpt = (cells + 255) / 256;

@@ -447,13 +447,14 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // Rows of up to 8*256-4 cells; returns hipErrorInvalidValue beyond that so
 // the caller falls back to the streaming kernel.  One wavefront per row
-// where the row fits 4 chunks of 128 cells in pairs (nx even, nx <= 508;
-// with topography 2 chunks, nx <= 252, as the extra per-cell terms would
-// leave one wave per SIMD), the outflow increments fit one lane per bin
-// (ns <= 64) and the arrays allow 16-byte accesses.
+// where the row fits 3 chunks of 128 cells in pairs (nx even, nx <= 380;
+// with topography 2 chunks, nx <= 252: longer rows leave one wave per SIMD
+// and spill, and run slower than the workgroup kernel), the outflow
+// increments fit one lane per bin (ns <= 64) and the arrays allow 16-byte
+// accesses.
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
   const int cells = a.d.nx + 4;
-  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 512) && a.d.ns <= 64 &&
+  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 384) && a.d.ns <= 64 &&
                     aligned16(a.c0) &&
                     aligned16(a.cout) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
                     aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
@@ -461,8 +462,7 @@ hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
     switch ((cells + 127) / 128) {
       case 1: return launch_wave<1>(s, a);
       case 2: return launch_wave<2>(s, a);
-      case 3: return launch_wave<3>(s, a);
-      default: return launch_wave<4>(s, a);
+      default: return launch_wave<3>(s, a);
     }
   }
   int cpt = (cells + 255) / 256;

@@ -93,6 +93,9 @@ struct ash3d_tr_ctx {
   hipEvent_t ndom_ev[3] = {};
   int ndom_state[3] = {0, 0, 0};
   bool capturing = false;
+  // the velocity ghosts hold Set_BC(1)'s values for the current interior
+  // velocities (cleared by every entry point that writes velocities)
+  bool vel_bc_ok = false;
   // umbrella source: device copies of the stencil + patch flux and of the
   // two wind planes, uploaded through one pinned staging buffer each
   double *umb_src = nullptr, *umb_wind = nullptr;
@@ -243,7 +246,13 @@ struct ProfScope {
 // Set_BC(1) on a state buffer (ts0 unless given).
 int set_bc1(ash3d_tr_ctx *x, bool velocities, double *state = nullptr) {
   ProfScope ps(x, ASH3D_PROF_BC);
-  if (velocities) HIPCHK(launch_set_bc1_vel(x->stream, x->d, x->periodic, x->vx, x->vy, x->vz, x->vf));
+  // Set_BC(1) on velocities is a projection (every ghost a copy of an
+  // interior or wrapped cell that the pass does not write), so it is
+  // skipped while the ghosts already hold its values; a captured graph
+  // always contains it
+  if (velocities && (x->capturing || !x->vel_bc_ok))
+    HIPCHK(launch_set_bc1_vel(x->stream, x->d, x->periodic, x->vx, x->vy, x->vz, x->vf));
+  if (velocities && !x->capturing) x->vel_bc_ok = true;
   HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, state ? state : x->ts0()));
   return 0;
 }
@@ -770,6 +779,7 @@ int ash3d_tr_sync_from_host(ash3d_tr_ctx *x, uint32_t mask) {
     if ((rc = upload(x, x->c, h.concen_pd, 2 * x->n_conc(), 0.0))) return rc;
     x->ts1_state = TS1_EXPLICIT;
   }
+  if (mask & (ASH3D_SYNC_WINDS | ASH3D_SYNC_VF)) x->vel_bc_ok = false;
   if (mask & ASH3D_SYNC_WINDS) {
     if ((rc = upload(x, x->vx, h.vx_pd, d.ncell, 0.0)) || (rc = upload(x, x->vy, h.vy_pd, d.ncell, 0.0)) ||
         (rc = upload(x, x->vz, h.vz_pd, d.ncell, 0.0)) || (rc = upload(x, x->vh, h.vh_pd, d.ncell, 0.0)))
@@ -1097,6 +1107,7 @@ int ash3d_tr_umbrella_winds_add(ash3d_tr_ctx *x, int32_t ibase, int32_t itop, co
   const double *parts[2] = {uvx, uvy};
   const size_t counts[2] = {plane, plane};
   if ((rc = upload_staged(x, 1, x->umb_wind, parts, counts, 2))) return rc;
+  x->vel_bc_ok = false;
   HIPCHK(launch_umbrella_winds(x->stream, d, x->vx, x->vy, ibase, itop, x->umb_wind, x->umb_wind + plane));
   return 0;
 }
@@ -1593,6 +1604,7 @@ int ash3d_tr_met_interp(ash3d_tr_ctx *x, double frac, int32_t use_vf) {
   if (use_vf && !(x->met_vf[0] && x->met_vf[1])) return fail(ASH3D_TR_EINVAL, "fall velocities were not pushed");
   const float *last[4] = {x->met[lst][0], x->met[lst][1], x->met[lst][2], x->met[lst][3]};
   const float *next[4] = {x->met[nxt][0], x->met[nxt][1], x->met[nxt][2], x->met[nxt][3]};
+  x->vel_bc_ok = false;
   HIPCHK(launch_met_interp(x->stream, x->d, last, next, frac, use_vf != 0, x->vx, x->vy, x->vz, x->vf));
   return 0;
 }

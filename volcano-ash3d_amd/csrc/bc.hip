@@ -35,8 +35,10 @@ __device__ inline bool ghost_cell(const Dims &d, int k, long long t, int &i, int
   }
   t -= rowcells;
   if (t >= 4LL * d.ny) return false;
-  const int col = (int)(t / d.ny);
-  j = (int)(t % d.ny) + 1;
+  // the four ghost columns of a row in adjacent lanes: i = -1, 0 and
+  // nx+1, nx+2 are two 16-byte pieces of the row
+  const int col = (int)(t & 3);
+  j = (int)(t >> 2) + 1;
   i = (col < 2) ? (col - 1) : (d.nx + (col - 1));
   return true;
 }
@@ -154,7 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_vel_all(Dims d, int periodic, double
 }
 
 // Neumann ghosts along one direction for all bins of ts0, all transverse
-// indices including ghosts; x applies the periodic override.
+// indices including ghosts; x applies the periodic override.  y and z: a
+// thread per line writes its four ghosts (lanes along the contiguous x);
+// x: a thread per ghost, the four of a row in adjacent lanes.
 __global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodic, double *c0) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   const int n = (int)blockIdx.y;
@@ -162,14 +166,16 @@ __global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodi
   if (dir == 0) { A = d.ny + 4; B = d.nz + 4; N = d.nx; }
   else if (dir == 1) { A = d.nx + 4; B = d.nz + 4; N = d.ny; }
   else { A = d.nx + 4; B = d.ny + 4; N = d.nz; }
-  if (t >= (long long)A * B) return;
-  const int a = (int)(t % A) - 1, b = (int)(t / A) - 1;
+  const int per = (dir == 0) ? 4 : 1;  // threads per line
+  if (t >= (long long)A * B * per) return;
+  const long long line = (dir == 0) ? (t >> 2) : t;
+  const int u0 = (dir == 0) ? (int)(t & 3) : 0, u1 = (dir == 0) ? u0 + 1 : 4;
+  const int a = (int)(line % A) - 1, b = (int)(line / A) - 1;
   double *c = c0 + d.ncell * n;
   const long long base = line_base(d, dir, a, b), st = line_stride(d, dir);
   const int p = (dir == 0) ? periodic : 0;
   const int g[4] = {-1, 0, N + 1, N + 2};
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = u0; u < u1; ++u) {
     int src;
     if (p) src = g[u] < 1 ? g[u] + N : g[u] - N;
     else src = g[u] < 1 ? 1 : N;
@@ -286,7 +292,7 @@ hipError_t launch_copy_ghosts(hipStream_t s, const Dims &d, const double *src, d
 
 hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0) {
   long long nl;
-  if (dir == 0) nl = (long long)(d.ny + 4) * (d.nz + 4);
+  if (dir == 0) nl = 4LL * (d.ny + 4) * (d.nz + 4);  // a thread per x ghost
   else if (dir == 1) nl = (long long)(d.nx + 4) * (d.nz + 4);
   else nl = (long long)(d.nx + 4) * (d.ny + 4);
   hipLaunchKernelGGL(k_neumann, dim3(blocks_for(nl), d.ns), dim3(kBlock), 0, s, d, dir, periodic, c0);
