@@ -202,7 +202,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_lines(AdvArgs A, const do
 // neighbouring segments) are read into registers before a barrier.  The
 // pending sums of each cell are formed in the same order as the one-lane
 // kernel above (e3, e2, e1, e0, then the direct term).
-template <int DIR, int NB, bool TOPO>
+// GM: whole lines per wave (no halos: the ring reads those cells itself
+// and nobody writes them).
+template <int DIR, int NB, bool TOPO, bool GM>
 __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double *__restrict__ qpre, double *lam) {
   constexpr int PF = 2;  // cells in flight
   const Dims d = A.d;
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
   // A.gy > 0: the waves of a workgroup take gy bin groups of the same whole
   // lines (as the forward sweep), else W segments of the lines
   const int G = (d.ns + NB - 1) / NB;
-  const bool gmode = A.gy > 0;
+  constexpr bool gmode = GM;
   const TileGroup tg = xcd_tile_group(blockIdx.x, gmode ? (G + A.gy - 1) / A.gy : G);
   if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
   const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
@@ -242,17 +244,19 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
   auto act = [&](int r) { return r >= rmin && r <= rmax; };
   // lambda of the halo cells (zero where mu and the direct term vanish)
   double hl0[NB], hl1[NB], hl2[NB], hl3[NB];
-  auto halo = [&](int r, int u) {
-    return (any_on && exists[u] && act(r)) ? lam[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
-  };
+  if constexpr (!GM) {
+    auto halo = [&](int r, int u) {
+      return (any_on && exists[u] && act(r)) ? lam[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
+    };
 #pragma unroll
-  for (int u = 0; u < NB; ++u) {
-    hl0[u] = halo(lo - 2, u);
-    hl1[u] = halo(lo - 1, u);
-    hl2[u] = halo(hi + 1, u);
-    hl3[u] = halo(hi + 2, u);
+    for (int u = 0; u < NB; ++u) {
+      hl0[u] = halo(lo - 2, u);
+      hl1[u] = halo(lo - 1, u);
+      hl2[u] = halo(hi + 1, u);
+      hl3[u] = halo(hi + 2, u);
+    }
+    if (W > 1) __syncthreads();
   }
-  if (W > 1) __syncthreads();
 
   if (any_on && lo <= hi) {
     const double dt = A.dt;
@@ -307,10 +311,12 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
         for (int u = 0; u < NB; ++u) {
           q0[u] = (ld && exists[u]) ? rs_q[p][u] : 0.0;
           double v = rs_l[p][u];
-          v = (r == lo - 2) ? hl0[u] : v;
-          v = (r == lo - 1) ? hl1[u] : v;
-          v = (r == hi + 1) ? hl2[u] : v;
-          v = (r == hi + 2) ? hl3[u] : v;
+          if constexpr (!GM) {
+            v = (r == lo - 2) ? hl0[u] : v;
+            v = (r == lo - 1) ? hl1[u] : v;
+            v = (r == hi + 1) ? hl2[u] : v;
+            v = (r == hi + 2) ? hl3[u] : v;
+          }
           l0[u] = v;
           vf0[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
         }
@@ -1020,10 +1026,14 @@ hipError_t adj_seg_dir(hipStream_t s, const AdvArgs &a_in, const double *qpre, d
     grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
     block = dim3(64, W);
   }
-  if (DIR == 1 && a.deld != nullptr)
-    hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true>), grid, block, 0, s, a, qpre, lam);
-  else
-    hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false>), grid, block, 0, s, a, qpre, lam);
+  const bool topo = DIR == 1 && a.deld != nullptr;
+  if (a.gy > 0) {
+    if (topo) hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true, true>), grid, block, 0, s, a, qpre, lam);
+    else hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false, true>), grid, block, 0, s, a, qpre, lam);
+  } else {
+    if (topo) hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true, false>), grid, block, 0, s, a, qpre, lam);
+    else hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false, false>), grid, block, 0, s, a, qpre, lam);
+  }
   return hipGetLastError();
 }
 

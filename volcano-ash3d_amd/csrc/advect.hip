@@ -31,7 +31,9 @@ constexpr int kBlock = 256;
 // The two cells below lo and above hi are also written by the neighbouring
 // segments, so they are read into registers before a workgroup barrier and
 // never re-read from memory.
-template <int DIR, int NB, bool TOPO>
+// GM: the bin groups of a tile in one workgroup, whole lines per wave (no
+// halos to exchange); otherwise line segments per wave.
+template <int DIR, int NB, bool TOPO, bool GM>
 __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   constexpr int PF = (NB >= 2) ? 2 : 4;  // cells in flight
   const Dims d = A.d;
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   // A.gy > 0: the waves of a workgroup take gy bin groups of the same lines
   // (whole lines, so the per-cell fields are read by all of them at once)
   const int G = (d.ns + NB - 1) / NB;
-  const bool gmode = A.gy > 0;
+  constexpr bool gmode = GM;
   const TileGroup tg = xcd_tile_group(blockIdx.x, gmode ? (G + A.gy - 1) / A.gy : G);
   if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
   const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
@@ -76,18 +78,21 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 
   // halo cells shared with the neighbouring segments (four named arrays so
   // that they stay in registers)
+  // (whole lines: the ring reads those cells itself, nobody writes them)
   double hq0[NB], hq1[NB], hq2[NB], hq3[NB];
-  auto halo = [&](int r, int u) {
-    return (any_on && exists[u] && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
-  };
+  if constexpr (!GM) {
+    auto halo = [&](int r, int u) {
+      return (any_on && exists[u] && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
+    };
 #pragma unroll
-  for (int u = 0; u < NB; ++u) {
-    hq0[u] = halo(lo - 2, u);
-    hq1[u] = halo(lo - 1, u);
-    hq2[u] = halo(hi + 1, u);
-    hq3[u] = halo(hi + 2, u);
+    for (int u = 0; u < NB; ++u) {
+      hq0[u] = halo(lo - 2, u);
+      hq1[u] = halo(lo - 1, u);
+      hq2[u] = halo(hi + 1, u);
+      hq3[u] = halo(hi + 2, u);
+    }
+    if (W > 1) __syncthreads();
   }
-  if (W > 1) __syncthreads();
 
   // outflow increments of the line ends, applied after the sweep
   double upd_lo[NB], upd_hi[NB];
@@ -149,10 +154,12 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
         for (int u = 0; u < NB; ++u) {
           // the halo cells come from the registers filled before the barrier
           double v = (ld && exists[u]) ? rs_q[p][u] : 0.0;
-          v = (r == lo - 2) ? hq0[u] : v;
-          v = (r == lo - 1) ? hq1[u] : v;
-          v = (r == hi + 1) ? hq2[u] : v;
-          v = (r == hi + 2) ? hq3[u] : v;
+          if constexpr (!GM) {
+            v = (r == lo - 2) ? hq0[u] : v;
+            v = (r == lo - 1) ? hq1[u] : v;
+            v = (r == hi + 1) ? hq2[u] : v;
+            v = (r == hi + 2) ? hq3[u] : v;
+          }
           q0[u] = v;
           vfc[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
         }
@@ -287,10 +294,14 @@ hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in) {
     grid = dim3((unsigned)(padded_tiles((long long)ntile * a.g.Nb) * groups));
     block = dim3(64, W);
   }
-  if (DIR != 2 && a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect<DIR, NB, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_advect<DIR, NB, false>), grid, block, 0, s, a);
+  const bool topo = DIR != 2 && a.deld != nullptr;
+  if (a.gy > 0) {
+    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_advect<DIR, NB, false, true>), grid, block, 0, s, a);
+  } else {
+    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_advect<DIR, NB, false, false>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 
