@@ -78,18 +78,21 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 
   // halo cells shared with the neighbouring segments (four named arrays so
   // that they stay in registers)
-  // (whole lines: the ring reads those cells itself, nobody writes them)
-  double hq0[NB], hq1[NB], hq2[NB], hq3[NB];
+  // (whole lines: the ring reads those cells itself, nobody writes them).
+  // The halo values wait in LDS (4*NB per thread, element-major), read back
+  // in the four iterations that need them.
+  extern __shared__ double s_halo[];
+  const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y, nthr = 64 * W;
   if constexpr (!GM) {
     auto halo = [&](int r, int u) {
       return (any_on && exists[u] && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
     };
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      hq0[u] = halo(lo - 2, u);
-      hq1[u] = halo(lo - 1, u);
-      hq2[u] = halo(hi + 1, u);
-      hq3[u] = halo(hi + 2, u);
+      s_halo[(4 * u + 0) * nthr + tid] = halo(lo - 2, u);
+      s_halo[(4 * u + 1) * nthr + tid] = halo(lo - 1, u);
+      s_halo[(4 * u + 2) * nthr + tid] = halo(hi + 1, u);
+      s_halo[(4 * u + 3) * nthr + tid] = halo(hi + 2, u);
     }
     if (W > 1) __syncthreads();
   }
@@ -155,10 +158,11 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
           // the halo cells come from the registers filled before the barrier
           double v = (ld && exists[u]) ? rs_q[p][u] : 0.0;
           if constexpr (!GM) {
-            v = (r == lo - 2) ? hq0[u] : v;
-            v = (r == lo - 1) ? hq1[u] : v;
-            v = (r == hi + 1) ? hq2[u] : v;
-            v = (r == hi + 2) ? hq3[u] : v;
+            // r, lo and hi are uniform across the wave: a scalar branch
+            if (r == lo - 2 || r == lo - 1 || r == hi + 1 || r == hi + 2) {
+              const int k = (r == lo - 2) ? 0 : (r == lo - 1) ? 1 : (r == hi + 1) ? 2 : 3;
+              v = s_halo[(4 * u + k) * nthr + tid];
+            }
           }
           q0[u] = v;
           vfc[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
@@ -299,8 +303,9 @@ hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in) {
     if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_advect<DIR, NB, false, true>), grid, block, 0, s, a);
   } else {
-    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, false>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_advect<DIR, NB, false, false>), grid, block, 0, s, a);
+    const size_t halo = sizeof(double) * 4 * NB * (size_t)block.x * block.y;
+    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, false>), grid, block, halo, s, a);
+    else hipLaunchKernelGGL((k_advect<DIR, NB, false, false>), grid, block, halo, s, a);
   }
   return hipGetLastError();
 }
