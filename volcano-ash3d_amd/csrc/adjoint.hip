@@ -243,17 +243,20 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
   const int ldlo = rmin - 2, ldhi = rmax + 2;  // cells the forward gathers
   auto act = [&](int r) { return r >= rmin && r <= rmax; };
   // lambda of the halo cells (zero where mu and the direct term vanish)
-  double hl0[NB], hl1[NB], hl2[NB], hl3[NB];
+  // (held in LDS, 4*NB per thread element-major, until the four
+  // iterations that read them)
+  extern __shared__ double s_halo[];
+  const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y, nthr = 64 * W;
   if constexpr (!GM) {
     auto halo = [&](int r, int u) {
       return (any_on && exists[u] && act(r)) ? lam[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
     };
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      hl0[u] = halo(lo - 2, u);
-      hl1[u] = halo(lo - 1, u);
-      hl2[u] = halo(hi + 1, u);
-      hl3[u] = halo(hi + 2, u);
+      s_halo[(4 * u + 0) * nthr + tid] = halo(lo - 2, u);
+      s_halo[(4 * u + 1) * nthr + tid] = halo(lo - 1, u);
+      s_halo[(4 * u + 2) * nthr + tid] = halo(hi + 1, u);
+      s_halo[(4 * u + 3) * nthr + tid] = halo(hi + 2, u);
     }
     if (W > 1) __syncthreads();
   }
@@ -312,10 +315,11 @@ __global__ __launch_bounds__(256) void k_adj_advect_seg(AdvArgs A, const double 
           q0[u] = (ld && exists[u]) ? rs_q[p][u] : 0.0;
           double v = rs_l[p][u];
           if constexpr (!GM) {
-            v = (r == lo - 2) ? hl0[u] : v;
-            v = (r == lo - 1) ? hl1[u] : v;
-            v = (r == hi + 1) ? hl2[u] : v;
-            v = (r == hi + 2) ? hl3[u] : v;
+            // r, lo and hi are uniform across the wave: a scalar branch
+            if (r == lo - 2 || r == lo - 1 || r == hi + 1 || r == hi + 2) {
+              const int k = (r == lo - 2) ? 0 : (r == lo - 1) ? 1 : (r == hi + 1) ? 2 : 3;
+              v = s_halo[(4 * u + k) * nthr + tid];
+            }
           }
           l0[u] = v;
           vf0[u] = (DIR == 2 && ld && exists[u]) ? rs_vf[p][u] : 0.0;
@@ -1031,8 +1035,9 @@ hipError_t adj_seg_dir(hipStream_t s, const AdvArgs &a_in, const double *qpre, d
     if (topo) hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true, true>), grid, block, 0, s, a, qpre, lam);
     else hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false, true>), grid, block, 0, s, a, qpre, lam);
   } else {
-    if (topo) hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true, false>), grid, block, 0, s, a, qpre, lam);
-    else hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false, false>), grid, block, 0, s, a, qpre, lam);
+    const size_t halo = sizeof(double) * 4 * NB * (size_t)block.x * block.y;
+    if (topo) hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, true, false>), grid, block, halo, s, a, qpre, lam);
+    else hipLaunchKernelGGL((k_adj_advect_seg<DIR, NB, false, false>), grid, block, halo, s, a, qpre, lam);
   }
   return hipGetLastError();
 }
