@@ -300,3 +300,46 @@ def test_tuning_does_not_change_results(pkg, oracle, tr, nb, x_tile):
     T.sync_to_host()
     T.close()
     assert_same(a, b)
+
+
+@pytest.mark.parametrize("between", ["nothing", "source", "source_then_step", "diffuse", "checkpoint",
+                                     "subgrid_step"])
+def test_glue_ghost_zeroing_before_any_reader(pkg, oracle, tr, between):
+    """The glue's zeroing of the concentration ghosts may wait for the next
+    step, whose Set_BC(1) rewrites every ghost; whatever else follows the
+    glue (a host read, a sweep, a checkpoint, the vent-column source and
+    then a read) sees the ghosts the reference's copy leaves, bitwise."""
+    st = pkg.workloads.random_state(16, 12, 9, 3, seed=5, latlon=True)
+    st.concen_pd[..., 1] = st.concen_pd[..., 0]
+    flux = np.abs(np.random.default_rng(2).standard_normal((st.nz + 1, st.ns)))
+    a, b = st.copy(), st.copy()
+    O, T = oracle.Oracle(a), tr.Transport(b)
+    for X in (O, T):
+        X.step_glue()
+        X.transport_step(1, True)
+        X.step_glue()
+    if between in ("source", "source_then_step"):
+        for X in (O, T):
+            X.source_column_add(5, 6, flux)
+    if between == "source_then_step":
+        for X in (O, T):
+            X.transport_step(2, True)
+    elif between == "diffuse":
+        for X in (O, T):
+            X.diffCN_z()
+    elif between == "checkpoint":
+        # a saved and reloaded ts0 (loading marks ts1 as the mirror of ts0,
+        # so only ts0 is compared)
+        T.checkpoints(1)
+        T.checkpoint_save(0)
+        T.checkpoint_load(0)
+    elif between == "subgrid_step":
+        for X in (O, T):
+            X.transport_step_subgrid(2, True, 5, 6, 3)
+    T.sync_to_host()
+    T.close()
+    if between == "checkpoint":
+        assert np.array_equal(a.concen_pd[..., 0], b.concen_pd[..., 0])
+        assert_same(a, b, names=ARRAYS[1:])
+    else:
+        assert_same(a, b)

@@ -96,6 +96,13 @@ struct ash3d_tr_ctx {
   // the velocity ghosts hold Set_BC(1)'s values for the current interior
   // velocities (cleared by every entry point that writes velocities)
   bool vel_bc_ok = false;
+  // the glue's zeroing of the ts0 ghosts, not yet launched: the next
+  // transport step's Set_BC(1) rewrites every concentration ghost from
+  // interior cells, so it stands in for it; every other entry point but
+  // the met/wind updates launches it first (the vent-column source too: it
+  // adds into the ghost cell above the column, which a later reader must
+  // see as zero plus the source)
+  bool ghost_zero_pending = false;
   // umbrella source: device copies of the stencil + patch flux and of the
   // two wind planes, uploaded through one pinned staging buffer each
   double *umb_src = nullptr, *umb_wind = nullptr;
@@ -195,9 +202,20 @@ int download(ash3d_tr_ctx *x, double *dst, const double *src, long long n) {
   return 0;
 }
 
-int check_ctx(ash3d_tr_ctx *x) {
+// Entry points that may leave the glue's ghost zeroing pending.
+int check_ctx_keep(ash3d_tr_ctx *x) {
   if (!x) return fail(ASH3D_TR_EINVAL, "null context");
   HIPCHK(hipSetDevice(x->device));
+  return 0;
+}
+
+int check_ctx(ash3d_tr_ctx *x) {
+  int rc = check_ctx_keep(x);
+  if (rc) return rc;
+  if (x->ghost_zero_pending) {
+    x->ghost_zero_pending = false;
+    HIPCHK(launch_zero_ghosts(x->stream, x->d, x->ts0()));
+  }
   return 0;
 }
 
@@ -965,7 +983,7 @@ static int step_diffusion(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
 
 int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
                   double imp_fac, const int32_t *is_aloft) {
-  int rc = check_ctx(x);
+  int rc = check_ctx_keep(x);
   if (rc) return rc;
   int32_t b[6];
   full_bounds(x, b);
@@ -974,6 +992,7 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   // re-zero concentration ghosts that the sweeps never write; only the
   // periodic x copies change, so only a periodic grid repeats the pass.
   if ((rc = set_bc1(x, true))) return rc;
+  x->ghost_zero_pending = false;  // every ts0 ghost rewritten
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0;
   if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false))) return rc;
@@ -985,7 +1004,7 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
 
 int ash3d_tr_step_keep(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusion, int32_t use_cn,
                        double imp_fac, const int32_t *is_aloft, int32_t slot_b, int32_t slot_z) {
-  int rc = check_ctx(x);
+  int rc = check_ctx_keep(x);
   if (rc) return rc;
   const int ns = (int)x->ckpt.size();
   if (slot_b < 0 || slot_b >= ns || slot_z < 0 || slot_z >= ns || slot_b == slot_z)
@@ -995,6 +1014,7 @@ int ash3d_tr_step_keep(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_di
   // ash3d_tr_step's advection, each sweep out of place: ts0 -> slot_b ->
   // slot_z -> ts0 (the same kernels and Set_BC(1) passes, so the same bits)
   if ((rc = set_bc1(x, true))) return rc;
+  x->ghost_zero_pending = false;  // every ts0 ghost rewritten
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0;
   double *sb = x->ckpt[slot_b], *sz = x->ckpt[slot_z];
@@ -1011,7 +1031,7 @@ int ash3d_tr_step_glue(ash3d_tr_ctx *x) {
   if (rc) return rc;
   ProfScope ps(x, ASH3D_PROF_GLUE);
   if (x->ts1_state == TS1_MIRROR) {
-    HIPCHK(launch_zero_ghosts(x->stream, x->d, x->ts0()));
+    x->ghost_zero_pending = true;  // launched by the next entry point unless a step comes first
   } else if (x->ts1_state == TS1_ZERO) {
     HIPCHK(launch_fill(x->stream, x->ts0(), x->n_conc(), 0.0));
   } else {
@@ -1096,7 +1116,7 @@ int ash3d_tr_umbrella_source_add(ash3d_tr_ctx *x, int32_t ivent, int32_t jvent, 
 
 int ash3d_tr_umbrella_winds_add(ash3d_tr_ctx *x, int32_t ibase, int32_t itop, const double *uvx,
                                 const double *uvy) {
-  int rc = check_ctx(x);
+  int rc = check_ctx_keep(x);
   if (rc) return rc;
   const Dims &d = x->d;
   if (!uvx || !uvy) return fail(ASH3D_TR_EINVAL, "null umbrella wind");
@@ -1569,7 +1589,7 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *x, int32_t itime) {
 }
 
 int ash3d_tr_met_push(ash3d_tr_ctx *x, const float *vx, const float *vy, const float *vz, const float *vf) {
-  int rc = check_ctx(x);
+  int rc = check_ctx_keep(x);
   if (rc) return rc;
   if (!vx || !vy || !vz) return fail(ASH3D_TR_EINVAL, "null met-step wind");
   const Dims &d = x->d;
@@ -1597,7 +1617,7 @@ int ash3d_tr_met_push(ash3d_tr_ctx *x, const float *vx, const float *vy, const f
 }
 
 int ash3d_tr_met_interp(ash3d_tr_ctx *x, double frac, int32_t use_vf) {
-  int rc = check_ctx(x);
+  int rc = check_ctx_keep(x);
   if (rc) return rc;
   if (x->met_count < 2) return fail(ASH3D_TR_EINVAL, "two met steps are needed (have %d)", x->met_count);
   const int nxt = x->met_next, lst = 1 - nxt;
