@@ -107,7 +107,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   double ga[M];                 // backward: dh_k -= ga_k*dh_{k+1}
   double ah[M], ch[M];          // final rows: ah_k*x_first + x_k + ch_k*x_last = dh_k
   double rho;
-  double kr[M], rk[M];  // adjoint: kappa of each row and its inverse
+  // adjoint: kappa of each row and its inverse, in LDS after the tile
+  // (row k of this lane at [k*NT + tid], its inverse at [(M+k)*NT + tid])
+  double *s_kap = s_tile + T * PITCH;
   {
     double av[M], bv[M], cv[M];
 #pragma unroll
@@ -132,8 +134,9 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
         rL[k] = rR[k] = 0.0;
       }
       if (ADJ) {
-        kr[k] = Q.kap[Q.st0 + Q.sta * ac + Q.stb * b + Q.str * (long long)(rmin + (real ? i0 : 0))];
-        rk[k] = 1.0 / kr[k];
+        const double kv = Q.kap[Q.st0 + Q.sta * ac + Q.stb * b + Q.str * (long long)(rmin + (real ? i0 : 0))];
+        s_kap[k * NT + tid] = kv;
+        s_kap[(M + k) * NT + tid] = 1.0 / kv;
       }
     }
 #pragma unroll
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       // forward: the explicit half-step B q; adjoint: K^{-1} lambda
-      const double d = ADJ ? q[k + 1] * rk[k]
+      const double d = ADJ ? q[k + 1] * s_kap[(M + k) * NT + tid]
                            : (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
       dh[k] = al[k] * d;
       if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
         if (i == 0) xm = xs[k];
         if (i == n - 1) xp = xs[k];
         const double v = (omt * rL[k]) * xm + (1.0 - omt * (rL[k] + rR[k])) * xs[k] + (omt * rR[k]) * xp;
-        if (i < n) trow[i] = kr[k] * v;
+        if (i < n) trow[i] = s_kap[k * NT + tid] * v;
       }
     } else {
 #pragma unroll
@@ -436,11 +439,12 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   }
 }
 
-template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = ((M <= 4 || (M <= 6 && !ADJ)) ? 2 : 1)>
+template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = (M <= 6 ? 2 : 1)>
 hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
   const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
-  const size_t lds = sizeof(double) * (size_t)T * (LP * M + 1);
+  // the tile, and for the adjoint each lane's row kappas and inverses
+  const size_t lds = sizeof(double) * ((size_t)T * (LP * M + 1) + (ADJ ? (size_t)2 * M * T * LP : 0));
   hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
                      q);
   return hipGetLastError();
