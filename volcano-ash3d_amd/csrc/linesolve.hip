@@ -445,6 +445,12 @@ hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
   // the tile, and for the adjoint each lane's row kappas and inverses
   const size_t lds = sizeof(double) * ((size_t)T * (LP * M + 1) + (ADJ ? (size_t)2 * M * T * LP : 0));
+  if (lds > 65536) {
+    // beyond the default dynamic LDS limit (the adjoint's 512-lane tiles)
+    hipError_t e = hipFuncSetAttribute((const void *)k_cn_part<LP, M, T, CONTIG, PD, ADJ>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
                      q);
   return hipGetLastError();
@@ -465,14 +471,10 @@ hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArg
     if (n <= 384) return launch_part<64, 6, 4, true, ADJ>(s, a, q);
     return launch_part<64, 8, 4, true, ADJ>(s, a, q);
   }
-  // (the adjoint keeps two more values per row: 4 rows per lane up to 256)
   if (shape == 1) {
     if (n <= 64) return launch_part<16, 4, 32, false, ADJ>(s, a, q);
     if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
-    if (n <= 256) {
-      if constexpr (ADJ) return launch_part<64, 4, 8, false, ADJ>(s, a, q);
-      else return launch_part<32, 8, 16, false, ADJ>(s, a, q);
-    }
+    if (n <= 256) return launch_part<32, 8, 16, false, ADJ>(s, a, q);
     return launch_part<64, 8, 4, false, ADJ>(s, a, q);
   }
   if (n <= 64) return launch_part<16, 4, 16, false, ADJ>(s, a, q);
