@@ -254,6 +254,11 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 
   // ---- bin-independent terms of the row
   double u[NC][2], dtv[NC][2], divp[NC][2], divm[NC][2];
+  // the four per-element row terms wait in LDS for the bin loop (wave-private:
+  // 4 * 2*NC values per lane, element-major), which frees their registers
+  extern __shared__ double s_row[];
+  double *srow = s_row + (long long)(threadIdx.x >> 6) * (8 * NC * 64) + l;
+  auto rt = [&](int which, int m, int e) -> double & { return srow[((which * NC + m) * 2 + e) * 64]; };
   double tf[NC][2], D[NC][2], tfm1[NC][2], Dm1[NC][2];
   {
     double vel[NC][2], sig[NC][2], kap[NC][2];
@@ -301,6 +306,10 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
         // divergence corrections (:245-246)
         divp[m][e] = (fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]));
         divm[m][e] = (fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]));
+        rt(0, m, e) = u[m][e];
+        rt(1, m, e) = dtv[m][e];
+        rt(2, m, e) = divp[m][e];
+        rt(3, m, e) = divm[m][e];
         if (!TOPO) tfm1[m][e] = Dm1[m][e] = 0.0;
       }
   }
@@ -353,12 +362,13 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
         const int c = cc[m][e];
         fss[m][e] = 0.0; fsr[m][e] = 0.0; fsl[m][e] = 0.0;
         if (lim != 0 && valid[m][e] && c >= rmin && c <= rmax + 1)
-          fss[m][e] = limited_flux(lim, dqm[m][e], dq[m][e], dqp[m][e], u[m][e], dtv[m][e]);
+          fss[m][e] = limited_flux(lim, dqm[m][e], dq[m][e], dqp[m][e], rt(0, m, e), rt(1, m, e));
         if (uin[m][e]) {
-          double r = fmax(0.0, u[m][e]) * dq[m][e];
-          double t = fmin(0.0, u[m][e]) * dq[m][e];
-          r = r + q[m][e] * divm[m][e];
-          t = t + qm1[m][e] * divp[m][e];
+          const double uu = rt(0, m, e);
+          double r = fmax(0.0, uu) * dq[m][e];
+          double t = fmin(0.0, uu) * dq[m][e];
+          r = r + q[m][e] * rt(3, m, e);
+          t = t + qm1[m][e] * rt(2, m, e);
           if (TOPO) {
             r = r + tf[m][e] * q[m][e] * D[m][e];
             t = t - tfm1[m][e] * qm1[m][e] * Dm1[m][e];
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const int c = cc[m][e];
-        upd[m][e] = uin[m][e] ? -dtv[m][e] * (fsln[m][e] + fsr[m][e] + fssn[m][e] - fss[m][e]) : 0.0;
+        upd[m][e] = uin[m][e] ? -rt(1, m, e) * (fsln[m][e] + fsr[m][e] + fssn[m][e] - fss[m][e]) : 0.0;
         res[m][e] = (on && c >= rmin && c <= rmax) ? q[m][e] + upd[m][e] : 0.0;
       }
     const long long bofs = d.ncell * (long long)n;
@@ -434,10 +444,11 @@ template <int NC>
 hipError_t launch_wave(hipStream_t s, const AdvArgs &a) {
   const long long rows = (long long)a.d.ny * a.d.nz;
   dim3 grid((unsigned)((rows + 3) / 4));
+  const size_t lds = sizeof(double) * 4 * 8 * NC * 64;  // 4 waves x 4 row terms x 2*NC per lane
   if (a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect_x_wave<NC, true>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_advect_x_wave<NC, true>), grid, dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((k_advect_x_wave<NC, false>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_advect_x_wave<NC, false>), grid, dim3(256), lds, s, a);
   return hipGetLastError();
 }
 
