@@ -1,14 +1,21 @@
-// advect_x (src/AdvectionHorz_DCU.f90:68-305) with one workgroup per x row.
+// advect_x (src/AdvectionHorz_DCU.f90:68-305) over whole x rows, in two
+// forms that give the same bits:
 //
-// x rows are contiguous in memory, so lanes map to cells of the row: every
-// load and store of concen is a coalesced row segment.  The row of one bin
-// is staged through LDS and the update is built in three LDS phases (dq;
-// fss, fs_l, fs_r; the cell update), each neighbour value read from LDS.
-// The workgroup loops over all grain bins, so the bin-independent terms of
-// the row (interface speeds usig, dt/kappa, the divergence corrections, the
+// k_advect_x_wave: one wavefront per row, the row held in registers as
+// 16-byte cell pairs with the neighbours moved by DPP wave rotations (the
+// default where nx is even and the row fits three chunks of 128 cells);
+//
+// k_advect_x_rows: one workgroup per row.  x rows are contiguous in
+// memory, so lanes map to cells of the row: every load and store of
+// concen is a coalesced row segment.  The row of one bin is staged through
+// LDS and the update is built in three LDS phases (dq; fss, fs_l, fs_r;
+// the cell update), each neighbour value read from LDS.
+//
+// Both loop over all grain bins, so the bin-independent terms of the row
+// (interface speeds usig, dt/kappa, the divergence corrections, the
 // topographic factors) are computed once per row.  All global reads of a
-// row happen before the barrier that precedes its in-place writes, and no
-// other workgroup touches the row.
+// row and bin happen before its in-place writes, and no other wavefront or
+// workgroup touches the row.
 //
 // Arithmetic: the reference expressions in the reference order, no
 // contraction -- bitwise equal to the streaming kernel and the oracle.
