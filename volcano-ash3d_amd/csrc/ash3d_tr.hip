@@ -971,11 +971,13 @@ int ash3d_tr_diffuse_vert(ash3d_tr_ctx *x, int32_t use_cn, double dt, double imp
 static int step_diffusion(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_cn, double imp_fac,
                           const int32_t *is_aloft, const int32_t *b) {
   int rc;
-  {
-    ProfScope ps(x, ASH3D_PROF_BC);
-    HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
-  }
   const int dfirst = (itime % 2 == 0) ? 0 : 1;
+  {
+    // Set_BC(2) where it is read or survives: the first direction's ghosts
+    // of the interior rows (the later Neumann passes rewrite the rest)
+    ProfScope ps(x, ASH3D_PROF_BC);
+    HIPCHK(launch_set_bc2_face(x->stream, x->d, x->periodic, dfirst, x->ts0()));
+  }
   if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, b, false))) return rc;
   if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, b))) return rc;
   return sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, b);
@@ -1746,13 +1748,15 @@ int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
   if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false))) return rc;
   if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false))) return rc;
   if (use_diffusion) {
+    const int dfirst = (itime % 2 == 0) ? 0 : 1;
     {
+      // as in step_diffusion: Set_BC(2) on the first direction's ghosts of
+      // the interior rows
       ProfScope ps(x, ASH3D_PROF_BC);
-      HIPCHK(launch_set_bc2(x->stream, x->d, x->periodic, x->ts0()));
+      HIPCHK(launch_set_bc2_face(x->stream, x->d, x->periodic, dfirst, x->ts0()));
     }
     const int32_t g[6] = {std::max(1, b[0] - 1), std::min(x->d.nx, b[1] + 1), std::max(1, b[2] - 1),
                           std::min(x->d.ny, b[3] + 1), std::max(1, b[4] - 1), std::min(x->d.nz, b[5] + 1)};
-    const int dfirst = (itime % 2 == 0) ? 0 : 1;
     if ((rc = sweep_diffuse(x, dfirst, use_cn, dt, imp_fac, is_aloft, g, false))) return rc;
     if ((rc = sweep_diffuse(x, 1 - dfirst, use_cn, dt, imp_fac, is_aloft, g))) return rc;
     if ((rc = sweep_diffuse(x, 2, use_cn, dt, imp_fac, is_aloft, g))) return rc;

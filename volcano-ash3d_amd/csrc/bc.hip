@@ -183,6 +183,35 @@ __global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodi
   }
 }
 
+// The part of Set_BC(2) (rule GH_BC2) that the diffusion of a step reads
+// or leaves in the final state when the sweeps start along dir (0: x, 1:
+// y): the dir ghosts of the interior rows.  Every other ghost the full
+// pass writes is rewritten, before anything reads it, by the Neumann
+// passes of the later sweeps (which cover all transverse indices,
+// ghosts included: the second horizontal direction's over every ghost
+// plane index, the z pass over every ghost plane).  dir 0: the four x
+// ghosts of a row in adjacent lanes; dir 1: lanes along x.
+__global__ __launch_bounds__(kBlock) void k_bc2_face(Dims d, int periodic, int dir, double *c0) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int n = (int)blockIdx.y;
+  double *c = c0 + d.ncell * n;
+  if (dir == 0) {
+    if (t >= 4LL * d.ny * d.nz) return;
+    const int col = (int)(t & 3);
+    const long long row = t >> 2;
+    const int j = (int)(row % d.ny) + 1, k = (int)(row / d.ny) + 1;
+    const int i = (col < 2) ? (col - 1) : (d.nx + (col - 1));
+    c[d.p3(i, j, k)] = c[d.p3(xmap(i, d.nx, periodic), j, k)];
+  } else {
+    if (t >= 4LL * d.nx * d.nz) return;
+    const int i = (int)(t % d.nx) + 1;
+    const long long rk = t / d.nx;
+    const int r = (int)(rk & 3), k = (int)(rk >> 2) + 1;
+    const int j = (r < 2) ? (r - 1) : (d.ny + (r - 1));
+    c[d.p3(i, j, k)] = c[d.p3(i, clampi(j, 1, d.ny), k)];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double v) {
   const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
   const long long stride = (long long)gridDim.x * kBlock;
@@ -280,6 +309,12 @@ hipError_t launch_set_bc1_conc(hipStream_t s, const Dims &d, int periodic, doubl
 
 hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0) {
   return launch_ghosts(s, d, periodic, GH_BC2, c0);
+}
+
+hipError_t launch_set_bc2_face(hipStream_t s, const Dims &d, int periodic, int dir, double *c0) {
+  const long long n = 4LL * (dir == 0 ? d.ny : d.nx) * d.nz;
+  hipLaunchKernelGGL(k_bc2_face, dim3(blocks_for(n), d.ns), dim3(kBlock), 0, s, d, periodic, dir, c0);
+  return hipGetLastError();
 }
 
 hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0) {

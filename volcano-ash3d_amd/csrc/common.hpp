@@ -161,6 +161,8 @@ hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double
                               double *vz, double *vf);
 hipError_t launch_set_bc1_conc(hipStream_t s, const Dims &d, int periodic, double *c0);
 hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0);
+// Set_BC(2) on the dir (0: x, 1: y) ghosts of the interior rows only
+hipError_t launch_set_bc2_face(hipStream_t s, const Dims &d, int periodic, int dir, double *c0);
 hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0);
 hipError_t launch_zero_ghosts(hipStream_t s, const Dims &d, double *c0);
 // dst's ghost cells (every bin) = src's
