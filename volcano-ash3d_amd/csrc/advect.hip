@@ -85,7 +85,9 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y, nthr = 64 * W;
   if constexpr (!GM) {
     auto halo = [&](int r, int u) {
-      return (any_on && exists[u] && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]] : 0.0;
+      const bool qin = !A.zg || (r >= 1 && r <= N);
+      return (any_on && exists[u] && qin && r >= ldlo && r <= ldhi) ? c[base + sL * (long long)(r + 1) + bofs[u]]
+                                                                      : 0.0;
     };
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
@@ -156,7 +158,8 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
           // the halo cells come from the registers filled before the barrier
-          double v = (ld && exists[u]) ? rs_q[p][u] : 0.0;
+          // (A.zg: ghost cells read as Set_BC(1)'s zeros)
+          double v = (ld && exists[u] && (!A.zg || (r >= 1 && r <= N))) ? rs_q[p][u] : 0.0;
           if constexpr (!GM) {
             // r, lo and hi are uniform across the wave: a scalar branch
             if (r == lo - 2 || r == lo - 1 || r == hi + 1 || r == hi + 2) {

@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
     for (int m = 0; m < CPT; ++m) {
       const int c = cc[m];
       q[m] = 0.0;
-      if (valid[m] && c >= rmin - 2 && c <= rmax + 2) q[m] = c0[base + (c + 1) + bofs];
+      if (valid[m] && c >= rmin - 2 && c <= rmax + 2 && (!A.zg || (c >= 1 && c <= nx)))
+        q[m] = c0[base + (c + 1) + bofs];
       if (valid[m]) s_q[c + 1] = q[m];
     }
     __syncthreads();
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 
   // per element: cell, row position, masks (bin-independent)
   int cc[NC][2];
-  bool valid[NC][2], ld[NC][2], uin[NC][2];
+  bool valid[NC][2], ld[NC][2], uin[NC][2], lq[NC][2];
   int pp[NC];       // row position of the pair, clamped into the row
   bool store[NC];   // an interior pair (both cells in 1..nx)
 #pragma unroll
@@ -246,6 +247,8 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
       valid[m][e] = p0 + e < W;
       ld[m][e] = valid[m][e] && c >= rmin - 2 && c <= rmax + 2;
       uin[m][e] = valid[m][e] && c >= rmin - 1 && c <= rmax + 1;
+      // concentrations gathered (A.zg: ghosts read as Set_BC(1)'s zeros)
+      lq[m][e] = ld[m][e] && (!A.zg || (c >= 1 && c <= nx));
     }
   }
 
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) q[m][e] = ld[m][e] ? qs[m][e] : 0.0;  // gather (:143)
+      for (int e = 0; e < 2; ++e) q[m][e] = lq[m][e] ? qs[m][e] : 0.0;  // gather (:143)
     fetch(qs, n + 2);
     // dq (:168-169)
     double qm1[NC][2], dq[NC][2];

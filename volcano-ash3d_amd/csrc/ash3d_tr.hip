@@ -261,8 +261,9 @@ struct ProfScope {
   }
 };
 
-// Set_BC(1) on a state buffer (ts0 unless given).
-int set_bc1(ash3d_tr_ctx *x, bool velocities, double *state = nullptr) {
+// Set_BC(1) on a state buffer (ts0 unless given); conc = false: the
+// velocity ghosts only.
+int set_bc1(ash3d_tr_ctx *x, bool velocities, double *state = nullptr, bool conc = true) {
   ProfScope ps(x, ASH3D_PROF_BC);
   // Set_BC(1) on velocities is a projection (every ghost a copy of an
   // interior or wrapped cell that the pass does not write), so it is
@@ -271,7 +272,7 @@ int set_bc1(ash3d_tr_ctx *x, bool velocities, double *state = nullptr) {
   if (velocities && (x->capturing || !x->vel_bc_ok))
     HIPCHK(launch_set_bc1_vel(x->stream, x->d, x->periodic, x->vx, x->vy, x->vz, x->vf));
   if (velocities && !x->capturing) x->vel_bc_ok = true;
-  HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, state ? state : x->ts0()));
+  if (conc) HIPCHK(launch_set_bc1_conc(x->stream, x->d, x->periodic, state ? state : x->ts0()));
   return 0;
 }
 
@@ -311,12 +312,14 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
 // lines, then the ghost cells copied from state): out ends as state would
 // after the in-place sweep, and state is left as it was.
 int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
-                 bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr, bool to_sink = true) {
+                 bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr, bool to_sink = true,
+                 bool zero_ghosts = false) {
   int rc = check_bounds(x, bounds);
   if (rc) return rc;
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if (with_bc && (rc = set_bc1(x, bc_vel, state))) return rc;
   AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, state ? state : x->ts0());
+  a.zg = (zero_ghosts && !x->periodic) ? 1 : 0;
   if (state && out) a.cout = out;
   if (state && to_sink) {
     const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
@@ -993,13 +996,18 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   // Set_BC(1) calls repeat velocity ghosts that cannot have changed, and
   // re-zero concentration ghosts that the sweeps never write; only the
   // periodic x copies change, so only a periodic grid repeats the pass.
-  if ((rc = set_bc1(x, true))) return rc;
+  // Without periodic x and with diffusion, the concentration part of the
+  // pass is not launched: the sweeps read the ghost cells along their
+  // lines as the zeros it would write, and the diffusion's Set_BC(2) and
+  // Neumann passes rewrite every ghost before anything else reads it.
+  const bool zg = use_diffusion && !x->periodic;
+  if ((rc = set_bc1(x, true, nullptr, !zg))) return rc;
   x->ghost_zero_pending = false;  // every ts0 ghost rewritten
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0;
-  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false))) return rc;
-  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false))) return rc;
-  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false))) return rc;
+  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, nullptr, nullptr, true, zg))) return rc;
+  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false, nullptr, nullptr, true, zg))) return rc;
+  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false, nullptr, nullptr, true, zg))) return rc;
   if (use_diffusion) return step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b);
   return 0;
 }

@@ -122,6 +122,8 @@ struct AdvArgs {
   int limiter;
   int gy;              // streamed sweeps: bin groups per workgroup along y (0: line segments along y)
   int xrows;           // x rows: 1 one wavefront per row where it applies, 0 one workgroup per row
+  int zg;              // 1: the concentration ghosts along the lines read as zero (the non-periodic
+                       // Set_BC(1) values) instead of being loaded
   AloftMask aloft;
 };
 
