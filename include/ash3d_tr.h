@@ -207,10 +207,10 @@ enum {
    * lines, so the per-cell fields are read once for all of them */
   ASH3D_TUNE_ADV_GROUPS_Y = 12,
   ASH3D_TUNE_ADV_GROUPS_Z = 13,
-  /* x advection rows (forward and adjoint): 1 (default) one wavefront
-   * per row, the row in registers (nx even, nx <= 380 or with
-   * ZScaling_ID 2 nx <= 252, at most 64 bins for the forward; other grids
-   * use 0), 0 one workgroup per row staged through LDS */
+  /* x advection rows: 1 (default) one wavefront per row, the row in
+   * registers (nx <= 380, or nx <= 252 with ZScaling_ID 2, at most 64
+   * bins; the adjoint for even nx only; other grids use 0), 0 one
+   * workgroup per row staged through LDS */
   ASH3D_TUNE_ADV_X_ROWS = 14
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial

@@ -1,10 +1,11 @@
 """x advection rows against the oracle, bitwise, with both row kernels:
-one wavefront per row (the row in registers as 16-byte cell pairs, the
-neighbours by DPP rotations across chunks of 128 cells) and one workgroup
-per row (the row staged through LDS).  Row lengths sit at and around the
-chunk edges (nx + 4 = 128, 256, 384); odd nx, nx > 380 (> 252 with
-topography) and more than 64 bins take the workgroup kernel whatever the
-setting."""
+one wavefront per row (the row in registers, as 16-byte cell pairs for
+even nx and single cells for odd nx, the neighbours by DPP rotations
+across chunks of 128 or 64 cells) and one workgroup per row (the row
+staged through LDS).  Row lengths sit at and around the chunk edges
+(nx + 4 = 64, 128, 256, 384 and the pair/single-cell limits); nx > 380
+(> 252 with topography) and more than 64 bins take the workgroup kernel
+whatever the setting."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,8 @@ from test_gpu_parity import assert_same, run_both
 
 pytestmark = pytest.mark.gpu
 
-NX = [2, 12, 124, 126, 252, 254, 380, 382, 508, 13, 510]
+NX = [2, 12, 124, 126, 252, 254, 380, 382, 508, 510,
+      3, 13, 59, 61, 123, 125, 251, 253, 379, 381]
 VARIANTS = {
     "cartesian": dict(latlon=False, periodic=False, zscaling_id=0),
     "lonlat_periodic": dict(latlon=True, periodic=True, zscaling_id=0),

@@ -197,14 +197,15 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
 
 // ---------------------------------------------------------------------
 // One wavefront per x row, the row in registers.  Lane l of chunk m holds
-// the cell pair c = -1 + 128 m + 2 l + e (e = 0, 1), loaded and stored as
-// one 16-byte access (nx even: every pair is two ghost or two interior
-// cells).  The neighbours c-1 and c+1 come from the same lane (the other
-// element of the pair) or from the next/previous lane by a whole-wave DPP
-// rotation, with the rotated value of the neighbouring chunk at lanes 0
-// and 63 and zero beyond the row (what the LDS version reads there).  No
-// LDS and no barriers: the row of the next bin is loaded while the current
-// one is updated.  Every load and store of the bin loop is issued on every
+// E cells c = -1 + 64 E m + E l + e (e < E): E = 2, the cell pair loaded
+// and stored as one 16-byte access (nx even: every pair is two ghost or
+// two interior cells); E = 1 for odd nx.  The neighbours c-1 and c+1 come
+// from the same lane (the other element of a pair) or from the
+// next/previous lane by a whole-wave DPP rotation, with the rotated value
+// of the neighbouring chunk at lanes 0 and 63 and zero beyond the row
+// (what the LDS version reads there).  The per-row terms wait in a
+// wave-private LDS region; no barriers: the row of the next bin is loaded
+// while the current one is updated.  Every load and store of the bin loop is issued on every
 // iteration (clamped addresses, masked-off pairs stored to junk), so the
 // wait before a row is used counts exactly the memory operations issued
 // after its loads.  The outflow increments of the row ends are gathered in
@@ -212,8 +213,23 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
 //
 // The arithmetic is that of k_advect_x_rows expression by expression.
 
-template <int NC, bool TOPO>
+template <int NC, int E, bool TOPO>
 __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
+  constexpr int CS = 64 * E;  // cells per chunk
+  // a lane's E consecutive cells (one 16-byte access for a pair)
+  auto ld_cells = [](const double *src, double (&out)[E]) {
+    if constexpr (E == 2) {
+      const double2 v = *(const double2 *)src;
+      out[0] = v.x;
+      out[1] = v.y;
+    } else {
+      out[0] = *src;
+    }
+  };
+  auto st_cells = [](double *dst, const double (&v)[E]) {
+    if constexpr (E == 2) *(double2 *)dst = make_double2(v[0], v[1]);
+    else *dst = v[0];
+  };
   const Dims d = A.d;
   const LineGeom g = A.g;
   const int nx = d.nx, W = nx + 4;
@@ -231,17 +247,17 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   const bool lane0 = l == 0, lane63 = l == 63;
 
   // per element: cell, row position, masks (bin-independent)
-  int cc[NC][2];
-  bool valid[NC][2], ld[NC][2], uin[NC][2], lq[NC][2];
+  int cc[NC][E];
+  bool valid[NC][E], ld[NC][E], uin[NC][E], lq[NC][E];
   int pp[NC];       // row position of the pair, clamped into the row
   bool store[NC];   // an interior pair (both cells in 1..nx)
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
-    const int p0 = 128 * m + 2 * l;
-    pp[m] = min(p0, W - 2);
-    store[m] = p0 >= 2 && p0 + 1 <= nx + 1;
+    const int p0 = CS * m + E * l;
+    pp[m] = min(p0, W - E);
+    store[m] = p0 >= 2 && p0 + E - 1 <= nx + 1;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int c = p0 + e - 1;
       cc[m][e] = c;
       valid[m][e] = p0 + e < W;
@@ -257,52 +273,54 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
       const long long bofs = d.ncell * (long long)n;
 #pragma unroll
       for (int m = 0; m < NC; ++m)
-        if (store[m]) *(double2 *)(co + base + pp[m] + bofs) = make_double2(0.0, 0.0);
+        if (store[m]) {
+          const double z[E] = {};
+          st_cells(co + base + pp[m] + bofs, z);
+        }
     }
     return;
   }
 
   // ---- bin-independent terms of the row
-  double u[NC][2], dtv[NC][2], divp[NC][2], divm[NC][2];
+  double u[NC][E], dtv[NC][E], divp[NC][E], divm[NC][E];
   // the four per-element row terms wait in LDS for the bin loop (wave-private:
   // 4 * 2*NC values per lane, element-major), which frees their registers
   extern __shared__ double s_row[];
-  double *srow = s_row + (long long)(threadIdx.x >> 6) * (8 * NC * 64) + l;
-  auto rt = [&](int which, int m, int e) -> double & { return srow[((which * NC + m) * 2 + e) * 64]; };
-  double tf[NC][2], D[NC][2], tfm1[NC][2], Dm1[NC][2];
+  double *srow = s_row + (long long)(threadIdx.x >> 6) * (4 * E * NC * 64) + l;
+  auto rt = [&](int which, int m, int e) -> double & { return srow[((which * NC + m) * E + e) * 64]; };
+  double tf[NC][E], D[NC][E], tfm1[NC][E], Dm1[NC][E];
   {
-    double vel[NC][2], sig[NC][2], kap[NC][2];
+    double vel[NC][E], sig[NC][E], kap[NC][E];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
       const long long o = base + pp[m];
-      const double2 v2 = *(const double2 *)(A.vel + o);
-      const double2 s2 = *(const double2 *)(A.sig + o);
-      const double2 k2 = *(const double2 *)(A.kap + o);
-      double2 d2 = make_double2(0.0, 0.0);
-      if (TOPO) d2 = *(const double2 *)(A.deld + (long long)pp[m] + (long long)(nx + 4) * (j + 1));
-      const double vv[2] = {v2.x, v2.y}, ss[2] = {s2.x, s2.y}, kk[2] = {k2.x, k2.y}, dd[2] = {d2.x, d2.y};
+      double vv[E], ss[E], kk[E], dd[E] = {};
+      ld_cells(A.vel + o, vv);
+      ld_cells(A.sig + o, ss);
+      ld_cells(A.kap + o, kk);
+      if (TOPO) ld_cells(A.deld + (long long)pp[m] + (long long)(nx + 4) * (j + 1), dd);
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
+      for (int e = 0; e < E; ++e) {
         vel[m][e] = ld[m][e] ? vv[e] : 0.0;
         sig[m][e] = ld[m][e] ? ss[e] : 0.0;
         kap[m][e] = ld[m][e] ? kk[e] : 1.0;
         D[m][e] = (TOPO && uin[m][e]) ? dd[e] : 0.0;
       }
     }
-    double velm1[NC][2], sigp1[NC][2];
+    double velm1[NC][E], sigp1[NC][E];
     row_prev<NC>(vel, velm1, lane0);
     if (TOPO) row_next<NC>(sig, sigp1, lane63);
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
+      for (int e = 0; e < E; ++e) {
         // usig (:159-162), dt/kappa (:155-156)
         u[m][e] = uin[m][e] ? 0.5 * (velm1[m][e] + vel[m][e]) * sig[m][e] : 0.0;
         dtv[m][e] = uin[m][e] ? A.dt / kap[m][e] : 0.0;
         // topographic factor 0.5*vel_I*(sig_I+sig_{I+1}) (:252-254)
         tf[m][e] = (TOPO && valid[m][e]) ? 0.5 * vel[m][e] * (sig[m][e] + sigp1[m][e]) : 0.0;
       }
-    double um1[NC][2], up1[NC][2];
+    double um1[NC][E], up1[NC][E];
     row_prev<NC>(u, um1, lane0);
     row_next<NC>(u, up1, lane63);
     if (TOPO) {
@@ -312,7 +330,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
+      for (int e = 0; e < E; ++e) {
         // divergence corrections (:245-246)
         divp[m][e] = (fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]));
         divm[m][e] = (fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]));
@@ -327,48 +345,46 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   // outflow increments: lane n keeps those of bin n (at most 64 bins)
   const bool of_lo = rmin == 1 && !A.periodic;
   const bool of_hi = rmax == nx && !A.periodic;
-  const int m_hi = (nx + 2) >> 7, l_hi = ((nx + 2) & 127) >> 1;  // cell nx+1: element 0
+  const int m_hi = (nx + 2) / CS, l_hi = ((nx + 2) % CS) / E;  // cell nx+1: element 0 (nx even for pairs)
   double acc_lo = 0.0, acc_hi = 0.0;
 
   const int lim = A.limiter;
   // two bins in flight: the rows of bins n and n+1 are in slots A and B;
   // slot A is refilled with bin n+2 as soon as bin n is gathered
-  double qa[NC][2], qb[NC][2];
-  auto fetch = [&](double (&qs)[NC][2], int n) {
+  double qa[NC][E], qb[NC][E];
+  auto fetch = [&](double (&qs)[NC][E], int n) {
     // a bin past the end or one that is off: a cached re-read of bin n-2
     const int nn = (n < d.ns && A.aloft.on(n)) ? n : max(0, min(n - 2, d.ns - 1));
     const long long bofs = d.ncell * (long long)nn;
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-      const double2 q2 = *(const double2 *)(c0 + base + pp[m] + bofs);
-      qs[m][0] = q2.x;
-      qs[m][1] = q2.y;
+      ld_cells(c0 + base + pp[m] + bofs, qs[m]);
     }
   };
-  auto bin = [&](double (&qs)[NC][2], int n) {
+  auto bin = [&](double (&qs)[NC][E], int n) {
     const bool on = A.aloft.on(n);
-    double q[NC][2];
+    double q[NC][E];
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) q[m][e] = lq[m][e] ? qs[m][e] : 0.0;  // gather (:143)
+      for (int e = 0; e < E; ++e) q[m][e] = lq[m][e] ? qs[m][e] : 0.0;  // gather (:143)
     fetch(qs, n + 2);
     // dq (:168-169)
-    double qm1[NC][2], dq[NC][2];
+    double qm1[NC][E], dq[NC][E];
     row_prev<NC>(q, qm1, lane0);
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) dq[m][e] = valid[m][e] ? q[m][e] - qm1[m][e] : 0.0;
-    double dqm[NC][2], dqp[NC][2];
+      for (int e = 0; e < E; ++e) dq[m][e] = valid[m][e] ? q[m][e] - qm1[m][e] : 0.0;
+    double dqm[NC][E], dqp[NC][E];
     row_prev<NC>(dq, dqm, lane0);
     row_next<NC>(dq, dqp, lane63);
     // limited flux and fluctuations at interface I = c (:178-259)
-    double fss[NC][2], fsr[NC][2], fsl[NC][2];
+    double fss[NC][E], fsr[NC][E], fsl[NC][E];
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
+      for (int e = 0; e < E; ++e) {
         const int c = cc[m][e];
         fss[m][e] = 0.0; fsr[m][e] = 0.0; fsl[m][e] = 0.0;
         if (lim != 0 && valid[m][e] && c >= rmin && c <= rmax + 1)
@@ -387,15 +403,15 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
           fsl[m][e] = t;
         }
       }
-    double fsln[NC][2], fssn[NC][2];
+    double fsln[NC][E], fssn[NC][E];
     row_next<NC>(fsl, fsln, lane63);
     row_next<NC>(fss, fssn, lane63);
     // cell update (:264-281), interior write and outflow (:285-294)
-    double upd[NC][2], res[NC][2];
+    double upd[NC][E], res[NC][E];
 #pragma unroll
     for (int m = 0; m < NC; ++m)
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
+      for (int e = 0; e < E; ++e) {
         const int c = cc[m][e];
         upd[m][e] = uin[m][e] ? -rt(1, m, e) * (fsln[m][e] + fsr[m][e] + fssn[m][e] - fss[m][e]) : 0.0;
         res[m][e] = (on && c >= rmin && c <= rmax) ? q[m][e] + upd[m][e] : 0.0;
@@ -404,15 +420,16 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
       double *dst = store[m] ? co + base + pp[m] + bofs : junk;
-      *(double2 *)dst = make_double2(res[m][0], res[m][1]);
+      st_cells(dst, res[m]);
     }
     if (on) {
-      // cell 0 is element 1 of lane 0, chunk 0; cell nx+1 element 0 of lane l_hi, chunk m_hi
+      // cell 0 (row position 1) is element 1 % E of lane 1 / E, chunk 0; cell
+      // nx+1 element 0 of lane l_hi, chunk m_hi
       double uh = upd[0][0];
 #pragma unroll
       for (int m = 0; m < NC; ++m)
         if (m == m_hi) uh = upd[m][0];
-      const double lo = lane_value(upd[0][1], 0);
+      const double lo = lane_value(upd[0][1 % E], 1 / E);
       const double hi = lane_value(uh, l_hi);
       if (l == n) { acc_lo = lo; acc_hi = hi; }
     }
@@ -450,15 +467,15 @@ hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt) {
   return hipGetLastError();
 }
 
-template <int NC>
+template <int NC, int E>
 hipError_t launch_wave(hipStream_t s, const AdvArgs &a) {
   const long long rows = (long long)a.d.ny * a.d.nz;
   dim3 grid((unsigned)((rows + 3) / 4));
-  const size_t lds = sizeof(double) * 4 * 8 * NC * 64;  // 4 waves x 4 row terms x 2*NC per lane
+  const size_t lds = sizeof(double) * 4 * 4 * E * NC * 64;  // 4 waves x 4 row terms x E*NC per lane
   if (a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect_x_wave<NC, true>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_advect_x_wave<NC, E, true>), grid, dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((k_advect_x_wave<NC, false>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_advect_x_wave<NC, E, false>), grid, dim3(256), lds, s, a);
   return hipGetLastError();
 }
 
@@ -468,22 +485,31 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // Rows of up to 8*256-4 cells; returns hipErrorInvalidValue beyond that so
 // the caller falls back to the streaming kernel.  One wavefront per row
-// where the row fits 3 chunks of 128 cells in pairs (nx even, nx <= 380;
+// (odd nx: single cells per lane, 6 chunks of 64 at most) where the row
+// fits 3 chunks of 128 cells in pairs (nx even, nx <= 380;
 // with topography 2 chunks, nx <= 252: longer rows leave one wave per SIMD
 // and spill, and run slower than the workgroup kernel), the outflow
 // increments fit one lane per bin (ns <= 64) and the arrays allow 16-byte
 // accesses.
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
   const int cells = a.d.nx + 4;
-  const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 384) && a.d.ns <= 64 &&
-                    aligned16(a.c0) &&
-                    aligned16(a.cout) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
-                    aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
-  if (wave) {
-    switch ((cells + 127) / 128) {
-      case 1: return launch_wave<1>(s, a);
-      case 2: return launch_wave<2>(s, a);
-      default: return launch_wave<3>(s, a);
+  const bool common = a.xrows == 1 && cells <= (a.deld ? 256 : 384) && a.d.ns <= 64;
+  if (common && a.d.nx % 2 == 0 && aligned16(a.c0) && aligned16(a.cout) && aligned16(a.vel) &&
+      aligned16(a.sig) && aligned16(a.kap) && aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld))) {
+    switch ((cells + 127) / 128) {  // cell pairs
+      case 1: return launch_wave<1, 2>(s, a);
+      case 2: return launch_wave<2, 2>(s, a);
+      default: return launch_wave<3, 2>(s, a);
+    }
+  }
+  if (common) {
+    switch ((cells + 63) / 64) {  // single cells (odd nx)
+      case 1: return launch_wave<1, 1>(s, a);
+      case 2: return launch_wave<2, 1>(s, a);
+      case 3: return launch_wave<3, 1>(s, a);
+      case 4: return launch_wave<4, 1>(s, a);
+      case 5: return launch_wave<5, 1>(s, a);
+      default: return launch_wave<6, 1>(s, a);
     }
   }
   int cpt = (cells + 255) / 256;

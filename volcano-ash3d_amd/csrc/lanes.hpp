@@ -103,6 +103,24 @@ __device__ inline void row_next(const double (&v)[NC][2], double (&o)[NC][2], bo
     o[m][1] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m] : 0.0) : r[m];
   }
 }
+// single cells per lane (p = 64 m + l): o = v at p-1 / p+1, the same
+// rotations with the neighbouring chunk's value at lanes 0 and 63
+template <int NC>
+__device__ inline void row_prev(const double (&v)[NC][1], double (&o)[NC][1], bool lane0) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRor1>(v[m][0]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) o[m][0] = lane0 ? (m > 0 ? r[m > 0 ? m - 1 : 0] : 0.0) : r[m];
+}
+template <int NC>
+__device__ inline void row_next(const double (&v)[NC][1], double (&o)[NC][1], bool lane63) {
+  double r[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) r[m] = dpp_f64<kWaveRol1>(v[m][0]);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) o[m][0] = lane63 ? (m + 1 < NC ? r[m + 1 < NC ? m + 1 : m] : 0.0) : r[m];
+}
 // o = v at p-2 (the previous lane's same element)
 template <int NC>
 __device__ inline void row_prev2(const double (&v)[NC][2], double (&o)[NC][2], bool lane0) {
