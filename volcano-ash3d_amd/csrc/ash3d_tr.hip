@@ -319,7 +319,8 @@ int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, c
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if (with_bc && (rc = set_bc1(x, bc_vel, state))) return rc;
   AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, state ? state : x->ts0());
-  a.zg = (zero_ghosts && !x->periodic) ? 1 : 0;
+  // (periodic x: the x ghosts are copies, never zeros)
+  a.zg = (zero_ghosts && !(x->periodic && dir == 0)) ? 1 : 0;
   if (state && out) a.cout = out;
   if (state && to_sink) {
     const long long face = std::max(x->n_yz(), std::max(x->n_xz(), x->n_xy()));
@@ -996,18 +997,28 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   // Set_BC(1) calls repeat velocity ghosts that cannot have changed, and
   // re-zero concentration ghosts that the sweeps never write; only the
   // periodic x copies change, so only a periodic grid repeats the pass.
-  // Without periodic x and with diffusion, the concentration part of the
-  // pass is not launched: the sweeps read the ghost cells along their
-  // lines as the zeros it would write, and the diffusion's Set_BC(2) and
-  // Neumann passes rewrite every ghost before anything else reads it.
-  const bool zg = use_diffusion && !x->periodic;
+  // With diffusion the concentration part of the pass is not launched:
+  // the sweeps read the ghost cells along their lines as the zeros it
+  // would write, and the diffusion's Set_BC(2) and Neumann passes rewrite
+  // every ghost before anything else reads it.
+  // With periodic x the x ghosts the x sweep reads are the periodic copies
+  // of the state it starts from: one pass over the x ghosts of the
+  // interior rows right before it (the Set_BC(2) rule there is the same
+  // copy) stands in for the full passes.
+  const bool zg = use_diffusion != 0;
   if ((rc = set_bc1(x, true, nullptr, !zg))) return rc;
   x->ghost_zero_pending = false;  // every ts0 ghost rewritten
   const int first = (itime % 2 == 0) ? 1 : 0;
-  const bool again = x->periodic != 0;
-  if ((rc = sweep_advect(x, first, dt, is_aloft, b, false, false, nullptr, nullptr, true, zg))) return rc;
-  if ((rc = sweep_advect(x, 1 - first, dt, is_aloft, b, again, false, nullptr, nullptr, true, zg))) return rc;
-  if ((rc = sweep_advect(x, 2, dt, is_aloft, b, again, false, nullptr, nullptr, true, zg))) return rc;
+  const bool again = x->periodic != 0 && !zg;
+  const int order[3] = {first, 1 - first, 2};
+  for (int q = 0; q < 3; ++q) {
+    const int dir = order[q];
+    if (zg && x->periodic && dir == 0) {
+      ProfScope ps(x, ASH3D_PROF_BC);
+      HIPCHK(launch_set_bc2_face(x->stream, x->d, 1, 0, x->ts0()));
+    }
+    if ((rc = sweep_advect(x, dir, dt, is_aloft, b, q > 0 && again, false, nullptr, nullptr, true, zg))) return rc;
+  }
   if (use_diffusion) return step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b);
   return 0;
 }
