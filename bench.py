@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X transport step on the single-forecast workload.
+"""Benchmark of the MI355X transport step on the large global workload.
 
-Workload (default): spherical lon-lat grid 0.25 deg, 360x240x50 cells,
-dz = 0.5 km, 15 grain bins (fall speeds log-spaced 0.01-5 m/s), zonal jet
-with seeded correlated perturbations, Kh = Kz = 500 m2/s, Crank-Nicolson,
-superbee, FP64.  One step = the reference's hot-path sequence
+Workload (default): periodic spherical lon-lat grid 0.25 deg, 1440x720x80
+cells from pole to pole, dz = 0.3 km, 20 grain bins (fall speeds
+log-spaced 0.005-8 m/s), two zonal jets with seeded correlated
+perturbations vanishing toward the poles, Kh = 2000 m2/s, Kz = 10 m2/s,
+Crank-Nicolson, superbee, FP64 (--config forecast: the 360x240x50x15
+single forecast).  One step = the reference's hot-path sequence
 (Set_BC(1); AdvectHorz; AdvectVert; Set_BC(2); DiffuseHorz; DiffuseVert)
 plus the driver glue, on device-resident state.
 
@@ -53,7 +55,10 @@ def step_bytes_per_bincell(st, use_diffusion):
     return per_cell / st.ns
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_forecast_360x240x50x15.md")
+def traffic_file(st):
+    """The committed FETCH_SIZE / WRITE_SIZE summary of this grid."""
+    name = {(1440, 720, 80, 20): "global", (360, 240, 50, 15): "forecast"}.get((st.nx, st.ny, st.nz, st.ns), "grid")
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_{name}_{st.nx}x{st.ny}x{st.nz}x{st.ns}.md")
 
 
 def measured_traffic(kind, st, tuning, line_solver, defaults):
@@ -63,9 +68,10 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
     diffusion sweeps (and tile shape for the partitioned solves), the same
     bins per thread for the streamed advection and row kernel for the x
     advection (tuned, else the library default); else None."""
-    if (st.nx, st.ny, st.nz, st.ns) != (360, 240, 50, 15) or not os.path.exists(TRAFFIC_FILE):
+    path = traffic_file(st)
+    if not os.path.exists(path):
         return None
-    text = open(TRAFFIC_FILE).read()
+    text = open(path).read()
     block = text.split("```json", 1)[1].split("```", 1)[0]
     rec = json.loads(block).get(kind)
     if rec is None:
@@ -99,23 +105,59 @@ def measured_traffic(kind, st, tuning, line_solver, defaults):
     return rec["traffic_bytes"]
 
 
-def cpu_baseline(st, use_diffusion, steps):
-    """Time the oracle (C port of the reference path, OpenMP rows in the
-    advection sweeps as in the reference) on a bounded sample."""
+def cpu_model():
+    """(CPU model name, logical CPUs this process may run on)."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return name, ncpu
+
+
+def cpu_baseline(make_state, use_diffusion, steps, steps_1t, sample):
+    """Time the oracle (a C port of the reference path: OpenMP rows in the
+    advection sweeps and serial diffusion, as in the reference) on a
+    bounded sample of the workload: 1 warm-up step, then `steps` timed
+    steps on every core this process may use and `steps_1t` timed steps on
+    one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    st = make_state()
+    model, ncpu = cpu_model()
+    oracle.set_num_threads(ncpu)
     o = oracle.Oracle(st)
-    o.step_glue()
-    o.transport_step(1, use_diffusion)  # warm-up
+    o.transport_step(1, use_diffusion)  # warm-up (first touch of every array)
     t0 = time.perf_counter()
     for it in range(2, 2 + steps):
         o.step_glue()
         o.transport_step(it, use_diffusion)
     dt = time.perf_counter() - t0
+    cores = oracle.num_threads()
     rate = st.ns * st.nx * st.ny * st.nz * steps / dt
-    return {"value": rate, "unit": "bin-cell-updates/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{steps} full transport steps (after 1 warm-up) of the same "
-                      f"{st.nx}x{st.ny}x{st.nz}x{st.ns} workload, {dt:.1f} s"}
+    oracle.set_num_threads(1)
+    t0 = time.perf_counter()
+    for it in range(2 + steps, 2 + steps + steps_1t):
+        o.step_glue()
+        o.transport_step(it, use_diffusion)
+    dt1 = time.perf_counter() - t0
+    oracle.set_num_threads(ncpu)
+    rate1 = st.ns * st.nx * st.ny * st.nz * steps_1t / dt1
+    return {"value": rate, "unit": "bin-cell-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "host_logical_cpus": ncpu,
+            "value_1thread": rate1,
+            "restates": "oracle/ash3d_oracle.c: Set_BC, AdvectHorz (advect_x/y), AdvectVert (advect_z), Set_BC(2), "
+                        "DiffuseHorz/DiffuseVert (diffCN_x/y/z with dgtsv) of src/Ash3d.F90:491-510, "
+                        "advection rows threaded with OpenMP and diffusion serial as in the reference",
+            "sample": f"{sample}: 1 warm-up step, then {steps} timed steps ({dt:.1f} s) on {cores} threads "
+                      f"and {steps_1t} timed step(s) ({dt1:.1f} s) on 1 thread"}
 
 
 def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="exact", tuning=None):
@@ -127,7 +169,7 @@ def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="
     adj = importlib.import_module("volcano-ash3d_amd.adjoint")
     import numpy as np
     st.concen_pd[...] = 0.0
-    dz = np.full(st.nz, 0.5)
+    dz = np.ascontiguousarray(st.dz_vec_pd[2:st.nz + 2])
     H = min(40, st.nz)
     frac = np.full(st.ns, 1.0 / st.ns)
     n_steps = periods * steps_per_period
@@ -253,25 +295,50 @@ def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
             "eager_ms_per_ensemble_step": 1e3 * t_eager / steps, "graph_ms_per_ensemble_step": 1e3 * t_graph / steps}
 
 
+WORKLOADS = {
+    "global": ("large global run: periodic lon-lat 0.25 deg 1440x720x80 (90S-90N), dz 0.3 km, 20 bins "
+               "(vf 0.005-8 m/s), two jets + seeded correlated noise vanishing at the poles, Kh=2000 m2/s, "
+               "Kz=10 m2/s, Crank-Nicolson, superbee, broad cloud at 20-24 km"),
+    "forecast": ("single forecast: lon-lat 0.25 deg 360x240x50, dz 0.5 km, 15 bins (vf 0.01-5 m/s), "
+                 "zonal jet + seeded perturbations, Kh=Kz=500 m2/s, Crank-Nicolson, superbee"),
+    "cartesian": "projected 100x100x30, 5 km, uniform wind, 1 bin, no diffusion, superbee",
+}
+
+
+def build_state(pkg, config, bins=None):
+    if config == "global":
+        return pkg.workloads.global_state(bins=bins), True
+    if config == "forecast":
+        st = pkg.workloads.config2_state()
+        if bins is not None:
+            st = pkg.workloads.config2_state(ns=len(bins))
+        return st, True
+    return pkg.workloads.config1_state(), False
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", choices=("forecast", "cartesian"), default="forecast")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", choices=tuple(WORKLOADS), default="global")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed CPU steps on all cores")
+    ap.add_argument("--cpu-steps-1t", type=int, default=1, help="timed CPU steps on one thread")
+    ap.add_argument("--cpu-bins", type=int, default=1,
+                    help="bins of the CPU sample on the full grid of the global workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-profile", action="store_true",
                     help="no HIP-event kernel timing (for hardware-counter runs); no per-kernel figures")
-    ap.add_argument("--ensemble-members", type=int, default=16,
+    ap.add_argument("--ensemble-members", type=int, default=0,
                     help="members of the ensemble leg (0: skip); each on its own context and stream")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-scale", type=float, default=1.0, help="horizontal size factor of the members")
     ap.add_argument("--line-solver", choices=("exact", "partitioned"), default="partitioned",
                     help="theta-scheme line solves: dgtsv order (bitwise the reference) or partitioned "
                          "(lanes of a wave per line; agrees to rounding)")
+    ap.add_argument("--adj-bins", type=int, default=2, help="bins of the adjoint leg's state (global workload)")
     ap.add_argument("--adj-periods", type=int, default=2)
     ap.add_argument("--adj-steps-per-period", type=int, default=10)
     args = ap.parse_args()
@@ -285,30 +352,25 @@ def main():
     tr = importlib.import_module("volcano-ash3d_amd.transport")
     tr.load()
     t0 = time.perf_counter()
-    if args.config == "forecast":
-        st = pkg.workloads.config2_state()
-        use_diffusion = True
-        workload = ("single forecast: lon-lat 0.25 deg 360x240x50, dz 0.5 km, 15 bins (vf 0.01-5 m/s), "
-                    "zonal jet + seeded perturbations, Kh=Kz=500 m2/s, Crank-Nicolson, superbee")
-    else:
-        st = pkg.workloads.config1_state()
-        use_diffusion = False
-        workload = "projected 100x100x30, 5 km, uniform wind, 1 bin, no diffusion, superbee"
-    st.concen_pd[..., 1] = st.concen_pd[..., 0]  # the glue copies ts1 into ts0 each step
-    cpu_state = None if args.no_cpu_baseline else st.copy()
-    adj_state = None if args.no_adjoint else st.copy()
+    st, use_diffusion = build_state(pkg, args.config)
+    workload = WORKLOADS[args.config]
     log(f"[bench] state built in {time.perf_counter() - t0:.1f} s, dt={st.dt} h, "
         f"{st.nbytes() / 1e9:.2f} GB host arrays")
 
+    t0 = time.perf_counter()
     T = tr.Transport(st)
     T.tune(line_solver=args.line_solver)
-    itime = 0
+    log(f"[bench] device context created (upload) in {time.perf_counter() - t0:.1f} s")
+    # ts0 holds the cloud, as after the driver glue: the first step runs
+    # without it, every later one is glue + step (src/Ash3d.F90:376-378, 491-510)
+    itime = 1
+    T.transport_step(itime, use_diffusion)
     for _ in range(args.warmup):
         itime += 1
         T.step_glue()
         T.transport_step(itime, use_diffusion)
     T.synchronize()
-    log(f"[bench] warm-up {args.warmup} steps done")
+    log(f"[bench] warm-up {args.warmup + 1} steps done")
     tuning = None
     if not args.no_tune:
         at = importlib.import_module("volcano-ash3d_amd.autotune")
@@ -332,7 +394,7 @@ def main():
         T.close()
         print(json.dumps({"steps": args.steps, "ms_per_step": 1e3 * wall / args.steps, "profile": None}))
         return
-    prof_steps = max(5, min(args.steps, 20))
+    prof_steps = max(4, min(args.steps, 10))
     T.profile(True)
     for _ in range(prof_steps):
         itime += 1
@@ -341,6 +403,9 @@ def main():
     T.synchronize()
     prof = T.profile_read()
     T.profile(False)
+    # the step's mass budget after the timed steps (device totals)
+    aloft, deposit, outflow = T.mass_totals()
+    negatives = None
 
     bincells = st.ns * st.nx * st.ny * st.nz
     rate = bincells * args.steps / wall
@@ -370,44 +435,67 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "avg_ms": dk["avg_ms"], "algorithmic_bytes": algorithmic_bytes(dominant, st),
-                "traffic_source": "profiles/hbm_traffic_forecast_360x240x50x15.md" if traffic else None}
+                "traffic_source": os.path.relpath(traffic_file(st), ROOT) if traffic else None}
     for k, v in kernels.items():
         t = measured_traffic(k, st, tuning, args.line_solver, defaults)
         if t:
             v["traffic_gbps"] = round(t / (v["avg_ms"] * 1e-3) / 1e9, 1)
     log("[bench] kernels: " + json.dumps(kernels))
-
     T.close()
+    del T
+    config = {"workload": workload, "nx": st.nx, "ny": st.ny, "nz": st.nz, "bins": st.ns,
+              "dt_h": st.dt, "diffusion": "crank-nicolson" if use_diffusion else "off",
+              "line_solver": args.line_solver if use_diffusion else None,
+              "winds": "static within the timed steps (the hourly refresh and per-step interpolation of "
+                       "MesoInterpolater are driver work outside the transport step)",
+              "parallelism": "single gpu",
+              "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults"}
+    budget = {"aloft_kg": float(aloft.sum()), "deposit_kg": float(deposit.sum()), "outflow_kg": float(outflow.sum())}
+    del st
+
     ensemble = None
-    if args.ensemble_members > 0 and not args.no_profile:
+    if args.ensemble_members > 0:
         log("[bench] ensemble leg ...")
         ensemble = ensemble_leg(pkg, tr, args.ensemble_members, args.ensemble_steps, args.ensemble_scale,
                                 args.line_solver)
         log("[bench] ensemble: " + json.dumps(ensemble))
     adjoint = None
-    if adj_state is not None:
+    if not args.no_adjoint:
         log("[bench] adjoint leg ...")
+        bins = None
+        if args.config == "global":
+            bins = list(range(0, 20, max(1, 20 // args.adj_bins)))[:args.adj_bins]
+        adj_state, _ = build_state(pkg, args.config, bins=bins)
         adjoint = adjoint_leg(pkg, adj_state, use_diffusion, args.adj_periods, args.adj_steps_per_period,
                               args.line_solver, tuning)
+        if bins is not None:
+            adjoint["bins"] = bins
+        del adj_state
         log("[bench] adjoint: " + json.dumps(adjoint))
     cpu = None
-    if cpu_state is not None:
+    if not args.no_cpu_baseline:
         log("[bench] timing the CPU oracle ...")
-        cpu = cpu_baseline(cpu_state, use_diffusion, args.cpu_steps)
-        log(f"[bench] cpu baseline {cpu['value']:.3e} bin-cell-updates/s on {cpu['cores']} threads")
+        if args.config == "global":
+            cbins = list(range(0, 20, max(1, 20 // args.cpu_bins)))[:args.cpu_bins]
+            sample = (f"the global workload's full 1440x720x80 grid with {len(cbins)} of its 20 bins "
+                      f"(bins {cbins})")
+            make = lambda: build_state(pkg, args.config, bins=cbins)[0]  # noqa: E731
+        else:
+            sample = f"the {args.config} workload"
+            make = lambda: build_state(pkg, args.config)[0]  # noqa: E731
+        cpu = cpu_baseline(make, use_diffusion, args.cpu_steps, args.cpu_steps_1t, sample)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} bin-cell-updates/s on {cpu['cores']} threads, "
+            f"{cpu['value_1thread']:.3e} on 1")
 
     out = {
         "metric": METRIC, "value": rate, "unit": "bin-cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": workload, "nx": st.nx, "ny": st.ny, "nz": st.nz, "bins": st.ns,
-                   "dt_h": st.dt, "diffusion": "crank-nicolson" if use_diffusion else "off",
-                   "line_solver": args.line_solver if use_diffusion else None,
-                   "parallelism": "single gpu",
-                   "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults"},
+        "config": config,
         "step_hbm": {"bytes_per_bincell": round(bpb, 2), "frac_of_peak": round(step_frac, 4)},
         "roofline": roofline,
         "kernels": kernels,
+        "mass_budget": budget,
         "adjoint": adjoint,
         "ensemble": ensemble,
         "cpu_baseline": cpu,

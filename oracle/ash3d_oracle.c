@@ -953,6 +953,16 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* threads of the OpenMP row loops from now on (the CPU baseline's
+ * one-thread figure) */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ---- wind time interpolation : src/MesoInterpolater.f90:203-230 -------- */
 /* v_pd = (real(last,8) + real(next - last,8) * frac) * MPS_2_KMPHR; the
  * difference is taken in FP32 (both operands are _sp arrays). */

@@ -151,6 +151,7 @@ void orc_dgtsv(int n, int nrhs, double *dl, double *d, double *du, double *b,
                int ldb, int *info);
 double orc_limiter_phi(int limiter, double theta);
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,8 @@ def load():
         lib.orc_limiter_phi.argtypes = [ctypes.c_int, ctypes.c_double]
         lib.orc_limiter_phi.restype = ctypes.c_double
         lib.orc_num_threads.restype = ctypes.c_int
+        lib.orc_set_num_threads.argtypes = [ctypes.c_int]
+        lib.orc_set_num_threads.restype = None
         M = ctypes.POINTER(_MetStep)
         lib.orc_met_interp.argtypes = [P, M, M, ctypes.c_double, ctypes.c_int]
         lib.orc_met_interp.restype = None
@@ -321,3 +323,7 @@ def limiter_phi(limiter: int, theta: float) -> float:
 
 def num_threads() -> int:
     return load().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    load().orc_set_num_threads(int(n))

@@ -127,7 +127,10 @@ def adjust_dt(st: TransportState, use_diffusion: bool, cfl: float = CFL,
         sy = np.minimum(st.sigma_ny_pd[2:nx + 2, 2:ny + 2, 2:nz + 2], st.sigma_ny_pd[2:nx + 2, 3:ny + 3, 2:nz + 2])
         vxmax_dx = (np.abs(st.vx_pd[I]) * sx / kap).max()
         vymax_dy = (np.abs(st.vy_pd[I]) * sy / kap).max()
-    vfmax = np.abs(st.vf_pd[2:nx + 2, 2:ny + 2, 2:nz + 2, :]).max(axis=3)
+    # max over bins one bin at a time (no temporary of the whole vf array)
+    vfmax = np.abs(st.vf_pd[2:nx + 2, 2:ny + 2, 2:nz + 2, 0])
+    for n in range(1, st.ns):
+        np.maximum(vfmax, np.abs(st.vf_pd[2:nx + 2, 2:ny + 2, 2:nz + 2, n]), out=vfmax)
     vzmax_dz = ((np.abs(st.vz_pd[I]) + vfmax) / dzp[None, None, :]).max()
     time_advect = 1.0 / max(vxmax_dx, vymax_dy, vzmax_dz)
     time_diffuse = DT_MAX
@@ -269,6 +272,86 @@ def ensemble_member(member: int, ns: int = 10, size_scale: float = 1.0, seed: in
     height_km = rng.uniform(8.0, 20.0)
     k = min(nz - 3.0, height_km / 0.5)
     gaussian_blob(st, (nx * 0.3, ny * 0.5, k), (nx * 0.08, ny * 0.1, nz * 0.12), 1.0e3)
+    return st
+
+
+def cloud_window(st: TransportState, center, widths, peak, cutoff: float = 6.0) -> None:
+    """A Gaussian cloud in ts0 (bin n scaled by 1 + 0.1 n), written only
+    inside +-cutoff widths of its centre (zero beyond), so that building
+    it costs the window, not the grid."""
+    nx, ny, nz = st.nx, st.ny, st.nz
+    lo = [max(1, int(math.floor(c - cutoff * w))) for c, w in zip(center, widths)]
+    hi = [min(m, int(math.ceil(c + cutoff * w))) for c, w, m in zip(center, widths, (nx, ny, nz))]
+    i = np.arange(lo[0], hi[0] + 1)[:, None, None]
+    j = np.arange(lo[1], hi[1] + 1)[None, :, None]
+    k = np.arange(lo[2], hi[2] + 1)[None, None, :]
+    g = np.exp(-0.5 * (((i - center[0]) / widths[0]) ** 2 + ((j - center[1]) / widths[1]) ** 2 +
+                       ((k - center[2]) / widths[2]) ** 2))
+    for n in range(st.ns):
+        st.concen_pd[lo[0] + 1:hi[0] + 2, lo[1] + 1:hi[1] + 2, lo[2] + 1:hi[2] + 2, n, 0] = peak * g * (1.0 + 0.1 * n)
+
+
+def global_jets_winds(st: TransportState, rng: np.random.Generator, jets=((40.0, 40.0), (30.0, -45.0)),
+                      jet_width_deg: float = 12.0, sigma_mps: float = 3.0, length_km: float = 200.0) -> None:
+    """Global wind: two zonal jets (peak m/s, latitude) of Gaussian
+    meridional shape times sin(pi z / 24 km), plus Gaussian-correlated
+    perturbations on u and v, all scaled by cos(latitude) so that the wind
+    vanishes toward the poles; w = 0.  The noise is correlated on a 45
+    degree-latitude cell width."""
+    nx, ny, nz = st.nx, st.ny, st.nz
+    zc = st.z_cc[2:nz + 2]
+    lat = st.lat_cc[2:ny + 2]
+    coslat = np.cos(lat * DEG2RAD)
+    dx45 = st.dy * math.cos(45.0 * DEG2RAD) * (st.dx_deg / st.dn_deg)
+    du = correlated_noise(nx, ny, dx45, st.dy, length_km, sigma_mps, rng)
+    dv = correlated_noise(nx, ny, dx45, st.dy, length_km, sigma_mps, rng)
+    prof = np.zeros(ny)
+    for peak, lat0 in jets:
+        prof += peak * np.exp(-((lat - lat0) / jet_width_deg) ** 2)
+    shape_z = np.sin(np.pi * zc / 24.0)
+    for kk in range(nz):
+        st.vx_pd[2:nx + 2, 2:ny + 2, kk + 2] = ((prof * shape_z[kk])[None, :] + du) * coslat[None, :] * MPS_2_KMPHR
+        st.vy_pd[2:nx + 2, 2:ny + 2, kk + 2] = dv * coslat[None, :] * MPS_2_KMPHR
+    st.vz_pd[...] = 0.0
+
+
+def global_state(seed: int = 5, ns: int = 20, nx: int = 1440, ny: int = 720, nz: int = 80,
+                 bins=None, pole_gap_deg: float = 0.25) -> TransportState:
+    """The large global run: a periodic 0.25 deg lon-lat grid of 1440x720
+    cells (rows of 179.5/720 deg from 89.75 S to 89.75 N), 80 levels of
+    0.3 km, 20 bins with |vf| log-spaced 0.005-8 m/s, two jets plus
+    correlated noise vanishing toward the poles, Kh = 2000 m2/s,
+    Kz = 10 m2/s, Crank-Nicolson, and a broad cloud aloft at 20-24 km (the
+    top of the grid), where a 6 h umbrella-style source puts it.
+
+    The rows stop pole_gap_deg short of each pole.  On a grid from exactly
+    90 S to 90 N the ghost row beyond each pole gets the negative of the
+    first row's volume from the spherical-shell formulas
+    (src/Calc_Mesh.f90:121-139), so the average volume at the polar
+    interface that diffCN_y and diff_y divide by
+    (src/Diffusion.F90:1052-1058) is exactly zero and the reference's y
+    diffusion returns NaN there.
+
+    A smaller grid of the same construction comes from nx, ny, nz; bins
+    selects a subset of the bins (indices into the ns log-spaced fall
+    speeds), e.g. for a CPU sample."""
+    vf_all = fall_velocities(ns, 0.005, 8.0)
+    vf = vf_all if bins is None else vf_all[list(bins)]
+    nb = len(vf)
+    st = TransportState(nx, ny, nz, nb, limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5)
+    de, dn = 360.0 / nx, (180.0 - 2.0 * pole_gap_deg) / ny
+    lonlat_mesh(st, -180.0, -90.0 + pole_gap_deg, de, dn, np.full(nz, 24.0 / nz))
+    st.is_periodic = True
+    st.dx_deg, st.dn_deg = de, dn
+    rng = np.random.default_rng(seed)
+    global_jets_winds(st, rng)
+    set_uniform_fall(st, vf)
+    set_diffusivity(st, 2000.0, 10.0)
+    st.use_diffusion = True
+    st.dt = adjust_dt(st, use_diffusion=True)
+    # the broad cloud: centred at 15 N, 40 x 15 degrees, at 22 km
+    cloud_window(st, (nx * 0.55, ny * (105.0 / 180.0), nz * (22.0 / 24.0)),
+                 (nx * (10.0 / 360.0), ny * (4.0 / 180.0), nz * (1.5 / 24.0)), 1.0e3)
     return st
 
 
