@@ -184,6 +184,21 @@ int ash3d_tr_source_column_add(ash3d_tr_ctx *ctx, int32_t ivent, int32_t jvent, 
  * the values are on the host. */
 int ash3d_tr_mass_totals(ash3d_tr_ctx *ctx, double *aloft, double *deposit, double *outflow);
 
+/* ash3d_tr_mass_totals without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and the same bits as ash3d_tr_mass_totals, with the
+ * partial sums in a fixed device array of the library (one per device,
+ * ASH3D_TR_NOALLOC_MAX_BINS * 128 * 3 doubles = 192 KiB, held until the
+ * process ends) and the result written to out, a device array of 3*ns
+ * doubles (aloft[0..ns), deposit[ns..2ns), outflow[2ns..3ns)), on the
+ * context's stream without waiting.  Contexts with more than
+ * ASH3D_TR_NOALLOC_MAX_BINS bins are rejected (ASH3D_TR_EINVAL).  Calls
+ * share their device's array: calls that overlap on one device (contexts
+ * on different streams) race; serialise them. */
+#define ASH3D_TR_NOALLOC_MAX_BINS 64
+int ash3d_tr_mass_totals_noalloc(ash3d_tr_ctx *ctx, double *out);
+
 /* Launch tuning (no counterpart in the reference): the number of grain
  * bins one thread carries through a sweep (1, 2, 4 or 8; 3 also for the
  * advection sweeps).  More bins share

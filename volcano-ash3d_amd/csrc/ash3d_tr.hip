@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -54,9 +55,44 @@ enum { TS1_EXPLICIT = 0, TS1_ZERO = 1, TS1_MIRROR = 2 };
 
 }  // namespace
 
+// Scratch of the allocation-free entry points (ash3d_tr_mass_totals_noalloc):
+// the per-workgroup partial sums of the mass totals, for at most
+// ASH3D_TR_NOALLOC_MAX_BINS bins.  Each device holds its own copy of this
+// array from the library's load until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in):
+// ash3d_tr_create resolves the device address once per device with
+// hipGetSymbolAddress and every use goes through that address.
+static constexpr long long kNoallocScratch = (long long)ASH3D_TR_NOALLOC_MAX_BINS * 128 * 3;
+static __device__ __align__(256) double scratch[kNoallocScratch];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `dev`, resolved on
+// the first call for that device.
+int resolve_scratch(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(ASH3D_TR_EINVAL, "device ordinal %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ASH3D_TR_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
 struct ash3d_tr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  double *scratch = nullptr;  // this device's address of the allocation-free scratch array
   Dims d{};
   int periodic = 0, zscaling = 0, limiter = ASH3D_LIM_SUPERBEE;
   ash3d_tr_desc host{};
@@ -713,6 +749,12 @@ int ash3d_tr_create(const ash3d_tr_desc *desc, ash3d_tr_ctx **out) {
     delete x;
     return fail(ASH3D_TR_EDEVICE, "cannot create a stream on device %d", desc->device);
   }
+  if ((rc = resolve_scratch(x->device, &x->scratch))) {
+    std::string msg = g_err;
+    ash3d_tr_destroy(x);
+    g_err = msg;
+    return rc;
+  }
   const Dims &d = x->d;
   if ((rc = dev_alloc(&x->c, 2 * x->n_conc())) || (rc = dev_alloc(&x->vx, d.ncell)) ||
       (rc = dev_alloc(&x->vy, d.ncell)) || (rc = dev_alloc(&x->vz, d.ncell)) || (rc = dev_alloc(&x->vh, d.ncell)) ||
@@ -1253,6 +1295,25 @@ int ash3d_tr_mass_totals(ash3d_tr_ctx *x, double *aloft, double *deposit, double
     if (deposit) deposit[n] = h[d.ns + n];
     if (outflow) outflow[n] = h[2 * d.ns + n];
   }
+  return 0;
+}
+
+int ash3d_tr_mass_totals_noalloc(ash3d_tr_ctx *x, double *out) {
+  int rc = check_ctx(x);
+  if (rc) return rc;
+  const Dims &d = x->d;
+  if (!out) return fail(ASH3D_TR_EINVAL, "null output");
+  if (d.ns > ASH3D_TR_NOALLOC_MAX_BINS)
+    return fail(ASH3D_TR_EINVAL, "ash3d_tr_mass_totals_noalloc handles at most %d bins (the context has %d)",
+                ASH3D_TR_NOALLOC_MAX_BINS, d.ns);
+  if (mass_partials_size(d) > kNoallocScratch)
+    return fail(ASH3D_TR_EINVAL, "mass-total partials of %lld doubles exceed the scratch of %lld",
+                mass_partials_size(d), kNoallocScratch);
+  if (!x->scratch) return fail(ASH3D_TR_EDEVICE, "allocation-free scratch not resolved on device %d", x->device);
+  // every partial is written by k_mass_partials before k_mass_final reads
+  // it on the same stream, so nothing needs clearing
+  HIPCHK(launch_mass_totals(x->stream, d, x->ts0(), x->kap, x->dep, x->ofyz1, x->ofyz2, x->ofxz1, x->ofxz2,
+                            x->ofxy2, x->scratch, out));
   return 0;
 }
 

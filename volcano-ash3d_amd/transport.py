@@ -139,6 +139,7 @@ _SIGS = {
     "ash3d_tr_step_glue": [ctypes.c_void_p],
     "ash3d_tr_source_column_add": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp],
     "ash3d_tr_mass_totals": [ctypes.c_void_p, _dp, _dp, _dp],
+    "ash3d_tr_mass_totals_noalloc": [ctypes.c_void_p, ctypes.c_void_p],
     "ash3d_tr_tune": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
@@ -393,6 +394,12 @@ class Transport:
         self._check(self.lib.ash3d_tr_mass_totals(self._ctx, a.ctypes.data_as(_dp), dpo.ctypes.data_as(_dp),
                                                   o.ctypes.data_as(_dp)))
         return a, dpo, o
+
+    def mass_totals_noalloc(self, out_ptr: int):
+        """ash3d_tr_mass_totals_noalloc: the per-bin totals into device
+        memory at out_ptr (3*ns doubles: aloft, deposit, outflow), queued on
+        the context's stream; the library allocates nothing."""
+        self._check(self.lib.ash3d_tr_mass_totals_noalloc(self._ctx, ctypes.c_void_p(int(out_ptr))))
 
     # -- captured steps ----------------------------------------------------
     def graph_capture(self, itime: int, use_diffusion: bool):
