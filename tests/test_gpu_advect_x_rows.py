@@ -3,9 +3,11 @@ one wavefront per row (the row in registers, as 16-byte cell pairs for
 even nx and single cells for odd nx, the neighbours by DPP rotations
 across chunks of 128 or 64 cells) and one workgroup per row (the row
 staged through LDS).  Row lengths sit at and around the chunk edges
-(nx + 4 = 64, 128, 256, 384 and the pair/single-cell limits); nx > 380
-(> 252 with topography) and more than 64 bins take the workgroup kernel
-whatever the setting."""
+(nx + 4 = 64, 128, 256, 384 and the pair/single-cell limits).  Longer
+even rows (nx > 380, > 252 with topography) take several wavefronts per
+row, a segment each with the halos exchanged through LDS, up to 8
+segments; odd ones, longer ones and more than 64 bins take the workgroup
+kernel whatever the setting."""
 import numpy as np
 import pytest
 
@@ -53,6 +55,34 @@ def test_x_rows_kernels(pkg, oracle, tr, nx, variant, xrows):
         st.bounds = (lo, max(lo, nx - 2), 2, 4, 2, 3)
         st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
     a, b = advect_x_both(pkg, oracle, tr, st, xrows)
+    assert_same(a, b)
+
+
+# segmented rows: 2 and 3 segments, the global grid's 1444-cell rows (4
+# segments; 6 with topography), a segment at its 380-cell cap, one cell
+# past it (5 segments), 8 segments and the first row past 8 (workgroup
+# kernel)
+NX_SEG = [382, 762, 1440, 1516, 1518, 3036, 3038]
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("nx", NX_SEG)
+def test_x_segmented_rows(pkg, oracle, tr, nx, variant):
+    ns = 5
+    st = pkg.workloads.random_state(nx, 3, 3, ns, seed=nx + 17 * len(variant), **VARIANTS[variant])
+    if variant == "subgrid_aloft":
+        st.bounds = (3, nx - 2, 2, 3, 2, 3)
+        st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
+    a, b = advect_x_both(pkg, oracle, tr, st, 1)
+    assert_same(a, b)
+
+
+@pytest.mark.parametrize("ns", [1, 2, 20, 64])
+def test_x_segmented_rows_bin_counts(pkg, oracle, tr, ns):
+    st = pkg.workloads.random_state(1440, 2, 3, ns, seed=100 + ns, latlon=True, periodic=True)
+    st.is_aloft = np.ones(ns, dtype=np.int32)
+    st.is_aloft[1::4] = 0 if ns > 2 else 1
+    a, b = advect_x_both(pkg, oracle, tr, st, 1)
     assert_same(a, b)
 
 

@@ -450,6 +450,278 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   }
 }
 
+
+// ---------------------------------------------------------------------
+// Rows longer than one wavefront's registers hold (nx even): WV waves per
+// row in one workgroup, wave w owning the row positions [S, S + L) with
+// S = w L (L even).  Each wave holds the extended range [S - 2, S + L + 2)
+// in the layout of k_advect_x_wave (NC chunks of 64 cell pairs, lane l of
+// chunk m: positions S - 2 + 128 m + 2 l + e), so every cell it stores has
+// its +-2 stencil in its own registers and the arithmetic is that kernel's.
+// The per-cell fields of the extended range are read from memory; the
+// concentrations of the two halo pairs are not, because the neighbouring
+// wave updates them in place: every wave writes the first and the last
+// pair of its own range to LDS (two slots, by bin parity) and reads its
+// halos from there after a barrier, one barrier per bin.  Global loads and
+// stores of concentrations touch only the wave's own range.
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(512) void k_advect_x_seg(AdvArgs A, int L) {
+  constexpr int E = 2, CS = 128;
+  auto ld_cells = [](const double *src, double (&out)[E]) {
+    const double2 v = *(const double2 *)src;
+    out[0] = v.x;
+    out[1] = v.y;
+  };
+  auto st_cells = [](double *dst, const double (&v)[E]) { *(double2 *)dst = make_double2(v[0], v[1]); };
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int nx = d.nx, W = nx + 4;
+  const int l = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6), WV = (int)(blockDim.x >> 6);
+  const long long row = blockIdx.x;
+  const int j = (int)(row % d.ny) + 1;
+  const int k = (int)(row / d.ny) + 1;
+  const int rmin = g.rmin, rmax = g.rmax;
+  const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
+  const long long base = d.p3(-1, j, k);  // cell -1 of the row
+  const double *c0 = A.c0;
+  double *co = A.cout;  // == c0 for the in-place sweep
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 1024)));
+  const bool lane0 = l == 0, lane63 = l == 63;
+  const int S = w * L, Send = min(S + L, W), X = S - 2;
+  // the halo pairs (positions S-2, S-1 and S+L, S+L+1) and the own edge
+  // pairs published to the neighbours (S, S+1 and S+L-2, S+L-1)
+  const bool has_lo = w > 0, has_hi = w + 1 < WV;
+  const int m_hh = (L + 2) / CS, l_hh = ((L + 2) % CS) / 2;  // upper halo pair
+  const int m_le = L / CS, l_le = (L % CS) / 2;              // own last pair
+
+  int cc[NC][E];
+  bool valid[NC][E], ld[NC][E], uin[NC][E], lq[NC][E];
+  int pf[NC], pq[NC];  // field and concentration offsets of the pair (clamped)
+  bool store[NC], halo_lo[NC], halo_hi[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int p0 = X + CS * m + 2 * l;
+    const bool own = p0 >= S && p0 < Send;
+    pf[m] = min(max(p0, 0), W - 2);
+    pq[m] = own ? p0 : S;
+    store[m] = own && p0 >= 2 && p0 + 1 <= nx + 1;
+    halo_lo[m] = has_lo && m == 0 && l == 0;
+    halo_hi[m] = has_hi && m == m_hh && l == l_hh;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int c = p0 + e - 1;
+      cc[m][e] = c;
+      valid[m][e] = p0 + e >= 0 && p0 + e < W && p0 + e < Send + 2;
+      ld[m][e] = valid[m][e] && c >= rmin - 2 && c <= rmax + 2;
+      uin[m][e] = valid[m][e] && c >= rmin - 1 && c <= rmax + 1;
+      lq[m][e] = ld[m][e] && (!A.zg || (c >= 1 && c <= nx));
+    }
+  }
+
+  if (!row_on) {
+    for (int n = 0; n < d.ns; ++n) {
+      const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (store[m]) {
+          const double z[E] = {};
+          st_cells(co + base + pq[m] + bofs, z);
+        }
+    }
+    return;
+  }
+
+  extern __shared__ double s_row[];
+  // wave-private row terms (4 * 2*NC values per lane), then the edge pairs:
+  // s_edge[((slot * WV + wave) * 2 + first/last) * 2 + e]
+  double *srow = s_row + (long long)w * (4 * E * NC * 64) + l;
+  double *s_edge = s_row + (long long)WV * (4 * E * NC * 64);
+  auto rt = [&](int which, int m, int e) -> double & { return srow[((which * NC + m) * E + e) * 64]; };
+  double tf[NC][E], D[NC][E], tfm1[NC][E], Dm1[NC][E];
+  {
+    double u[NC][E], dtv[NC][E];
+    double vel[NC][E], sig[NC][E], kap[NC][E];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const long long o = base + pf[m];
+      double vv[E], ss[E], kk[E], dd[E] = {};
+      ld_cells(A.vel + o, vv);
+      ld_cells(A.sig + o, ss);
+      ld_cells(A.kap + o, kk);
+      if (TOPO) ld_cells(A.deld + (long long)pf[m] + (long long)(nx + 4) * (j + 1), dd);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        vel[m][e] = ld[m][e] ? vv[e] : 0.0;
+        sig[m][e] = ld[m][e] ? ss[e] : 0.0;
+        kap[m][e] = ld[m][e] ? kk[e] : 1.0;
+        D[m][e] = (TOPO && uin[m][e]) ? dd[e] : 0.0;
+      }
+    }
+    double velm1[NC][E], sigp1[NC][E];
+    row_prev<NC>(vel, velm1, lane0);
+    if (TOPO) row_next<NC>(sig, sigp1, lane63);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        // usig (:159-162), dt/kappa (:155-156)
+        u[m][e] = uin[m][e] ? 0.5 * (velm1[m][e] + vel[m][e]) * sig[m][e] : 0.0;
+        dtv[m][e] = uin[m][e] ? A.dt / kap[m][e] : 0.0;
+        // topographic factor 0.5*vel_I*(sig_I+sig_{I+1}) (:252-254)
+        tf[m][e] = (TOPO && valid[m][e]) ? 0.5 * vel[m][e] * (sig[m][e] + sigp1[m][e]) : 0.0;
+      }
+    double um1[NC][E], up1[NC][E];
+    row_prev<NC>(u, um1, lane0);
+    row_next<NC>(u, up1, lane63);
+    if (TOPO) {
+      row_prev<NC>(tf, tfm1, lane0);
+      row_prev<NC>(D, Dm1, lane0);
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        // divergence corrections (:245-246)
+        rt(0, m, e) = u[m][e];
+        rt(1, m, e) = dtv[m][e];
+        rt(2, m, e) = (fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]));
+        rt(3, m, e) = (fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]));
+        if (!TOPO) tfm1[m][e] = Dm1[m][e] = 0.0;
+      }
+  }
+
+  // outflow increments (wave 0 owns cell 0, the last wave cell nx+1): lane
+  // n keeps those of bin n (at most 64 bins)
+  const bool of_lo = rmin == 1 && !A.periodic && w == 0;
+  const bool of_hi = rmax == nx && !A.periodic && w == WV - 1;
+  const int o_hi = nx + 2 - X;  // extended offset of cell nx+1 (element 0) in the last wave
+  const int m_hi = o_hi / CS, l_hi = (o_hi % CS) / E;
+  double acc_lo = 0.0, acc_hi = 0.0;
+
+  const int lim = A.limiter;
+  double qa[NC][E], qb[NC][E];
+  auto fetch = [&](double (&qs)[NC][E], int n) {
+    const int nn = (n < d.ns && A.aloft.on(n)) ? n : max(0, min(n - 2, d.ns - 1));
+    const long long bofs = d.ncell * (long long)nn;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) ld_cells(c0 + base + pq[m] + bofs, qs[m]);
+  };
+  auto bin = [&](double (&qs)[NC][E], int n) {
+    const bool on = A.aloft.on(n);
+    double *edge = s_edge + (long long)(n & 1) * WV * 4;
+    // publish the own first and last pairs, then take the halos
+    if (l == 1) {
+      edge[(w * 2 + 0) * 2 + 0] = qs[0][0];
+      edge[(w * 2 + 0) * 2 + 1] = qs[0][1];
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+      if (m == m_le && l == l_le) {
+        edge[(w * 2 + 1) * 2 + 0] = qs[m][0];
+        edge[(w * 2 + 1) * 2 + 1] = qs[m][1];
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    double hlo[E] = {}, hhi[E] = {};
+    if (has_lo) {
+      hlo[0] = edge[((w - 1) * 2 + 1) * 2 + 0];
+      hlo[1] = edge[((w - 1) * 2 + 1) * 2 + 1];
+    }
+    if (has_hi) {
+      hhi[0] = edge[((w + 1) * 2 + 0) * 2 + 0];
+      hhi[1] = edge[((w + 1) * 2 + 0) * 2 + 1];
+    }
+    double q[NC][E];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double v = halo_lo[m] ? hlo[e] : (halo_hi[m] ? hhi[e] : qs[m][e]);
+        q[m][e] = lq[m][e] ? v : 0.0;  // gather (:143)
+      }
+    fetch(qs, n + 2);
+    // dq (:168-169)
+    double qm1[NC][E], dq[NC][E];
+    row_prev<NC>(q, qm1, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) dq[m][e] = valid[m][e] ? q[m][e] - qm1[m][e] : 0.0;
+    double dqm[NC][E], dqp[NC][E];
+    row_prev<NC>(dq, dqm, lane0);
+    row_next<NC>(dq, dqp, lane63);
+    // limited flux and fluctuations at interface I = c (:178-259)
+    double fss[NC][E], fsr[NC][E], fsl[NC][E];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = cc[m][e];
+        fss[m][e] = 0.0; fsr[m][e] = 0.0; fsl[m][e] = 0.0;
+        if (lim != 0 && valid[m][e] && c >= rmin && c <= rmax + 1)
+          fss[m][e] = limited_flux(lim, dqm[m][e], dq[m][e], dqp[m][e], rt(0, m, e), rt(1, m, e));
+        if (uin[m][e]) {
+          const double uu = rt(0, m, e);
+          double r = fmax(0.0, uu) * dq[m][e];
+          double t = fmin(0.0, uu) * dq[m][e];
+          r = r + q[m][e] * rt(3, m, e);
+          t = t + qm1[m][e] * rt(2, m, e);
+          if (TOPO) {
+            r = r + tf[m][e] * q[m][e] * D[m][e];
+            t = t - tfm1[m][e] * qm1[m][e] * Dm1[m][e];
+          }
+          fsr[m][e] = r;
+          fsl[m][e] = t;
+        }
+      }
+    double fsln[NC][E], fssn[NC][E];
+    row_next<NC>(fsl, fsln, lane63);
+    row_next<NC>(fss, fssn, lane63);
+    // cell update (:264-281), interior write and outflow (:285-294)
+    double upd[NC][E], res[NC][E];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = cc[m][e];
+        upd[m][e] = uin[m][e] ? -rt(1, m, e) * (fsln[m][e] + fsr[m][e] + fssn[m][e] - fss[m][e]) : 0.0;
+        res[m][e] = (on && c >= rmin && c <= rmax) ? q[m][e] + upd[m][e] : 0.0;
+      }
+    const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      double *dst = store[m] ? co + base + pq[m] + bofs : junk;
+      st_cells(dst, res[m]);
+    }
+    if (on) {
+      // cell 0 (position 1): element 1 of lane 1, chunk 0 of wave 0; cell
+      // nx+1: element 0 of lane l_hi, chunk m_hi of the last wave
+      double uh = upd[0][0];
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (m == m_hi) uh = upd[m][0];
+      const double lo = lane_value(upd[0][1], 1);
+      const double hi = lane_value(uh, l_hi < 64 ? l_hi : 0);
+      if (l == n) { acc_lo = lo; acc_hi = hi; }
+    }
+  };
+  fetch(qa, 0);
+  fetch(qb, 1);
+  int n = 0;
+  for (; n + 1 < d.ns; n += 2) {
+    bin(qa, n);
+    bin(qb, n + 1);
+  }
+  if (n < d.ns) bin(qa, n);
+  // outflow (:285-294): bin n from lane n
+  if (l < d.ns && A.aloft.on(l)) {
+    const long long of = (long long)(j + 1) + (long long)(d.ny + 4) * ((long long)(k + 1) + (long long)(d.nz + 4) * l);
+    if (of_lo) A.of1[of] = A.of1[of] + acc_lo;
+    if (of_hi) A.of2[of] = A.of2[of] + acc_hi;
+  }
+}
+
 template <int CPT>
 hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt) {
   const size_t shmem = sizeof(double) * 6 * (size_t)(a.d.nx + 4);
@@ -481,6 +753,31 @@ hipError_t launch_wave(hipStream_t s, const AdvArgs &a) {
 
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// Segmented rows: WV = ceil(W / (128 NC - 4)) waves, L the even segment
+// length; the dynamic LDS holds each wave's row terms and the edge pairs.
+template <int NC>
+hipError_t launch_seg(hipStream_t s, const AdvArgs &a) {
+  const int W = a.d.nx + 4;
+  const int cap = 128 * NC - 4;
+  const int WV = (W + cap - 1) / cap;
+  if (WV > 8) return hipErrorInvalidValue;
+  int L = (W + WV - 1) / WV;
+  L += L & 1;
+  if (L > cap || (WV - 1) * L >= W) return hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * ((size_t)WV * 4 * 2 * NC * 64 + 2 * (size_t)WV * 4);
+  const void *kern = a.deld != nullptr ? (const void *)k_advect_x_seg<NC, true> : (const void *)k_advect_x_seg<NC, false>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid((unsigned)((long long)a.d.ny * a.d.nz));
+  if (a.deld != nullptr)
+    hipLaunchKernelGGL((k_advect_x_seg<NC, true>), grid, dim3(64 * WV), lds, s, a, L);
+  else
+    hipLaunchKernelGGL((k_advect_x_seg<NC, false>), grid, dim3(64 * WV), lds, s, a, L);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 // Rows of up to 8*256-4 cells; returns hipErrorInvalidValue beyond that so
@@ -501,6 +798,14 @@ hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
       case 2: return launch_wave<2, 2>(s, a);
       default: return launch_wave<3, 2>(s, a);
     }
+  }
+  // longer rows (nx even): several waves per row, a segment each
+  if (a.xrows == 1 && a.d.ns <= 64 && a.d.nx % 2 == 0 && !common && aligned16(a.c0) && aligned16(a.cout) &&
+      aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) && aligned16(a.junk) &&
+      (a.deld == nullptr || aligned16(a.deld))) {
+    hipError_t e = a.deld != nullptr ? launch_seg<2>(s, a) : launch_seg<3>(s, a);
+    if (e != hipErrorInvalidValue) return e;
+    (void)hipGetLastError();
   }
   if (common) {
     switch ((cells + 63) / 64) {  // single cells (odd nx)
