@@ -226,7 +226,14 @@ enum {
    * registers (nx <= 380, or nx <= 252 with ZScaling_ID 2, at most 64
    * bins; the adjoint for even nx only; other grids use 0), 0 one
    * workgroup per row staged through LDS */
-  ASH3D_TUNE_ADV_X_ROWS = 14
+  ASH3D_TUNE_ADV_X_ROWS = 14,
+  /* longest active line (3..2048 cells) that the partitioned solves take,
+   * for x lines (contiguous in memory; default 2048) and for y and z lines
+   * (strided; default 512); lines beyond 512 cells run over several waves
+   * per line (SPIKE coupling of the waves' parts), longer lines in dgtsv
+   * order */
+  ASH3D_TUNE_PART_MAX_LINE = 15,
+  ASH3D_TUNE_PART_MAX_LINE_STRIDED = 16
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

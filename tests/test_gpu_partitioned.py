@@ -52,16 +52,55 @@ def test_sweeps(pkg, oracle, tr, dirn, ns, periodic):
 
 
 @pytest.mark.parametrize("alt", [0, 1])
-@pytest.mark.parametrize("shape", [(400, 130, 70), (300, 300, 20), (200, 60, 150), (520, 40, 3)])
+@pytest.mark.parametrize("shape", [(400, 130, 70), (300, 300, 20), (200, 60, 150), (520, 40, 3),
+                                   (40, 30, 80), (36, 24, 90)])
 def test_tile_shapes(pkg, oracle, tr, shape, alt):
-    """x lines of 400 / 300 / 200 / 520 cells (the last beyond 512: dgtsv
-    order), y lines of 130 / 300 / 60 / 40, z lines of 70 / 20 / 150 / 3,
-    in both tile shapes of every line-length class."""
+    """x lines of 400 / 300 / 200 / 520 cells (the last beyond 512: three
+    waves per line), y lines of 130 / 300 / 60 / 40, z lines of 70 / 20 /
+    150 / 3 / 80 / 90 (5 and 6 rows per lane), in both tile shapes of
+    every line-length class."""
     nx, ny, nz = shape
     st = pkg.workloads.random_state(nx, ny, nz, 2, seed=sum(shape), latlon=True, use_cn=True)
     st.dt *= 20.0
     a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",), ("diffCN_z",)],
                     part_shape_x=alt, part_shape_y=alt, part_shape_z=alt)
+    assert_same(a, b, tol=TOL)
+
+
+# lines longer than one wave (several waves per line, SPIKE coupling):
+# 3, 4, 6 and 8 parts, contiguous (x) and strided (y) lines, the first
+# line length past each part count, subgrid ranges, theta 0.5 and 1
+LONG = [(600, 4, 3), (769, 3, 2), (1000, 2, 2), (1440, 2, 2), (1537, 2, 2), (2048, 2, 2)]
+
+
+@pytest.mark.parametrize("nx,ny,nz", LONG)
+def test_long_x_lines(pkg, oracle, tr, nx, ny, nz):
+    st = pkg.workloads.random_state(nx, ny, nz, 3, seed=nx, latlon=True, periodic=nx % 2 == 0, use_cn=True)
+    st.dt *= 20.0
+    a, b = run_both(pkg, oracle, tr, st, [("Set_BC", 2), ("diffCN_x",)])
+    assert_same(a, b, tol=TOL)
+
+
+@pytest.mark.parametrize("ny", [513, 720, 769, 1024, 1500, 2048])
+@pytest.mark.parametrize("strided_max", [512, 2048])
+def test_long_y_lines(pkg, oracle, tr, ny, strided_max):
+    """Long strided lines: dgtsv order (the default) and several waves per
+    line when the partitioned solves are allowed to take them."""
+    st = pkg.workloads.random_state(9, ny, 2, 3, seed=ny, latlon=True, use_cn=True)
+    st.dt *= 20.0
+    a, b = run_both(pkg, oracle, tr, st, [("Set_BC", 2), ("diffCN_y",)], part_max_line_strided=strided_max)
+    assert_same(a, b, tol=TOL)
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0])
+def test_long_lines_subgrid_and_theta(pkg, oracle, tr, theta):
+    """Active ranges of 700 and 610 cells inside 900- and 800-cell lines,
+    bins not aloft, backward Euler."""
+    st = pkg.workloads.random_state(900, 800, 2, 3, seed=11, latlon=True, imp_fac=theta)
+    st.is_aloft = np.array([1, 0, 1], dtype=np.int32)
+    st.bounds = (101, 800, 95, 704, 1, 2)
+    st.dt *= 10.0
+    a, b = run_both(pkg, oracle, tr, st, [("diffCN_x",), ("diffCN_y",)], part_max_line_strided=2048)
     assert_same(a, b, tol=TOL)
 
 

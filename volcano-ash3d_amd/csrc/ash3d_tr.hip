@@ -194,6 +194,12 @@ struct ash3d_tr_ctx {
   // tile shape of the partitioned solves, by direction (measured on the
   // 360x240x50 forecast grid: 2-line x workgroups, 16-line y tiles)
   int part_shape[3] = {1, 1, 0};
+  // longest active line the partitioned solves take (longer lines: dgtsv
+  // order); lines beyond 512 cells span several waves.  Strided lines
+  // (y, z) default to 512: their multi-wave tiles hold only 1-2 lines, too
+  // few for coalesced rows (720-cell y lines: 17.2 ms against 13.6 ms in
+  // dgtsv order on the 1440x720x80x20 global grid)
+  int part_max_line = 2048, part_max_line_strided = 512;
   // streamed advection: bin groups per workgroup (0: line segments); the
   // z default is the autotuner's choice on the 360x240x50x15 forecast grid
   int adv_gy[3] = {0, 0, 4};
@@ -495,7 +501,9 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   *rc = 0;
   const int n = a.g.rmax - a.g.rmin + 1;
   a.sel = 0;
-  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > 512) return 0;
+  // (x lines are contiguous in memory; y and z lines strided)
+  const int nmax = adjoint ? 512 : (dir == 0 ? x->part_max_line : x->part_max_line_strided);
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > nmax) return 0;
   const int hn = host_ndom(x, dir);
   if (hn > 0) return 0;
   const Dims &o = x->d;
@@ -1220,6 +1228,11 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
   if (param >= ASH3D_TUNE_PART_SHAPE_X && param <= ASH3D_TUNE_PART_SHAPE_Z) {
     if (value != 0 && value != 1) return fail(ASH3D_TR_EINVAL, "tile shape must be 0 or 1 (got %d)", value);
     x->part_shape[param - ASH3D_TUNE_PART_SHAPE_X] = value;
+    return 0;
+  }
+  if (param == ASH3D_TUNE_PART_MAX_LINE || param == ASH3D_TUNE_PART_MAX_LINE_STRIDED) {
+    if (value < 3 || value > 2048) return fail(ASH3D_TR_EINVAL, "longest partitioned line must be 3..2048 (got %d)", value);
+    (param == ASH3D_TUNE_PART_MAX_LINE ? x->part_max_line : x->part_max_line_strided) = value;
     return 0;
   }
   if (param == ASH3D_TUNE_PART_BINS) {

@@ -25,6 +25,20 @@
 // (coalesced both ways) while the next bin's tile is already in flight.
 // Sweeps with any line that is not a diagonally dominant M-matrix (negative
 // diffusivities, non-finite factors) are left to the dgtsv-order kernels.
+//
+// Lines longer than one wave's 64 x M rows (WV > 1) are split into WV
+// parts, one wave each, and solved by the SPIKE algorithm (Polizzi &
+// Sameh, Parallel Computing 32, 2006) on top of the scheme above: every
+// part drops its coupling to the neighbouring parts (the sub-diagonal
+// entry alpha of its first row, the super-diagonal entry gamma of its
+// last row) and solves its own system A_w y = d with the partitioned
+// solve; the spikes v = A_w^{-1} e_first and w = A_w^{-1} e_last depend on
+// the matrix only and are formed once per line.  Then
+//   x = y - alpha * x_last(previous part) * v - gamma * x_first(next part) * w,
+// and the 2 WV end values x_first, x_last of the parts solve a small
+// system R z = (y_first, y_last, ...) whose inverse is formed once per
+// line; per bin the parts exchange their y end values through LDS (one
+// barrier) and each lane corrects its rows.
 #include <type_traits>
 
 #include "common.hpp"
@@ -62,13 +76,16 @@ struct Lev {
   static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
 };
 
-template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ>
-__global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
-  constexpr int NT = T * LP;
-  constexpr int NPAD = LP * M;
+template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ, int WV>
+__global__ __launch_bounds__(T *LP *WV) void k_cn_part(DiffArgs A, PartArgs Q) {
+  constexpr int LPL = LP * WV;  // lanes per line
+  constexpr int NT = T * LPL;
+  constexpr int NPAD = LPL * M;
   constexpr int PITCH = NPAD + 1;
   constexpr int LEV = Lev<LP>::value;
+  constexpr int NR = 2 * WV;  // unknowns of the parts' reduced system
   static_assert(PD == 1 || PD == 2, "one or two bins in flight");
+  static_assert(WV == 1 || (LP == 64 && !ADJ), "several parts per line: whole waves, forward solves");
   extern __shared__ double s_tile[];  // T lines x PITCH
   // a sweep with lines that are not diagonally dominant M-matrix rows runs
   // the dgtsv-order kernel instead (launched after this one; uniform exit)
@@ -83,8 +100,10 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   const int b = (int)(tg.tile / tiles_a) + 1;
   const int a_first = (int)(tg.tile % tiles_a) * T + 1;
   const int tid = threadIdx.x;
-  const int ln = tid / LP;  // line of this lane within the tile
-  const int p = tid % LP;   // segment of this lane
+  const int ln = tid / LPL;  // line of this lane within the tile
+  const int p = tid % LPL;   // segment of this lane
+  const int pw = p % LP;     // segment within the part (the PCR group)
+  const int wv = p / LP;     // part of the line (wave)
   const int a = a_first + ln;
   const bool has_line = a <= g.Na;
   const int ac = has_line ? a : g.Na;  // an existing line for masked lanes' reads
@@ -107,6 +126,7 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   double ga[M];                 // backward: dh_k -= ga_k*dh_{k+1}
   double ah[M], ch[M];          // final rows: ah_k*x_first + x_k + ch_k*x_last = dh_k
   double rho;
+  double cpl_a = 0.0, cpl_c = 0.0;  // dropped couplings (first lane / last lane of a part)
   // adjoint: kappa of each row and its inverse, in LDS after the tile
   // (row k of this lane at [k*NT + tid], its inverse at [(M+k)*NT + tid])
   double *s_kap = s_tile + T * PITCH;
@@ -132,6 +152,11 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
       } else {
         av[k] = 0.0; bv[k] = 1.0; cv[k] = 0.0;
         rL[k] = rR[k] = 0.0;
+      }
+      if (WV > 1) {
+        // each part's own system: drop the couplings to the other parts
+        if (k == 0 && pw == 0) { cpl_a = av[0]; av[0] = 0.0; }
+        if (k == M - 1 && pw == LP - 1) { cpl_c = cv[M - 1]; cv[M - 1] = 0.0; }
       }
       if (ADJ) {
         const double kv = Q.kap[Q.st0 + Q.sta * ac + Q.stb * b + Q.str * (long long)(rmin + (real ? i0 : 0))];
@@ -180,14 +205,14 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
         // equation 2p+1: 2p (own eq 0) and 2p+2 (lane p+1, eq 0)
         const double uA = __shfl_up(RA[1], 1, LP), uB = __shfl_up(RB[1], 1, LP), uC = __shfl_up(RC[1], 1, LP);
         const double dA = __shfl_down(RA[0], 1, LP), dB = __shfl_down(RB[0], 1, LP), dC = __shfl_down(RC[0], 1, LP);
-        const bool lo = p >= 1, hi = p + 1 < LP;
+        const bool lo = pw >= 1, hi = pw + 1 < LP;
         mA[0] = lo ? uA : 0.0; mB[0] = lo ? uB : 1.0; mC[0] = lo ? uC : 0.0;
         pA[0] = RA[1]; pB[0] = RB[1]; pC[0] = RC[1];
         mA[1] = RA[0]; mB[1] = RB[0]; mC[1] = RC[0];
         pA[1] = hi ? dA : 0.0; pB[1] = hi ? dB : 1.0; pC[1] = hi ? dC : 0.0;
       } else {
         const int h = 1 << (lev - 1);
-        const bool lo = p >= h, hi = p + h < LP;
+        const bool lo = pw >= h, hi = pw + h < LP;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           const double uA = __shfl_up(RA[j], h, LP), uB = __shfl_up(RB[j], h, LP), uC = __shfl_up(RC[j], h, LP);
@@ -249,20 +274,13 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
 
   // ---- solve this lane's segment of its line from the tile in LDS and
   // leave the solution there
-  auto solve = [&]() {
-    double q[M + 2];
-#pragma unroll
-    for (int k = 0; k < M + 2; ++k) {
-      const int i = p * M + k - 1;
-      q[k] = trow[min(max(i, 0), n - 1)];
-    }
+  // the solve of this part's system for the right-hand side d of the
+  // lane's rows (segment elimination, PCR of the segment ends, interior)
+  auto local = [&](const double (&d)[M], double (&xs)[M]) {
     double dh[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-      // forward: the explicit half-step B q; adjoint: K^{-1} lambda
-      const double d = ADJ ? q[k + 1] * s_kap[(M + k) * NT + tid]
-                           : (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
-      dh[k] = al[k] * d;
+      dh[k] = al[k] * d[k];
       if (k >= 2) dh[k] = dh[k] - be[k] * dh[k > 0 ? k - 1 : 0];
     }
 #pragma unroll
@@ -277,17 +295,17 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
         double uD, dD, t;
         lane_updown<LP, 1>(D[1], uD, t);
         lane_updown<LP, 1>(D[0], t, dD);
-        mD[0] = p >= 1 ? uD : 0.0;
+        mD[0] = pw >= 1 ? uD : 0.0;
         pD[0] = D[1];
         mD[1] = D[0];
-        pD[1] = p + 1 < LP ? dD : 0.0;
+        pD[1] = pw + 1 < LP ? dD : 0.0;
       } else {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           double uD, dD;
           lane_updown<LP, h>(D[j], uD, dD);
-          mD[j] = p >= h ? uD : 0.0;
-          pD[j] = p + h < LP ? dD : 0.0;
+          mD[j] = pw >= h ? uD : 0.0;
+          pD[j] = pw + h < LP ? dD : 0.0;
         }
       }
 #pragma unroll
@@ -301,9 +319,118 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
     if constexpr (LEV > 5) level(std::integral_constant<int, 5>{});
     if constexpr (LEV > 6) level(std::integral_constant<int, 6>{});
     const double xf = D[0] * invB[0], xl = D[1] * invB[1];
-    double xs[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) xs[k] = (k == 0) ? xf : (k == M - 1) ? xl : dh[k] - ah[k] * xf - ch[k] * xl;
+  };
+
+  // ---- SPIKE set-up (several parts per line): spikes of the lane's rows,
+  // the parts' couplings and spike ends in LDS, the inverse of the reduced
+  // system formed by the line's first lane (its rows for x_last of the
+  // previous part and x_first of the next one are read from LDS per bin)
+  double vS[M], wS[M];
+  double alpha_w = 0.0, gamma_w = 0.0;
+  double *s_cpl = s_tile + T * PITCH + (ADJ ? (size_t)2 * M * NT : 0);  // [T][WV][6]
+  double *s_Ri = s_cpl + T * WV * 6;                                     // [T][NR][NR]
+  double *s_Rm = s_Ri + T * NR * NR;                                     // [T][NR][NR]
+  double *s_y = s_Rm + T * NR * NR;                                      // [T][NR]
+  if constexpr (WV > 1) {
+    double e1[M], e2[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      e1[k] = (k == 0 && pw == 0) ? 1.0 : 0.0;
+      e2[k] = (k == M - 1 && pw == LP - 1) ? 1.0 : 0.0;
+    }
+    local(e1, vS);
+    local(e2, wS);
+    alpha_w = lane_value(cpl_a, 0);
+    gamma_w = lane_value(cpl_c, LP - 1);
+    double *cp = s_cpl + (ln * WV + wv) * 6;
+    if (pw == 0) { cp[0] = alpha_w; cp[1] = vS[0]; cp[2] = wS[0]; }
+    if (pw == LP - 1) { cp[3] = gamma_w; cp[4] = vS[M - 1]; cp[5] = wS[M - 1]; }
+    __syncthreads();
+    // R z = y with z = (x_first(0), x_last(0), x_first(1), ...): row 2u
+    // (first row of part u) and 2u+1 (its last row) couple x_last(u-1) =
+    // z[2u-1] through alpha_u * spike v and x_first(u+1) = z[2u+2] through
+    // gamma_u * spike w.  R^{-1} by Gauss-Jordan (R is I plus couplings
+    // below one in size for these dominant matrices), one thread of the
+    // line per entry of R and of R^{-1}, two barriers per column
+    double *Ri = s_Ri + ln * NR * NR, *Rm = s_Rm + ln * NR * NR;
+    const bool gj = p < NR * NR;
+    const int r_ = p / NR, c_ = p % NR;
+    double rm = 0.0, ri = 0.0;
+    if (gj) {
+      rm = ri = (r_ == c_) ? 1.0 : 0.0;
+      const int u = r_ / 2;
+      const double *cu = s_cpl + (ln * WV + u) * 6;
+      const bool first = (r_ % 2) == 0;
+      if (u > 0 && c_ == 2 * u - 1) rm = cu[0] * (first ? cu[1] : cu[4]);
+      if (u + 1 < WV && c_ == 2 * u + 2) rm = cu[3] * (first ? cu[2] : cu[5]);
+      Rm[p] = rm;
+      Ri[p] = ri;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int c0 = 0; c0 < NR; ++c0) {
+      double piv = 1.0, f = 0.0, pm = 0.0, pi = 0.0;
+      if (gj) {
+        piv = Rm[c0 * NR + c0];
+        f = Rm[r_ * NR + c0];
+        pm = Rm[c0 * NR + c_];
+        pi = Ri[c0 * NR + c_];
+      }
+      __syncthreads();
+      if (gj) {
+        const double nm = pm / piv, ni = pi / piv;
+        if (r_ == c0) {
+          rm = nm;
+          ri = ni;
+        } else {
+          rm = rm - f * nm;
+          ri = ri - f * ni;
+        }
+        Rm[p] = rm;
+        Ri[p] = ri;
+      }
+      __syncthreads();
+    }
+  }
+
+  auto solve = [&]() {
+    double q[M + 2];
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) {
+      const int i = p * M + k - 1;
+      q[k] = trow[min(max(i, 0), n - 1)];
+    }
+    double d[M], xs[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      // forward: the explicit half-step B q; adjoint: K^{-1} lambda
+      d[k] = ADJ ? q[k + 1] * s_kap[(M + k) * NT + tid]
+                 : (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
+    }
+    local(d, xs);
+    if constexpr (WV > 1) {
+      // every part's y end values, then the coupled correction
+      double *yv = s_y + ln * NR;
+      if (pw == 0) yv[2 * wv] = xs[0];
+      if (pw == LP - 1) yv[2 * wv + 1] = xs[M - 1];
+      __syncthreads();
+      // rows 2 wv - 1 and 2 wv + 2 of R^{-1} (zero rows at the line's ends)
+      const double *Ri = s_Ri + ln * NR * NR;
+      const double *rL_ = Ri + (wv > 0 ? 2 * wv - 1 : 0) * NR, *rF_ = Ri + (wv + 1 < WV ? 2 * wv + 2 : 0) * NR;
+      double xlp = 0.0, xfn = 0.0;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const double y = yv[c];
+        xlp = xlp + rL_[c] * y;
+        xfn = xfn + rF_[c] * y;
+      }
+      if (wv == 0) xlp = 0.0;
+      if (wv + 1 == WV) xfn = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) xs[k] = xs[k] - alpha_w * xlp * vS[k] - gamma_w * xfn * wS[k];
+    }
     if constexpr (ADJ) {
       // lambda' = K B u with u = A^{-1} K^{-1} lambda: B's stencil needs the
       // last row of the segment before and the first row of the one after;
@@ -439,28 +566,53 @@ __global__ __launch_bounds__(T *LP) void k_cn_part(DiffArgs A, PartArgs Q) {
   }
 }
 
-template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = (M <= 6 ? 2 : 1)>
+template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = (M <= 6 ? 2 : 1), int WV = 1>
 hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
   const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
-  // the tile, and for the adjoint each lane's row kappas and inverses
-  const size_t lds = sizeof(double) * ((size_t)T * (LP * M + 1) + (ADJ ? (size_t)2 * M * T * LP : 0));
+  // the tile, for the adjoint each lane's row kappas and inverses, for
+  // several parts per line the couplings, the reduced inverse and the end values
+  const size_t lds = sizeof(double) * ((size_t)T * (LP * WV * M + 1) + (ADJ ? (size_t)2 * M * T * LP * WV : 0) +
+                                       (size_t)T * WV * 6 + (size_t)T * 8 * WV * WV + (size_t)T * 2 * WV);
   if (lds > 65536) {
     // beyond the default dynamic LDS limit (the adjoint's 512-lane tiles)
-    hipError_t e = hipFuncSetAttribute((const void *)k_cn_part<LP, M, T, CONTIG, PD, ADJ>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_cn_part<LP, M, T, CONTIG, PD, ADJ, WV>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ>), dim3((unsigned)padded_tiles(tiles * G)), dim3(T * LP), lds, s, a,
-                     q);
+  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ, WV>), dim3((unsigned)padded_tiles(tiles * G)),
+                     dim3(T * LP * WV), lds, s, a, q);
   return hipGetLastError();
 }
+
+// bins in flight of the multi-wave line solves
+#ifndef ASH3D_PDL
+#define ASH3D_PDL 2
+#endif
+constexpr int PDL = ASH3D_PDL;
 
 template <bool ADJ>
 hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArgs &q, bool contig, int n, int shape) {
   // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
   // shape 1: contiguous lines two per workgroup (fewer waves held at each
   // barrier), strided lines in wider tiles (longer coalesced row segments)
+  if constexpr (!ADJ) {
+    // lines over several waves (SPIKE coupling of the parts)
+    // (4 rows per lane, 256 per wave; contiguous lines one per workgroup,
+    // strided lines in tiles of 2-4 along the contiguous index)
+    if (n > 512) {
+      if (contig) {
+        if (n <= 768) return launch_part<64, 4, 1, true, false, PDL, 3>(s, a, q);
+        if (n <= 1024) return launch_part<64, 4, 1, true, false, PDL, 4>(s, a, q);
+        if (n <= 1536) return launch_part<64, 4, 1, true, false, PDL, 6>(s, a, q);
+        return launch_part<64, 4, 1, true, false, PDL, 8>(s, a, q);
+      }
+      if (n <= 768) return launch_part<64, 4, 2, false, false, PDL, 3>(s, a, q);
+      if (n <= 1024) return launch_part<64, 4, 2, false, false, PDL, 4>(s, a, q);
+      if (n <= 1536) return launch_part<64, 4, 1, false, false, PDL, 6>(s, a, q);
+      return launch_part<64, 4, 1, false, false, PDL, 8>(s, a, q);
+    }
+  }
   if (contig) {
     if (shape == 1) {
       if (n <= 256) return launch_part<64, 4, 2, true, ADJ>(s, a, q);
@@ -471,13 +623,20 @@ hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArg
     if (n <= 384) return launch_part<64, 6, 4, true, ADJ>(s, a, q);
     return launch_part<64, 8, 4, true, ADJ>(s, a, q);
   }
+  // lines of 65..96 cells in 16 lanes of 5 or 6 rows (less padding than
+  // 32 lanes of 4: 80-cell columns 11.5 -> 6.5 ms a sweep on the
+  // 1440x720x80x20 global grid)
   if (shape == 1) {
     if (n <= 64) return launch_part<16, 4, 32, false, ADJ>(s, a, q);
+    if (n <= 80) return launch_part<16, 5, 16, false, ADJ>(s, a, q);
+    if (n <= 96) return launch_part<16, 6, 16, false, ADJ>(s, a, q);
     if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
     if (n <= 256) return launch_part<32, 8, 16, false, ADJ>(s, a, q);
     return launch_part<64, 8, 4, false, ADJ>(s, a, q);
   }
   if (n <= 64) return launch_part<16, 4, 16, false, ADJ>(s, a, q);
+  if (n <= 80) return launch_part<16, 5, 32, false, ADJ>(s, a, q);
+  if (n <= 96) return launch_part<16, 6, 32, false, ADJ>(s, a, q);
   if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
   if (n <= 256) return launch_part<64, 4, 8, false, ADJ>(s, a, q);
   return launch_part<64, 8, 4, false, ADJ>(s, a, q);
@@ -493,7 +652,7 @@ hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayou
   q.lr0 = lay.lr0; q.lra = lay.lra; q.lrb = lay.lrb; q.lrr = lay.lrr; q.lr_ncell = lay.lr_ncell;
   q.kap = lay.kap;
   q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
-  if (n < 3 || n > 512) return hipErrorInvalidValue;
+  if (n < 3 || n > (adjoint ? 512 : 2048)) return hipErrorInvalidValue;
   if (adjoint && !q.kap) return hipErrorInvalidValue;
   return adjoint ? launch_cn_part_shapes<true>(s, a, q, lay.str == 1, n, shape)
                  : launch_cn_part_shapes<false>(s, a, q, lay.str == 1, n, shape);

@@ -561,7 +561,8 @@ class Transport:
     # -- launch tuning ----------------------------------------------------
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z",
                    "x_tile", "line_solver", "part_bins", "part_shape_x", "part_shape_y", "part_shape_z",
-                   "adv_groups_y", "adv_groups_z", "adv_x_rows")
+                   "adv_groups_y", "adv_groups_z", "adv_x_rows", "part_max_line",
+                   "part_max_line_strided")
     LINE_SOLVERS = ("exact", "partitioned")
 
     def tune(self, **kw):
