@@ -453,3 +453,21 @@ def test_lonlat_shell_volume(pkg):
         vol = st.kappa_pd[2:38, 2:20, k + 2].sum()
         exp = 4.0 / 3.0 * np.pi * ((R + z[k + 1]) ** 3 - (R + z[k]) ** 3)
         np.testing.assert_allclose(vol, exp, rtol=1e-12)
+
+
+def test_pole_to_pole_grid_y_diffusion_is_nan(pkg, oracle):
+    """On a lon-lat grid from exactly 90 S to 90 N the ghost row beyond
+    each pole gets the negated volume of the first row
+    (src/Calc_Mesh.f90:121-139), so the average volume diffCN_y divides by
+    at the polar interface (src/Diffusion.F90:1052-1058) is zero and the
+    reference's y diffusion returns NaN in the polar rows; the global
+    workload's rows stop 0.25 deg short of the poles and stay finite."""
+    st = pkg.workloads.global_state(nx=8, ny=720, nz=2, ns=1, pole_gap_deg=0.0)
+    kap = st.kappa_pd[4, :, 3]
+    assert kap[1] == -kap[2] and kap[-2] == -kap[-3]  # ghost row 0 = -row 1 (and at the north pole)
+    oracle.Oracle(st).diffCN_y()
+    q = st.concen_pd[2:10, 2:722, 2:4, 0, 0]
+    assert np.isnan(q[:, 0]).all() and np.isnan(q[:, -1]).all()
+    st = pkg.workloads.global_state(nx=8, ny=720, nz=2, ns=1)
+    oracle.Oracle(st).diffCN_y()
+    assert np.isfinite(st.concen_pd).all()

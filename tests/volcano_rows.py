@@ -1,0 +1,60 @@
+"""Tiny hand-built rows and columns for the pins of the advection details
+(tests/test_oracle_pins.py on the oracle, tests/test_gpu_pins.py through
+the HIP library).  A backend is either the oracle module or a
+(transport module, "gpu") pair; each helper builds the state, runs one
+sweep and returns the interior cells and the outflows of the row."""
+import numpy as np
+
+from importlib import import_module
+
+_state = import_module("volcano-ash3d_amd.state")
+
+
+def _run(backend, st, sweep):
+    if isinstance(backend, tuple):
+        tr, _ = backend
+        T = tr.Transport(st)
+        getattr(T, sweep)()
+        T.sync_to_host()
+        T.close()
+    else:
+        getattr(backend.Oracle(st), sweep)()
+    return st
+
+
+def advect_x_row(backend, q, vel, sig, kap, dt, limiter, deld=None):
+    """advect_x of one row of N = len(q) cells (ny = nz = 1, one bin):
+    q and vel on cells 1..N (the velocity ghosts get Set_BC(1)'s edge
+    values), sig, kap and deld on cells -1..N+2.  Returns (q after the
+    sweep on cells 1..N, west outflow, east outflow)."""
+    N = len(q)
+    st = _state.TransportState(N, 1, 1, 1, limiter=limiter, zscaling_id=2 if deld is not None else 0)
+    st.dt = dt
+    st.vx_pd[...] = 0.0
+    st.vx_pd[2:N + 2, 2, 2] = vel
+    st.sigma_nx_pd[:, 2, 2] = sig
+    st.kappa_pd[:, 2, 2] = kap
+    if deld is not None:
+        st.deldxond_cc[:, 2] = deld
+    st.concen_pd[2:N + 2, 2, 2, 0, 0] = q
+    _run(backend, st, "advect_x")
+    return (st.concen_pd[2:N + 2, 2, 2, 0, 0].copy(), float(st.outflow_yz1_pd[2, 2, 0]),
+            float(st.outflow_yz2_pd[2, 2, 0]))
+
+
+def advect_z_column(backend, q, vf, sig, kap, dt, kmin, kmax, limiter):
+    """advect_z of one column of nz = len(q) cells (nx = ny = 1, one bin)
+    with active levels kmin..kmax, vz = vh = 0 and a uniform fall speed vf.
+    Returns (q after the sweep on cells 1..nz, bottom outflow (deposit),
+    top outflow)."""
+    nz = len(q)
+    st = _state.TransportState(1, 1, nz, 1, limiter=limiter)
+    st.dt = dt
+    st.vf_pd[...] = vf
+    st.sigma_nz_pd[...] = sig
+    st.kappa_pd[...] = kap
+    st.concen_pd[2, 2, 2:nz + 2, 0, 0] = q
+    st.bounds = (1, 1, 1, 1, kmin, kmax)
+    _run(backend, st, "advect_z")
+    return (st.concen_pd[2, 2, 2:nz + 2, 0, 0].copy(), float(st.outflow_xy1_pd[2, 2, 0]),
+            float(st.outflow_xy2_pd[2, 2, 0]))
