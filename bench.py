@@ -61,47 +61,33 @@ def traffic_file(st):
     return os.path.join(ROOT, "profiles", f"hbm_traffic_{name}_{st.nx}x{st.ny}x{st.nz}x{st.ns}.md")
 
 
-def measured_traffic(kind, st, tuning, line_solver, defaults):
+def effective_knobs(kind, st, tuning):
+    """The launch settings that decide which kernel a sweep kind runs and
+    how (from the context's settings after tuning)."""
+    if kind == "advect_x":
+        return {"adv_x_rows": tuning["adv_x_rows"]}
+    if kind in ("advect_y", "advect_z"):
+        return {"nb": tuning["nb_" + kind], "adv_groups": tuning["adv_groups_" + kind[-1]]}
+    d = kind[-1]
+    n = {"x": st.nx, "y": st.ny, "z": st.nz}[d]
+    nmax = tuning["part_max_line"] if d == "x" else tuning["part_max_line_strided"]
+    if tuning["line_solver"] == 1 and 3 <= n <= nmax:
+        return {"solver": "partitioned", "part_shape": tuning["part_shape_" + d], "part_bins": tuning["part_bins"]}
+    return {"solver": "dgtsv-order", "nb": tuning["nb_diffuse_" + d]} | ({"x_tile": tuning["x_tile"]} if d == "x" else {})
+
+
+def measured_traffic(kind, st, tuning):
     """HBM bytes per launch of a kernel kind from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
-    on this grid with the same kernel: the same line solver for the
-    diffusion sweeps (and tile shape for the partitioned solves), the same
-    bins per thread for the streamed advection and row kernel for the x
-    advection (tuned, else the library default); else None."""
+    FETCH_SIZE / WRITE_SIZE summary of this grid (copy-calibrated), when it
+    was measured with the same effective launch settings; else None."""
     path = traffic_file(st)
     if not os.path.exists(path):
         return None
     text = open(path).read()
     block = text.split("```json", 1)[1].split("```", 1)[0]
     rec = json.loads(block).get(kind)
-    if rec is None:
+    if rec is None or rec.get("knobs") != effective_knobs(kind, st, tuning):
         return None
-    if kind.startswith("diffuse"):
-        if rec.get("line_solver") != line_solver or (tuning or {}).get("part_bins", [0])[0] not in (0, None):
-            return None
-        knob = "part_shape_" + kind[-1]
-        shape = (tuning or {}).get(knob, [None])[0]
-        if shape is None:
-            shape = defaults.get(knob)
-        if line_solver == "partitioned" and rec.get("part_shape") != shape:
-            return None
-    if kind == "advect_x":
-        xr = (tuning or {}).get("adv_x_rows", [None])[0]
-        if xr is None:
-            xr = defaults.get("adv_x_rows")
-        if rec.get("x_rows", 0) != xr:
-            return None
-    if rec["bins_per_thread"] is not None:
-        nb = (tuning or {}).get("nb_" + kind, [None])[0]
-        if nb is None:
-            nb = defaults.get(kind)
-        if nb != rec["bins_per_thread"]:
-            return None
-        gy = (tuning or {}).get("adv_groups_" + kind[-1], [None])[0]
-        if gy is None:
-            gy = defaults.get("adv_groups_" + kind[-1])
-        if gy != rec.get("adv_groups", 0):
-            return None
     return rec["traffic_bytes"]
 
 
@@ -331,6 +317,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-profile", action="store_true",
                     help="no HIP-event kernel timing (for hardware-counter runs); no per-kernel figures")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="hardware-counter runs: start with one step glue, which copies one concentration level "
+                         "(the host's ts1, zeros) device to device, a copy of known size to scale FETCH_SIZE / "
+                         "WRITE_SIZE by; the timed steps then start from a zero state (same memory traffic)")
     ap.add_argument("--ensemble-members", type=int, default=0,
                     help="members of the ensemble leg (0: skip); each on its own context and stream")
     ap.add_argument("--ensemble-steps", type=int, default=10)
@@ -364,6 +354,9 @@ def main():
     # ts0 holds the cloud, as after the driver glue: the first step runs
     # without it, every later one is glue + step (src/Ash3d.F90:376-378, 491-510)
     itime = 1
+    if args.copy_calibration:
+        T.step_glue()
+        log(f"[bench] calibration copy: {8 * st.ns * (st.nx + 4) * (st.ny + 4) * (st.nz + 4)} bytes device to device")
     T.transport_step(itime, use_diffusion)
     for _ in range(args.warmup):
         itime += 1
@@ -428,18 +421,18 @@ def main():
                    key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dominant]
     achieved = dk["gbps"]
-    # the library's bins per thread and tile shapes on this grid
-    defaults = {"advect_y": 3, "advect_z": 2, "adv_groups_y": 0, "adv_groups_z": 4, "adv_x_rows": 1,
-                "part_shape_x": 1, "part_shape_y": 1, "part_shape_z": 0}
-    traffic = measured_traffic(dominant, st, tuning, args.line_solver, defaults)
+    # the launch settings in force (library defaults, or the autotuner's)
+    settings = T.tuning()
+    traffic = measured_traffic(dominant, st, settings)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "avg_ms": dk["avg_ms"], "algorithmic_bytes": algorithmic_bytes(dominant, st),
                 "traffic_source": os.path.relpath(traffic_file(st), ROOT) if traffic else None}
     for k, v in kernels.items():
-        t = measured_traffic(k, st, tuning, args.line_solver, defaults)
+        t = measured_traffic(k, st, settings) if algorithmic_bytes(k, st) else None
         if t:
             v["traffic_gbps"] = round(t / (v["avg_ms"] * 1e-3) / 1e9, 1)
+            v["traffic_over_algorithmic"] = round(t / algorithmic_bytes(k, st), 3)
     log("[bench] kernels: " + json.dumps(kernels))
     T.close()
     del T
@@ -449,7 +442,8 @@ def main():
               "winds": "static within the timed steps (the hourly refresh and per-step interpolation of "
                        "MesoInterpolater are driver work outside the transport step)",
               "parallelism": "single gpu",
-              "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults"}
+              "tuning": {k: v[0] for k, v in tuning.items()} if tuning else "defaults",
+              "settings": settings}
     budget = {"aloft_kg": float(aloft.sum()), "deposit_kg": float(deposit.sum()), "outflow_kg": float(outflow.sum())}
     del st
 

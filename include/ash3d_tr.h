@@ -243,6 +243,9 @@ enum {
  * so the results agree with dgtsv's to rounding (about 1e-14 relative). */
 enum { ASH3D_LINE_SOLVER_EXACT = 0, ASH3D_LINE_SOLVER_PARTITIONED = 1 };
 int ash3d_tr_tune(ash3d_tr_ctx *ctx, int32_t param, int32_t value);
+/* The current value of a launch setting (the library's default until
+ * ash3d_tr_tune changes it). */
+int ash3d_tr_tune_get(ash3d_tr_ctx *ctx, int32_t param, int32_t *value);
 
 /* Kernel timing.  No counterpart in the reference, whose only timers are
  * wall-clock reads around the whole run (src/Ash3d.F90:176-177, :727-732).

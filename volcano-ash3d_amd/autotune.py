@@ -1,10 +1,13 @@
 """Launch autotuner for the transport step.
 
-For each sweep kind it times every bins-per-thread setting (the number of
-grain bins one GPU thread carries along a line), and for the x-line solves
-the LDS tile width, with the library's HIP-event kernel timers, on the live context, and keeps the fastest.  The tuning
-steps advance the context's state like ordinary steps; results of a step do
-not depend on the setting (the kernels are bitwise identical across it).
+For each sweep kind it times the launch settings that do not change the
+results (bins one GPU thread carries along a line, bin groups per
+workgroup, row kernels, tile shapes of the partitioned line solves, the
+longest line those take) with the library's HIP-event kernel timers, on the
+live context.  The library's default is timed first and kept unless another
+setting is faster by more than `margin` (2 % by default), so that timing
+noise does not flip the choice.  The tuning steps advance the context's
+state like ordinary steps; results of a step do not depend on the setting.
 Choices can be cached in a JSON file keyed by grid shape and bin count.
 """
 from __future__ import annotations
@@ -24,6 +27,9 @@ ADV_X_ROWS = (1, 0)
 # the partitioned line solves have one setting for the three directions:
 # bins per workgroup (0: all bins of the tile's lines in one workgroup)
 PART_BINS = (0, 8, 4)
+# longest line of the partitioned solves: single-wave lines only (512) or
+# lines over several waves too (2048); only tried where a line is longer
+PART_MAX_LINE = (512, 2048)
 
 
 def _key(st):
@@ -31,10 +37,12 @@ def _key(st):
 
 
 def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8), itime0: int = 1000,
-             cache_path: str | None = None, line_solver: str = "exact") -> dict:
+             cache_path: str | None = None, line_solver: str = "exact", margin: float = 0.02) -> dict:
     """line_solver: the solver the context runs ("exact" tunes the
-    dgtsv-order kernels' bins per thread and x tile, "partitioned" the
-    bins per workgroup of the partitioned solves)."""
+    dgtsv-order kernels' bins per thread and x tile, "partitioned" also the
+    tile shapes, bins per workgroup and longest lines of the partitioned
+    solves; the dgtsv-order settings stay tuned there for the lines the
+    partitioned solves leave to them)."""
     st = T.st
     if cache_path and os.path.exists(cache_path):
         cached = json.load(open(cache_path)).get(_key(st))
@@ -43,10 +51,14 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             return {p: (v, None) for p, v in cached.items()}
     chosen = {}
     itime = itime0
-    params = [p for p in PARAMS if not (line_solver == "partitioned" and p[1].startswith("diffuse"))]
+    params = list(PARAMS)
     if line_solver == "partitioned":
         params += [("part_shape_x", "diffuse_x"), ("part_shape_y", "diffuse_y"), ("part_shape_z", "diffuse_z"),
                    ("part_bins", ("diffuse_x", "diffuse_y", "diffuse_z"))]
+        if st.nx > 512:
+            params.append(("part_max_line", "diffuse_x"))
+        if st.ny > 512 or st.nz > 512:
+            params.append(("part_max_line_strided", ("diffuse_y", "diffuse_z")))
     for param, kind in params:
         kinds = kind if isinstance(kind, tuple) else (kind,)
         if kinds[0].startswith("diffuse") and not use_diffusion:
@@ -54,7 +66,10 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
         values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
                   ADV_GROUPS if param.startswith("adv_groups") else
                   ADV_X_ROWS if param == "adv_x_rows" else
+                  PART_MAX_LINE if param.startswith("part_max_line") else
                   (0, 1) if param.startswith("part_shape") else candidates)
+        default = T.tune_get(param)
+        values = (default,) + tuple(v for v in values if v != default)
         times = {}
         for nb in values:
             if param.startswith("nb_") and nb > 1 and nb // 2 >= st.ns:
@@ -71,6 +86,8 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             T.profile(False)
             times[nb] = sum(prof[k][0] / max(prof[k][1], 1) for k in kinds)
         best = min(times, key=times.get)
+        if times[best] > (1.0 - margin) * times[default]:
+            best = default
         T.tune(**{param: best})
         chosen[param] = (best, times)
     if cache_path:

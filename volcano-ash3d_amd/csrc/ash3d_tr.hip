@@ -1260,6 +1260,29 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
   return 0;
 }
 
+int ash3d_tr_tune_get(ash3d_tr_ctx *x, int32_t param, int32_t *value) {
+  int rc = check_ctx_keep(x);
+  if (rc) return rc;
+  if (!value) return fail(ASH3D_TR_EINVAL, "null output");
+  switch (param) {
+    case ASH3D_TUNE_NB_ADVECT_X: case ASH3D_TUNE_NB_ADVECT_Y: case ASH3D_TUNE_NB_ADVECT_Z:
+      *value = x->nb_adv[param - ASH3D_TUNE_NB_ADVECT_X]; return 0;
+    case ASH3D_TUNE_NB_DIFFUSE_X: case ASH3D_TUNE_NB_DIFFUSE_Y: case ASH3D_TUNE_NB_DIFFUSE_Z:
+      *value = x->nb_diff[param - ASH3D_TUNE_NB_DIFFUSE_X]; return 0;
+    case ASH3D_TUNE_X_TILE: *value = x->x_tile; return 0;
+    case ASH3D_TUNE_LINE_SOLVER: *value = x->line_solver; return 0;
+    case ASH3D_TUNE_PART_BINS: *value = x->part_bins; return 0;
+    case ASH3D_TUNE_PART_SHAPE_X: case ASH3D_TUNE_PART_SHAPE_Y: case ASH3D_TUNE_PART_SHAPE_Z:
+      *value = x->part_shape[param - ASH3D_TUNE_PART_SHAPE_X]; return 0;
+    case ASH3D_TUNE_ADV_GROUPS_Y: *value = x->adv_gy[1]; return 0;
+    case ASH3D_TUNE_ADV_GROUPS_Z: *value = x->adv_gy[2]; return 0;
+    case ASH3D_TUNE_ADV_X_ROWS: *value = x->adv_xrows; return 0;
+    case ASH3D_TUNE_PART_MAX_LINE: *value = x->part_max_line; return 0;
+    case ASH3D_TUNE_PART_MAX_LINE_STRIDED: *value = x->part_max_line_strided; return 0;
+    default: return fail(ASH3D_TR_EINVAL, "unknown tuning parameter %d", param);
+  }
+}
+
 int ash3d_tr_profile(ash3d_tr_ctx *x, int32_t enable) {
   int rc = check_ctx(x);
   if (rc) return rc;

@@ -141,6 +141,7 @@ _SIGS = {
     "ash3d_tr_mass_totals": [ctypes.c_void_p, _dp, _dp, _dp],
     "ash3d_tr_mass_totals_noalloc": [ctypes.c_void_p, ctypes.c_void_p],
     "ash3d_tr_tune": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
+    "ash3d_tr_tune_get": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "ash3d_tr_profile": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_profile_read": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_int64)],
@@ -572,6 +573,16 @@ class Transport:
             if name == "line_solver" and isinstance(value, str):
                 value = self.LINE_SOLVERS.index(value)
             self._check(self.lib.ash3d_tr_tune(self._ctx, self.TUNE_PARAMS.index(name), int(value)))
+
+    def tune_get(self, name: str) -> int:
+        """The current value of a launch setting (see tune)."""
+        v = ctypes.c_int32()
+        self._check(self.lib.ash3d_tr_tune_get(self._ctx, self.TUNE_PARAMS.index(name), ctypes.byref(v)))
+        return int(v.value)
+
+    def tuning(self) -> dict:
+        """Every launch setting with its current value."""
+        return {name: self.tune_get(name) for name in self.TUNE_PARAMS}
 
     # -- kernel timing ----------------------------------------------------
     def profile(self, enable: bool):
