@@ -146,89 +146,88 @@ def cpu_baseline(make_state, use_diffusion, steps, steps_1t, sample):
                       f"and {steps_1t} timed step(s) ({dt1:.1f} s) on 1 thread"}
 
 
-def adjoint_leg(pkg, st, use_diffusion, periods, steps_per_period, line_solver="exact", tuning=None):
-    """Time a source-inversion gradient on the same grid: forward run with
-    checkpoints (and the kept advection-sweep inputs), then the checkpointed
-    reverse sweep.  Returns the adjoint/forward time ratio and the per-step
-    times of ash3d_tr_adj_step (sweep inputs recomputed),
-    ash3d_tr_adj_step_kept and ash3d_tr_step on resident states."""
-    adj = importlib.import_module("volcano-ash3d_amd.adjoint")
-    import numpy as np
-    st.concen_pd[...] = 0.0
-    dz = np.ascontiguousarray(st.dz_vec_pd[2:st.nz + 2])
-    H = min(40, st.nz)
-    frac = np.full(st.ns, 1.0 / st.ns)
-    n_steps = periods * steps_per_period
-    rng = np.random.default_rng(0)
-    s = 1e6 * rng.random((periods, H))
-
-    def timed(segment):
-        inv = adj.SourceInversion(st, st.nx // 4, st.ny // 2, n_heights=H, n_periods=periods,
-                                  steps_per_period=steps_per_period, frac=frac, dz=dz,
-                                  obs_every=steps_per_period // 2, use_diffusion=use_diffusion, segment=segment)
-        # the forward leg's line solver and launch settings
-        inv.T.tune(line_solver=line_solver, **{k: v[0] for k, v in (tuning or {}).items()})
-        w = np.random.default_rng(1).random((len(inv.obs_steps), st.nx, st.ny))
-        inv.forward(s)  # warm-up (allocations, first touch)
-        inv.gradient(w)
-        inv.T.synchronize()
+def timed_gradient(inv, s, w, reps=1):
+    """(forward seconds, gradient seconds, gradient) after one warm-up pair."""
+    inv.forward(s)
+    inv.gradient(w)
+    inv.T.synchronize()
+    t_f = t_g = 0.0
+    for _ in range(reps):
         t0 = time.perf_counter()
         inv.forward(s)
         inv.T.synchronize()
-        t_fwd = time.perf_counter() - t0
+        t_f += time.perf_counter() - t0
         t0 = time.perf_counter()
         g = inv.gradient(w)
         inv.T.synchronize()
-        return inv, g, t_fwd, time.perf_counter() - t0
+        t_g += time.perf_counter() - t0
+    return t_f / reps, t_g / reps, g
 
-    # square-root checkpointing (replays every segment but the last)
-    K_sqrt = max(1, int(round(n_steps ** 0.5)))
-    inv, _, f_rep, g_rep = timed(K_sqrt)
+
+def inversion_leg(pkg, line_solver, tuning, revolve_slots=40):
+    """The adjoint source inversion at its full size (240x160x50, 8 bins,
+    40 x 24 source table, 12 h = 552 steps, column loads every 30 min,
+    squared misfit): forward run and checkpointed gradient with the
+    segment plan of the default 64 GB device budget, and with binomial
+    checkpointing in revolve_slots slots."""
+    adj = importlib.import_module("volcano-ash3d_amd.adjoint")
+    import numpy as np
+    st, (iv, jv), spp, frac, dz, s_true = pkg.workloads.inversion_case()
+    P, H = s_true.shape
+    out = {"workload": f"source inversion 240x160x50, 8 bins, {H} heights x {P} periods of 30 min "
+                       f"({P * spp} steps of {st.dt * 3600:.1f} s, 12 h), column loads every 30 min, "
+                       "J = sum W (L - obs)^2 with 10 % noise", "line_solver": line_solver}
+    rng = np.random.default_rng(11)
+    for label, kw in (("segments", {}), ("revolve", {"schedule": "revolve", "revolve_slots": revolve_slots})):
+        inv = adj.SourceInversion(st, iv, jv, n_heights=H, n_periods=P, steps_per_period=spp, frac=frac, dz=dz,
+                                  obs_every=spp, use_diffusion=True, **kw)
+        inv.T.tune(line_solver=line_solver, **{k: v[0] for k, v in (tuning or {}).items()})
+        if label == "segments":
+            truth = inv.forward(s_true)
+            obs = truth * (1.0 + 0.1 * rng.standard_normal(truth.shape))
+            weight = 1.0 / (0.1 * truth.max()) ** 2
+            s0 = 0.6 * s_true + 0.3 * s_true.max() * rng.random(s_true.shape)
+            L = inv.forward(s0)
+            w = 2.0 * weight * (L - obs)
+        t_f, t_g, g = timed_gradient(inv, s0, w)
+        rec = {"forward_s": round(t_f, 4), "gradient_s": round(t_g, 4), "ratio": round(t_g / t_f, 3),
+               "checkpoint_states": inv.n_slots,
+               "state_gb": round(inv.state_bytes / 1e9, 3)}
+        if label == "segments":
+            rec.update(segments=inv.n_seg, segment_steps=inv.K, budget_gb=64)
+        else:
+            rec.update(free_slots=inv.S, recomputed_steps=inv.forward_steps)
+        out[label] = rec
+        inv.close()
+    return out
+
+
+def global_adjoint_leg(pkg, bins, n_steps, slots, line_solver, tuning):
+    """Binomial checkpointing on the global grid (a bin subset of its full
+    1440x720x80 grid): a source at 20-24 km over n_steps steps, loads every
+    n_steps/4 steps, reverse sweep in `slots` free slots."""
+    adj = importlib.import_module("volcano-ash3d_amd.adjoint")
+    import numpy as np
+    st = pkg.workloads.global_state(bins=bins)
+    st.concen_pd[...] = 0.0
+    dz = np.ascontiguousarray(st.dz_vec_pd[2:st.nz + 2])
+    P = 4
+    spp = n_steps // P
+    H = min(60, st.nz)
+    inv = adj.SourceInversion(st, st.nx // 2, st.ny // 2, n_heights=H, n_periods=P, steps_per_period=spp,
+                              frac=np.full(st.ns, 1.0 / st.ns), dz=dz, obs_every=spp, use_diffusion=True,
+                              schedule="revolve", revolve_slots=slots)
+    inv.T.tune(line_solver=line_solver, **{k: v[0] for k, v in (tuning or {}).items()})
+    rng = np.random.default_rng(0)
+    s = 1e6 * rng.random((P, H))
+    w = rng.random((len(inv.obs_steps), st.nx, st.ny))
+    t_f, t_g, g = timed_gradient(inv, s, w)
+    out = {"workload": f"global 1440x720x80 grid, bins {bins} of 20, {P * spp} steps, {H} source heights x {P} periods, "
+                       f"binomial checkpointing in {slots} free slots of {inv.state_bytes / 1e9:.2f} GB",
+           "forward_s": round(t_f, 4), "gradient_s": round(t_g, 4), "ratio": round(t_g / t_f, 3),
+           "recomputed_steps": inv.forward_steps, "steps": inv.n_steps}
     inv.close()
-    # default: segment length from the device-memory budget (all step inputs fit here)
-    inv, g, t_fwd, t_grad = timed(None)
-    # kernel-level: one adjoint step vs one forward step on a resident state
-    T = inv.T
-    T.profile(True)
-    reps = 5
-    T.checkpoint_load(0)
-    T.synchronize()
-    t0 = time.perf_counter()
-    for r in range(reps):
-        T.adj_step(1 + r, use_diffusion)
-    T.synchronize()
-    t_adj = (time.perf_counter() - t0) / reps
-    prof = T.profile_read()
-    T.profile(False)
-    # the adjoint step with the sweep inputs kept by the forward step (what
-    # the inversion's reverse sweep runs for every kept step)
-    sb, sz = inv._snaps(0)
-    T.checkpoint_load(0)
-    T.transport_step_keep(1, use_diffusion, sb, sz)
-    T.checkpoint_load(0)
-    T.synchronize()
-    t0 = time.perf_counter()
-    for r in range(reps):
-        T.adj_step_kept(1, use_diffusion, sb, sz)
-    T.synchronize()
-    t_adj_kept = (time.perf_counter() - t0) / reps
-    T.checkpoint_load(0)
-    T.synchronize()
-    t0 = time.perf_counter()
-    for r in range(reps):
-        T.transport_step(1 + r, use_diffusion)
-    T.synchronize()
-    t_step = (time.perf_counter() - t0) / reps
-    inv.close()
-    return {"workload": f"source inversion, {periods} periods x {steps_per_period} steps, {H} source heights, "
-                        f"column loads every {steps_per_period // 2} steps, every step input kept on the device",
-            "line_solver": line_solver,
-            "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4), "ratio": round(t_grad / t_fwd, 3),
-            "ratio_sqrt_checkpointing": round(g_rep / f_rep, 3), "sqrt_segment": K_sqrt,
-            "adj_step_ms": round(1e3 * t_adj, 3), "adj_step_kept_ms": round(1e3 * t_adj_kept, 3),
-            "step_ms": round(1e3 * t_step, 3), "step_ratio": round(t_adj / t_step, 3),
-            "step_ratio_kept": round(t_adj_kept / t_step, 3), "grad_norm": float(np.abs(g).max()),
-            "adj_step_kernels_ms": {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}}
+    return out
 
 
 def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
@@ -328,9 +327,9 @@ def main():
     ap.add_argument("--line-solver", choices=("exact", "partitioned"), default="partitioned",
                     help="theta-scheme line solves: dgtsv order (bitwise the reference) or partitioned "
                          "(lanes of a wave per line; agrees to rounding)")
-    ap.add_argument("--adj-bins", type=int, default=2, help="bins of the adjoint leg's state (global workload)")
-    ap.add_argument("--adj-periods", type=int, default=2)
-    ap.add_argument("--adj-steps-per-period", type=int, default=10)
+    ap.add_argument("--adj-bins", type=int, default=2, help="bins of the global adjoint leg's state")
+    ap.add_argument("--adj-steps", type=int, default=240, help="steps of the global adjoint leg")
+    ap.add_argument("--adj-slots", type=int, default=60, help="free checkpoint slots of the global adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transport step shards "
@@ -455,17 +454,14 @@ def main():
         log("[bench] ensemble: " + json.dumps(ensemble))
     adjoint = None
     if not args.no_adjoint:
-        log("[bench] adjoint leg ...")
-        bins = None
+        log("[bench] adjoint legs ...")
+        adjoint = {"inversion": inversion_leg(pkg, args.line_solver, tuning)}
+        log("[bench] inversion: " + json.dumps(adjoint["inversion"]))
         if args.config == "global":
             bins = list(range(0, 20, max(1, 20 // args.adj_bins)))[:args.adj_bins]
-        adj_state, _ = build_state(pkg, args.config, bins=bins)
-        adjoint = adjoint_leg(pkg, adj_state, use_diffusion, args.adj_periods, args.adj_steps_per_period,
-                              args.line_solver, tuning)
-        if bins is not None:
-            adjoint["bins"] = bins
-        del adj_state
-        log("[bench] adjoint: " + json.dumps(adjoint))
+            adjoint["global"] = global_adjoint_leg(pkg, bins, args.adj_steps, args.adj_slots, args.line_solver,
+                                                   tuning)
+            log("[bench] global adjoint: " + json.dumps(adjoint["global"]))
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU oracle ...")

@@ -1,0 +1,73 @@
+"""Checkpoint plans of the source inversion's reverse sweep (host logic,
+no device): the segment choice stays inside the device-memory budget, and
+the binomial schedule reverses every step in order from a state it holds,
+within its slots and with the recomputation count of the binomial bound."""
+import importlib
+import math
+
+import pytest
+
+adj = importlib.import_module("volcano-ash3d_amd.adjoint")
+
+
+def simulate(n, S):
+    """Replays revolve_actions: checks every reversed step's input is the
+    current state, saves only into slots 1..S, steps reversed n-1..0.
+    Returns the number of recomputed forward steps."""
+    slots, cur, done, fwd = {0: 0}, None, [], 0
+    for a in adj.revolve_actions(n, S):
+        if a[0] == "advance":
+            _, t0, t1, sl = a
+            assert slots[sl] == t0 and t1 > t0
+            cur, fwd = t1, fwd + t1 - t0
+        elif a[0] == "save":
+            assert 1 <= a[1] <= S and cur == a[2]
+            slots[a[1]] = a[2]
+        else:
+            _, t, sl = a
+            if sl is not None:
+                assert slots[sl] == t
+                cur = t
+            assert cur == t
+            done.append(t)
+    assert done == list(reversed(range(n)))
+    return fwd
+
+
+@pytest.mark.parametrize("n,S", [(1, 0), (2, 0), (7, 0), (10, 1), (10, 2), (100, 3), (540, 10), (540, 40),
+                                 (5760, 20), (5760, 100), (37, 5)])
+def test_revolve_schedule(n, S):
+    fwd = simulate(n, S)
+    if S == 0:
+        assert fwd == n * (n - 1) // 2  # every step recomputed from the start
+    else:
+        r = 0
+        while adj.beta(S, r) < n:
+            r += 1
+        # each step is advanced at most r times
+        assert fwd <= r * n
+        # ... and binomially few are: the count of the binomial bound
+        # r*n - beta(S+1, r-1) (Griewank & Walther, Proposition 1)
+        assert fwd <= r * n - math.comb(S + r, r - 1) + n
+
+
+@pytest.mark.parametrize("n_steps,per_step,budget_states", [(20, 3, 1000), (540, 3, 400), (540, 3, 60), (5760, 1, 50),
+                                                         (5760, 3, 120), (100, 3, 3)])
+def test_plan_stays_in_budget(n_steps, per_step, budget_states):
+    sb = 1.0e9
+    plan = adj.plan_checkpoints(n_steps, per_step, sb, budget_states * sb)
+    assert plan["n_slots"] <= budget_states
+    if plan["schedule"] == "segments":
+        assert plan["K"] * plan["n_seg"] >= n_steps
+    else:
+        assert plan["S"] >= 0
+
+
+def test_plan_store_all_and_errors():
+    assert adj.plan_checkpoints(20, 3, 1.0, 1000.0)["K"] == 20
+    assert adj.plan_checkpoints(20, 3, 1.0, 1000.0, segment=5)["n_seg"] == 4
+    assert adj.plan_checkpoints(20, 3, 1.0, 1000.0, schedule="revolve", revolve_slots=4)["n_slots"] == 6
+    with pytest.raises(ValueError, match="at least 2"):
+        adj.plan_checkpoints(100, 3, 1.0, 1.5)
+    with pytest.raises(ValueError, match="schedule"):
+        adj.plan_checkpoints(10, 1, 1.0, 100.0, schedule="x")

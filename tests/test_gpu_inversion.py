@@ -29,12 +29,12 @@ def adj(pkg):
     return importlib.import_module("volcano-ash3d_amd.adjoint")
 
 
-def make(pkg, adj, segment=None, use_cn=True, periodic=False, seed=7, keep=True):
+def make(pkg, adj, segment=None, use_cn=True, periodic=False, seed=7, keep=True, **kw):
     st = pkg.workloads.random_state(20, 16, 10, 3, seed=seed, latlon=True, periodic=periodic, use_cn=use_cn)
     st.concen_pd[...] = 0.0
     dz = 0.5 + 0.05 * np.arange(10)
     inv = adj.SourceInversion(st, 8, 7, n_heights=6, n_periods=3, steps_per_period=4, frac=[0.5, 0.3, 0.2],
-                              dz=dz, obs_every=2, use_diffusion=True, segment=segment, keep_snapshots=keep)
+                              dz=dz, obs_every=2, use_diffusion=True, segment=segment, keep_snapshots=keep, **kw)
     return inv
 
 
@@ -71,6 +71,27 @@ def test_gradient_independent_of_segment(pkg, adj):
         grads.append(inv.gradient(w))
         inv.close()
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def test_revolve_matches_segments(pkg, adj):
+    """Binomial checkpointing with 0, 1, 2 and 5 free slots gives the
+    segment schedule's loads and gradient bit for bit (the adjoint steps
+    see the same step inputs), and recomputes the forward steps the
+    schedule counts."""
+    rng = np.random.default_rng(8)
+    s = 1e4 * rng.random((3, 6))
+    ref = make(pkg, adj, segment=5, periodic=True, seed=9)
+    w = np.random.default_rng(10).random((len(ref.obs_steps), 20, 16))
+    L0, g0 = ref.forward(s), ref.gradient(w)
+    ref.close()
+    for S in (0, 1, 2, 5):
+        inv = make(pkg, adj, periodic=True, seed=9, schedule="revolve", revolve_slots=S)
+        assert inv.schedule == "revolve" and inv.n_slots == S + 2
+        L, g = inv.forward(s), inv.gradient(w)
+        fwd = inv.forward_steps
+        inv.close()
+        assert np.array_equal(L, L0) and np.array_equal(g, g0), S
+        assert fwd == sum(a[2] - a[1] for a in adj.revolve_actions(12, S) if a[0] == "advance")
 
 
 @pytest.mark.parametrize("periodic", [False, True])

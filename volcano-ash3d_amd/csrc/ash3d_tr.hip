@@ -1057,7 +1057,6 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   // copy) stands in for the full passes.
   const bool zg = use_diffusion != 0;
   if ((rc = set_bc1(x, true, nullptr, !zg))) return rc;
-  x->ghost_zero_pending = false;  // every ts0 ghost rewritten
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0 && !zg;
   const int order[3] = {first, 1 - first, 2};
@@ -1069,7 +1068,11 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
     }
     if ((rc = sweep_advect(x, dir, dt, is_aloft, b, q > 0 && again, false, nullptr, nullptr, true, zg))) return rc;
   }
-  if (use_diffusion) return step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b);
+  if (use_diffusion && (rc = step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b))) return rc;
+  // every ts0 ghost has been rewritten (Set_BC(1), or the diffusion's
+  // Set_BC(2) and Neumann passes); a step that failed part-way leaves the
+  // glue's zeroing pending for the next entry point
+  x->ghost_zero_pending = false;
   return 0;
 }
 
@@ -1085,7 +1088,7 @@ int ash3d_tr_step_keep(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_di
   // ash3d_tr_step's advection, each sweep out of place: ts0 -> slot_b ->
   // slot_z -> ts0 (the same kernels and Set_BC(1) passes, so the same bits)
   if ((rc = set_bc1(x, true))) return rc;
-  x->ghost_zero_pending = false;  // every ts0 ghost rewritten
+  x->ghost_zero_pending = false;  // Set_BC(1) above rewrote every ts0 ghost
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0;
   double *sb = x->ckpt[slot_b], *sz = x->ckpt[slot_z];

@@ -355,6 +355,39 @@ def global_state(seed: int = 5, ns: int = 20, nx: int = 1440, ny: int = 720, nz:
     return st
 
 
+def inversion_case(seed: int = 4, nx: int = 240, ny: int = 160, nz: int = 50, ns: int = 8,
+                   n_heights: int = 40, n_periods: int = 24, hours: float = 12.0):
+    """The adjoint source-inversion workload: lon-lat 0.25 deg 240x160x50
+    (20-60 N), dz 0.5 km, 8 bins (|vf| log-spaced 0.01-5 m/s, equal
+    fractions), the forecast's zonal jet with seeded perturbations,
+    Kh = Kz = 500 m2/s, Crank-Nicolson, an empty domain, a vent at grid
+    cell (nx/4, ny/2).  The unknown source is a rate table of n_heights
+    levels x n_periods periods over `hours` (30 min periods over 12 h); dt
+    is the largest whole fraction of a period within the CFL / diffusion
+    limit.  Returns (state, vent (i, j), steps per period, bin fractions,
+    dz (nz), the true source table (a lognormal height profile centred at
+    9 km, sigma_ln 0.35, times a Gaussian pulse in time))."""
+    st = TransportState(nx, ny, nz, ns, limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5)
+    lonlat_mesh(st, 180.0, 20.0, 0.25, 0.25, np.full(nz, 0.5))
+    rng = np.random.default_rng(seed)
+    zonal_jet_winds(st, rng)
+    set_uniform_fall(st, fall_velocities(ns, 0.01, 5.0))
+    set_diffusivity(st, 500.0, 500.0)
+    st.use_diffusion = True
+    dt_max = adjust_dt(st, use_diffusion=True)
+    period = hours / n_periods
+    spp = int(math.ceil(period / dt_max))
+    st.dt = period / spp
+    frac = np.full(ns, 1.0 / ns)
+    dz = np.full(nz, 0.5)
+    zh = (np.arange(n_heights) + 0.5) * 0.5
+    prof = np.exp(-(np.log(zh / 9.0)) ** 2 / (2 * 0.35 ** 2))
+    tp = (np.arange(n_periods) + 0.5) * period
+    env = np.exp(-0.5 * ((tp - 3.0) / 1.5) ** 2)
+    s_true = 1.0e6 * np.outer(env, prof)
+    return st, (nx // 4, ny // 2), spp, frac, dz, s_true
+
+
 def random_state(nx, ny, nz, ns, seed, latlon=False, periodic=False, zscaling_id=0,
                  limiter=LIM_SUPERBEE, use_cn=True, imp_fac=0.5, diffusivity=True,
                  cfl=0.8) -> TransportState:
