@@ -231,10 +231,12 @@ def global_adjoint_leg(pkg, bins, n_steps, slots, line_solver, tuning):
 
 
 def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
-    """Ensemble throughput: one context (own stream) per member on the one
-    GPU, all members stepped together; eager launches against replays of
-    the captured glue + step graphs."""
-    import numpy as np
+    """Ensemble throughput: one context per member on the one GPU (grids of
+    120-400 x 80-300 x 40-60 cells, 10 bins, each member its own dt), all
+    members stepped together, four ways: each member's own steps (eager
+    launches, then replays of its captured glue + step graphs), and the
+    batched ensemble step (ash3d_tr_ensemble_step: one launch per sweep
+    kind and kernel variant over all members, eager, then as one graph)."""
     t0 = time.perf_counter()
     states = [pkg.workloads.ensemble_member(m, size_scale=size_scale) for m in range(members)]
     log(f"[bench] ensemble: {members} members built in {time.perf_counter() - t0:.1f} s")
@@ -242,6 +244,7 @@ def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
     for T in Ts:
         T.tune(line_solver=line_solver)
     work = sum(st.ns * st.nx * st.ny * st.nz for st in states)
+    step_bytes = sum(step_bytes_per_bincell(st, True) * st.ns * st.nx * st.ny * st.nz for st in states)
     for T in Ts:
         T.st.concen_pd[..., 1] = T.st.concen_pd[..., 0]
         T.sync_from_host(tr.SYNC_CONCEN)
@@ -249,35 +252,55 @@ def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
         T.transport_step(1, True)
     for T in Ts:
         T.synchronize()
+    E = None
 
     def run(mode, it0):
         t = time.perf_counter()
         for s_ in range(steps):
             it = it0 + s_
+            if mode.startswith("batched"):
+                E.step(it, use_graph=mode == "batched_graph")
+                continue
             for T in Ts:
                 if mode == "graph":
                     T.graph_launch(it)
                 else:
                     T.step_glue()
                     T.transport_step(it, True)
+        if E is not None:
+            E.synchronize()
         for T in Ts:
             T.synchronize()
         return time.perf_counter() - t
 
-    run("eager", 2)  # warm-up
-    t_eager = run("eager", 2 + steps)
-    for T in Ts:
-        T.graph_capture(2, True)
-        T.graph_capture(3, True)
-    run("graph", 2 + 2 * steps)  # warm-up
-    t_graph = run("graph", 2 + 3 * steps)
+    it = 2
+    t = {}
+    for mode in ("eager", "graph", "batched_eager", "batched_graph"):
+        if mode == "graph":
+            for T in Ts:
+                T.graph_capture(2, True)
+                T.graph_capture(3, True)
+        if mode == "batched_eager":
+            E = tr.Ensemble(Ts)
+        run(mode, it)  # warm-up
+        it += steps
+        t[mode] = run(mode, it)
+        it += steps
+        log(f"[bench] ensemble {mode}: {1e3 * t[mode] / steps:.2f} ms per ensemble step")
+    launches = E.info(2)
+    E.close()
     sizes = [(st.nx, st.ny, st.nz) for st in states]
     for T in Ts:
         T.close()
-    return {"members": members, "steps": steps, "bin_cells_per_step": work,
-            "grid_sizes": f"{min(sizes)}..{max(sizes)}, 10 bins",
-            "eager_rate": work * steps / t_eager, "graph_rate": work * steps / t_graph,
-            "eager_ms_per_ensemble_step": 1e3 * t_eager / steps, "graph_ms_per_ensemble_step": 1e3 * t_graph / steps}
+    out = {"members": members, "steps": steps, "bin_cells_per_step": work,
+           "grid_sizes": f"{min(sizes)}..{max(sizes)}, 10 bins",
+           "algorithmic_bytes_per_step": step_bytes,
+           "launches_per_step": {"batched": launches[0], "members_own": launches[1]}}
+    for mode, v in t.items():
+        out[f"{mode}_rate"] = work * steps / v
+        out[f"{mode}_ms_per_ensemble_step"] = 1e3 * v / steps
+        out[f"{mode}_hbm_frac"] = step_bytes * steps / v / 8.0e12
+    return out
 
 
 WORKLOADS = {

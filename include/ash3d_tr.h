@@ -283,6 +283,40 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *ctx, int32_t itime, double dt, int32_t 
                            double imp_fac, const int32_t *is_aloft);
 int ash3d_tr_graph_launch(ash3d_tr_ctx *ctx, int32_t itime);
 
+/* ---- Batched ensembles --------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per process
+ * (an ensemble forecast runs the executable once per member).  An ensemble
+ * steps the states of n contexts on one device -- wind-perturbed members
+ * on grids of different sizes, each with its own dt -- as batched launches:
+ * every sweep kind of the step (each advection direction, each Set_BC(2)
+ * face or Neumann pass, each diffusion direction) is one launch over all
+ * members for each kernel variant the members' grids select, the members'
+ * argument blocks and workgroup ranges held in device memory.  One
+ * ensemble step of parity itime % 2 does, for every member m, what
+ * ash3d_tr_step_glue(m) followed by ash3d_tr_step(m, itime, dt[m], 1, 1,
+ * imp_fac, is_aloft[m]) does, bit for bit; with use_graph it is one HIP
+ * graph launch (captured at create, one per parity).
+ *   Requirements, checked at create and at every step: members on one
+ * device, with diffusion by the partitioned Crank-Nicolson line solves
+ * (ASH3D_TUNE_LINE_SOLVER = ASH3D_LINE_SOLVER_PARTITIONED) on diagonally
+ * dominant lines of 3 to the solver's maximum active cells, and the last
+ * call on each member a step (ts1 mirrored, as for graph_launch).
+ * is_aloft is null (every bin of every member aloft) or n pointers, each
+ * null or an ns-long mask.  The members' tuning at create fixes the
+ * kernels; the step orders itself after earlier calls on each member's
+ * stream, and later calls on the members after it.  Members must outlive
+ * the ensemble, and must not be stepped on their own between ensemble
+ * steps with another dt (the step rebuilds line factors that changed, at a
+ * cost).  ensemble_info gives the launches of one step, batched and as
+ * the members' own steps would issue them. */
+typedef struct ash3d_tr_ens ash3d_tr_ens;
+int ash3d_tr_ensemble_create(ash3d_tr_ctx *const *members, int32_t n, const double *dt, double imp_fac,
+                             const int32_t *const *is_aloft, ash3d_tr_ens **out);
+int ash3d_tr_ensemble_step(ash3d_tr_ens *ens, int32_t itime, int32_t use_graph);
+int ash3d_tr_ensemble_synchronize(ash3d_tr_ens *ens);
+int ash3d_tr_ensemble_info(ash3d_tr_ens *ens, int32_t itime, int32_t *batched_launches, int32_t *member_launches);
+int ash3d_tr_ensemble_destroy(ash3d_tr_ens *ens);
+
 /* ---- Checkpoints ------------------------------------------------------
  * No counterpart in the reference (which only writes output files).  Slots
  * of device memory, each one copy of concen_pd(:,:,:,:,ts0).  Save copies

@@ -43,12 +43,13 @@ def advect_x_row(backend, q, vel, sig, kap, dt, limiter, deld=None):
 
 
 def advect_z_column(backend, q, vf, sig, kap, dt, kmin, kmax, limiter):
-    """advect_z of one column of nz = len(q) cells (nx = ny = 1, one bin)
-    with active levels kmin..kmax, vz = vh = 0 and a uniform fall speed vf.
-    Returns (q after the sweep on cells 1..nz, bottom outflow (deposit),
-    top outflow)."""
+    """advect_z of one column of nz = len(q) cells (the column i = j = 1
+    of a 2 x 1 grid, the bounds leaving out column 2; one bin) with active
+    levels kmin..kmax, vz = vh = 0 and a uniform fall speed vf.  Returns (q
+    after the sweep on cells 1..nz, bottom outflow (deposit), top
+    outflow)."""
     nz = len(q)
-    st = _state.TransportState(1, 1, nz, 1, limiter=limiter)
+    st = _state.TransportState(2, 1, nz, 1, limiter=limiter)
     st.dt = dt
     st.vf_pd[...] = vf
     st.sigma_nz_pd[...] = sig

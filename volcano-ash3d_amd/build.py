@@ -27,17 +27,36 @@ HIPFLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile each source to an object (in parallel, up to jobs at once)
+    and link the shared library."""
+    import concurrent.futures
+    import tempfile
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    flags = [f for f in HIPFLAGS if f != "-shared"]
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libash3d_tr.so")
+    with tempfile.TemporaryDirectory(prefix="ash3d_build_") as work:
+        def compile_one(src):
+            obj = os.path.join(work, os.path.splitext(src)[0] + ".o")
+            cmd = [hipcc, *flags, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            return obj, r
+        with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+            results = list(ex.map(compile_one, SOURCES))
+        for obj, r in results:
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError(f"hipcc failed compiling {os.path.basename(obj)}")
+        cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[o for o, _ in results], "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed linking libash3d_tr.so")
     os.replace(tmp, LIB)
     return LIB
 

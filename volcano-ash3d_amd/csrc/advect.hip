@@ -34,7 +34,7 @@ constexpr int kBlock = 256;
 // GM: the bin groups of a tile in one workgroup, whole lines per wave (no
 // halos to exchange); otherwise line segments per wave.
 template <int DIR, int NB, bool TOPO, bool GM>
-__global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
+__device__ __forceinline__ void advect_body(const AdvArgs &A, const long long vb, const long long vn) {
   constexpr int PF = (NB >= 2) ? 2 : 4;  // cells in flight
   const Dims d = A.d;
   const LineGeom g = A.g;
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   // (whole lines, so the per-cell fields are read by all of them at once)
   const int G = (d.ns + NB - 1) / NB;
   constexpr bool gmode = GM;
-  const TileGroup tg = xcd_tile_group(blockIdx.x, gmode ? (G + A.gy - 1) / A.gy : G);
+  const TileGroup tg = xcd_tile_group(vb, gmode ? (G + A.gy - 1) / A.gy : G, vn);
   if (tg.tile >= (long long)ntile * g.Nb) return;  // padding workgroup (uniform)
   const int a = (int)(tg.tile % ntile) * 64 + (int)threadIdx.x + 1;
   const int b = (int)(tg.tile / ntile) + 1;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   const bool line_on = lane_ok && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   const double *c = A.c0;
   double *co = A.cout;  // == c0 for the in-place sweep
-  double *junk = A.junk + (threadIdx.x + 64 * threadIdx.y + 256 * (blockIdx.x % (kJunk / 256)));
+  double *junk = A.junk + (threadIdx.x + 64 * threadIdx.y + 256 * (vb % (kJunk / 256)));
 
   bool exists[NB], on[NB];
   long long bofs[NB];  // bin offset (an existing bin for the others)
@@ -279,8 +279,20 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
   }
 }
 
+template <int DIR, int NB, bool TOPO, bool GM>
+__global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
+  advect_body<DIR, NB, TOPO, GM>(A, blockIdx.x, gridDim.x);
+}
+// the same sweep over the members of an ensemble (one launch)
+template <int DIR, int NB, bool TOPO, bool GM>
+__global__ __launch_bounds__(256) void k_advect_batch(Batch<AdvArgs> B) {
+  const int m = batch_member(B);
+  const AdvArgs &A = B.args[m];
+  advect_body<DIR, NB, TOPO, GM>(A, (long long)blockIdx.x - B.first[m], B.first[m + 1] - B.first[m]);
+}
+
 template <int DIR, int NB>
-hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in) {
+hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in, LaunchSpec *sp = nullptr) {
   AdvArgs a = a_in;
   const int ntile = (a.g.Na + 63) / 64;
   const long long groups = (a.d.ns + NB - 1) / NB;
@@ -302,23 +314,24 @@ hipError_t launch_dir(hipStream_t s, const AdvArgs &a_in) {
     block = dim3(64, W);
   }
   const bool topo = DIR != 2 && a.deld != nullptr;
-  if (a.gy > 0) {
-    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_advect<DIR, NB, false, true>), grid, block, 0, s, a);
-  } else {
-    const size_t halo = sizeof(double) * 4 * NB * (size_t)block.x * block.y;
-    if (topo) hipLaunchKernelGGL((k_advect<DIR, NB, true, false>), grid, block, halo, s, a);
-    else hipLaunchKernelGGL((k_advect<DIR, NB, false, false>), grid, block, halo, s, a);
-  }
-  return hipGetLastError();
+  const size_t lds = a.gy > 0 ? 0 : sizeof(double) * 4 * NB * (size_t)block.x * block.y;
+  if (a.gy > 0)
+    return issue(s, sp, topo ? (const void *)k_advect<DIR, NB, true, true> : (const void *)k_advect<DIR, NB, false, true>,
+                 topo ? (const void *)k_advect_batch<DIR, NB, true, true>
+                      : (const void *)k_advect_batch<DIR, NB, false, true>,
+                 grid, block, lds, a);
+  return issue(s, sp, topo ? (const void *)k_advect<DIR, NB, true, false> : (const void *)k_advect<DIR, NB, false, false>,
+               topo ? (const void *)k_advect_batch<DIR, NB, true, false>
+                    : (const void *)k_advect_batch<DIR, NB, false, false>,
+               grid, block, lds, a);
 }
 
 template <int NB>
-hipError_t launch_nb(hipStream_t s, const AdvArgs &a) {
+hipError_t launch_nb(hipStream_t s, const AdvArgs &a, LaunchSpec *sp = nullptr) {
   switch (a.g.dir) {
-    case 0: return launch_dir<0, NB>(s, a);
-    case 1: return launch_dir<1, NB>(s, a);
-    default: return launch_dir<2, NB>(s, a);
+    case 0: return launch_dir<0, NB>(s, a, sp);
+    case 1: return launch_dir<1, NB>(s, a, sp);
+    default: return launch_dir<2, NB>(s, a, sp);
   }
 }
 
@@ -337,19 +350,23 @@ int pick_bins_per_thread(int ns, int kind, int dir) {
   return nb;
 }
 
-hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb) {
+static hipError_t advect_any(hipStream_t s, const AdvArgs &a, int nb, LaunchSpec *sp) {
   if (a.g.dir == 0) {
-    hipError_t e = launch_advect_x_rows(s, a);
+    hipError_t e = sp ? spec_advect_x_rows(a, sp) : launch_advect_x_rows(s, a);
     if (e != hipErrorInvalidValue) return e;
     (void)hipGetLastError();  // row too long for one workgroup: stream it
   }
   switch (nb) {
-    case 1: return launch_nb<1>(s, a);
-    case 2: return launch_nb<2>(s, a);
-    case 3: return launch_nb<3>(s, a);
-    case 4: return launch_nb<4>(s, a);
-    default: return launch_nb<8>(s, a);
+    case 1: return launch_nb<1>(s, a, sp);
+    case 2: return launch_nb<2>(s, a, sp);
+    case 3: return launch_nb<3>(s, a, sp);
+    case 4: return launch_nb<4>(s, a, sp);
+    default: return launch_nb<8>(s, a, sp);
   }
 }
+
+hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb) { return advect_any(s, a, nb, nullptr); }
+
+hipError_t spec_advect(const AdvArgs &a, int nb, LaunchSpec *sp) { return advect_any(nullptr, a, nb, sp); }
 
 }  // namespace ash3d

@@ -19,6 +19,8 @@
 //
 // Arithmetic: the reference expressions in the reference order, no
 // contraction -- bitwise equal to the streaming kernel and the oracle.
+#include <cstring>
+
 #include "common.hpp"
 #include "dcu.hpp"
 #include "lanes.hpp"
@@ -28,7 +30,7 @@ namespace ash3d {
 namespace {
 
 template <int CPT, bool TOPO>
-__global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
+__device__ __forceinline__ void advect_x_rows_body(const AdvArgs &A, const long long vb) {
   extern __shared__ double smem[];
   const Dims d = A.d;
   const LineGeom g = A.g;
@@ -41,8 +43,8 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
   double *s_tf = smem + 4 * W;
   double *s_D = smem + 5 * W;
 
-  const int j = (int)(blockIdx.x % (unsigned)d.ny) + 1;
-  const int k = (int)(blockIdx.x / (unsigned)d.ny) + 1;
+  const int j = (int)(vb % d.ny) + 1;
+  const int k = (int)(vb / d.ny) + 1;
   const int rmin = g.rmin, rmax = g.rmax;
   const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
   const long long base = d.p3(-1, j, k);  // cell -1 of the row
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
 // The arithmetic is that of k_advect_x_rows expression by expression.
 
 template <int NC, int E, bool TOPO>
-__global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
+__device__ __forceinline__ void advect_x_wave_body(const AdvArgs &A, const long long vb) {
   constexpr int CS = 64 * E;  // cells per chunk
   // a lane's E consecutive cells (one 16-byte access for a pair)
   auto ld_cells = [](const double *src, double (&out)[E]) {
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   const LineGeom g = A.g;
   const int nx = d.nx, W = nx + 4;
   const int l = (int)(threadIdx.x & 63);
-  const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long long row = vb * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (row >= (long long)d.ny * d.nz) return;  // whole wavefront
   const int j = (int)(row % d.ny) + 1;
   const int k = (int)(row / d.ny) + 1;
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   const long long base = d.p3(-1, j, k);  // cell -1 of the row
   const double *c0 = A.c0;
   double *co = A.cout;  // == c0 for the in-place sweep
-  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 512)));
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (vb % (kJunk / 512)));
   const bool lane0 = l == 0, lane63 = l == 63;
 
   // per element: cell, row position, masks (bin-independent)
@@ -464,8 +466,13 @@ __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
 // pair of its own range to LDS (two slots, by bin parity) and reads its
 // halos from there after a barrier, one barrier per bin.  Global loads and
 // stores of concentrations touch only the wave's own range.
+struct SegArgs {
+  AdvArgs A;
+  int L;  // even segment length
+};
+
 template <int NC, bool TOPO>
-__global__ __launch_bounds__(512) void k_advect_x_seg(AdvArgs A, int L) {
+__device__ __forceinline__ void advect_x_seg_body(const AdvArgs &A, const int L, const long long vb) {
   constexpr int E = 2, CS = 128;
   auto ld_cells = [](const double *src, double (&out)[E]) {
     const double2 v = *(const double2 *)src;
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(512) void k_advect_x_seg(AdvArgs A, int L) {
   const LineGeom g = A.g;
   const int nx = d.nx, W = nx + 4;
   const int l = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6), WV = (int)(blockDim.x >> 6);
-  const long long row = blockIdx.x;
+  const long long row = vb;
   const int j = (int)(row % d.ny) + 1;
   const int k = (int)(row / d.ny) + 1;
   const int rmin = g.rmin, rmax = g.rmax;
@@ -485,7 +492,7 @@ __global__ __launch_bounds__(512) void k_advect_x_seg(AdvArgs A, int L) {
   const long long base = d.p3(-1, j, k);  // cell -1 of the row
   const double *c0 = A.c0;
   double *co = A.cout;  // == c0 for the in-place sweep
-  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 1024)));
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (vb % (kJunk / 1024)));
   const bool lane0 = l == 0, lane63 = l == 63;
   const int S = w * L, Send = min(S + L, W), X = S - 2;
   // the halo pairs (positions S-2, S-1 and S+L, S+L+1) and the own edge
@@ -722,33 +729,60 @@ __global__ __launch_bounds__(512) void k_advect_x_seg(AdvArgs A, int L) {
   }
 }
 
+// one kernel per body: a launch for one member and a batched launch over
+// the members of an ensemble (workgroup index within the member's launch)
+template <int CPT, bool TOPO>
+__global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
+  advect_x_rows_body<CPT, TOPO>(A, blockIdx.x);
+}
+template <int CPT, bool TOPO>
+__global__ __launch_bounds__(256) void k_advect_x_rows_batch(Batch<AdvArgs> B) {
+  const int m = batch_member(B);
+  const AdvArgs &A = B.args[m];
+  advect_x_rows_body<CPT, TOPO>(A, (long long)blockIdx.x - B.first[m]);
+}
+template <int NC, int E, bool TOPO>
+__global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
+  advect_x_wave_body<NC, E, TOPO>(A, blockIdx.x);
+}
+template <int NC, int E, bool TOPO>
+__global__ __launch_bounds__(256) void k_advect_x_wave_batch(Batch<AdvArgs> B) {
+  const int m = batch_member(B);
+  const AdvArgs &A = B.args[m];
+  advect_x_wave_body<NC, E, TOPO>(A, (long long)blockIdx.x - B.first[m]);
+}
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(512) void k_advect_x_seg(SegArgs S) {
+  advect_x_seg_body<NC, TOPO>(S.A, S.L, blockIdx.x);
+}
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(512) void k_advect_x_seg_batch(Batch<SegArgs> B) {
+  const int m = batch_member(B);
+  const SegArgs &S = B.args[m];
+  advect_x_seg_body<NC, TOPO>(S.A, S.L, (long long)blockIdx.x - B.first[m]);
+}
+
 template <int CPT>
-hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt) {
+hipError_t launch_cpt(hipStream_t s, const AdvArgs &a, int bt, LaunchSpec *sp) {
   const size_t shmem = sizeof(double) * 6 * (size_t)(a.d.nx + 4);
   dim3 grid((unsigned)((long long)a.d.ny * a.d.nz));
-  const void *kern = a.deld != nullptr ? (const void *)k_advect_x_rows<CPT, true>
-                                       : (const void *)k_advect_x_rows<CPT, false>;
-  if (shmem > 65536) {
-    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
-  }
-  if (a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect_x_rows<CPT, true>), grid, dim3(bt), shmem, s, a);
-  else
-    hipLaunchKernelGGL((k_advect_x_rows<CPT, false>), grid, dim3(bt), shmem, s, a);
-  return hipGetLastError();
+  const bool topo = a.deld != nullptr;
+  return issue(s, sp, topo ? (const void *)k_advect_x_rows<CPT, true> : (const void *)k_advect_x_rows<CPT, false>,
+               topo ? (const void *)k_advect_x_rows_batch<CPT, true> : (const void *)k_advect_x_rows_batch<CPT, false>,
+               grid, dim3(bt), shmem, a);
 }
 
 template <int NC, int E>
-hipError_t launch_wave(hipStream_t s, const AdvArgs &a) {
+hipError_t launch_wave(hipStream_t s, const AdvArgs &a, LaunchSpec *sp) {
   const long long rows = (long long)a.d.ny * a.d.nz;
   dim3 grid((unsigned)((rows + 3) / 4));
   const size_t lds = sizeof(double) * 4 * 4 * E * NC * 64;  // 4 waves x 4 row terms x E*NC per lane
-  if (a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect_x_wave<NC, E, true>), grid, dim3(256), lds, s, a);
-  else
-    hipLaunchKernelGGL((k_advect_x_wave<NC, E, false>), grid, dim3(256), lds, s, a);
-  return hipGetLastError();
+  const bool topo = a.deld != nullptr;
+  return issue(s, sp,
+               topo ? (const void *)k_advect_x_wave<NC, E, true> : (const void *)k_advect_x_wave<NC, E, false>,
+               topo ? (const void *)k_advect_x_wave_batch<NC, E, true>
+                    : (const void *)k_advect_x_wave_batch<NC, E, false>,
+               grid, dim3(256), lds, a);
 }
 
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
@@ -756,7 +790,7 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // Segmented rows: WV = ceil(W / (128 NC - 4)) waves, L the even segment
 // length; the dynamic LDS holds each wave's row terms and the edge pairs.
 template <int NC>
-hipError_t launch_seg(hipStream_t s, const AdvArgs &a) {
+hipError_t launch_seg(hipStream_t s, const AdvArgs &a, LaunchSpec *sp) {
   const int W = a.d.nx + 4;
   const int cap = 128 * NC - 4;
   const int WV = (W + cap - 1) / cap;
@@ -765,17 +799,11 @@ hipError_t launch_seg(hipStream_t s, const AdvArgs &a) {
   L += L & 1;
   if (L > cap || (WV - 1) * L >= W) return hipErrorInvalidValue;
   const size_t lds = sizeof(double) * ((size_t)WV * 4 * 2 * NC * 64 + 2 * (size_t)WV * 4);
-  const void *kern = a.deld != nullptr ? (const void *)k_advect_x_seg<NC, true> : (const void *)k_advect_x_seg<NC, false>;
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
   dim3 grid((unsigned)((long long)a.d.ny * a.d.nz));
-  if (a.deld != nullptr)
-    hipLaunchKernelGGL((k_advect_x_seg<NC, true>), grid, dim3(64 * WV), lds, s, a, L);
-  else
-    hipLaunchKernelGGL((k_advect_x_seg<NC, false>), grid, dim3(64 * WV), lds, s, a, L);
-  return hipGetLastError();
+  const bool topo = a.deld != nullptr;
+  return issue(s, sp, topo ? (const void *)k_advect_x_seg<NC, true> : (const void *)k_advect_x_seg<NC, false>,
+               topo ? (const void *)k_advect_x_seg_batch<NC, true> : (const void *)k_advect_x_seg_batch<NC, false>,
+               grid, dim3(64 * WV), lds, SegArgs{a, L});
 }
 
 }  // namespace
@@ -788,33 +816,33 @@ hipError_t launch_seg(hipStream_t s, const AdvArgs &a) {
 // and spill, and run slower than the workgroup kernel), the outflow
 // increments fit one lane per bin (ns <= 64) and the arrays allow 16-byte
 // accesses.
-hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
+static hipError_t advect_x_rows_any(hipStream_t s, const AdvArgs &a, LaunchSpec *sp) {
   const int cells = a.d.nx + 4;
   const bool common = a.xrows == 1 && cells <= (a.deld ? 256 : 384) && a.d.ns <= 64;
   if (common && a.d.nx % 2 == 0 && aligned16(a.c0) && aligned16(a.cout) && aligned16(a.vel) &&
       aligned16(a.sig) && aligned16(a.kap) && aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld))) {
     switch ((cells + 127) / 128) {  // cell pairs
-      case 1: return launch_wave<1, 2>(s, a);
-      case 2: return launch_wave<2, 2>(s, a);
-      default: return launch_wave<3, 2>(s, a);
+      case 1: return launch_wave<1, 2>(s, a, sp);
+      case 2: return launch_wave<2, 2>(s, a, sp);
+      default: return launch_wave<3, 2>(s, a, sp);
     }
   }
   // longer rows (nx even): several waves per row, a segment each
   if (a.xrows == 1 && a.d.ns <= 64 && a.d.nx % 2 == 0 && !common && aligned16(a.c0) && aligned16(a.cout) &&
       aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) && aligned16(a.junk) &&
       (a.deld == nullptr || aligned16(a.deld))) {
-    hipError_t e = a.deld != nullptr ? launch_seg<2>(s, a) : launch_seg<3>(s, a);
+    hipError_t e = a.deld != nullptr ? launch_seg<2>(s, a, sp) : launch_seg<3>(s, a, sp);
     if (e != hipErrorInvalidValue) return e;
     (void)hipGetLastError();
   }
   if (common) {
     switch ((cells + 63) / 64) {  // single cells (odd nx)
-      case 1: return launch_wave<1, 1>(s, a);
-      case 2: return launch_wave<2, 1>(s, a);
-      case 3: return launch_wave<3, 1>(s, a);
-      case 4: return launch_wave<4, 1>(s, a);
-      case 5: return launch_wave<5, 1>(s, a);
-      default: return launch_wave<6, 1>(s, a);
+      case 1: return launch_wave<1, 1>(s, a, sp);
+      case 2: return launch_wave<2, 1>(s, a, sp);
+      case 3: return launch_wave<3, 1>(s, a, sp);
+      case 4: return launch_wave<4, 1>(s, a, sp);
+      case 5: return launch_wave<5, 1>(s, a, sp);
+      default: return launch_wave<6, 1>(s, a, sp);
     }
   }
   int cpt = (cells + 255) / 256;
@@ -824,11 +852,15 @@ hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) {
   int bt = (cells + cpt - 1) / cpt;
   bt = ((bt + 63) / 64) * 64;
   switch (cpt) {
-    case 1: return launch_cpt<1>(s, a, bt);
-    case 2: return launch_cpt<2>(s, a, bt);
-    case 4: return launch_cpt<4>(s, a, bt);
-    default: return launch_cpt<8>(s, a, bt);
+    case 1: return launch_cpt<1>(s, a, bt, sp);
+    case 2: return launch_cpt<2>(s, a, bt, sp);
+    case 4: return launch_cpt<4>(s, a, bt, sp);
+    default: return launch_cpt<8>(s, a, bt, sp);
   }
 }
+
+hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a) { return advect_x_rows_any(s, a, nullptr); }
+
+hipError_t spec_advect_x_rows(const AdvArgs &a, LaunchSpec *sp) { return advect_x_rows_any(nullptr, a, sp); }
 
 }  // namespace ash3d

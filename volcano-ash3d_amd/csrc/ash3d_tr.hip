@@ -492,20 +492,10 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
   return x->ndom_h[dir];
 }
 
-// Launches the partitioned solves of a sweep when they apply.  Returns 0
-// when they were not launched (the caller runs the dgtsv-order kernel as
-// usual), 1 when launched with the choice left to the device (the caller
-// launches the dgtsv-order kernel too, a.sel = 2), 2 when launched for a
-// sweep known to be dominant (the caller launches nothing else).
-int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjoint = false) {
-  *rc = 0;
-  const int n = a.g.rmax - a.g.rmin + 1;
-  a.sel = 0;
-  // (x lines are contiguous in memory; y and z lines strided)
-  const int nmax = adjoint ? 512 : (dir == 0 ? x->part_max_line : x->part_max_line_strided);
-  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > nmax) return 0;
-  const int hn = host_ndom(x, dir);
-  if (hn > 0) return 0;
+// Cell offsets of the partitioned solves of direction dir: the state in
+// its own layout, the row factor cache in the sweep's (for x lines the
+// x<->y transposed one, a.d)
+PartLayout part_layout(const ash3d_tr_ctx *x, int dir, const DiffArgs &a) {
   const Dims &o = x->d;
   PartLayout lay;
   lay.st0 = 1 + o.sy + o.sz;
@@ -521,11 +511,59 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   }
   lay.lr_ncell = a.d.ncell;
   lay.kap = x->kap;
+  return lay;
+}
+
+// Launches the partitioned solves of a sweep when they apply.  Returns 0
+// when they were not launched (the caller runs the dgtsv-order kernel as
+// usual), 1 when launched with the choice left to the device (the caller
+// launches the dgtsv-order kernel too, a.sel = 2), 2 when launched for a
+// sweep known to be dominant (the caller launches nothing else).
+int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjoint = false) {
+  *rc = 0;
+  const int n = a.g.rmax - a.g.rmin + 1;
+  a.sel = 0;
+  // (x lines are contiguous in memory; y and z lines strided)
+  const int nmax = adjoint ? 512 : (dir == 0 ? x->part_max_line : x->part_max_line_strided);
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > nmax) return 0;
+  const int hn = host_ndom(x, dir);
+  if (hn > 0) return 0;
+  const PartLayout lay = part_layout(x, dir, a);
   a.sel = 1;
   hipError_t e = launch_diff_cn_part(x->stream, a, lay, x->part_bins, x->part_shape[dir], adjoint);
   if (e != hipSuccess) *rc = fail(ASH3D_TR_EDEVICE, "partitioned line solve: %s", hipGetErrorString(e));
   a.sel = 2;
   return hn == 0 ? 2 : 1;
+}
+
+// Arguments of a diffusion sweep of direction dir over the state ts0
+DiffArgs diff_args(ash3d_tr_ctx *x, int dir, double dt, double imp_fac, const int32_t *is_aloft,
+                   const int32_t *bounds) {
+  DiffArgs a{};
+  a.d = x->d;
+  a.g = make_line_geom(x->d, dir, bounds);
+  a.c0 = x->ts0();
+  a.scratch_b = x->ts1();
+  a.dt = dt;
+  a.theta = imp_fac;
+  a.flag = x->flag;
+  make_mask(x, is_aloft, &a.aloft);
+  a.kap = x->kap;
+  a.kk = dir == 0 ? x->kx : (dir == 1 ? x->ky : x->kz);
+  a.sig = dir == 0 ? x->sx : (dir == 1 ? x->sy : x->sz);
+  return a;
+}
+
+// x lines as the y lines of the x<->y transposed grid: dims, line
+// geometry, kappa, sigma and K of the transposed copies
+void transpose_x_args(const ash3d_tr_ctx *x, const int32_t *bounds, DiffArgs &a) {
+  const Dims &d = x->d;
+  const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
+  a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
+  a.g = make_line_geom(a.d, 1, tb);
+  a.kap = x->tkap;
+  a.sig = x->tsx;
+  a.kk = x->tkx;
 }
 
 // neumann = false: the caller has just run Set_BC(2), whose ghosts are the
@@ -541,18 +579,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     ProfScope ps(x, ASH3D_PROF_BC);
     HIPCHK(launch_neumann_dir(x->stream, x->d, dir, x->periodic, x->ts0()));
   }
-  DiffArgs a{};
-  a.d = x->d;
-  a.g = make_line_geom(x->d, dir, bounds);
-  a.c0 = x->ts0();
-  a.scratch_b = x->ts1();
-  a.dt = dt;
-  a.theta = imp_fac;
-  a.flag = x->flag;
-  make_mask(x, is_aloft, &a.aloft);
-  a.kap = x->kap;
-  a.kk = dir == 0 ? x->kx : (dir == 1 ? x->ky : x->kz);
-  a.sig = dir == 0 ? x->sx : (dir == 1 ? x->sy : x->sz);
+  DiffArgs a = diff_args(x, dir, dt, imp_fac, is_aloft, bounds);
   if (use_cn && a.g.rmax - a.g.rmin + 1 <= 1) {
     // ncells <= 1: the reference zero-fills ts1 and copies nothing (:770, :909)
     x->ts1_state = TS1_ZERO;
@@ -563,12 +590,7 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     // layout, the state read and written in place through LDS row tiles
     const Dims &d = x->d;
     if ((rc = ensure_tgeom(x))) return rc;
-    const int32_t tb[6] = {bounds[2], bounds[3], bounds[0], bounds[1], bounds[4], bounds[5]};
-    a.d = make_dims(d.ny, d.nx, d.nz, d.ns);
-    a.g = make_line_geom(a.d, 1, tb);
-    a.kap = x->tkap;
-    a.sig = x->tsx;
-    a.kk = x->tkx;
+    transpose_x_args(x, bounds, a);
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for both kernels
@@ -1882,6 +1904,332 @@ int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
   }
   return 0;
 }
+
+}  // extern "C"
+
+// ---- batched ensembles ------------------------------------------------------
+
+// One batched launch of an ensemble step: the members (n of them) whose
+// launch in this slot of the step selects the same kernel, block and LDS;
+// their argument blocks and workgroup offsets at byte offsets args_off and
+// first_off of the ensemble's device buffer.
+struct EnsGroup {
+  LaunchSpec spec;  // the first member's
+  size_t args_off = 0, first_off = 0;
+  int n = 0;
+  long long total = 0;  // workgroups along x, all members
+  unsigned gy = 1;      // grid.y: the largest of the members'
+};
+
+struct ash3d_tr_ens {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<ash3d_tr_ctx *> m;
+  std::vector<double> dt;
+  std::vector<std::vector<int32_t>> aloft;  // per member (empty: every bin aloft)
+  double imp_fac = 0.5;
+  std::vector<EnsGroup> plan[2];  // by step parity
+  int member_launches[2] = {0, 0};
+  unsigned char *dev = nullptr;  // argument blocks and offsets of both plans
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> ready;  // per member
+};
+
+namespace ash3d {
+
+struct BatchAny {
+  const void *args;
+  const long long *first;
+  int n;
+};
+static_assert(sizeof(BatchAny) == sizeof(Batch<AdvArgs>), "batch descriptor layout");
+
+hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const long long *dev_first,
+                        int n, long long total, unsigned grid_y) {
+  if (n <= 0 || total <= 0) return hipSuccess;
+  if (total > 0x7fffffffLL || grid_y == 0 || grid_y > 65535) return hipErrorInvalidValue;
+  BatchAny b{dev_args, dev_first, n};
+  void *kargs[] = {(void *)&b};
+  hipError_t e = hipLaunchKernel(sp.batch, dim3((unsigned)total, grid_y), sp.block, kargs, sp.lds, s);
+  return e != hipSuccess ? e : hipGetLastError();
+}
+
+}  // namespace ash3d
+
+namespace {
+
+// slots of one glue + step: [x ghosts before x, periodic only] around the
+// horizontal advection pair, advection z, Set_BC(2) face, diffusion of the
+// first direction, Neumann + diffusion of the second, Neumann + diffusion z
+constexpr int kEnsSlots = 10;
+
+const int32_t *ens_aloft(const ash3d_tr_ens *e, size_t k) {
+  return e->aloft[k].empty() ? nullptr : e->aloft[k].data();
+}
+
+// Brings member x's caches for a step with dt, imp_fac, is_aloft up to date
+// on its stream (velocity ghosts, transposed geometry, row and LU factors)
+// and checks that the step takes only launches the ensemble batches.
+int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_aloft) {
+  int rc = check_ctx_keep(x);
+  if (rc) return rc;
+  if (x->ts1_state != TS1_MIRROR)
+    return fail(ASH3D_TR_EINVAL, "ensemble member: the last call must be a step (ts1 mirrored)");
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED)
+    return fail(ASH3D_TR_EINVAL, "ensemble member: the line solver must be the partitioned one");
+  if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
+  if ((rc = set_bc1(x, true, nullptr, false))) return rc;
+  if ((rc = ensure_tgeom(x))) return rc;
+  int32_t b[6];
+  full_bounds(x, b);
+  for (int dir = 0; dir < 3; ++dir) {
+    DiffArgs a = diff_args(x, dir, dt, imp_fac, is_aloft, b);
+    if (dir == 0) transpose_x_args(x, b, a);
+    const int n = a.g.rmax - a.g.rmin + 1;
+    const int nmax = dir == 0 ? x->part_max_line : x->part_max_line_strided;
+    if (n < 3 || n > std::min(nmax, 2048))
+      return fail(ASH3D_TR_EINVAL, "ensemble member: %d active cells along dir %d (partitioned solves take 3..%d)",
+                  n, dir, std::min(nmax, 2048));
+    if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
+  }
+  return 0;
+}
+
+// The launches of member x's glue + step of parity itime % 2, one per slot
+// (args_size 0 where the member has none), after ens_prepare.
+int ens_member_specs(ash3d_tr_ctx *x, int itime, double dt, double imp_fac, const int32_t *is_aloft,
+                     LaunchSpec *slots) {
+  int rc = 0;
+  for (int q = 0; q < kEnsSlots; ++q) slots[q] = LaunchSpec{};
+  int32_t b[6];
+  full_bounds(x, b);
+  int q = 0;
+  const int first = (itime % 2 == 0) ? 1 : 0;
+  const int order[3] = {first, 1 - first, 2};
+  for (int i = 0; i < 3; ++i) {
+    const int dir = order[i];
+    if (dir == 0) {
+      // (periodic x: the x ghosts the sweep reads are the periodic copies)
+      if (x->periodic) spec_set_bc2_face(x->d, 1, 0, x->ts0(), &slots[q]);
+      ++q;
+    }
+    AdvArgs a = advect_args(x, dir, dt, is_aloft, b, x->ts0());
+    a.zg = (x->periodic && dir == 0) ? 0 : 1;
+    hipError_t e = spec_advect(a, x->nb_adv[dir], &slots[q++]);
+    if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble advection dir %d: %s", dir, hipGetErrorString(e));
+  }
+  const int dfirst = (itime % 2 == 0) ? 0 : 1;
+  for (int i = 0; i < 3; ++i) {
+    const int dir = i == 0 ? dfirst : (i == 1 ? 1 - dfirst : 2);
+    if (i == 0) spec_set_bc2_face(x->d, x->periodic, dfirst, x->ts0(), &slots[q++]);
+    else spec_neumann_dir(x->d, dir, x->periodic, x->ts0(), &slots[q++]);
+    DiffArgs a = diff_args(x, dir, dt, imp_fac, is_aloft, b);
+    if (dir == 0) transpose_x_args(x, b, a);
+    if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
+    a.sel = 1;
+    hipError_t e = spec_diff_cn_part(a, part_layout(x, dir, a), x->part_bins, x->part_shape[dir], &slots[q++]);
+    if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble diffusion dir %d: %s", dir, hipGetErrorString(e));
+  }
+  return q == kEnsSlots ? 0 : fail(ASH3D_TR_EINVAL, "ensemble step slots");
+}
+
+int ens_run_plan(ash3d_tr_ens *e, int p) {
+  for (const EnsGroup &g : e->plan[p])
+    HIPCHK(launch_batch(e->stream, g.spec, e->dev + g.args_off, (const long long *)(e->dev + g.first_off), g.n,
+                        g.total, g.gy));
+  return 0;
+}
+
+int ens_destroy(ash3d_tr_ens *e) {
+  if (!e) return 0;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (int k = 0; k < 2; ++k)
+    if (e->graph[k]) hipGraphExecDestroy(e->graph[k]);
+  if (e->dev) hipFree(e->dev);
+  if (e->done) hipEventDestroy(e->done);
+  for (hipEvent_t v : e->ready)
+    if (v) hipEventDestroy(v);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return 0;
+}
+
+int ens_build(ash3d_tr_ens *e) {
+  int rc = 0;
+  const size_t n = e->m.size();
+  for (size_t k = 0; k < n; ++k)
+    if ((rc = ens_prepare(e->m[k], e->dt[k], e->imp_fac, ens_aloft(e, k)))) return rc;
+  // every member's line factors are diagonally dominant M-matrix rows (the
+  // partitioned kernels exit at once otherwise)
+  for (size_t k = 0; k < n; ++k) {
+    ash3d_tr_ctx *x = e->m[k];
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int dir = 0; dir < 3; ++dir) {
+      int h = 0;
+      HIPCHK(hipMemcpy(&h, x->lu_ndom[dir], sizeof(int), hipMemcpyDeviceToHost));
+      if (h != 0)
+        return fail(ASH3D_TR_EINVAL, "ensemble member %zu: %d lines along dir %d are not diagonally dominant", k, h,
+                    dir);
+    }
+  }
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  std::vector<unsigned char> host;
+  std::vector<LaunchSpec> specs(n * kEnsSlots);
+  for (int p = 0; p < 2; ++p) {
+    const int itime = p == 0 ? 2 : 1;
+    for (size_t k = 0; k < n; ++k)
+      if ((rc = ens_member_specs(e->m[k], itime, e->dt[k], e->imp_fac, ens_aloft(e, k), &specs[k * kEnsSlots])))
+        return rc;
+    e->plan[p].clear();
+    e->member_launches[p] = 0;
+    for (int q = 0; q < kEnsSlots; ++q) {
+      std::vector<std::vector<size_t>> groups;  // member indices, by kernel / block / LDS
+      for (size_t k = 0; k < n; ++k) {
+        const LaunchSpec &sp = specs[k * kEnsSlots + q];
+        if (sp.args_size == 0) continue;
+        ++e->member_launches[p];
+        if (sp.grid.z != 1 || sp.grid.x == 0) return fail(ASH3D_TR_EINVAL, "ensemble launch shape");
+        size_t gi = 0;
+        for (; gi < groups.size(); ++gi) {
+          const LaunchSpec &o = specs[groups[gi][0] * kEnsSlots + q];
+          if (o.batch == sp.batch && o.block.x == sp.block.x && o.block.y == sp.block.y &&
+              o.block.z == sp.block.z && o.lds == sp.lds && o.args_size == sp.args_size)
+            break;
+        }
+        if (gi == groups.size()) groups.emplace_back();
+        groups[gi].push_back(k);
+      }
+      for (const auto &mem : groups) {
+        EnsGroup g;
+        g.spec = specs[mem[0] * kEnsSlots + q];
+        g.n = (int)mem.size();
+        g.args_off = align(host.size());
+        g.first_off = align(g.args_off + (size_t)g.n * g.spec.args_size);
+        host.resize(align(g.first_off + sizeof(long long) * (size_t)(g.n + 1)), 0);
+        long long tot = 0;
+        for (int i = 0; i <= g.n; ++i) {
+          std::memcpy(&host[g.first_off + sizeof(long long) * i], &tot, sizeof tot);
+          if (i == g.n) break;
+          const LaunchSpec &sp = specs[mem[i] * kEnsSlots + q];
+          std::memcpy(&host[g.args_off + (size_t)i * sp.args_size], sp.args, sp.args_size);
+          tot += sp.grid.x;
+          g.gy = std::max(g.gy, sp.grid.y);
+        }
+        g.total = tot;
+        e->plan[p].push_back(g);
+      }
+    }
+  }
+  HIPCHK(hipMalloc((void **)&e->dev, host.size()));
+  HIPCHK(hipMemcpy(e->dev, host.data(), host.size(), hipMemcpyHostToDevice));
+  for (int p = 0; p < 2; ++p)
+    for (const EnsGroup &g : e->plan[p])
+      if (g.spec.lds > 65536)
+        HIPCHK(hipFuncSetAttribute(g.spec.batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.spec.lds));
+  // one graph per parity
+  for (int p = 0; p < 2; ++p) {
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc2 = ens_run_plan(e, p);
+    hipGraph_t gr = nullptr;
+    hipError_t er = hipStreamEndCapture(e->stream, &gr);
+    if (rc2) {
+      if (gr) hipGraphDestroy(gr);
+      return rc2;
+    }
+    if (er != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble stream capture: %s", hipGetErrorString(er));
+    er = hipGraphInstantiate(&e->graph[p], gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (er != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble graph instantiation: %s", hipGetErrorString(er));
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ash3d_tr_ensemble_create(ash3d_tr_ctx *const *members, int32_t n, const double *dt, double imp_fac,
+                             const int32_t *const *is_aloft, ash3d_tr_ens **out) {
+  if (!out) return fail(ASH3D_TR_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (!members || !dt || n < 1) return fail(ASH3D_TR_EINVAL, "ensemble: members and dt of n >= 1 members");
+  for (int k = 0; k < n; ++k) {
+    if (!members[k]) return fail(ASH3D_TR_EINVAL, "ensemble: null member %d", k);
+    if (members[k]->device != members[0]->device) return fail(ASH3D_TR_EINVAL, "ensemble: members on two devices");
+    for (int j = 0; j < k; ++j)
+      if (members[j] == members[k]) return fail(ASH3D_TR_EINVAL, "ensemble: member %d repeated", k);
+  }
+  ash3d_tr_ens *e = new ash3d_tr_ens();
+  e->device = members[0]->device;
+  e->m.assign(members, members + n);
+  e->dt.assign(dt, dt + n);
+  e->imp_fac = imp_fac;
+  e->aloft.resize(n);
+  for (int k = 0; k < n; ++k)
+    if (is_aloft && is_aloft[k]) e->aloft[k].assign(is_aloft[k], is_aloft[k] + members[k]->d.ns);
+  int rc = 0;
+  if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->done, hipEventDisableTiming) != hipSuccess) {
+    ens_destroy(e);
+    return fail(ASH3D_TR_EDEVICE, "ensemble stream / event creation");
+  }
+  e->ready.assign(n, nullptr);
+  for (int k = 0; k < n; ++k)
+    if (hipEventCreateWithFlags(&e->ready[k], hipEventDisableTiming) != hipSuccess) {
+      ens_destroy(e);
+      return fail(ASH3D_TR_EDEVICE, "ensemble event creation");
+    }
+  if ((rc = ens_build(e))) {
+    std::string msg = g_err;
+    ens_destroy(e);
+    g_err = msg;
+    return rc;
+  }
+  *out = e;
+  return 0;
+}
+
+int ash3d_tr_ensemble_step(ash3d_tr_ens *e, int32_t itime, int32_t use_graph) {
+  if (!e) return fail(ASH3D_TR_EINVAL, "null ensemble");
+  int rc = 0;
+  const int p = ((itime % 2) + 2) % 2;
+  for (size_t k = 0; k < e->m.size(); ++k) {
+    ash3d_tr_ctx *x = e->m[k];
+    if ((rc = ens_prepare(x, e->dt[k], e->imp_fac, ens_aloft(e, k)))) return rc;
+    HIPCHK(hipEventRecord(e->ready[k], x->stream));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ready[k], 0));
+  }
+  HIPCHK(hipSetDevice(e->device));
+  if (use_graph) HIPCHK(hipGraphLaunch(e->graph[p], e->stream));
+  else if ((rc = ens_run_plan(e, p))) return rc;
+  HIPCHK(hipEventRecord(e->done, e->stream));
+  for (ash3d_tr_ctx *x : e->m) {
+    HIPCHK(hipStreamWaitEvent(x->stream, e->done, 0));
+    // as after ash3d_tr_step_glue + ash3d_tr_step
+    x->ts1_state = TS1_MIRROR;
+    x->ghost_zero_pending = false;
+  }
+  return 0;
+}
+
+int ash3d_tr_ensemble_synchronize(ash3d_tr_ens *e) {
+  if (!e) return fail(ASH3D_TR_EINVAL, "null ensemble");
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int ash3d_tr_ensemble_info(ash3d_tr_ens *e, int32_t itime, int32_t *batched_launches, int32_t *member_launches) {
+  if (!e) return fail(ASH3D_TR_EINVAL, "null ensemble");
+  const int p = ((itime % 2) + 2) % 2;
+  if (batched_launches) *batched_launches = (int32_t)e->plan[p].size();
+  if (member_launches) *member_launches = e->member_launches[p];
+  return 0;
+}
+
+int ash3d_tr_ensemble_destroy(ash3d_tr_ens *e) { return ens_destroy(e); }
 
 }  // extern "C"
 

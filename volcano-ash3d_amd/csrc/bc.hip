@@ -159,9 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_vel_all(Dims d, int periodic, double
 // indices including ghosts; x applies the periodic override.  y and z: a
 // thread per line writes its four ghosts (lanes along the contiguous x);
 // x: a thread per ghost, the four of a row in adjacent lanes.
-__global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodic, double *c0) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-  const int n = (int)blockIdx.y;
+__device__ __forceinline__ void neumann_body(const GhostArgs &G, const long long t, const int n) {
+  const Dims d = G.d;
+  const int dir = G.dir, periodic = G.periodic;
+  double *c0 = G.c0;
   int A, B, N;
   if (dir == 0) { A = d.ny + 4; B = d.nz + 4; N = d.nx; }
   else if (dir == 1) { A = d.nx + 4; B = d.nz + 4; N = d.ny; }
@@ -191,10 +192,10 @@ __global__ __launch_bounds__(kBlock) void k_neumann(Dims d, int dir, int periodi
 // ghosts included: the second horizontal direction's over every ghost
 // plane index, the z pass over every ghost plane).  dir 0: the four x
 // ghosts of a row in adjacent lanes; dir 1: lanes along x.
-__global__ __launch_bounds__(kBlock) void k_bc2_face(Dims d, int periodic, int dir, double *c0) {
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-  const int n = (int)blockIdx.y;
-  double *c = c0 + d.ncell * n;
+__device__ __forceinline__ void bc2_face_body(const GhostArgs &G, const long long t, const int n) {
+  const Dims d = G.d;
+  const int dir = G.dir, periodic = G.periodic;
+  double *c = G.c0 + d.ncell * n;
   if (dir == 0) {
     if (t >= 4LL * d.ny * d.nz) return;
     const int col = (int)(t & 3);
@@ -210,6 +211,27 @@ __global__ __launch_bounds__(kBlock) void k_bc2_face(Dims d, int periodic, int d
     const int j = (r < 2) ? (r - 1) : (d.ny + (r - 1));
     c[d.p3(i, j, k)] = c[d.p3(i, clampi(j, 1, d.ny), k)];
   }
+}
+
+// one launch per member (bin blockIdx.y) and one over the members of an
+// ensemble (grid.y = the largest bin count)
+__global__ __launch_bounds__(kBlock) void k_neumann(GhostArgs G) {
+  neumann_body(G, (long long)blockIdx.x * kBlock + threadIdx.x, (int)blockIdx.y);
+}
+__global__ __launch_bounds__(kBlock) void k_neumann_batch(Batch<GhostArgs> B) {
+  const int m = batch_member(B);
+  const GhostArgs &G = B.args[m];
+  if ((int)blockIdx.y >= G.d.ns) return;
+  neumann_body(G, ((long long)blockIdx.x - B.first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
+}
+__global__ __launch_bounds__(kBlock) void k_bc2_face(GhostArgs G) {
+  bc2_face_body(G, (long long)blockIdx.x * kBlock + threadIdx.x, (int)blockIdx.y);
+}
+__global__ __launch_bounds__(kBlock) void k_bc2_face_batch(Batch<GhostArgs> B) {
+  const int m = batch_member(B);
+  const GhostArgs &G = B.args[m];
+  if ((int)blockIdx.y >= G.d.ns) return;
+  bc2_face_body(G, ((long long)blockIdx.x - B.first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double v) {
@@ -311,9 +333,15 @@ hipError_t launch_set_bc2(hipStream_t s, const Dims &d, int periodic, double *c0
   return launch_ghosts(s, d, periodic, GH_BC2, c0);
 }
 
+void spec_set_bc2_face(const Dims &d, int periodic, int dir, double *c0, LaunchSpec *sp) {
+  const long long n = 4LL * (dir == 0 ? d.ny : d.nx) * d.nz;
+  (void)issue(nullptr, sp, (const void *)k_bc2_face, (const void *)k_bc2_face_batch, dim3(blocks_for(n), d.ns),
+              dim3(kBlock), 0, GhostArgs{d, periodic, dir, c0});
+}
+
 hipError_t launch_set_bc2_face(hipStream_t s, const Dims &d, int periodic, int dir, double *c0) {
   const long long n = 4LL * (dir == 0 ? d.ny : d.nx) * d.nz;
-  hipLaunchKernelGGL(k_bc2_face, dim3(blocks_for(n), d.ns), dim3(kBlock), 0, s, d, periodic, dir, c0);
+  hipLaunchKernelGGL(k_bc2_face, dim3(blocks_for(n), d.ns), dim3(kBlock), 0, s, GhostArgs{d, periodic, dir, c0});
   return hipGetLastError();
 }
 
@@ -325,12 +353,20 @@ hipError_t launch_copy_ghosts(hipStream_t s, const Dims &d, const double *src, d
   return launch_ghosts(s, d, 0, GH_COPY, dst, src);
 }
 
+static long long neumann_threads(const Dims &d, int dir) {
+  if (dir == 0) return 4LL * (d.ny + 4) * (d.nz + 4);  // a thread per x ghost
+  if (dir == 1) return (long long)(d.nx + 4) * (d.nz + 4);
+  return (long long)(d.nx + 4) * (d.ny + 4);
+}
+
+void spec_neumann_dir(const Dims &d, int dir, int periodic, double *c0, LaunchSpec *sp) {
+  (void)issue(nullptr, sp, (const void *)k_neumann, (const void *)k_neumann_batch,
+              dim3(blocks_for(neumann_threads(d, dir)), d.ns), dim3(kBlock), 0, GhostArgs{d, periodic, dir, c0});
+}
+
 hipError_t launch_neumann_dir(hipStream_t s, const Dims &d, int dir, int periodic, double *c0) {
-  long long nl;
-  if (dir == 0) nl = 4LL * (d.ny + 4) * (d.nz + 4);  // a thread per x ghost
-  else if (dir == 1) nl = (long long)(d.nx + 4) * (d.nz + 4);
-  else nl = (long long)(d.nx + 4) * (d.ny + 4);
-  hipLaunchKernelGGL(k_neumann, dim3(blocks_for(nl), d.ns), dim3(kBlock), 0, s, d, dir, periodic, c0);
+  hipLaunchKernelGGL(k_neumann, dim3(blocks_for(neumann_threads(d, dir)), d.ns), dim3(kBlock), 0, s,
+                     GhostArgs{d, periodic, dir, c0});
   return hipGetLastError();
 }
 

@@ -4,6 +4,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstring>
 #include <stdint.h>
 
 #include <string>
@@ -95,11 +97,44 @@ struct TileGroup {
   long long tile;
   int group;
 };
-__device__ inline TileGroup xcd_tile_group(long long b, int G) {
-  const long long per = (long long)gridDim.x / kXcds;
+__device__ inline TileGroup xcd_tile_group(long long b, int G, long long nblocks) {
+  const long long per = nblocks / kXcds;
   const long long it = (b % kXcds) * per + b / kXcds;
   return TileGroup{it / G, (int)(it % G)};
 }
+__device__ inline TileGroup xcd_tile_group(long long b, int G) { return xcd_tile_group(b, G, gridDim.x); }
+
+// Batched launches (ensembles): one launch covers the workgroups of several
+// members (contexts), member m owning workgroups first[m] .. first[m+1]-1
+// of the launch, args[m] its argument block.  Every member's count is a
+// multiple of kXcds, so a workgroup sits on the XCD it would have in the
+// member's own launch.  A kernel's body takes the workgroup index and
+// count within its member's launch.
+template <class Args>
+struct Batch {
+  const Args *__restrict__ args;
+  const long long *__restrict__ first;  // n + 1 entries
+  int n;
+};
+template <class Args>
+__device__ inline int batch_member(const Batch<Args> &B) {
+  const long long x = blockIdx.x;
+  int lo = 0, hi = B.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (B.first[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Argument block of the ghost-cell passes that a step launches (Set_BC(2)
+// on one face, the Neumann ghosts of one direction).
+struct GhostArgs {
+  Dims d;
+  int periodic, dir;
+  double *c0;
+};
 
 constexpr int kJunk = 4096;
 
@@ -158,6 +193,43 @@ struct DiffArgs {
   AloftMask aloft;
 };
 
+// A kernel launch described instead of issued: the kernel for one member
+// (single), the same kernel over a Batch of members (batch), the launch
+// shape and dynamic LDS of one member, and its argument block (args_size
+// bytes of args; 0 when the sweep needs no launch).  Launches with the
+// same batch kernel, block and LDS can run as one batched launch.
+struct LaunchSpec {
+  const void *single = nullptr, *batch = nullptr;
+  dim3 grid, block;
+  size_t lds = 0;
+  alignas(16) unsigned char args[512];
+  size_t args_size = 0;
+};
+// Fill sp (kernels, shape, LDS, argument block) or, with sp null, launch
+// the single kernel.
+template <class Args>
+hipError_t issue(hipStream_t s, LaunchSpec *sp, const void *single, const void *batch, dim3 grid, dim3 block,
+                 size_t lds, const Args &args) {
+  if (sp) {
+    static_assert(sizeof(Args) <= sizeof(sp->args), "argument block");
+    sp->single = single;
+    sp->batch = batch;
+    sp->grid = grid;
+    sp->block = block;
+    sp->lds = lds;
+    std::memcpy(sp->args, &args, sizeof args);
+    sp->args_size = sizeof args;
+    return hipSuccess;
+  }
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(single, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  void *kargs[] = {(void *)&args};
+  hipError_t e = hipLaunchKernel(single, grid, block, kargs, lds, s);
+  return e != hipSuccess ? e : hipGetLastError();
+}
+
 // launchers (defined next to their kernels)
 hipError_t launch_set_bc1_vel(hipStream_t s, const Dims &d, int periodic, double *vx, double *vy,
                               double *vz, double *vf);
@@ -180,7 +252,22 @@ hipError_t launch_umbrella_winds(hipStream_t s, const Dims &d, double *vx, doubl
 hipError_t launch_source_add(hipStream_t s, const Dims &d, double *c0, int iv, int jv, double dt,
                              const double *flux);
 hipError_t launch_advect(hipStream_t s, const AdvArgs &a, int nb);
+
+// the launch launch_advect(s, a, nb) would issue
+hipError_t spec_advect(const AdvArgs &a, int nb, LaunchSpec *sp);
+// Set_BC(2) on one face (launch_set_bc2_face), the Neumann ghosts of one
+// direction (launch_neumann_dir)
+void spec_set_bc2_face(const Dims &d, int periodic, int dir, double *c0, LaunchSpec *sp);
+void spec_neumann_dir(const Dims &d, int dir, int periodic, double *c0, LaunchSpec *sp);
+// one batched launch: members' argument blocks (device, each sp.args_size
+// bytes) and workgroup offsets (device, n + 1), total workgroups, and the
+// launch shape (block, LDS, grid.y/z) of the members
+hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const long long *dev_first,
+                        int n, long long total, unsigned grid_y);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
+// the launch launch_advect_x_rows would issue (hipErrorInvalidValue: the
+// row needs the streamed kernel)
+hipError_t spec_advect_x_rows(const AdvArgs &a, LaunchSpec *sp);
 hipError_t launch_diff_explicit(hipStream_t s, const DiffArgs &a, int nb);
 hipError_t launch_diff_cn(hipStream_t s, const DiffArgs &a, int nb);
 // the row factors L, R of every cell of the sweep's lines (bin-independent)
@@ -198,6 +285,8 @@ struct PartLayout {
 // kappa_l R_l = kappa_{l+1} L_{l+1} makes K^{-1} A and K^{-1} B symmetric)
 hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
                                bool adjoint = false);
+// the launch of the forward partitioned solves (launch_diff_cn_part)
+hipError_t spec_diff_cn_part(const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape, LaunchSpec *sp);
 // the LU factors of every line of the sweep (reads a.lr)
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom);
 // x-line solves on the original state layout (geometry, K and scratch in

@@ -76,8 +76,14 @@ struct Lev {
   static constexpr int value = (LP == 16) ? 5 : (LP == 32) ? 6 : 7;  // log2(2*LP)
 };
 
+struct CnPartArgs {
+  DiffArgs A;
+  PartArgs Q;
+};
+
 template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ, int WV>
-__global__ __launch_bounds__(T *LP *WV) void k_cn_part(DiffArgs A, PartArgs Q) {
+__device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &Q, const long long vb,
+                                             const long long vn) {
   constexpr int LPL = LP * WV;  // lanes per line
   constexpr int NT = T * LPL;
   constexpr int NPAD = LPL * M;
@@ -95,7 +101,7 @@ __global__ __launch_bounds__(T *LP *WV) void k_cn_part(DiffArgs A, PartArgs Q) {
   const long long ncell = A.d.ncell;
   const int tiles_a = (g.Na + T - 1) / T;
   const int G = (ns + Q.bins_wg - 1) / Q.bins_wg;
-  const Item tg = xcd_block_item(blockIdx.x, gridDim.x, G);
+  const Item tg = xcd_block_item(vb, vn, G);
   if (tg.tile >= (long long)tiles_a * g.Nb) return;  // padding workgroup (uniform)
   const int b = (int)(tg.tile / tiles_a) + 1;
   const int a_first = (int)(tg.tile % tiles_a) * T + 1;
@@ -465,7 +471,7 @@ __global__ __launch_bounds__(T *LP *WV) void k_cn_part(DiffArgs A, PartArgs Q) {
   // tile load (PF stores per lane, masked-off ones to the junk buffer), so
   // that the compiler can count the memory operations in flight exactly
   // and wait for one bin's loads without waiting for the next one's.
-  double *junk = A.junk + (tid + NT * (blockIdx.x % (kJunk / NT)));
+  double *junk = A.junk + (tid + NT * (vb % (kJunk / NT)));
   auto zero_cells = [&](int w, int r_lo, int r_hi) {
     const int len = r_hi - r_lo + 1;
     if (len <= 0) return;
@@ -566,23 +572,31 @@ __global__ __launch_bounds__(T *LP *WV) void k_cn_part(DiffArgs A, PartArgs Q) {
   }
 }
 
+template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ, int WV>
+__global__ __launch_bounds__(T *LP *WV) void k_cn_part(CnPartArgs P) {
+  cn_part_body<LP, M, T, CONTIG, PD, ADJ, WV>(P.A, P.Q, blockIdx.x, gridDim.x);
+}
+// the same solves over the members of an ensemble (one launch)
+template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ, int WV>
+__global__ __launch_bounds__(T *LP *WV) void k_cn_part_batch(Batch<CnPartArgs> B) {
+  const int m = batch_member(B);
+  const CnPartArgs &P = B.args[m];
+  cn_part_body<LP, M, T, CONTIG, PD, ADJ, WV>(P.A, P.Q, (long long)blockIdx.x - B.first[m],
+                                               B.first[m + 1] - B.first[m]);
+}
+
 template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = (M <= 6 ? 2 : 1), int WV = 1>
-hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
+hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q, LaunchSpec *sp) {
   const long long tiles = (long long)((a.g.Na + T - 1) / T) * a.g.Nb;
   const int G = (a.d.ns + q.bins_wg - 1) / q.bins_wg;
   // the tile, for the adjoint each lane's row kappas and inverses, for
   // several parts per line the couplings, the reduced inverse and the end values
   const size_t lds = sizeof(double) * ((size_t)T * (LP * WV * M + 1) + (ADJ ? (size_t)2 * M * T * LP * WV : 0) +
                                        (size_t)T * WV * 6 + (size_t)T * 8 * WV * WV + (size_t)T * 2 * WV);
-  if (lds > 65536) {
-    // beyond the default dynamic LDS limit (the adjoint's 512-lane tiles)
-    hipError_t e = hipFuncSetAttribute((const void *)k_cn_part<LP, M, T, CONTIG, PD, ADJ, WV>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((k_cn_part<LP, M, T, CONTIG, PD, ADJ, WV>), dim3((unsigned)padded_tiles(tiles * G)),
-                     dim3(T * LP * WV), lds, s, a, q);
-  return hipGetLastError();
+  // (beyond 64 KiB of dynamic LDS for the adjoint's 512-lane tiles)
+  return issue(s, sp, (const void *)k_cn_part<LP, M, T, CONTIG, PD, ADJ, WV>,
+               (const void *)k_cn_part_batch<LP, M, T, CONTIG, PD, ADJ, WV>, dim3((unsigned)padded_tiles(tiles * G)),
+               dim3(T * LP * WV), lds, CnPartArgs{a, q});
 }
 
 // bins in flight of the multi-wave line solves
@@ -592,7 +606,8 @@ hipError_t launch_part(hipStream_t s, const DiffArgs &a, const PartArgs &q) {
 constexpr int PDL = ASH3D_PDL;
 
 template <bool ADJ>
-hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArgs &q, bool contig, int n, int shape) {
+hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArgs &q, bool contig, int n, int shape,
+                                 LaunchSpec *sp) {
   // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
   // shape 1: contiguous lines two per workgroup (fewer waves held at each
   // barrier), strided lines in wider tiles (longer coalesced row segments)
@@ -602,50 +617,50 @@ hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArg
     // strided lines in tiles of 2-4 along the contiguous index)
     if (n > 512) {
       if (contig) {
-        if (n <= 768) return launch_part<64, 4, 1, true, false, PDL, 3>(s, a, q);
-        if (n <= 1024) return launch_part<64, 4, 1, true, false, PDL, 4>(s, a, q);
-        if (n <= 1536) return launch_part<64, 4, 1, true, false, PDL, 6>(s, a, q);
-        return launch_part<64, 4, 1, true, false, PDL, 8>(s, a, q);
+        if (n <= 768) return launch_part<64, 4, 1, true, false, PDL, 3>(s, a, q, sp);
+        if (n <= 1024) return launch_part<64, 4, 1, true, false, PDL, 4>(s, a, q, sp);
+        if (n <= 1536) return launch_part<64, 4, 1, true, false, PDL, 6>(s, a, q, sp);
+        return launch_part<64, 4, 1, true, false, PDL, 8>(s, a, q, sp);
       }
-      if (n <= 768) return launch_part<64, 4, 2, false, false, PDL, 3>(s, a, q);
-      if (n <= 1024) return launch_part<64, 4, 2, false, false, PDL, 4>(s, a, q);
-      if (n <= 1536) return launch_part<64, 4, 1, false, false, PDL, 6>(s, a, q);
-      return launch_part<64, 4, 1, false, false, PDL, 8>(s, a, q);
+      if (n <= 768) return launch_part<64, 4, 2, false, false, PDL, 3>(s, a, q, sp);
+      if (n <= 1024) return launch_part<64, 4, 2, false, false, PDL, 4>(s, a, q, sp);
+      if (n <= 1536) return launch_part<64, 4, 1, false, false, PDL, 6>(s, a, q, sp);
+      return launch_part<64, 4, 1, false, false, PDL, 8>(s, a, q, sp);
     }
   }
   if (contig) {
     if (shape == 1) {
-      if (n <= 256) return launch_part<64, 4, 2, true, ADJ>(s, a, q);
-      if (n <= 384) return launch_part<64, 6, 2, true, ADJ>(s, a, q);
-      return launch_part<64, 8, 2, true, ADJ>(s, a, q);
+      if (n <= 256) return launch_part<64, 4, 2, true, ADJ>(s, a, q, sp);
+      if (n <= 384) return launch_part<64, 6, 2, true, ADJ>(s, a, q, sp);
+      return launch_part<64, 8, 2, true, ADJ>(s, a, q, sp);
     }
-    if (n <= 256) return launch_part<64, 4, 4, true, ADJ>(s, a, q);
-    if (n <= 384) return launch_part<64, 6, 4, true, ADJ>(s, a, q);
-    return launch_part<64, 8, 4, true, ADJ>(s, a, q);
+    if (n <= 256) return launch_part<64, 4, 4, true, ADJ>(s, a, q, sp);
+    if (n <= 384) return launch_part<64, 6, 4, true, ADJ>(s, a, q, sp);
+    return launch_part<64, 8, 4, true, ADJ>(s, a, q, sp);
   }
   // lines of 65..96 cells in 16 lanes of 5 or 6 rows (less padding than
   // 32 lanes of 4: 80-cell columns 11.5 -> 6.5 ms a sweep on the
   // 1440x720x80x20 global grid)
   if (shape == 1) {
-    if (n <= 64) return launch_part<16, 4, 32, false, ADJ>(s, a, q);
-    if (n <= 80) return launch_part<16, 5, 16, false, ADJ>(s, a, q);
-    if (n <= 96) return launch_part<16, 6, 16, false, ADJ>(s, a, q);
-    if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
-    if (n <= 256) return launch_part<32, 8, 16, false, ADJ>(s, a, q);
-    return launch_part<64, 8, 4, false, ADJ>(s, a, q);
+    if (n <= 64) return launch_part<16, 4, 32, false, ADJ>(s, a, q, sp);
+    if (n <= 80) return launch_part<16, 5, 16, false, ADJ>(s, a, q, sp);
+    if (n <= 96) return launch_part<16, 6, 16, false, ADJ>(s, a, q, sp);
+    if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q, sp);
+    if (n <= 256) return launch_part<32, 8, 16, false, ADJ>(s, a, q, sp);
+    return launch_part<64, 8, 4, false, ADJ>(s, a, q, sp);
   }
-  if (n <= 64) return launch_part<16, 4, 16, false, ADJ>(s, a, q);
-  if (n <= 80) return launch_part<16, 5, 32, false, ADJ>(s, a, q);
-  if (n <= 96) return launch_part<16, 6, 32, false, ADJ>(s, a, q);
-  if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q);
-  if (n <= 256) return launch_part<64, 4, 8, false, ADJ>(s, a, q);
-  return launch_part<64, 8, 4, false, ADJ>(s, a, q);
+  if (n <= 64) return launch_part<16, 4, 16, false, ADJ>(s, a, q, sp);
+  if (n <= 80) return launch_part<16, 5, 32, false, ADJ>(s, a, q, sp);
+  if (n <= 96) return launch_part<16, 6, 32, false, ADJ>(s, a, q, sp);
+  if (n <= 128) return launch_part<32, 4, 16, false, ADJ>(s, a, q, sp);
+  if (n <= 256) return launch_part<64, 4, 8, false, ADJ>(s, a, q, sp);
+  return launch_part<64, 8, 4, false, ADJ>(s, a, q, sp);
 }
 
 }  // namespace
 
-hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
-                               bool adjoint) {
+static hipError_t diff_cn_part_any(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
+                                   bool adjoint, LaunchSpec *sp) {
   const int n = a.g.rmax - a.g.rmin + 1;
   PartArgs q;
   q.st0 = lay.st0; q.sta = lay.sta; q.stb = lay.stb; q.str = lay.str;
@@ -654,8 +669,17 @@ hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayou
   q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
   if (n < 3 || n > (adjoint ? 512 : 2048)) return hipErrorInvalidValue;
   if (adjoint && !q.kap) return hipErrorInvalidValue;
-  return adjoint ? launch_cn_part_shapes<true>(s, a, q, lay.str == 1, n, shape)
-                 : launch_cn_part_shapes<false>(s, a, q, lay.str == 1, n, shape);
+  return adjoint ? launch_cn_part_shapes<true>(s, a, q, lay.str == 1, n, shape, sp)
+                 : launch_cn_part_shapes<false>(s, a, q, lay.str == 1, n, shape, sp);
+}
+
+hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
+                               bool adjoint) {
+  return diff_cn_part_any(s, a, lay, bins_wg, shape, adjoint, nullptr);
+}
+
+hipError_t spec_diff_cn_part(const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape, LaunchSpec *sp) {
+  return diff_cn_part_any(nullptr, a, lay, bins_wg, shape, false, sp);
 }
 
 }  // namespace ash3d

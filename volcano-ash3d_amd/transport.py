@@ -186,6 +186,13 @@ _SIGS = {
     "ash3d_tr_graph_capture": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_double, _ip],
     "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
+    "ash3d_tr_ensemble_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, _dp, ctypes.c_double,
+                                 ctypes.POINTER(_ip), ctypes.POINTER(ctypes.c_void_p)],
+    "ash3d_tr_ensemble_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
+    "ash3d_tr_ensemble_synchronize": [ctypes.c_void_p],
+    "ash3d_tr_ensemble_info": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                               ctypes.POINTER(ctypes.c_int32)],
+    "ash3d_tr_ensemble_destroy": [ctypes.c_void_p],
     "ash3d_tr_met_push": [ctypes.c_void_p, _fp, _fp, _fp, _fp],
     "ash3d_tr_met_interp": [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32],
     "ash3d_tr_adjust_dt": [ctypes.c_void_p, ctypes.POINTER(DtParams), _dp, _dp],
@@ -595,6 +602,63 @@ class Transport:
             self._check(self.lib.ash3d_tr_profile_read(self._ctx, k, ctypes.byref(ms), ctypes.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+
+class Ensemble:
+    """Batched steps of several Transport contexts on one device
+    (ash3d_tr_ensemble_*): each step() does, for every member, what
+    step_glue() + transport_step(itime, True) does, as one batched launch
+    per sweep kind and kernel variant (one HIP graph with use_graph).
+    Every member must have taken a step first, with the partitioned line
+    solver and Crank-Nicolson diffusion; each member's dt, imp_fac and
+    is_aloft are taken at construction."""
+
+    def __init__(self, members, imp_fac: float | None = None):
+        self.lib = load()
+        self.members = list(members)
+        n = len(self.members)
+        if imp_fac is None:
+            imp_fac = float(self.members[0].st.imp_fac)
+        ctxs = (ctypes.c_void_p * n)(*[m._ctx.value for m in self.members])
+        self._dt = np.array([float(m.st.dt) for m in self.members], dtype=np.float64)
+        self._masks = [np.ascontiguousarray(m.st.is_aloft, dtype=np.int32) for m in self.members]
+        masks = (_ip * n)(*[a.ctypes.data_as(_ip) for a in self._masks])
+        self._ens = ctypes.c_void_p()
+        self._check(self.lib.ash3d_tr_ensemble_create(ctxs, n, self._dt.ctypes.data_as(_dp), float(imp_fac),
+                                                      masks, ctypes.byref(self._ens)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise Ash3dError(rc, self.lib.ash3d_tr_last_error().decode())
+
+    def step(self, itime: int, use_graph: bool = True):
+        self._check(self.lib.ash3d_tr_ensemble_step(self._ens, int(itime), int(bool(use_graph))))
+
+    def synchronize(self):
+        self._check(self.lib.ash3d_tr_ensemble_synchronize(self._ens))
+
+    def info(self, itime: int = 1):
+        """(batched launches, the members' own launches) of one step."""
+        b, m = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.ash3d_tr_ensemble_info(self._ens, int(itime), ctypes.byref(b), ctypes.byref(m)))
+        return b.value, m.value
+
+    def close(self):
+        if self._ens:
+            self.lib.ash3d_tr_ensemble_destroy(self._ens)
+            self._ens = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def write_esri_ascii(path: str, field: np.ndarray, mask=None, fill_value: str = "-9999.", is_latlon=False,
