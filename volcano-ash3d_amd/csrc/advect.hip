@@ -285,10 +285,12 @@ __global__ __launch_bounds__(256) void k_advect(AdvArgs A) {
 }
 // the same sweep over the members of an ensemble (one launch)
 template <int DIR, int NB, bool TOPO, bool GM>
-__global__ __launch_bounds__(256) void k_advect_batch(Batch<AdvArgs> B) {
-  const int m = batch_member(B);
-  const AdvArgs &A = B.args[m];
-  advect_body<DIR, NB, TOPO, GM>(A, (long long)blockIdx.x - B.first[m], B.first[m + 1] - B.first[m]);
+__global__ __launch_bounds__(256) void k_advect_batch(
+    const ASH3D_CONSTANT AdvArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const AdvArgs &A = *(const AdvArgs *)(args + m);
+  advect_body<DIR, NB, TOPO, GM>(A, (long long)blockIdx.x - first[m], first[m + 1] - first[m]);
 }
 
 template <int DIR, int NB>

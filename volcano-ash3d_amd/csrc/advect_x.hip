@@ -736,30 +736,36 @@ __global__ __launch_bounds__(256) void k_advect_x_rows(AdvArgs A) {
   advect_x_rows_body<CPT, TOPO>(A, blockIdx.x);
 }
 template <int CPT, bool TOPO>
-__global__ __launch_bounds__(256) void k_advect_x_rows_batch(Batch<AdvArgs> B) {
-  const int m = batch_member(B);
-  const AdvArgs &A = B.args[m];
-  advect_x_rows_body<CPT, TOPO>(A, (long long)blockIdx.x - B.first[m]);
+__global__ __launch_bounds__(256) void k_advect_x_rows_batch(
+    const ASH3D_CONSTANT AdvArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const AdvArgs &A = *(const AdvArgs *)(args + m);
+  advect_x_rows_body<CPT, TOPO>(A, (long long)blockIdx.x - first[m]);
 }
 template <int NC, int E, bool TOPO>
 __global__ __launch_bounds__(256) void k_advect_x_wave(AdvArgs A) {
   advect_x_wave_body<NC, E, TOPO>(A, blockIdx.x);
 }
 template <int NC, int E, bool TOPO>
-__global__ __launch_bounds__(256) void k_advect_x_wave_batch(Batch<AdvArgs> B) {
-  const int m = batch_member(B);
-  const AdvArgs &A = B.args[m];
-  advect_x_wave_body<NC, E, TOPO>(A, (long long)blockIdx.x - B.first[m]);
+__global__ __launch_bounds__(256) void k_advect_x_wave_batch(
+    const ASH3D_CONSTANT AdvArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const AdvArgs &A = *(const AdvArgs *)(args + m);
+  advect_x_wave_body<NC, E, TOPO>(A, (long long)blockIdx.x - first[m]);
 }
 template <int NC, bool TOPO>
 __global__ __launch_bounds__(512) void k_advect_x_seg(SegArgs S) {
   advect_x_seg_body<NC, TOPO>(S.A, S.L, blockIdx.x);
 }
 template <int NC, bool TOPO>
-__global__ __launch_bounds__(512) void k_advect_x_seg_batch(Batch<SegArgs> B) {
-  const int m = batch_member(B);
-  const SegArgs &S = B.args[m];
-  advect_x_seg_body<NC, TOPO>(S.A, S.L, (long long)blockIdx.x - B.first[m]);
+__global__ __launch_bounds__(512) void k_advect_x_seg_batch(
+    const ASH3D_CONSTANT SegArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const SegArgs &S = *(const SegArgs *)(args + m);
+  advect_x_seg_body<NC, TOPO>(S.A, S.L, (long long)blockIdx.x - first[m]);
 }
 
 template <int CPT>

@@ -1911,11 +1911,12 @@ int ash3d_tr_step_subgrid(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use
 
 // One batched launch of an ensemble step: the members (n of them) whose
 // launch in this slot of the step selects the same kernel, block and LDS;
-// their argument blocks and workgroup offsets at byte offsets args_off and
-// first_off of the ensemble's device buffer.
+// their argument blocks, the member of each workgroup and the members'
+// first workgroups at byte offsets args_off, member_off and first_off of
+// the ensemble's device buffer.
 struct EnsGroup {
   LaunchSpec spec;  // the first member's
-  size_t args_off = 0, first_off = 0;
+  size_t args_off = 0, member_off = 0, first_off = 0;
   int n = 0;
   long long total = 0;  // workgroups along x, all members
   unsigned gy = 1;      // grid.y: the largest of the members'
@@ -1938,19 +1939,11 @@ struct ash3d_tr_ens {
 
 namespace ash3d {
 
-struct BatchAny {
-  const void *args;
-  const long long *first;
-  int n;
-};
-static_assert(sizeof(BatchAny) == sizeof(Batch<AdvArgs>), "batch descriptor layout");
-
-hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const long long *dev_first,
-                        int n, long long total, unsigned grid_y) {
-  if (n <= 0 || total <= 0) return hipSuccess;
+hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const int *dev_member,
+                        const long long *dev_first, long long total, unsigned grid_y) {
+  if (total <= 0) return hipSuccess;
   if (total > 0x7fffffffLL || grid_y == 0 || grid_y > 65535) return hipErrorInvalidValue;
-  BatchAny b{dev_args, dev_first, n};
-  void *kargs[] = {(void *)&b};
+  void *kargs[] = {(void *)&dev_args, (void *)&dev_member, (void *)&dev_first};
   hipError_t e = hipLaunchKernel(sp.batch, dim3((unsigned)total, grid_y), sp.block, kargs, sp.lds, s);
   return e != hipSuccess ? e : hipGetLastError();
 }
@@ -2036,8 +2029,8 @@ int ens_member_specs(ash3d_tr_ctx *x, int itime, double dt, double imp_fac, cons
 
 int ens_run_plan(ash3d_tr_ens *e, int p) {
   for (const EnsGroup &g : e->plan[p])
-    HIPCHK(launch_batch(e->stream, g.spec, e->dev + g.args_off, (const long long *)(e->dev + g.first_off), g.n,
-                        g.total, g.gy));
+    HIPCHK(launch_batch(e->stream, g.spec, e->dev + g.args_off, (const int *)(e->dev + g.member_off),
+                        (const long long *)(e->dev + g.first_off), g.total, g.gy));
   return 0;
 }
 
@@ -2105,19 +2098,27 @@ int ens_build(ash3d_tr_ens *e) {
         EnsGroup g;
         g.spec = specs[mem[0] * kEnsSlots + q];
         g.n = (int)mem.size();
+        long long tot = 0;
+        for (size_t k : mem) {
+          tot += specs[k * kEnsSlots + q].grid.x;
+          g.gy = std::max(g.gy, specs[k * kEnsSlots + q].grid.y);
+        }
+        if (tot > 0x7fffffffLL) return fail(ASH3D_TR_EINVAL, "ensemble launch of %lld workgroups", tot);
+        g.total = tot;
         g.args_off = align(host.size());
         g.first_off = align(g.args_off + (size_t)g.n * g.spec.args_size);
-        host.resize(align(g.first_off + sizeof(long long) * (size_t)(g.n + 1)), 0);
-        long long tot = 0;
+        g.member_off = align(g.first_off + sizeof(long long) * (size_t)(g.n + 1));
+        host.resize(align(g.member_off + sizeof(int) * (size_t)tot), 0);
+        long long f = 0;
         for (int i = 0; i <= g.n; ++i) {
-          std::memcpy(&host[g.first_off + sizeof(long long) * i], &tot, sizeof tot);
+          std::memcpy(&host[g.first_off + sizeof(long long) * i], &f, sizeof f);
           if (i == g.n) break;
           const LaunchSpec &sp = specs[mem[i] * kEnsSlots + q];
           std::memcpy(&host[g.args_off + (size_t)i * sp.args_size], sp.args, sp.args_size);
-          tot += sp.grid.x;
-          g.gy = std::max(g.gy, sp.grid.y);
+          for (long long w = 0; w < (long long)sp.grid.x; ++w)
+            std::memcpy(&host[g.member_off + sizeof(int) * (size_t)(f + w)], &i, sizeof i);
+          f += sp.grid.x;
         }
-        g.total = tot;
         e->plan[p].push_back(g);
       }
     }

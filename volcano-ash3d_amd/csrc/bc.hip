@@ -218,20 +218,24 @@ __device__ __forceinline__ void bc2_face_body(const GhostArgs &G, const long lon
 __global__ __launch_bounds__(kBlock) void k_neumann(GhostArgs G) {
   neumann_body(G, (long long)blockIdx.x * kBlock + threadIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(kBlock) void k_neumann_batch(Batch<GhostArgs> B) {
-  const int m = batch_member(B);
-  const GhostArgs &G = B.args[m];
+__global__ __launch_bounds__(kBlock) void k_neumann_batch(
+    const ASH3D_CONSTANT GhostArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const GhostArgs &G = *(const GhostArgs *)(args + m);
   if ((int)blockIdx.y >= G.d.ns) return;
-  neumann_body(G, ((long long)blockIdx.x - B.first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
+  neumann_body(G, ((long long)blockIdx.x - first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
 }
 __global__ __launch_bounds__(kBlock) void k_bc2_face(GhostArgs G) {
   bc2_face_body(G, (long long)blockIdx.x * kBlock + threadIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(kBlock) void k_bc2_face_batch(Batch<GhostArgs> B) {
-  const int m = batch_member(B);
-  const GhostArgs &G = B.args[m];
+__global__ __launch_bounds__(kBlock) void k_bc2_face_batch(
+    const ASH3D_CONSTANT GhostArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const GhostArgs &G = *(const GhostArgs *)(args + m);
   if ((int)blockIdx.y >= G.d.ns) return;
-  bc2_face_body(G, ((long long)blockIdx.x - B.first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
+  bc2_face_body(G, ((long long)blockIdx.x - first[m]) * kBlock + threadIdx.x, (int)blockIdx.y);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double v) {

@@ -105,28 +105,20 @@ __device__ inline TileGroup xcd_tile_group(long long b, int G, long long nblocks
 __device__ inline TileGroup xcd_tile_group(long long b, int G) { return xcd_tile_group(b, G, gridDim.x); }
 
 // Batched launches (ensembles): one launch covers the workgroups of several
-// members (contexts), member m owning workgroups first[m] .. first[m+1]-1
-// of the launch, args[m] its argument block.  Every member's count is a
-// multiple of kXcds, so a workgroup sits on the XCD it would have in the
-// member's own launch.  A kernel's body takes the workgroup index and
-// count within its member's launch.
-template <class Args>
-struct Batch {
-  const Args *__restrict__ args;
-  const long long *__restrict__ first;  // n + 1 entries
-  int n;
-};
-template <class Args>
-__device__ inline int batch_member(const Batch<Args> &B) {
-  const long long x = blockIdx.x;
-  int lo = 0, hi = B.n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (B.first[mid] <= x) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
+// members (contexts).  A batched kernel takes the members' argument blocks
+// (args), the member of each workgroup (member[blockIdx.x]) and each
+// member's first workgroup (first, n + 1 entries): member m owns
+// workgroups first[m] .. first[m+1]-1 and reads args[m].  The three are
+// kernel parameters in the constant address space (ASH3D_CONSTANT), so
+// that the argument blocks are read with scalar loads and the pointers in
+// them are known to address global memory: the kernels issue global, not
+// flat, loads and stores, as with arguments passed by value.  Every member's
+// workgroup count is a multiple of kXcds where its kernel maps workgroups
+// to XCDs, so a workgroup sits on the XCD it would have in the member's
+// own launch.  A kernel's body takes the workgroup index and count within
+// its member's launch.
+
+#define ASH3D_CONSTANT __attribute__((address_space(4)))
 
 // Argument block of the ghost-cell passes that a step launches (Set_BC(2)
 // on one face, the Neumann ghosts of one direction).
@@ -260,10 +252,11 @@ hipError_t spec_advect(const AdvArgs &a, int nb, LaunchSpec *sp);
 void spec_set_bc2_face(const Dims &d, int periodic, int dir, double *c0, LaunchSpec *sp);
 void spec_neumann_dir(const Dims &d, int dir, int periodic, double *c0, LaunchSpec *sp);
 // one batched launch: members' argument blocks (device, each sp.args_size
-// bytes) and workgroup offsets (device, n + 1), total workgroups, and the
-// launch shape (block, LDS, grid.y/z) of the members
-hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const long long *dev_first,
-                        int n, long long total, unsigned grid_y);
+// bytes), the member of each of the total workgroups and the members'
+// first workgroups (device), and the launch shape (block, LDS) of the
+// members with grid.y
+hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const int *dev_member,
+                        const long long *dev_first, long long total, unsigned grid_y);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
 // the launch launch_advect_x_rows would issue (hipErrorInvalidValue: the
 // row needs the streamed kernel)

@@ -578,11 +578,13 @@ __global__ __launch_bounds__(T *LP *WV) void k_cn_part(CnPartArgs P) {
 }
 // the same solves over the members of an ensemble (one launch)
 template <int LP, int M, int T, bool CONTIG, int PD, bool ADJ, int WV>
-__global__ __launch_bounds__(T *LP *WV) void k_cn_part_batch(Batch<CnPartArgs> B) {
-  const int m = batch_member(B);
-  const CnPartArgs &P = B.args[m];
-  cn_part_body<LP, M, T, CONTIG, PD, ADJ, WV>(P.A, P.Q, (long long)blockIdx.x - B.first[m],
-                                               B.first[m + 1] - B.first[m]);
+__global__ __launch_bounds__(T *LP *WV) void k_cn_part_batch(
+    const ASH3D_CONSTANT CnPartArgs *__restrict__ args, const ASH3D_CONSTANT int *__restrict__ member,
+    const ASH3D_CONSTANT long long *__restrict__ first) {
+  const int m = member[blockIdx.x];
+  const CnPartArgs &P = *(const CnPartArgs *)(args + m);
+  cn_part_body<LP, M, T, CONTIG, PD, ADJ, WV>(P.A, P.Q, (long long)blockIdx.x - first[m],
+                                               first[m + 1] - first[m]);
 }
 
 template <int LP, int M, int T, bool CONTIG, bool ADJ, int PD = (M <= 6 ? 2 : 1), int WV = 1>
