@@ -233,7 +233,13 @@ enum {
    * per line (SPIKE coupling of the waves' parts), longer lines in dgtsv
    * order */
   ASH3D_TUNE_PART_MAX_LINE = 15,
-  ASH3D_TUNE_PART_MAX_LINE_STRIDED = 16
+  ASH3D_TUNE_PART_MAX_LINE_STRIDED = 16,
+  /* advection of the step with diffusion (ash3d_tr_step, graph and
+   * ensemble steps): 1 (default) the second horizontal sweep and the
+   * vertical sweep in one pass (the first sweep writes the ts1 slot, the
+   * fused pass reads it with the halo along its lines and writes ts0;
+   * bitwise the split sweeps), 0 three separate in-place sweeps */
+  ASH3D_TUNE_ADV_FUSED = 17
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits
@@ -266,7 +272,8 @@ enum {
   ASH3D_PROF_ADJ_ADVECT = 9,   /* adjoint advection sweeps */
   ASH3D_PROF_ADJ_DIFFUSE = 10, /* adjoint diffusion sweeps */
   ASH3D_PROF_COPY = 11,        /* state snapshots and checkpoints */
-  ASH3D_PROF_NKINDS = 12
+  ASH3D_PROF_ADVECT_HZ = 12,   /* fused second horizontal + vertical advection */
+  ASH3D_PROF_NKINDS = 13
 };
 int ash3d_tr_profile(ash3d_tr_ctx *ctx, int32_t enable);
 int ash3d_tr_profile_read(ash3d_tr_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libash3d_tr.so")
 SOURCES = ["ash3d_tr.hip", "bc.hip", "advect.hip", "advect_x.hip", "diffuse.hip", "totals.hip", "transpose.hip",
-           "adjoint.hip", "meteo.hip", "linesolve.hip", "products.hip", "source.hip"]
+           "adjoint.hip", "meteo.hip", "linesolve.hip", "products.hip", "source.hip", "advect_hz.hip"]
 ARCH = os.environ.get("ASH3D_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the kernels evaluate every expression with the

@@ -204,6 +204,12 @@ struct ash3d_tr_ctx {
   // z default is the autotuner's choice on the 360x240x50x15 forecast grid
   int adv_gy[3] = {0, 0, 4};
   int adv_xrows = 1;
+  int adv_fused = 1;  // second horizontal + vertical advection in one pass (step with diffusion)
+  // dt / kappa per cell for the fused pass, valid for dtv_dt
+  double *dtvc = nullptr;
+  double dtv_dt = 0.0;
+  bool dtv_ok = false;
+  double graph_dtv[2] = {-1.0, -1.0};  // dt of each captured graph's fused pass (< 0: none)
   // kernel timing (ash3d_tr_profile): event pairs per kernel kind
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -353,6 +359,38 @@ AdvArgs advect_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft
 // state and writes its result to out (every interior cell of the sweep's
 // lines, then the ghost cells copied from state): out ends as state would
 // after the in-place sweep, and state is left as it was.
+// The fused advection of the step with diffusion: the first sweep (dir)
+// from ts0 into the ts1 slot, its ghosts read as Set_BC(1)'s zeros (x on a
+// periodic grid: the copies Set_BC(2) leaves), and the arguments of the
+// pass that runs the second horizontal sweep (dir2) and the vertical
+// sweep from ts1 into ts0
+AdvArgs fused_first_args(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds) {
+  AdvArgs a = advect_args(x, dir, dt, is_aloft, bounds, x->ts0());
+  a.zg = (x->periodic && dir == 0) ? 0 : 1;
+  a.cout = x->ts1();
+  return a;
+}
+void fused_hz_args(ash3d_tr_ctx *x, int dir2, double dt, const int32_t *is_aloft, const int32_t *bounds, AdvArgs *h,
+                   AdvArgs *z) {
+  *h = advect_args(x, dir2, dt, is_aloft, bounds, x->ts1());
+  *z = advect_args(x, 2, dt, is_aloft, bounds, x->ts1());
+  z->cout = x->ts0();
+  h->zg = z->zg = 1;
+  h->dtv = z->dtv = x->dtvc;
+}
+// dt / kappa of every cell for the fused pass, recomputed when dt changes
+// (a geometry upload invalidates it)
+int ensure_dtv(ash3d_tr_ctx *x, double dt) {
+  int rc = 0;
+  if (!x->dtvc && (rc = dev_alloc(&x->dtvc, x->d.ncell))) return rc;
+  if (!x->dtv_ok || x->dtv_dt != dt) {
+    HIPCHK(launch_cell_dtv(x->stream, x->kap, x->dtvc, x->d.ncell, dt));
+    x->dtv_ok = true;
+    x->dtv_dt = dt;
+  }
+  return 0;
+}
+
 int sweep_advect(ash3d_tr_ctx *x, int dir, double dt, const int32_t *is_aloft, const int32_t *bounds,
                  bool with_bc, bool bc_vel, double *state = nullptr, double *out = nullptr, bool to_sink = true,
                  bool zero_ghosts = false) {
@@ -829,7 +867,7 @@ int ash3d_tr_destroy(ash3d_tr_ctx *x) {
                     x->ddx, x->ddy, x->ofxz1, x->ofxz2, x->ofyz1, x->ofyz2, x->ofxy1, x->ofxy2, x->dep,
                     x->fac, x->flux, x->tot, x->tq, x->tkap, x->tsx, x->tkx, x->lam, x->snap_b,
                     x->snap_z, x->adj_sb, x->sink, x->src_grad, x->col, x->lr[0], x->lr[1], x->lr[2],
-                    x->lu[0], x->lu[1], x->lu[2], x->junk, x->prod};
+                    x->lu[0], x->lu[1], x->lu[2], x->junk, x->prod, x->dtvc};
   for (double *p : ptrs)
     if (p) hipFree(p);
   for (double *p : x->ckpt)
@@ -881,6 +919,7 @@ int ash3d_tr_sync_from_host(ash3d_tr_ctx *x, uint32_t mask) {
   }
   if (mask & ASH3D_SYNC_VF)
     if ((rc = upload(x, x->vf, h.vf_pd, x->n_conc(), 0.0))) return rc;
+  if (mask & ASH3D_SYNC_GEOMETRY) x->dtv_ok = false;
   if (mask & (ASH3D_SYNC_GEOMETRY | ASH3D_SYNC_DIFFK)) {
     x->tgeom_valid = false;
     x->lr_ok[0] = x->lr_ok[1] = x->lr_ok[2] = false;
@@ -1082,13 +1121,36 @@ int ash3d_tr_step(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t use_diffusi
   const int first = (itime % 2 == 0) ? 1 : 0;
   const bool again = x->periodic != 0 && !zg;
   const int order[3] = {first, 1 - first, 2};
-  for (int q = 0; q < 3; ++q) {
-    const int dir = order[q];
-    if (zg && x->periodic && dir == 0) {
+  if (zg && x->adv_fused) {
+    // the first sweep into the ts1 slot, then the second horizontal and
+    // the vertical sweep in one pass from ts1 into ts0
+    if ((rc = ensure_dtv(x, dt))) return rc;
+    if (x->capturing) x->graph_dtv[itime % 2] = dt;
+    if (x->periodic && first == 0) {
       ProfScope ps(x, ASH3D_PROF_BC);
       HIPCHK(launch_set_bc2_face(x->stream, x->d, 1, 0, x->ts0()));
     }
-    if ((rc = sweep_advect(x, dir, dt, is_aloft, b, q > 0 && again, false, nullptr, nullptr, true, zg))) return rc;
+    {
+      AdvArgs a = fused_first_args(x, first, dt, is_aloft, b);
+      ProfScope ps(x, ASH3D_PROF_ADVECT_X + first);
+      HIPCHK(launch_advect(x->stream, a, x->nb_adv[first]));
+    }
+    {
+      AdvArgs h, z;
+      fused_hz_args(x, 1 - first, dt, is_aloft, b, &h, &z);
+      ProfScope ps(x, ASH3D_PROF_ADVECT_HZ);
+      HIPCHK(launch_advect_hz(x->stream, h, z));
+    }
+    x->ts1_state = TS1_MIRROR;
+  } else {
+    for (int q = 0; q < 3; ++q) {
+      const int dir = order[q];
+      if (zg && x->periodic && dir == 0) {
+        ProfScope ps(x, ASH3D_PROF_BC);
+        HIPCHK(launch_set_bc2_face(x->stream, x->d, 1, 0, x->ts0()));
+      }
+      if ((rc = sweep_advect(x, dir, dt, is_aloft, b, q > 0 && again, false, nullptr, nullptr, true, zg))) return rc;
+    }
   }
   if (use_diffusion && (rc = step_diffusion(x, itime, dt, use_cn, imp_fac, is_aloft, b))) return rc;
   // every ts0 ghost has been rewritten (Set_BC(1), or the diffusion's
@@ -1244,6 +1306,11 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     x->adv_gy[param == ASH3D_TUNE_ADV_GROUPS_Y ? 1 : 2] = value;
     return 0;
   }
+  if (param == ASH3D_TUNE_ADV_FUSED) {
+    if (value != 0 && value != 1) return fail(ASH3D_TR_EINVAL, "fused advection must be 0 or 1 (got %d)", value);
+    x->adv_fused = value;
+    return 0;
+  }
   if (param == ASH3D_TUNE_ADV_X_ROWS) {
     if (value != 0 && value != 1)
       return fail(ASH3D_TR_EINVAL, "x rows kernel must be 0 (workgroup rows) or 1 (wavefront rows) (got %d)", value);
@@ -1304,6 +1371,7 @@ int ash3d_tr_tune_get(ash3d_tr_ctx *x, int32_t param, int32_t *value) {
     case ASH3D_TUNE_ADV_X_ROWS: *value = x->adv_xrows; return 0;
     case ASH3D_TUNE_PART_MAX_LINE: *value = x->part_max_line; return 0;
     case ASH3D_TUNE_PART_MAX_LINE_STRIDED: *value = x->part_max_line_strided; return 0;
+    case ASH3D_TUNE_ADV_FUSED: *value = x->adv_fused; return 0;
     default: return fail(ASH3D_TR_EINVAL, "unknown tuning parameter %d", param);
   }
 }
@@ -1688,6 +1756,8 @@ int ash3d_tr_graph_capture(ash3d_tr_ctx *x, int32_t itime, double dt, int32_t us
     x->graph[slot] = nullptr;
   }
   for (int k = 0; k < 3; ++k) x->lu_args_set[k] = false;
+  x->graph_dtv[slot] = -1.0;
+  if (use_diffusion && x->adv_fused && (rc = ensure_dtv(x, dt))) return rc;  // (outside the capture)
   HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
   x->capturing = true;
   int rc2 = ash3d_tr_step_glue(x);
@@ -1721,6 +1791,7 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *x, int32_t itime) {
   // (eagerly, ahead of the graph on the stream) what a geometry/K upload
   // or an eager step with another dt or range replaced since the capture
   if (x->graph_tgeom[slot] && (rc = ensure_tgeom(x))) return rc;
+  if (x->graph_dtv[slot] >= 0.0 && (rc = ensure_dtv(x, x->graph_dtv[slot]))) return rc;
   for (int k = 0; k < 3; ++k) {
     if (!x->graph_lu_on[slot][k]) continue;
     DiffArgs a = x->graph_lu[slot][k];
@@ -1974,6 +2045,7 @@ int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_al
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if ((rc = set_bc1(x, true, nullptr, false))) return rc;
   if ((rc = ensure_tgeom(x))) return rc;
+  if (x->adv_fused && (rc = ensure_dtv(x, dt))) return rc;
   int32_t b[6];
   full_bounds(x, b);
   for (int dir = 0; dir < 3; ++dir) {
@@ -2000,17 +2072,30 @@ int ens_member_specs(ash3d_tr_ctx *x, int itime, double dt, double imp_fac, cons
   int q = 0;
   const int first = (itime % 2 == 0) ? 1 : 0;
   const int order[3] = {first, 1 - first, 2};
-  for (int i = 0; i < 3; ++i) {
-    const int dir = order[i];
-    if (dir == 0) {
-      // (periodic x: the x ghosts the sweep reads are the periodic copies)
-      if (x->periodic) spec_set_bc2_face(x->d, 1, 0, x->ts0(), &slots[q]);
-      ++q;
+  if (x->adv_fused) {
+    // the first sweep into ts1 and the fused pass, in the slots of the
+    // first and second sweeps of the split advection
+    if (first == 0 && x->periodic) spec_set_bc2_face(x->d, 1, 0, x->ts0(), &slots[0]);
+    hipError_t e = spec_advect(fused_first_args(x, first, dt, is_aloft, b), x->nb_adv[first], &slots[first == 0 ? 1 : 0]);
+    if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble advection dir %d: %s", first, hipGetErrorString(e));
+    AdvArgs h, z;
+    fused_hz_args(x, 1 - first, dt, is_aloft, b, &h, &z);
+    if ((e = spec_advect_hz(h, z, &slots[2])) != hipSuccess)
+      return fail(ASH3D_TR_EDEVICE, "ensemble fused advection: %s", hipGetErrorString(e));
+    q = 4;
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      const int dir = order[i];
+      if (dir == 0) {
+        // (periodic x: the x ghosts the sweep reads are the periodic copies)
+        if (x->periodic) spec_set_bc2_face(x->d, 1, 0, x->ts0(), &slots[q]);
+        ++q;
+      }
+      AdvArgs a = advect_args(x, dir, dt, is_aloft, b, x->ts0());
+      a.zg = (x->periodic && dir == 0) ? 0 : 1;
+      hipError_t e = spec_advect(a, x->nb_adv[dir], &slots[q++]);
+      if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble advection dir %d: %s", dir, hipGetErrorString(e));
     }
-    AdvArgs a = advect_args(x, dir, dt, is_aloft, b, x->ts0());
-    a.zg = (x->periodic && dir == 0) ? 0 : 1;
-    hipError_t e = spec_advect(a, x->nb_adv[dir], &slots[q++]);
-    if (e != hipSuccess) return fail(ASH3D_TR_EDEVICE, "ensemble advection dir %d: %s", dir, hipGetErrorString(e));
   }
   const int dfirst = (itime % 2 == 0) ? 0 : 1;
   for (int i = 0; i < 3; ++i) {

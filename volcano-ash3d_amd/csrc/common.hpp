@@ -151,6 +151,7 @@ struct AdvArgs {
   int xrows;           // x rows: 1 one wavefront per row where it applies, 0 one workgroup per row
   int zg;              // 1: the concentration ghosts along the lines read as zero (the non-periodic
                        // Set_BC(1) values) instead of being loaded
+  const double *dtv;   // dt / kappa per cell (the fused pass; null elsewhere)
   AloftMask aloft;
 };
 
@@ -194,7 +195,7 @@ struct LaunchSpec {
   const void *single = nullptr, *batch = nullptr;
   dim3 grid, block;
   size_t lds = 0;
-  alignas(16) unsigned char args[512];
+  alignas(16) unsigned char args[1024];
   size_t args_size = 0;
 };
 // Fill sp (kernels, shape, LDS, argument block) or, with sp null, launch
@@ -257,6 +258,13 @@ void spec_neumann_dir(const Dims &d, int dir, int periodic, double *c0, LaunchSp
 // members with grid.y
 hipError_t launch_batch(hipStream_t s, const LaunchSpec &sp, const void *dev_args, const int *dev_member,
                         const long long *dev_first, long long total, unsigned grid_y);
+// the second horizontal sweep (h: lines along x or y, input h.c0) and the
+// vertical sweep (z: output z.cout) in one pass, for the full-grid step
+// that reads concentration ghosts as zeros (advect_hz.hip)
+hipError_t launch_advect_hz(hipStream_t s, const AdvArgs &h, const AdvArgs &z);
+// out[c] = dt / kap[c] for the n cells (the fused pass's cached dt/kappa)
+hipError_t launch_cell_dtv(hipStream_t s, const double *kap, double *out, long long n, double dt);
+hipError_t spec_advect_hz(const AdvArgs &h, const AdvArgs &z, LaunchSpec *sp);
 hipError_t launch_advect_x_rows(hipStream_t s, const AdvArgs &a);
 // the launch launch_advect_x_rows would issue (hipErrorInvalidValue: the
 // row needs the streamed kernel)

@@ -202,7 +202,7 @@ _SIGS = {
                               ctypes.c_double, _ip],
 }
 PROF_KINDS = ("advect_x", "advect_y", "advect_z", "diffuse_x", "diffuse_y", "diffuse_z", "bc", "glue",
-              "transpose", "adj_advect", "adj_diffuse", "copy")
+              "transpose", "adj_advect", "adj_diffuse", "copy", "advect_hz")
 CKPT_SAVE, CKPT_LOAD = 0, 1
 EXPORTS = tuple(_SIGS) + ("ash3d_tr_last_error", "ash3d_tr_build_info")
 
@@ -570,7 +570,7 @@ class Transport:
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z",
                    "x_tile", "line_solver", "part_bins", "part_shape_x", "part_shape_y", "part_shape_z",
                    "adv_groups_y", "adv_groups_z", "adv_x_rows", "part_max_line",
-                   "part_max_line_strided")
+                   "part_max_line_strided", "adv_fused")
     LINE_SOLVERS = ("exact", "partitioned")
 
     def tune(self, **kw):
