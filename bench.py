@@ -44,6 +44,11 @@ def algorithmic_bytes(kind, st):
         return (16 * ns + 24) * ncell          # q r/w; vel, sigma, kappa
     if kind == "advect_z":
         return (24 * ns + 32) * ncell + 24 * ns * st.nx * st.ny  # + vf; vz, vh, sigma, kappa; deposit
+    if kind == "advect_hz":
+        # the fused second horizontal + vertical pass: q r/w and vf per
+        # bin-cell; vel, sigma (horizontal), vz, vh, sigma_z, dt/kappa per
+        # cell; deposit and outflows per column
+        return (24 * ns + 48) * ncell + 24 * ns * st.nx * st.ny
     if kind.startswith("diffuse"):
         return (16 * ns + 24) * ncell          # q r/w; kappa, sigma, K
     return None
@@ -64,10 +69,13 @@ def traffic_file(st):
 def effective_knobs(kind, st, tuning):
     """The launch settings that decide which kernel a sweep kind runs and
     how (from the context's settings after tuning)."""
+    if kind == "advect_hz":
+        return {"adv_fused": tuning["adv_fused"]}
     if kind == "advect_x":
-        return {"adv_x_rows": tuning["adv_x_rows"]}
+        return {"adv_x_rows": tuning["adv_x_rows"], "adv_fused": tuning["adv_fused"]}
     if kind in ("advect_y", "advect_z"):
-        return {"nb": tuning["nb_" + kind], "adv_groups": tuning["adv_groups_" + kind[-1]]}
+        return {"nb": tuning["nb_" + kind], "adv_groups": tuning["adv_groups_" + kind[-1]],
+                "adv_fused": tuning["adv_fused"]}
     d = kind[-1]
     n = {"x": st.nx, "y": st.ny, "z": st.nz}[d]
     nmax = tuning["part_max_line"] if d == "x" else tuning["part_max_line_strided"]
@@ -230,7 +238,7 @@ def global_adjoint_leg(pkg, bins, n_steps, slots, line_solver, tuning):
     return out
 
 
-def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
+def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver, streams=0):
     """Ensemble throughput: one context per member on the one GPU (grids of
     120-400 x 80-300 x 40-60 cells, 10 bins, each member its own dt), all
     members stepped together, four ways: each member's own steps (eager
@@ -281,7 +289,7 @@ def ensemble_leg(pkg, tr, members, steps, size_scale, line_solver):
                 T.graph_capture(2, True)
                 T.graph_capture(3, True)
         if mode == "batched_eager":
-            E = tr.Ensemble(Ts)
+            E = tr.Ensemble(Ts, streams=streams)
         run(mode, it)  # warm-up
         it += steps
         t[mode] = run(mode, it)
@@ -343,10 +351,12 @@ def main():
                     help="hardware-counter runs: start with one step glue, which copies one concentration level "
                          "(the host's ts1, zeros) device to device, a copy of known size to scale FETCH_SIZE / "
                          "WRITE_SIZE by; the timed steps then start from a zero state (same memory traffic)")
-    ap.add_argument("--ensemble-members", type=int, default=0,
+    ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the ensemble leg (0: skip); each on its own context and stream")
-    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--ensemble-scale", type=float, default=1.0, help="horizontal size factor of the members")
+    ap.add_argument("--ensemble-streams", type=int, default=0,
+                    help="stream groups of the batched ensemble (0: the library's default)")
     ap.add_argument("--line-solver", choices=("exact", "partitioned"), default="partitioned",
                     help="theta-scheme line solves: dgtsv order (bitwise the reference) or partitioned "
                          "(lanes of a wave per line; agrees to rounding)")
@@ -473,7 +483,7 @@ def main():
     if args.ensemble_members > 0:
         log("[bench] ensemble leg ...")
         ensemble = ensemble_leg(pkg, tr, args.ensemble_members, args.ensemble_steps, args.ensemble_scale,
-                                args.line_solver)
+                                args.line_solver, args.ensemble_streams)
         log("[bench] ensemble: " + json.dumps(ensemble))
     adjoint = None
     if not args.no_adjoint:

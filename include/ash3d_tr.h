@@ -235,10 +235,11 @@ enum {
   ASH3D_TUNE_PART_MAX_LINE = 15,
   ASH3D_TUNE_PART_MAX_LINE_STRIDED = 16,
   /* advection of the step with diffusion (ash3d_tr_step, graph and
-   * ensemble steps): 1 (default) the second horizontal sweep and the
-   * vertical sweep in one pass (the first sweep writes the ts1 slot, the
-   * fused pass reads it with the halo along its lines and writes ts0;
-   * bitwise the split sweeps), 0 three separate in-place sweeps */
+   * ensemble steps): 1 the second horizontal sweep and the vertical sweep
+   * in one pass (the first sweep writes the ts1 slot, the fused pass reads
+   * it with the halo along its lines and writes ts0; bitwise the split
+   * sweeps), 0 three separate in-place sweeps; default 1 on grids of at
+   * least 2^24 interior cells, 0 below */
   ASH3D_TUNE_ADV_FUSED = 17
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
@@ -309,7 +310,10 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *ctx, int32_t itime);
  * dominant lines of 3 to the solver's maximum active cells, and the last
  * call on each member a step (ts1 mirrored, as for graph_launch).
  * is_aloft is null (every bin of every member aloft) or n pointers, each
- * null or an ns-long mask.  The members' tuning at create fixes the
+ * null or an ns-long mask.  n_streams (0: 4; at most 16) splits the
+ * members into that many groups of about equal work, each with its own
+ * stream, so that one group's launches fill the others' tails; the graph
+ * holds the groups as parallel branches.  The members' tuning at create fixes the
  * kernels; the step orders itself after earlier calls on each member's
  * stream, and later calls on the members after it.  Members must outlive
  * the ensemble, and must not be stepped on their own between ensemble
@@ -318,7 +322,7 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *ctx, int32_t itime);
  * the members' own steps would issue them. */
 typedef struct ash3d_tr_ens ash3d_tr_ens;
 int ash3d_tr_ensemble_create(ash3d_tr_ctx *const *members, int32_t n, const double *dt, double imp_fac,
-                             const int32_t *const *is_aloft, ash3d_tr_ens **out);
+                             const int32_t *const *is_aloft, int32_t n_streams, ash3d_tr_ens **out);
 int ash3d_tr_ensemble_step(ash3d_tr_ens *ens, int32_t itime, int32_t use_graph);
 int ash3d_tr_ensemble_synchronize(ash3d_tr_ens *ens);
 int ash3d_tr_ensemble_info(ash3d_tr_ens *ens, int32_t itime, int32_t *batched_launches, int32_t *member_launches);

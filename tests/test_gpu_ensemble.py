@@ -42,20 +42,20 @@ def small_members(pkg):
     return sts
 
 
-def contexts(tr, sts):
+def contexts(tr, sts, **tune):
     Ts = []
     for st in sts:
         T = tr.Transport(st)
-        T.tune(line_solver="partitioned")
+        T.tune(line_solver="partitioned", **tune)
         T.transport_step(1, True)
         Ts.append(T)
     return Ts
 
 
-def run_individual(tr, st, steps):
+def run_individual(tr, st, steps, **tune):
     b = st.copy()
     T = tr.Transport(b)
-    T.tune(line_solver="partitioned")
+    T.tune(line_solver="partitioned", **tune)
     T.transport_step(1, True)
     for it in range(2, 2 + steps):
         T.step_glue()
@@ -75,10 +75,10 @@ def run_oracle(oracle, st, steps):
     return a
 
 
-def run_batched(tr, sts, steps, use_graph=True, between=None):
+def run_batched(tr, sts, steps, use_graph=True, between=None, streams=0, **tune):
     cs = [st.copy() for st in sts]
-    Ts = contexts(tr, cs)
-    E = tr.Ensemble(Ts)
+    Ts = contexts(tr, cs, **tune)
+    E = tr.Ensemble(Ts, streams=streams)
     for it in range(2, 2 + steps):
         E.step(it, use_graph)
         if between:
@@ -103,6 +103,28 @@ def test_batched_matches_members_and_oracle(pkg, oracle, tr):
         assert_same(run_oracle(oracle, st, 3), out[k], tol=TOL)
     for b, m in info:
         assert b < m
+
+
+@pytest.mark.parametrize("streams", [1, 2, 6])
+def test_stream_groups(pkg, tr, streams):
+    """One, two and six stream groups (graph branches) give the bits of
+    the members' own steps."""
+    sts = small_members(pkg)
+    out, _ = run_batched(tr, sts, 2, streams=streams)
+    for k, st in enumerate(sts):
+        assert_same(run_individual(tr, st, 2), out[k])
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_advection_modes(pkg, oracle, tr, fused):
+    """Members with the split sweeps and with the fused second horizontal +
+    vertical pass (the batched fused kernel): bitwise the members' own
+    steps in the same mode, 1e-12 of the oracle."""
+    sts = small_members(pkg)
+    out, _ = run_batched(tr, sts, 2, adv_fused=fused)
+    for k, st in enumerate(sts):
+        assert_same(run_individual(tr, st, 2, adv_fused=fused), out[k])
+        assert_same(run_oracle(oracle, st, 2), out[k], tol=TOL)
 
 
 def test_graph_replays_match_eager(pkg, tr):
@@ -154,6 +176,8 @@ def test_refuses_members_it_cannot_batch(pkg, tr):
     T.tune(line_solver="partitioned")
     with pytest.raises(tr.Ash3dError):  # the same member twice
         tr.Ensemble([T, T])
+    with pytest.raises(tr.Ash3dError):  # more than 16 stream groups
+        tr.Ensemble([T], streams=17)
     E = tr.Ensemble([T])
     E.step(2)
     E.synchronize()

@@ -15,13 +15,16 @@ from __future__ import annotations
 import json
 import os
 
-PARAMS = (("adv_x_rows", "advect_x"), ("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("adv_groups_y", "advect_y"),
+PARAMS = (("adv_fused", ("advect_x", "advect_y", "advect_z", "advect_hz")), ("adv_x_rows", "advect_x"), ("nb_advect_y", "advect_y"), ("nb_advect_z", "advect_z"), ("adv_groups_y", "advect_y"),
           ("adv_groups_z", "advect_z"), ("nb_diffuse_x", "diffuse_x"), ("x_tile", "diffuse_x"),
           ("nb_diffuse_y", "diffuse_y"), ("nb_diffuse_z", "diffuse_z"))
 X_TILES = (8, 16, 32)
 # streamed advection: line segments per workgroup (0) or up to 4 bin groups
 # of the tile's lines per workgroup
 ADV_GROUPS = (0, 4)
+# advection of the step with diffusion: the fused second horizontal +
+# vertical pass (1) or three split sweeps (0), timed as a whole step's
+# advection
 # x advection rows: one wavefront per row (1) or one workgroup per row (0)
 ADV_X_ROWS = (1, 0)
 # the partitioned line solves have one setting for the three directions:
@@ -63,7 +66,12 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
         kinds = kind if isinstance(kind, tuple) else (kind,)
         if kinds[0].startswith("diffuse") and not use_diffusion:
             continue
+        if param == "adv_fused" and not use_diffusion:
+            continue  # the fused pass belongs to the step with diffusion
+        if param in ("nb_advect_z", "adv_groups_z") and use_diffusion and T.tune_get("adv_fused"):
+            continue  # the vertical sweep runs inside the fused pass
         values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
+                  (1, 0) if param == "adv_fused" else
                   ADV_GROUPS if param.startswith("adv_groups") else
                   ADV_X_ROWS if param == "adv_x_rows" else
                   PART_MAX_LINE if param.startswith("part_max_line") else
@@ -84,7 +92,11 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
                 T.transport_step(itime, use_diffusion)
             prof = T.profile_read()
             T.profile(False)
-            times[nb] = sum(prof[k][0] / max(prof[k][1], 1) for k in kinds)
+            if param == "adv_fused":
+                # the whole advection of a step (the kinds it launches differ)
+                times[nb] = sum(prof[k][0] for k in kinds) / steps
+            else:
+                times[nb] = sum(prof[k][0] / max(prof[k][1], 1) for k in kinds)
         best = min(times, key=times.get)
         if times[best] > (1.0 - margin) * times[default]:
             best = default

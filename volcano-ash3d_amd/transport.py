@@ -187,7 +187,7 @@ _SIGS = {
                                ctypes.c_double, _ip],
     "ash3d_tr_graph_launch": [ctypes.c_void_p, ctypes.c_int32],
     "ash3d_tr_ensemble_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, _dp, ctypes.c_double,
-                                 ctypes.POINTER(_ip), ctypes.POINTER(ctypes.c_void_p)],
+                                 ctypes.POINTER(_ip), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)],
     "ash3d_tr_ensemble_step": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     "ash3d_tr_ensemble_synchronize": [ctypes.c_void_p],
     "ash3d_tr_ensemble_info": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
@@ -611,9 +611,10 @@ class Ensemble:
     per sweep kind and kernel variant (one HIP graph with use_graph).
     Every member must have taken a step first, with the partitioned line
     solver and Crank-Nicolson diffusion; each member's dt, imp_fac and
-    is_aloft are taken at construction."""
+    is_aloft are taken at construction; streams (0: the library's 4)
+    stream groups of about equal work run side by side."""
 
-    def __init__(self, members, imp_fac: float | None = None):
+    def __init__(self, members, imp_fac: float | None = None, streams: int = 0):
         self.lib = load()
         self.members = list(members)
         n = len(self.members)
@@ -625,7 +626,7 @@ class Ensemble:
         masks = (_ip * n)(*[a.ctypes.data_as(_ip) for a in self._masks])
         self._ens = ctypes.c_void_p()
         self._check(self.lib.ash3d_tr_ensemble_create(ctxs, n, self._dt.ctypes.data_as(_dp), float(imp_fac),
-                                                      masks, ctypes.byref(self._ens)))
+                                                      masks, int(streams), ctypes.byref(self._ens)))
 
     def _check(self, rc):
         if rc != 0:
