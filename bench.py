@@ -355,6 +355,8 @@ def main():
                     help="members of the ensemble leg (0: skip); each on its own context and stream")
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--ensemble-scale", type=float, default=1.0, help="horizontal size factor of the members")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="run the ensemble leg alone and print its record (no bench line)")
     ap.add_argument("--ensemble-streams", type=int, default=0,
                     help="stream groups of the batched ensemble (0: the library's default)")
     ap.add_argument("--line-solver", choices=("exact", "partitioned"), default="partitioned",
@@ -373,6 +375,12 @@ def main():
     pkg = importlib.import_module("volcano-ash3d_amd")
     tr = importlib.import_module("volcano-ash3d_amd.transport")
     tr.load()
+    if args.only_ensemble:
+        # the ensemble leg alone (for traces of it); no bench line
+        ens = ensemble_leg(pkg, tr, args.ensemble_members or 64, args.ensemble_steps, args.ensemble_scale,
+                           args.line_solver, args.ensemble_streams)
+        log("[bench] ensemble: " + json.dumps(ens))
+        return
     t0 = time.perf_counter()
     st, use_diffusion = build_state(pkg, args.config)
     workload = WORKLOADS[args.config]
