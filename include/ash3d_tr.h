@@ -302,13 +302,17 @@ int ash3d_tr_graph_launch(ash3d_tr_ctx *ctx, int32_t itime);
  * argument blocks and workgroup ranges held in device memory.  One
  * ensemble step of parity itime % 2 does, for every member m, what
  * ash3d_tr_step_glue(m) followed by ash3d_tr_step(m, itime, dt[m], 1, 1,
- * imp_fac, is_aloft[m]) does, bit for bit; with use_graph it is one HIP
- * graph launch (captured at create, one per parity).
+ * imp_fac, is_aloft[m]) does, bit for bit -- the glue only for members
+ * whose last state change was a step: a member glued by the caller (to
+ * insert a source between glue and step, as the reference's time loop
+ * does) is stepped from its state as it is.  With use_graph the step is
+ * one HIP graph launch (captured at create, one per parity).
  *   Requirements, checked at create and at every step: members on one
  * device, with diffusion by the partitioned Crank-Nicolson line solves
  * (ASH3D_TUNE_LINE_SOLVER = ASH3D_LINE_SOLVER_PARTITIONED) on diagonally
  * dominant lines of 3 to the solver's maximum active cells, and the last
- * call on each member a step (ts1 mirrored, as for graph_launch).
+ * state change on each member a step or a step glue (source insertion may
+ * follow the glue).
  * is_aloft is null (every bin of every member aloft) or n pointers, each
  * null or an ns-long mask.  n_streams (0: 4; at most 16) splits the
  * members into that many groups of about equal work, each with its own

@@ -127,6 +127,50 @@ def test_advection_modes(pkg, oracle, tr, fused):
         assert_same(run_oracle(oracle, st, 2), out[k], tol=TOL)
 
 
+def test_glue_and_source_between_steps(pkg, tr):
+    """The reference's loop body between steps on the members -- step glue,
+    then the vent-column source -- followed by the batched step: bitwise
+    the members' own glue, source and step."""
+    sts = small_members(pkg)
+
+    def flux(st, it):
+        rng = np.random.default_rng(100 * it + st.nx)
+        return np.asfortranarray(rng.random((st.nz + 1, st.ns)) * 50.0)
+
+    def vent(st):
+        return max(1, st.nx // 3), max(1, st.ny // 2)
+
+    own = []
+    for st in sts:
+        b = st.copy()
+        T = tr.Transport(b)
+        T.tune(line_solver="partitioned")
+        T.transport_step(1, True)
+        for it in range(2, 5):
+            T.step_glue()
+            T.source_column_add(*vent(st), flux(st, it))
+            T.transport_step(it, True)
+        T.sync_to_host()
+        T.close()
+        own.append(b)
+    cs = [st.copy() for st in sts]
+    Ts = contexts(tr, cs)
+    E = tr.Ensemble(Ts)
+    for it in range(2, 5):
+        for T, st in zip(Ts, sts):
+            T.step_glue()
+            T.source_column_add(*vent(st), flux(st, it))
+        E.step(it)
+    E.synchronize()
+    for T in Ts:
+        T.sync_to_host()
+    E.close()
+    for T in Ts:
+        T.close()
+    for a, b in zip(own, cs):
+        assert_same(a, b)
+
+
 def test_graph_replays_match_eager(pkg, tr):
     sts = small_members(pkg)
     seen = {True: [], False: []}

@@ -2049,8 +2049,11 @@ const int32_t *ens_aloft(const ash3d_tr_ens *e, size_t k) {
 int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_aloft) {
   int rc = check_ctx_keep(x);
   if (rc) return rc;
-  if (x->ts1_state != TS1_MIRROR)
-    return fail(ASH3D_TR_EINVAL, "ensemble member: the last call must be a step (ts1 mirrored)");
+  // after a step (ts1 mirrored) the ensemble step's glue only zeroes ghost
+  // cells the step rewrites; after a glue (ts1 zero) the member steps from
+  // ts0 as it is
+  if (x->ts1_state != TS1_MIRROR && x->ts1_state != TS1_ZERO)
+    return fail(ASH3D_TR_EINVAL, "ensemble member: the last call must be a step or a step glue");
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED)
     return fail(ASH3D_TR_EINVAL, "ensemble member: the line solver must be the partitioned one");
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
