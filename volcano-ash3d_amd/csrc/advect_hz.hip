@@ -146,30 +146,36 @@ struct ZWin {
 // j0 + w*TY .. + TY-1, each lane its TY cells of every level: the y
 // stencil of those cells is in the lane's own registers (rows j-2..j+2
 // loaded per level; the rows of the neighbouring waves come through L1).
-// x lines second (DIR2 = 0): a workgroup of 4 waves owns 64 columns x 4
-// rows, wave w row j0+w, lane l the cell i0+l, its x neighbours loaded
-// from the row (through L1), wrapped on a periodic grid.  Each wave takes
+// x lines second (DIR2 = 0): a workgroup of 4 waves owns 256 columns of
+// one row, wave w the columns i0 + 64w .., lane l one cell, its x
+// neighbours loaded from the row (through L1), wrapped on a periodic
+// grid.  Each wave takes
 // NB consecutive bins, so that the bin-independent fields of a level are
 // loaded once for them.  Every wave streams its cells up the levels on
 // its own (no barriers); the vertical sweep of each owned column and bin
 // runs in a ZWin.
 constexpr int kHzWaves = 4;
 
-template <int DIR2, bool TOPO, int TY, int NB>
+template <int DIR2, bool TOPO, int TY, int NB, bool WL>
 __device__ __forceinline__ void advect_hz_body(const AdvArgs &H, const AdvArgs &Z, const long long vb,
                                                const long long vn) {
   const Dims d = Z.d;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
-  constexpr int ROWS = DIR2 == 1 ? kHzWaves * TY : kHzWaves;  // rows of a workgroup's tile
-  constexpr int R = DIR2 == 1 ? TY : 1;                         // rows per lane
-  const int tiles_x = (nx + 63) / 64, tiles_y = (ny + ROWS - 1) / ROWS;
+  constexpr int R = DIR2 == 1 ? TY : 1;  // rows per lane
+  constexpr int COLS = (WL ? 64 * kHzWaves : 64);           // columns of a workgroup's tile
+  constexpr int ROWS = WL ? R : (DIR2 == 1 ? kHzWaves * TY : kHzWaves);  // rows of a workgroup's tile
+  const int tiles_x = (nx + COLS - 1) / COLS, tiles_y = (ny + ROWS - 1) / ROWS;
   const int G = (d.ns + NB - 1) / NB;
   const TileGroup tg = xcd_tile_group(vb, G, vn);
   if (tg.tile >= (long long)tiles_x * tiles_y) return;  // padding workgroup (uniform)
   const int n0 = tg.group * NB;
   const int l = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-  const int i = (int)(tg.tile % tiles_x) * 64 + 1 + l;
-  const int jw = (int)(tg.tile / tiles_x) * ROWS + 1 + w * R;  // first row of this wave
+  // WL: the waves side by side along x (the rows a workgroup reads are
+  // contiguous over 64 * waves cells); else stacked along y
+  const int i0w = (int)(tg.tile % tiles_x) * COLS + 1 + (WL ? 64 * w : 0);  // first column of this wave
+  const int i = i0w + l;
+  const int jw = (int)(tg.tile / tiles_x) * ROWS + 1 + (WL ? 0 : w * R);  // first row of this wave
+  if (i0w > nx) return;  // the whole wave is past the grid (no barriers follow)
   const int ic = min(i, nx);
   const bool lane_in = i <= nx;
   const int lim = Z.limiter;
@@ -379,11 +385,23 @@ struct HzArgs {
 #ifndef ASH3D_HZ_NB
 #define ASH3D_HZ_NB 2
 #endif
+#ifndef ASH3D_HZ_NBX
+#define ASH3D_HZ_NBX 3
+#endif
 constexpr int kHzTYy = ASH3D_HZ_TY;  // rows per lane, y lines second
-constexpr int kHzNB = ASH3D_HZ_NB;   // bins per wave
+// bins per wave: y lines second, x lines second (measured on 1440x720x80x20:
+// two bins with two rows per lane for y+z, three bins for x+z)
+template <int DIR2>
+constexpr int kHzNB = DIR2 == 1 ? ASH3D_HZ_NB : ASH3D_HZ_NBX;
+// waves of a workgroup: stacked along y for y lines (the halo rows of a
+// wave are the neighbouring waves' rows, read through the same L1;
+// side by side: 17.7 against 15.7 ms), side by side along x for x lines
+// (a workgroup reads 256 contiguous cells of each row; 14.3 against 14.4)
+template <int DIR2>
+constexpr bool kHzWL = DIR2 == 0;
 template <int DIR2, bool TOPO>
 __global__ __launch_bounds__(64 * kHzWaves) void k_advect_hz(HzArgs A) {
-  advect_hz_body<DIR2, TOPO, kHzTYy, kHzNB>(A.h, A.z, blockIdx.x, gridDim.x);
+  advect_hz_body<DIR2, TOPO, kHzTYy, kHzNB<DIR2>, kHzWL<DIR2>>(A.h, A.z, blockIdx.x, gridDim.x);
 }
 template <int DIR2, bool TOPO>
 __global__ __launch_bounds__(64 * kHzWaves) void k_advect_hz_batch(
@@ -391,14 +409,16 @@ __global__ __launch_bounds__(64 * kHzWaves) void k_advect_hz_batch(
     const ASH3D_CONSTANT long long *__restrict__ first) {
   const int m = member[blockIdx.x];
   const HzArgs &A = *(const HzArgs *)(args + m);
-  advect_hz_body<DIR2, TOPO, kHzTYy, kHzNB>(A.h, A.z, (long long)blockIdx.x - first[m], first[m + 1] - first[m]);
+  advect_hz_body<DIR2, TOPO, kHzTYy, kHzNB<DIR2>, kHzWL<DIR2>>(A.h, A.z, (long long)blockIdx.x - first[m],
+                                                               first[m + 1] - first[m]);
 }
 
 template <int DIR2>
 hipError_t hz_dir(hipStream_t s, const AdvArgs &h, const AdvArgs &z, LaunchSpec *sp) {
-  const int rows = DIR2 == 1 ? kHzWaves * kHzTYy : kHzWaves;
-  const long long tiles = (long long)((z.d.nx + 63) / 64) * ((z.d.ny + rows - 1) / rows);
-  const dim3 grid((unsigned)(padded_tiles(tiles) * ((z.d.ns + kHzNB - 1) / kHzNB))), block(64 * kHzWaves);
+  const int r = DIR2 == 1 ? kHzTYy : 1;
+  const int cols = kHzWL<DIR2> ? 64 * kHzWaves : 64, rows = kHzWL<DIR2> ? r : kHzWaves * r;
+  const long long tiles = (long long)((z.d.nx + cols - 1) / cols) * ((z.d.ny + rows - 1) / rows);
+  const dim3 grid((unsigned)(padded_tiles(tiles) * ((z.d.ns + kHzNB<DIR2> - 1) / kHzNB<DIR2>))), block(64 * kHzWaves);
   const bool topo = h.deld != nullptr;
   return issue(s, sp, topo ? (const void *)k_advect_hz<DIR2, true> : (const void *)k_advect_hz<DIR2, false>,
                topo ? (const void *)k_advect_hz_batch<DIR2, true> : (const void *)k_advect_hz_batch<DIR2, false>,
