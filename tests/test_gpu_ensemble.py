@@ -225,5 +225,13 @@ def test_refuses_members_it_cannot_batch(pkg, tr):
     E = tr.Ensemble([T])
     E.step(2)
     E.synchronize()
+    # new diffusivities after the ensemble was built: the line factors are
+    # rebuilt and checked again -- negative ones make lines that the
+    # partitioned solves cannot take, and the step says so
+    T.st.kx[...] = -5.0
+    T.sync_from_host(tr.SYNC_DIFFK)
+    with pytest.raises(tr.Ash3dError) as ex:
+        E.step(3)
+    assert ex.value.code == 1
     E.close()
     T.close()

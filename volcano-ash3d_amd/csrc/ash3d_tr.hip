@@ -121,6 +121,7 @@ struct ash3d_tr_ctx {
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
+  long long lu_count = 0;  // factorisations launched (any direction)
   // host copy of each direction's non-dominant line count (pinned, read
   // back asynchronously after a factorisation): 0 unknown, 1 in flight
   // (ndom_ev), 2 known.  When known, a sweep launches only the solver it
@@ -466,6 +467,7 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
     HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir]));
+    ++x->lu_count;
     x->ndom_state[dir] = 0;
     x->lu_ok[dir] = true;
     x->lu_dt[dir] = a.dt;
@@ -2008,6 +2010,7 @@ struct ash3d_tr_ens {
   std::vector<int> group_of;  // per member
   std::vector<hipEvent_t> fork, join;  // per stream group
   std::vector<ash3d_tr_ctx *> m;
+  std::vector<bool> fused;  // the member's planned advection: the fused pass
   std::vector<double> dt;
   std::vector<std::vector<int32_t>> aloft;  // per member (empty: every bin aloft)
   double imp_fac = 0.5;
@@ -2046,9 +2049,14 @@ const int32_t *ens_aloft(const ash3d_tr_ens *e, size_t k) {
 // Brings member x's caches for a step with dt, imp_fac, is_aloft up to date
 // on its stream (velocity ghosts, transposed geometry, row and LU factors)
 // and checks that the step takes only launches the ensemble batches.
-int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_aloft) {
+// fused: the member's launches use the fused advection pass (its dt/kappa
+// cache is kept current); refactored (optional): set when a line
+// factorisation had to be launched (the caller checks dominance again).
+int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_aloft, bool fused,
+                bool *refactored = nullptr) {
   int rc = check_ctx_keep(x);
   if (rc) return rc;
+  const long long lu_before = x->lu_count;
   // after a step (ts1 mirrored) the ensemble step's glue only zeroes ghost
   // cells the step rewrites; after a glue (ts1 zero) the member steps from
   // ts0 as it is
@@ -2059,7 +2067,7 @@ int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_al
   if (!(dt >= 0.0)) return fail(ASH3D_TR_EINVAL, "dt must be >= 0 (got %g)", dt);
   if ((rc = set_bc1(x, true, nullptr, false))) return rc;
   if ((rc = ensure_tgeom(x))) return rc;
-  if (x->adv_fused && (rc = ensure_dtv(x, dt))) return rc;
+  if (fused && (rc = ensure_dtv(x, dt))) return rc;
   int32_t b[6];
   full_bounds(x, b);
   for (int dir = 0; dir < 3; ++dir) {
@@ -2071,6 +2079,21 @@ int ens_prepare(ash3d_tr_ctx *x, double dt, double imp_fac, const int32_t *is_al
       return fail(ASH3D_TR_EINVAL, "ensemble member: %d active cells along dir %d (partitioned solves take 3..%d)",
                   n, dir, std::min(nmax, 2048));
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
+  }
+  if (refactored) *refactored = x->lu_count != lu_before;
+  return 0;
+}
+
+// Every line of member x's three directions is a diagonally dominant
+// M-matrix row (the partitioned kernels exit at once otherwise); waits for
+// the member's stream.
+int ens_check_dominant(ash3d_tr_ctx *x, size_t k) {
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int dir = 0; dir < 3; ++dir) {
+    int h = 0;
+    HIPCHK(hipMemcpy(&h, x->lu_ndom[dir], sizeof(int), hipMemcpyDeviceToHost));
+    if (h != 0)
+      return fail(ASH3D_TR_EINVAL, "ensemble member %zu: %d lines along dir %d are not diagonally dominant", k, h, dir);
   }
   return 0;
 }
@@ -2166,21 +2189,13 @@ int ens_destroy(ash3d_tr_ens *e) {
 int ens_build(ash3d_tr_ens *e) {
   int rc = 0;
   const size_t n = e->m.size();
-  for (size_t k = 0; k < n; ++k)
-    if ((rc = ens_prepare(e->m[k], e->dt[k], e->imp_fac, ens_aloft(e, k)))) return rc;
-  // every member's line factors are diagonally dominant M-matrix rows (the
-  // partitioned kernels exit at once otherwise)
+  e->fused.resize(n);
   for (size_t k = 0; k < n; ++k) {
-    ash3d_tr_ctx *x = e->m[k];
-    HIPCHK(hipStreamSynchronize(x->stream));
-    for (int dir = 0; dir < 3; ++dir) {
-      int h = 0;
-      HIPCHK(hipMemcpy(&h, x->lu_ndom[dir], sizeof(int), hipMemcpyDeviceToHost));
-      if (h != 0)
-        return fail(ASH3D_TR_EINVAL, "ensemble member %zu: %d lines along dir %d are not diagonally dominant", k, h,
-                    dir);
-    }
+    e->fused[k] = e->m[k]->adv_fused != 0;
+    if ((rc = ens_prepare(e->m[k], e->dt[k], e->imp_fac, ens_aloft(e, k), e->fused[k]))) return rc;
   }
+  for (size_t k = 0; k < n; ++k)
+    if ((rc = ens_check_dominant(e->m[k], k))) return rc;
   auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
   // stream groups of about equal work (bin-cells), the largest members
   // first; members are laid out largest first inside each launch too
@@ -2348,7 +2363,9 @@ int ash3d_tr_ensemble_step(ash3d_tr_ens *e, int32_t itime, int32_t use_graph) {
   const int p = ((itime % 2) + 2) % 2;
   for (size_t k = 0; k < e->m.size(); ++k) {
     ash3d_tr_ctx *x = e->m[k];
-    if ((rc = ens_prepare(x, e->dt[k], e->imp_fac, ens_aloft(e, k)))) return rc;
+    bool refactored = false;
+    if ((rc = ens_prepare(x, e->dt[k], e->imp_fac, ens_aloft(e, k), e->fused[k], &refactored))) return rc;
+    if (refactored && (rc = ens_check_dominant(x, k))) return rc;
     HIPCHK(hipEventRecord(e->ready[k], x->stream));
     HIPCHK(hipStreamWaitEvent(e->streams[0], e->ready[k], 0));
   }
