@@ -240,7 +240,18 @@ enum {
    * it with the halo along its lines and writes ts0; bitwise the split
    * sweeps), 0 three separate in-place sweeps; default 1 on grids of at
    * least 2^24 interior cells, 0 below */
-  ASH3D_TUNE_ADV_FUSED = 17
+  ASH3D_TUNE_ADV_FUSED = 17,
+  /* partitioned mode, eager steps: y and z theta-scheme sweeps whose lines
+   * are too long for the partitioned solves are solved by Jacobi sweeps
+   * in one pass when at most this many (0..7; 0 never; default 7) bring
+   * the error under 2^-52 of the line's largest value (rows with
+   * rho = max (|DL| + |DU|) / |D| need the least K + 1 with
+   * rho^(K+1) <= 2^-52 (1 - rho)); otherwise dgtsv order */
+  ASH3D_TUNE_JACOBI_SWEEPS = 18,
+  /* read only: the Jacobi sweeps the last y (z) theta-scheme sweep took,
+   * 0 when it used another solver */
+  ASH3D_TUNE_JACOBI_LAST_Y = 19,
+  ASH3D_TUNE_JACOBI_LAST_Z = 20
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

@@ -33,6 +33,8 @@ PART_BINS = (0, 8, 4)
 # longest line of the partitioned solves: single-wave lines only (512) or
 # lines over several waves too (2048); only tried where a line is longer
 PART_MAX_LINE = (512, 2048)
+# most Jacobi sweeps for strided lines beyond the partitioned solves (0: never)
+JACOBI_SWEEPS = 7
 
 
 def _key(st):
@@ -62,6 +64,8 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             params.append(("part_max_line", "diffuse_x"))
         if st.ny > 512 or st.nz > 512:
             params.append(("part_max_line_strided", ("diffuse_y", "diffuse_z")))
+            # Jacobi sweeps for the strided lines left to dgtsv order
+            params.append(("jacobi_sweeps", ("diffuse_y", "diffuse_z")))
     for param, kind in params:
         kinds = kind if isinstance(kind, tuple) else (kind,)
         if kinds[0].startswith("diffuse") and not use_diffusion:
@@ -72,6 +76,7 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             continue  # the vertical sweep runs inside the fused pass
         values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
                   (1, 0) if param == "adv_fused" else
+                  (JACOBI_SWEEPS, 0) if param == "jacobi_sweeps" else
                   ADV_GROUPS if param.startswith("adv_groups") else
                   ADV_X_ROWS if param == "adv_x_rows" else
                   PART_MAX_LINE if param.startswith("part_max_line") else

@@ -112,12 +112,14 @@ struct ash3d_tr_ctx {
   bool lr_ok[3] = {false, false, false};
   // LU factors of the theta-scheme lines per direction (one allocation:
   // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags, one
-  // INFO per line and the count of lines that are not diagonally
-  // dominant), valid for the key (dt, theta, rmin, rmax)
+  // INFO per line, the count of lines that are not diagonally dominant
+  // and the largest row ratio (|DL| + |DU|) / |D| as double bits), valid
+  // for the key (dt, theta, rmin, rmax)
   double *lu[3] = {nullptr, nullptr, nullptr};
   uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
   int *lu_sing[3] = {nullptr, nullptr, nullptr};
   int *lu_ndom[3] = {nullptr, nullptr, nullptr};
+  unsigned long long *lu_rho[3] = {nullptr, nullptr, nullptr};
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
@@ -125,8 +127,10 @@ struct ash3d_tr_ctx {
   // host copy of each direction's non-dominant line count (pinned, read
   // back asynchronously after a factorisation): 0 unknown, 1 in flight
   // (ndom_ev), 2 known.  When known, a sweep launches only the solver it
-  // needs instead of both with the choice made on the device.
+  // needs instead of both with the choice made on the device.  The row
+  // ratio comes back with it (rho_h, in the same pinned block).
   int *ndom_h = nullptr;
+  unsigned long long *rho_h = nullptr;
   hipEvent_t ndom_ev[3] = {};
   int ndom_state[3] = {0, 0, 0};
   bool capturing = false;
@@ -206,6 +210,11 @@ struct ash3d_tr_ctx {
   int adv_gy[3] = {0, 0, 4};
   int adv_xrows = 1;
   int adv_fused = 1;  // second horizontal + vertical advection in one pass (step with diffusion)
+  // most Jacobi sweeps a y or z theta-scheme sweep may take in place of
+  // the dgtsv-order solve (0: never), and the sweeps the last y and z
+  // theta-scheme sweeps took (0: another solver)
+  int jacobi_sweeps = kJacobiMaxIters + 1;
+  int jacobi_last[3] = {0, 0, 0};
   // dt / kappa per cell for the fused pass, valid for dtv_dt
   double *dtvc = nullptr;
   double dtv_dt = 0.0;
@@ -458,15 +467,17 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   int rc = 0;
   const long long nc = x->d.ncell, lines = (long long)a.g.Na * a.g.Nb;
   if (!x->lu[dir]) {
-    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 1))) return rc;
+    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 2))) return rc;
     x->lu_piv[dir] = (uint8_t *)(x->lu[dir] + 4 * nc);
     x->lu_sing[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8);
     x->lu_ndom[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2);
+    x->lu_rho[dir] = (unsigned long long *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 1);
   }
   if (!x->junk && (rc = dev_alloc(&x->junk, kJunk))) return rc;
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
-    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir]));
+    HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir],
+                          x->lu_rho[dir]));
     ++x->lu_count;
     x->ndom_state[dir] = 0;
     x->lu_ok[dir] = true;
@@ -510,9 +521,12 @@ int ensure_fac(ash3d_tr_ctx *x, int nb) {
 int host_ndom(ash3d_tr_ctx *x, int dir) {
   if (x->capturing || !x->lu_ok[dir]) return -1;
   if (x->ndom_state[dir] == 0) {
-    if (!x->ndom_h && hipHostMalloc((void **)&x->ndom_h, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-      x->ndom_h = nullptr;
-      return -1;
+    if (!x->ndom_h) {
+      if (hipHostMalloc((void **)&x->ndom_h, 64, hipHostMallocDefault) != hipSuccess) {
+        x->ndom_h = nullptr;
+        return -1;
+      }
+      x->rho_h = (unsigned long long *)(x->ndom_h + 4);
     }
     if (!x->ndom_ev[dir] && hipEventCreateWithFlags(&x->ndom_ev[dir], hipEventDisableTiming) != hipSuccess) {
       x->ndom_ev[dir] = nullptr;
@@ -520,6 +534,8 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
     }
     if (hipMemcpyAsync(x->ndom_h + dir, x->lu_ndom[dir], sizeof(int), hipMemcpyDeviceToHost, x->stream) !=
             hipSuccess ||
+        hipMemcpyAsync(x->rho_h + dir, x->lu_rho[dir], sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       x->stream) != hipSuccess ||
         hipEventRecord(x->ndom_ev[dir], x->stream) != hipSuccess)
       return -1;
     x->ndom_state[dir] = 1;
@@ -530,6 +546,32 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
     x->ndom_state[dir] = 2;
   }
   return x->ndom_h[dir];
+}
+
+// Jacobi sweeps for a theta-scheme sweep of y or z lines that would
+// otherwise be solved in dgtsv order (partitioned mode, eager steps): the
+// K of launch_diff_cn_jacobi (K + 1 sweeps from zero), or -1.  K + 1
+// sweeps leave an error of at most rho^(K+1) of the line's largest
+// solution value (rho: the largest row ratio (|DL| + |DU|) / |D| of the
+// sweep, the max norm of the Jacobi iteration matrix); K is the least
+// with rho^(K+1) <= 2^-52 (1 - rho), the rounding of the sweeps growing
+// at most by 1 / (1 - rho).  -1 while the factors' ratio is not known on
+// the host (the first sweep after a factorisation) and in graph captures.
+int jacobi_iters(ash3d_tr_ctx *x, int dir) {
+  if (dir == 0 || x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0) return -1;
+  if (host_ndom(x, dir) != 0) return -1;
+  double rho;
+  std::memcpy(&rho, x->rho_h + dir, sizeof rho);
+  if (!(rho >= 0.0 && rho < 1.0)) return -1;
+  const double bound = std::ldexp(1.0, -52) * (1.0 - rho);
+  double e = rho;
+  int k = 0;
+  while (e > bound && k < kJacobiMaxIters) {
+    e *= rho;
+    ++k;
+  }
+  if (e > bound || k + 1 > x->jacobi_sweeps) return -1;
+  return k;
 }
 
 // Cell offsets of the partitioned solves of direction dir: the state in
@@ -664,7 +706,10 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);  // one record for both kernels
     const int tp = try_cn_partitioned(x, dir, a, &rc);
     if (rc) return rc;
-    if (tp != 2) HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
+    const int k = tp == 0 ? jacobi_iters(x, dir) : -1;
+    x->jacobi_last[dir] = k + 1;
+    if (k >= 0) HIPCHK(launch_diff_cn_jacobi(x->stream, a, x->nb_diff[dir], k));
+    else if (tp != 2) HIPCHK(launch_diff_cn(x->stream, a, x->nb_diff[dir]));
   } else {
     ProfScope ps(x, ASH3D_PROF_DIFFUSE_X + dir);
     HIPCHK(launch_diff_explicit(x->stream, a, x->nb_diff[dir]));
@@ -1317,6 +1362,14 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     x->adv_fused = value;
     return 0;
   }
+  if (param == ASH3D_TUNE_JACOBI_SWEEPS) {
+    if (value < 0 || value > kJacobiMaxIters + 1)
+      return fail(ASH3D_TR_EINVAL, "Jacobi sweeps must be 0..%d (got %d)", kJacobiMaxIters + 1, value);
+    x->jacobi_sweeps = value;
+    return 0;
+  }
+  if (param == ASH3D_TUNE_JACOBI_LAST_Y || param == ASH3D_TUNE_JACOBI_LAST_Z)
+    return fail(ASH3D_TR_EINVAL, "tuning parameter %d can only be read", param);
   if (param == ASH3D_TUNE_ADV_X_ROWS) {
     if (value != 0 && value != 1)
       return fail(ASH3D_TR_EINVAL, "x rows kernel must be 0 (workgroup rows) or 1 (wavefront rows) (got %d)", value);
@@ -1378,6 +1431,9 @@ int ash3d_tr_tune_get(ash3d_tr_ctx *x, int32_t param, int32_t *value) {
     case ASH3D_TUNE_PART_MAX_LINE: *value = x->part_max_line; return 0;
     case ASH3D_TUNE_PART_MAX_LINE_STRIDED: *value = x->part_max_line_strided; return 0;
     case ASH3D_TUNE_ADV_FUSED: *value = x->adv_fused; return 0;
+    case ASH3D_TUNE_JACOBI_SWEEPS: *value = x->jacobi_sweeps; return 0;
+    case ASH3D_TUNE_JACOBI_LAST_Y: *value = x->jacobi_last[1]; return 0;
+    case ASH3D_TUNE_JACOBI_LAST_Z: *value = x->jacobi_last[2]; return 0;
     default: return fail(ASH3D_TR_EINVAL, "unknown tuning parameter %d", param);
   }
 }

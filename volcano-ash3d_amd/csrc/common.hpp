@@ -289,7 +289,15 @@ hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayou
 // the launch of the forward partitioned solves (launch_diff_cn_part)
 hipError_t spec_diff_cn_part(const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape, LaunchSpec *sp);
 // the LU factors of every line of the sweep (reads a.lr)
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom);
+// (*rho: the largest row ratio (|DL| + |DU|) / |D| over the sweep's lines as
+// double bits, 1.0 when a line is not diagonally dominant; may be null)
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom,
+                          unsigned long long *rho);
+// theta-scheme solves of y or z lines by K + 1 Jacobi sweeps (0 <= k <=
+// kJacobiMaxIters), in one pass; for sweeps whose rows are all diagonally
+// dominant (nb: bins per thread, 1, 2 or 4)
+constexpr int kJacobiMaxIters = 6;
+hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
 // (some L or R negative or not finite, or theta < 0): the partitioned
 // solves, which eliminate without pivoting, are used only when it is zero.
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, int *ndom) {
+__global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, int *ndom,
+                                                   unsigned long long *rho) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   double *fF = lu, *fD = lu + d.ncell, *fDU = fD + d.ncell, *fDL = fDU + d.ncell;
   int singular = 0;
   bool dom = true;
+  double rho_line = 0.0;  // largest (|DL| + |DU|) / |D| of the line's rows
   double Dc = 0.0, DUc = 0.0;
   // row factors PF rows ahead
   constexpr int PF = 8;
@@ -119,6 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
       // L, R >= 0 and finite, theta >= 0: every row has D >= 1 + |DL| + |DU|
       // in exact arithmetic (false for NaN too)
       dom = dom && LeftFac >= 0.0 && RightFac >= 0.0 && LeftFac <= DBL_MAX && RightFac <= DBL_MAX && th >= 0.0;
+      rho_line = fmax(rho_line, (fabs(DLn) + fabs(DUn)) / fabs(Dn));
       if (l == 1) {
         Dc = Dn;
         DUc = DUn;
@@ -162,9 +165,16 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   fD[base + sL * (long long)(rmin + n)] = Dc;
   sing[gid] = singular;
   if (!dom) atomicAdd(ndom, 1);
+  // the sweep's largest row ratio (non-negative doubles order as their bit
+  // patterns); lines that are not dominant count as 1
+  if (rho) {
+    const double r = (dom && rho_line < 1.0) ? rho_line : 1.0;
+    atomicMax(rho, (unsigned long long)__double_as_longlong(r));
+  }
 }
 
 __global__ void k_zero_i32(int *p) { *p = 0; }
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 template <int DIR, int NB>
 __global__ __launch_bounds__(kBlock) void k_diff_explicit(DiffArgs A) {
@@ -424,6 +434,177 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
     } else {
       for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
       for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bofs] = 0.0;
+    }
+  }
+}
+
+// theta-scheme line solve by Jacobi sweeps, for sweeps whose rows are
+// strongly diagonally dominant.  Row t of a line (the matrix of k_line_lu:
+// DL = -th L, D, DU = -th R) gives x_t = y_t + a_t x_{t-1} + b_t x_{t+1}
+// with y = B / D, a = th L / D, b = th R / D (a_1 = b_n = 0).  The sweeps
+// x(0) = y, x(m) = y + a x(m-1)[t-1] + b x(m-1)[t+1] leave, after K of
+// them, an error of at most rho^(K+1) of the line's largest |x| (rho =
+// max (|DL| + |DU|) / |D| = max (a + b), the max norm of the iteration
+// matrix); the host picks K.  The K sweeps run in one pass along the line,
+// sweep m lagging m rows behind the rows entering: row s enters (its
+// right-hand side B from q at rows s-1..s+1, as the dgtsv-order kernel
+// forms it), sweep m updates row s-m from sweep m-1's values at rows s-m-1
+// and s-m+1, and sweep K's row s-K is stored, in place (every q it
+// overwrites has been read).  One thread per line and NB bins, lanes along
+// the contiguous transverse index: q is read and written once per
+// bin-cell, the row factors L, R once per bin group.
+template <int DIR, int NB, int K>
+__global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
+  constexpr int PF = 2;                 // rows loaded ahead
+  constexpr int NY = K + 1 > 3 ? K + 1 : 3;  // y of rows s .. s-NY+1 (sweep 0 and the rows sweeps use)
+  constexpr int NW = K > 1 ? K - 1 : 1;  // windows of sweeps 1 .. K-1
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
+  const long long gid = tg.tile * kBlock + threadIdx.x;
+  if (gid >= (long long)g.Na * g.Nb) return;
+  const int a = (int)(gid % g.Na) + 1;
+  const int b = (int)(gid / g.Na) + 1;
+  const int n0 = tg.group * NB;
+  const int N = g.N, rmin = g.rmin, rmax = g.rmax;
+  const int n = rmax - rmin + 1;
+  const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  double *c = A.c0;
+  bool exists[NB], on[NB];
+  long long bofs[NB];  // bin offset (an existing bin's for the others)
+  bool any_on = false;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    exists[u] = (n0 + u) < d.ns;
+    on[u] = exists[u] && line_on && A.aloft.on(n0 + u);
+    any_on = any_on || on[u];
+    bofs[u] = d.ncell * (long long)min(n0 + u, d.ns - 1);
+  }
+  if (any_on) {
+    const double th = A.theta, th1 = 1.0 - th;
+    const double *Lf = A.lr, *Rf = A.lr + d.ncell;
+    auto ofs = [&](int l) { return base + sL * (long long)(rmin + l); };  // row l: cell rmin-1+l
+    // rows ahead: slot p holds L, R of row s and q of row s+1
+    double rL[PF], rR[PF], rq[PF][NB];
+    auto fetch = [&](int slot, int l) {
+      const long long o = ofs(min(l, n));
+      rL[slot] = Lf[o];
+      rR[slot] = Rf[o];
+      const long long oq = ofs(min(l + 1, n));
+#pragma unroll
+      for (int u = 0; u < NB; ++u) rq[slot][u] = c[oq + bofs[u]];
+    };
+    double qc[NB], qp[NB];  // q of rows s and s-1 as row s enters
+#pragma unroll
+    for (int u = 0; u < NB; ++u) qc[u] = qp[u] = c[ofs(1) + bofs[u]];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) fetch(p, 1 + p);
+    double y[NY][NB], ca[K + 1], cb[K + 1];  // index j: row s-j
+    double xw[NW][3][NB];                    // sweep m: rows s-m, s-m-1, s-m-2
+#pragma unroll
+    for (int j = 0; j < NY; ++j)
+#pragma unroll
+      for (int u = 0; u < NB; ++u) y[j][u] = 0.0;
+#pragma unroll
+    for (int j = 0; j <= K; ++j) ca[j] = cb[j] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NW; ++m)
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int u = 0; u < NB; ++u) xw[m][e][u] = 0.0;
+    for (int s0 = 1; s0 <= n + K; s0 += PF) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const int s = s0 + p;
+        if (s > n + K) break;
+        const double L = rL[p], R = rR[p];
+        double qn[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) qn[u] = rq[p][u];
+        fetch(p, s + PF);
+#pragma unroll
+        for (int j = NY - 1; j > 0; --j)
+#pragma unroll
+          for (int u = 0; u < NB; ++u) y[j][u] = y[j - 1][u];
+#pragma unroll
+        for (int j = K; j > 0; --j) {
+          ca[j] = ca[j - 1];
+          cb[j] = cb[j - 1];
+        }
+        if (s <= n) {
+          const double C = L + R;
+          double D = 1.0 + th * C;
+          if (s == 1) D = D - th * L;
+          if (s == n) D = D - th * R;
+          const double rD = 1.0 / D;
+          ca[0] = (s > 1) ? (th * L) * rD : 0.0;
+          cb[0] = (s < n) ? (th * R) * rD : 0.0;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const double ql = (s == 1) ? qc[u] : qp[u];
+            const double qr = (s == n) ? qc[u] : qn[u];
+            y[0][u] = ((th1 * L * ql + (1.0 - th1 * C) * qc[u]) + th1 * R * qr) * rD;
+          }
+        } else {
+          ca[0] = cb[0] = 0.0;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) y[0][u] = 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          qp[u] = qc[u];
+          qc[u] = qn[u];
+        }
+        if constexpr (K == 0) {
+          if (s <= n) {
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+              if (on[u]) c[ofs(s) + bofs[u]] = y[0][u];
+          }
+        } else {
+          // sweep m at row t = s-m from sweep m-1 at rows t-1 and t+1
+          // (sweep 0: y at rows s-m-1, s-m+1; later sweeps: their windows)
+#pragma unroll
+          for (int m = 1; m <= K; ++m) {
+            const int t = s - m;
+            const bool tin = t >= 1 && t <= n;
+            double xm[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+              const double lo = (m == 1) ? y[2][u] : xw[m > 1 ? m - 2 : 0][2][u];
+              const double hi = (m == 1) ? y[0][u] : xw[m > 1 ? m - 2 : 0][0][u];
+              xm[u] = tin ? (y[m][u] + ca[m] * lo) + cb[m] * hi : 0.0;
+            }
+            if (m < K) {
+              const int w = m - 1 < NW ? m - 1 : 0;
+#pragma unroll
+              for (int u = 0; u < NB; ++u) {
+                xw[w][2][u] = xw[w][1][u];
+                xw[w][1][u] = xw[w][0][u];
+                xw[w][0][u] = xm[u];
+              }
+            } else if (tin) {
+#pragma unroll
+              for (int u = 0; u < NB; ++u)
+                if (on[u]) c[ofs(t) + bofs[u]] = xm[u];
+            }
+          }
+        }
+      }
+    }
+  }
+  // cells the reference leaves at zero (zero fill of ts1, then copy)
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    if (!exists[u]) continue;
+    const long long bo = d.ncell * (long long)(n0 + u);
+    if (!on[u]) {
+      for (int r = 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
+    } else {
+      for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
+      for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
     }
   }
 }
@@ -762,14 +943,16 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
   return hipGetLastError();
 }
 
-hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom) {
+hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom,
+                          unsigned long long *rho) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)((lines + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, ndom);
+  if (rho) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, rho);
   switch (a.g.dir) {
-    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
-    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
-    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom); break;
+    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
+    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
+    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
   }
   return hipGetLastError();
 }
@@ -781,6 +964,42 @@ hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &or
     case 2: return launch_cn_xrows_tx<2>(s, a, x, tx);
     case 4: return launch_cn_xrows_tx<4>(s, a, x, tx);
     default: return launch_cn_xrows_tx<8>(s, a, x, tx);
+  }
+}
+
+namespace {
+
+template <int DIR, int NB>
+hipError_t launch_jacobi_k(hipStream_t s, const DiffArgs &a, int k, dim3 grid) {
+  switch (k) {
+    case 0: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 0>), grid, dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 1>), grid, dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 2>), grid, dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 3>), grid, dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 4>), grid, dim3(kBlock), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 5>), grid, dim3(kBlock), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 6>), grid, dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int NB>
+hipError_t launch_jacobi_nb(hipStream_t s, const DiffArgs &a, int k) {
+  const long long lines = (long long)a.g.Na * a.g.Nb;
+  dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
+  return a.g.dir == 1 ? launch_jacobi_k<1, NB>(s, a, k, grid) : launch_jacobi_k<2, NB>(s, a, k, grid);
+}
+
+}  // namespace
+
+hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k) {
+  if (a.g.dir != 1 && a.g.dir != 2) return hipErrorInvalidValue;
+  if (k < 0 || k > kJacobiMaxIters) return hipErrorInvalidValue;
+  switch (nb) {
+    case 1: return launch_jacobi_nb<1>(s, a, k);
+    case 2: return launch_jacobi_nb<2>(s, a, k);
+    default: return launch_jacobi_nb<4>(s, a, k);
   }
 }
 

@@ -570,7 +570,8 @@ class Transport:
     TUNE_PARAMS = ("nb_advect_x", "nb_advect_y", "nb_advect_z", "nb_diffuse_x", "nb_diffuse_y", "nb_diffuse_z",
                    "x_tile", "line_solver", "part_bins", "part_shape_x", "part_shape_y", "part_shape_z",
                    "adv_groups_y", "adv_groups_z", "adv_x_rows", "part_max_line",
-                   "part_max_line_strided", "adv_fused")
+                   "part_max_line_strided", "adv_fused", "jacobi_sweeps", "jacobi_last_y",
+                   "jacobi_last_z")
     LINE_SOLVERS = ("exact", "partitioned")
 
     def tune(self, **kw):
