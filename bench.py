@@ -80,7 +80,10 @@ def effective_knobs(kind, st, tuning):
     n = {"x": st.nx, "y": st.ny, "z": st.nz}[d]
     nmax = tuning["part_max_line"] if d == "x" else tuning["part_max_line_strided"]
     if tuning["line_solver"] == 1 and 3 <= n <= nmax:
-        return {"solver": "partitioned", "part_shape": tuning["part_shape_" + d], "part_bins": tuning["part_bins"]}
+        part = {"solver": "partitioned", "part_shape": tuning["part_shape_" + d], "part_bins": tuning["part_bins"]}
+        if d == "x" and tuning.get("jacobi_last_x", 0) > 0:
+            part |= {"jacobi_x": tuning["jacobi_x"], "sweeps": tuning["jacobi_last_x"]}
+        return part
     if d != "x" and tuning.get("jacobi_last_" + d, 0) > 0:
         return {"solver": "jacobi", "sweeps": tuning["jacobi_last_" + d], "nb": min(tuning["nb_diffuse_" + d], 4)}
     return {"solver": "dgtsv-order", "nb": tuning["nb_diffuse_" + d]} | ({"x_tile": tuning["x_tile"]} if d == "x" else {})

@@ -246,12 +246,22 @@ enum {
    * in one pass when at most this many (0..7; 0 never; default 7) bring
    * the error under 2^-52 of the line's largest value (rows with
    * rho = max (|DL| + |DU|) / |D| need the least K + 1 with
-   * rho^(K+1) <= 2^-52 (1 - rho)); otherwise dgtsv order */
+   * rho^(K+1) <= 2^-52 (1 - rho)); otherwise dgtsv order.  Also the most
+   * sweeps of the x lines' Jacobi sweeps (ASH3D_TUNE_JACOBI_X) */
   ASH3D_TUNE_JACOBI_SWEEPS = 18,
   /* read only: the Jacobi sweeps the last y (z) theta-scheme sweep took,
    * 0 when it used another solver */
   ASH3D_TUNE_JACOBI_LAST_Y = 19,
-  ASH3D_TUNE_JACOBI_LAST_Z = 20
+  ASH3D_TUNE_JACOBI_LAST_Z = 20,
+  /* partitioned mode, eager steps, x lines: the lines whose rows have
+   * rho <= 5.79e-3 (seven sweeps suffice) take Jacobi sweeps (as many as
+   * the largest rho among them asks for), the partitioned solves the
+   * others; 1 (default) where the partitioned solves would run over
+   * several waves per line (lines longer than 512 cells), 2 at any
+   * length, 0 never */
+  ASH3D_TUNE_JACOBI_X = 21,
+  /* read only: the Jacobi sweeps of the last x theta-scheme sweep (0: none) */
+  ASH3D_TUNE_JACOBI_LAST_X = 22
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial
  * pivoting) in dgtsv's order: bitwise the reference.  PARTITIONED splits

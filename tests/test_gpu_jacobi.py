@@ -1,6 +1,8 @@
 """Jacobi sweeps for the y and z theta-scheme lines that the partitioned
 solves leave to dgtsv order (line_solver="partitioned", lines longer than
-part_max_line_strided), against the oracle's dgtsv-order solves.
+part_max_line_strided), and for the x lines that seven sweeps solve (the
+partitioned solves taking the other x lines), against the oracle's
+dgtsv-order solves.
 
 Row t of a line gives x_t = y_t + a_t x_{t-1} + b_t x_{t+1} (y = B / D,
 a = th L / D, b = th R / D); K + 1 sweeps from zero leave an error of at
@@ -22,6 +24,11 @@ dgtsv order.
 * The first sweep after a factorisation (rho not yet on the host) runs in
   dgtsv order; full steps with glue and a source over several steps stay
   within the bar.
+* x lines (one workgroup per line, the rows across its lanes and waves):
+  every line, some or none of them with rho <= 5.79e-3 (per-line rho
+  recomputed here; the sweeps those lines take pinned), periodic and not,
+  lines of 600 cells over three waves with subgrid bounds and a mask;
+  lines of one wave stay in the partitioned solves by default.
 """
 import math
 
@@ -45,21 +52,21 @@ def tr(pkg):
 
 
 def line_factors(st, dirn):
-    """Row factors L, R of every interior cell for lines along dirn (1: y,
-    2: z), src/Diffusion.F90:807-816: kds_I = 0.5 (K_{I-1} + K_I) sig_I /
-    (0.5 (kap_{I-1} + kap_I)), L = dt kds_c sig_c / kap_c, R = dt
-    kds_{c+1} sig_{c+1} / kap_c.  Arrays indexed [line a, line b, row]."""
+    """Row factors L, R of every interior cell for lines along dirn (0: x,
+    1: y, 2: z), src/Diffusion.F90:807-816: kds_I = 0.5 (K_{I-1} + K_I)
+    sig_I / (0.5 (kap_{I-1} + kap_I)), L = dt kds_c sig_c / kap_c, R = dt
+    kds_{c+1} sig_{c+1} / kap_c.  Arrays indexed [line a, line b, row]
+    (x lines: a = j, b = k; y lines: i, k; z lines: i, j)."""
     nx, ny, nz = st.nx, st.ny, st.nz
-    kk = st.ky if dirn == 1 else st.kz
-    sg = st.sigma_ny_pd if dirn == 1 else st.sigma_nz_pd
+    kk = (st.kx, st.ky, st.kz)[dirn]
+    sg = (st.sigma_nx_pd, st.sigma_ny_pd, st.sigma_nz_pd)[dirn]
     # cells 0..n+1 in every dimension (kappa and sigma carry two ghosts, K one)
     kap = st.kappa_pd[1:nx + 3, 1:ny + 3, 1:nz + 3]
     sig = sg[1:nx + 3, 1:ny + 3, 1:nz + 3]
     K = kk[0:nx + 2, 0:ny + 2, 0:nz + 2]
-    if dirn == 1:  # (x, z, y)
-        kap, sig, K = (np.transpose(v[1:nx + 1, :, 1:nz + 1], (0, 2, 1)) for v in (kap, sig, K))
-    else:  # (x, y, z)
-        kap, sig, K = (v[1:nx + 1, 1:ny + 1, :] for v in (kap, sig, K))
+    idx = [slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1)]
+    idx[dirn] = slice(None)
+    kap, sig, K = (np.moveaxis(v[tuple(idx)], dirn, -1) for v in (kap, sig, K))
     n = kap.shape[2] - 2
     kds = 0.5 * (K[..., 0:n + 1] + K[..., 1:n + 2]) * (sig[..., 1:n + 2] / (0.5 * (kap[..., 0:n + 1] + kap[..., 1:n + 2])))
     kc = kap[..., 1:n + 1]
@@ -70,6 +77,11 @@ def line_factors(st, dirn):
 
 def line_rho(st, dirn, rmin=1, rmax=None):
     """max (|DL| + |DU|) / |D| over rows rmin..rmax of every line."""
+    return float(per_line_rho(st, dirn, rmin, rmax).max())
+
+
+def per_line_rho(st, dirn, rmin=1, rmax=None):
+    """max (|DL| + |DU|) / |D| over rows rmin..rmax, per line."""
     L, R = line_factors(st, dirn)
     rmax = L.shape[2] if rmax is None else rmax
     L, R = L[..., rmin - 1:rmax], R[..., rmin - 1:rmax]
@@ -81,7 +93,17 @@ def line_rho(st, dirn, rmin=1, rmax=None):
     DL[..., 0] = 0.0
     DU = th * R
     DU[..., -1] = 0.0
-    return float(((np.abs(DL) + np.abs(DU)) / np.abs(D)).max())
+    return ((np.abs(DL) + np.abs(DU)) / np.abs(D)).max(axis=-1)
+
+
+RHO_LINE_MAX = 5.79e-3  # lines whose rho lets seven sweeps suffice
+
+
+def x_split(st, rmin=1, rmax=None):
+    """(lines taking the Jacobi sweeps, all lines, the sweeps they take)."""
+    r = per_line_rho(st, 0, rmin, rmax)
+    j = r <= RHO_LINE_MAX
+    return int(j.sum()), r.size, expected_sweeps(float(r[j].max())) if j.any() else 0
 
 
 def expected_sweeps(rho, max_sweeps=MAX_SWEEPS):
@@ -265,3 +287,72 @@ def test_multistep_with_glue_and_source(pkg, oracle, tr, periodic):
     print("sweeps per step (y, z):", seen, flush=True)
     assert all(y > 0 and z > 0 for y, z in seen[1:]), seen
     assert_same(a, b, names=ARRAYS, tol=TOL)
+
+
+def x_state(pkg, nx, ny, nz, ns, scale, seed, periodic=False, theta=0.5):
+    """Lon-lat grid from about 30-40 deg N: the rows' rho grows with
+    latitude (1 / cos^2), so a dt scale can put the line split anywhere."""
+    st = pkg.workloads.random_state(nx, ny, nz, ns, seed=seed, latlon=True, periodic=periodic, imp_fac=theta)
+    st.dt *= scale
+    return st
+
+
+# (periodic, dt scale): all lines, 238 and 19 of 240, none; periodic: all,
+# 1 of 240, none
+X_SPLITS = [(False, 30.0), (False, 150.0), (False, 300.0), (False, 1000.0), (True, 1e4), (True, 3e5),
+            (True, 1e6)]
+
+
+@pytest.mark.parametrize("periodic,scale", X_SPLITS)
+def test_x_lines_split(pkg, oracle, tr, periodic, scale):
+    """x lines (jacobi_x=2: any length): every line, some, or none in
+    Jacobi sweeps, the others in the partitioned solves."""
+    st = x_state(pkg, 24, 40, 6, 3, scale, seed=29, periodic=periodic)
+    nj, nl, want = x_split(st)
+    seen = []
+    a, b = run(pkg, oracle, tr, st, [("Set_BC", 2), ("diffCN_x",), ("diffCN_x",), ("diffCN_x",)], jacobi_x=2,
+               after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
+    print(f"{nj} of {nl} lines in {want} sweeps; seen {seen}", flush=True)
+    assert seen[1] == 0 and seen[2] == seen[3] == want, (seen, want)
+    assert_same(a, b, tol=TOL)
+
+
+def test_x_lines_split_covers_both_kinds(pkg):
+    """The scales above put some lines on each side (checked on the CPU)."""
+    for periodic in (False, True):
+        kinds = set()
+        for p, scale in X_SPLITS:
+            if p == periodic:
+                nj, nl, _ = x_split(x_state(pkg, 24, 40, 6, 3, scale, seed=29, periodic=p))
+                kinds.add("all" if nj == nl else "none" if nj == 0 else "some")
+        assert kinds == {"all", "some", "none"}, (periodic, kinds)
+
+
+@pytest.mark.parametrize("theta,scale", [(0.5, 200.0), (1.0, 100.0)])
+@pytest.mark.parametrize("nb", [1, 2, 4])
+def test_long_x_lines(pkg, oracle, tr, theta, scale, nb):
+    """Lines of 600 cells (default jacobi_x=1: beyond one wave), 68 of 72
+    lines in Jacobi sweeps, subgrid bounds and an IsAloft mask, 1, 2 and 4
+    bins per thread."""
+    st = x_state(pkg, 600, 24, 3, 5, scale, seed=31, theta=theta)
+    st.is_aloft = np.array([1, 1, 0, 1, 1], dtype=np.int32)
+    st.bounds = (3, 590, 2, 23, 1, 3)
+    nj, nl, want = x_split(st, 3, 590)
+    seen = []
+    a, b = run(pkg, oracle, tr, st, [("diffCN_x",)] * 3, nb_diffuse_x=nb,
+               after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
+    print(f"{nj} of {nl} lines in {want} sweeps; seen {seen}", flush=True)
+    assert 0 < nj < nl
+    assert seen == [0, want, want], (seen, want)
+    assert_same(a, b, tol=TOL)
+
+
+def test_short_x_lines_keep_the_partitioned_solves(pkg, oracle, tr):
+    """jacobi_x=1 (default): x lines of one wave stay in the partitioned
+    solves."""
+    st = x_state(pkg, 24, 40, 6, 3, 30.0, seed=29)
+    seen = []
+    a, b = run(pkg, oracle, tr, st, [("diffCN_x",)] * 3,
+               after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
+    assert seen == [0, 0, 0]
+    assert_same(a, b, tol=TOL)

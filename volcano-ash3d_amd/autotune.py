@@ -64,8 +64,12 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             params.append(("part_max_line", "diffuse_x"))
         if st.ny > 512 or st.nz > 512:
             params.append(("part_max_line_strided", ("diffuse_y", "diffuse_z")))
-            # Jacobi sweeps for the strided lines left to dgtsv order
-            params.append(("jacobi_sweeps", ("diffuse_y", "diffuse_z")))
+        if st.nx > 512:
+            # the lines that seven Jacobi sweeps solve off the multi-wave partitioned solves
+            params.append(("jacobi_x", "diffuse_x"))
+        if max(st.nx, st.ny, st.nz) > 512:
+            # Jacobi sweeps: strided lines left to dgtsv order, x lines as above
+            params.append(("jacobi_sweeps", ("diffuse_x", "diffuse_y", "diffuse_z")))
     for param, kind in params:
         kinds = kind if isinstance(kind, tuple) else (kind,)
         if kinds[0].startswith("diffuse") and not use_diffusion:
@@ -77,6 +81,7 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
         values = (X_TILES if param == "x_tile" else PART_BINS if param == "part_bins" else
                   (1, 0) if param == "adv_fused" else
                   (JACOBI_SWEEPS, 0) if param == "jacobi_sweeps" else
+                  (1, 0) if param == "jacobi_x" else
                   ADV_GROUPS if param.startswith("adv_groups") else
                   ADV_X_ROWS if param == "adv_x_rows" else
                   PART_MAX_LINE if param.startswith("part_max_line") else

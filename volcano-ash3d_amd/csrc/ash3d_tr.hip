@@ -112,14 +112,18 @@ struct ash3d_tr_ctx {
   bool lr_ok[3] = {false, false, false};
   // LU factors of the theta-scheme lines per direction (one allocation:
   // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags, one
-  // INFO per line, the count of lines that are not diagonally dominant
-  // and the largest row ratio (|DL| + |DU|) / |D| as double bits), valid
-  // for the key (dt, theta, rmin, rmax)
+  // INFO per line, the count of lines that are not diagonally dominant,
+  // the largest row ratio (|DL| + |DU|) / |D| as double bits, per line a
+  // flag for the lines seven Jacobi sweeps solve and the largest ratio
+  // among those), valid for the key (dt, theta, rmin, rmax)
   double *lu[3] = {nullptr, nullptr, nullptr};
   uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
   int *lu_sing[3] = {nullptr, nullptr, nullptr};
   int *lu_ndom[3] = {nullptr, nullptr, nullptr};
   unsigned long long *lu_rho[3] = {nullptr, nullptr, nullptr};
+  uint8_t *lu_jl[3] = {nullptr, nullptr, nullptr};
+  unsigned long long *lu_rhoj[3] = {nullptr, nullptr, nullptr};
+  int *lu_njl[3] = {nullptr, nullptr, nullptr};
   double lu_dt[3] = {0.0, 0.0, 0.0}, lu_th[3] = {0.0, 0.0, 0.0};
   int lu_rmin[3] = {0, 0, 0}, lu_rmax[3] = {0, 0, 0};
   bool lu_ok[3] = {false, false, false};
@@ -128,9 +132,10 @@ struct ash3d_tr_ctx {
   // back asynchronously after a factorisation): 0 unknown, 1 in flight
   // (ndom_ev), 2 known.  When known, a sweep launches only the solver it
   // needs instead of both with the choice made on the device.  The row
-  // ratio comes back with it (rho_h, in the same pinned block).
-  int *ndom_h = nullptr;
-  unsigned long long *rho_h = nullptr;
+  // ratios and the flagged line count come back with it (njl_h, rho_h,
+  // rhoj_h, in the same pinned block).
+  int *ndom_h = nullptr, *njl_h = nullptr;
+  unsigned long long *rho_h = nullptr, *rhoj_h = nullptr;
   hipEvent_t ndom_ev[3] = {};
   int ndom_state[3] = {0, 0, 0};
   bool capturing = false;
@@ -215,6 +220,10 @@ struct ash3d_tr_ctx {
   // theta-scheme sweeps took (0: another solver)
   int jacobi_sweeps = kJacobiMaxIters + 1;
   int jacobi_last[3] = {0, 0, 0};
+  // x lines: 1 the lines that seven sweeps solve take them and the
+  // partitioned solves the rest, when these run over several waves per
+  // line (longer than 512 cells); 2 at any length; 0 never
+  int jacobi_x = 1;
   // dt / kappa per cell for the fused pass, valid for dtv_dt
   double *dtvc = nullptr;
   double dtv_dt = 0.0;
@@ -467,17 +476,21 @@ int ensure_lu(ash3d_tr_ctx *x, int dir, DiffArgs &a) {
   int rc = 0;
   const long long nc = x->d.ncell, lines = (long long)a.g.Na * a.g.Nb;
   if (!x->lu[dir]) {
-    if ((rc = dev_alloc(&x->lu[dir], 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 2))) return rc;
+    const long long tail = 4 * nc + (nc + 7) / 8 + (lines + 1) / 2;
+    if ((rc = dev_alloc(&x->lu[dir], tail + 4 + (lines + 7) / 8))) return rc;
     x->lu_piv[dir] = (uint8_t *)(x->lu[dir] + 4 * nc);
     x->lu_sing[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8);
-    x->lu_ndom[dir] = (int *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2);
-    x->lu_rho[dir] = (unsigned long long *)(x->lu[dir] + 4 * nc + (nc + 7) / 8 + (lines + 1) / 2 + 1);
+    x->lu_ndom[dir] = (int *)(x->lu[dir] + tail);
+    x->lu_rho[dir] = (unsigned long long *)(x->lu[dir] + tail + 1);
+    x->lu_rhoj[dir] = (unsigned long long *)(x->lu[dir] + tail + 2);
+    x->lu_njl[dir] = (int *)(x->lu[dir] + tail + 3);
+    x->lu_jl[dir] = (uint8_t *)(x->lu[dir] + tail + 4);
   }
   if (!x->junk && (rc = dev_alloc(&x->junk, kJunk))) return rc;
   if (!x->lu_ok[dir] || x->lu_dt[dir] != a.dt || x->lu_th[dir] != a.theta || x->lu_rmin[dir] != a.g.rmin ||
       x->lu_rmax[dir] != a.g.rmax) {
     HIPCHK(launch_line_lu(x->stream, a, x->lu[dir], x->lu_piv[dir], x->lu_sing[dir], x->lu_ndom[dir],
-                          x->lu_rho[dir]));
+                          x->lu_rho[dir], x->lu_jl[dir], x->lu_rhoj[dir], x->lu_njl[dir]));
     ++x->lu_count;
     x->ndom_state[dir] = 0;
     x->lu_ok[dir] = true;
@@ -522,11 +535,13 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
   if (x->capturing || !x->lu_ok[dir]) return -1;
   if (x->ndom_state[dir] == 0) {
     if (!x->ndom_h) {
-      if (hipHostMalloc((void **)&x->ndom_h, 64, hipHostMallocDefault) != hipSuccess) {
+      if (hipHostMalloc((void **)&x->ndom_h, 128, hipHostMallocDefault) != hipSuccess) {
         x->ndom_h = nullptr;
         return -1;
       }
-      x->rho_h = (unsigned long long *)(x->ndom_h + 4);
+      x->njl_h = x->ndom_h + 4;
+      x->rho_h = (unsigned long long *)(x->ndom_h + 8);
+      x->rhoj_h = x->rho_h + 3;
     }
     if (!x->ndom_ev[dir] && hipEventCreateWithFlags(&x->ndom_ev[dir], hipEventDisableTiming) != hipSuccess) {
       x->ndom_ev[dir] = nullptr;
@@ -536,6 +551,10 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
             hipSuccess ||
         hipMemcpyAsync(x->rho_h + dir, x->lu_rho[dir], sizeof(unsigned long long), hipMemcpyDeviceToHost,
                        x->stream) != hipSuccess ||
+        hipMemcpyAsync(x->rhoj_h + dir, x->lu_rhoj[dir], sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       x->stream) != hipSuccess ||
+        hipMemcpyAsync(x->njl_h + dir, x->lu_njl[dir], sizeof(int), hipMemcpyDeviceToHost, x->stream) !=
+            hipSuccess ||
         hipEventRecord(x->ndom_ev[dir], x->stream) != hipSuccess)
       return -1;
     x->ndom_state[dir] = 1;
@@ -557,11 +576,7 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
 // with rho^(K+1) <= 2^-52 (1 - rho), the rounding of the sweeps growing
 // at most by 1 / (1 - rho).  -1 while the factors' ratio is not known on
 // the host (the first sweep after a factorisation) and in graph captures.
-int jacobi_iters(ash3d_tr_ctx *x, int dir) {
-  if (dir == 0 || x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0) return -1;
-  if (host_ndom(x, dir) != 0) return -1;
-  double rho;
-  std::memcpy(&rho, x->rho_h + dir, sizeof rho);
+int jacobi_k(double rho, int max_sweeps) {
   if (!(rho >= 0.0 && rho < 1.0)) return -1;
   const double bound = std::ldexp(1.0, -52) * (1.0 - rho);
   double e = rho;
@@ -570,8 +585,30 @@ int jacobi_iters(ash3d_tr_ctx *x, int dir) {
     e *= rho;
     ++k;
   }
-  if (e > bound || k + 1 > x->jacobi_sweeps) return -1;
+  if (e > bound || k + 1 > max_sweeps) return -1;
   return k;
+}
+
+int jacobi_iters(ash3d_tr_ctx *x, int dir) {
+  if (dir == 0 || x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0) return -1;
+  if (host_ndom(x, dir) != 0) return -1;
+  double rho;
+  std::memcpy(&rho, x->rho_h + dir, sizeof rho);
+  return jacobi_k(rho, x->jacobi_sweeps);
+}
+
+// x lines of n active cells: the K of the Jacobi sweeps for the lines that
+// seven sweeps solve (the factorisation's flags; rho over those lines),
+// the partitioned solves taking the others, or -1 (no Jacobi sweeps: the
+// x-line solves as before).  Only where the partitioned solves certainly
+// run for the sweep (every line dominant, n within their range).
+int jacobi_x_iters(ash3d_tr_ctx *x, int n) {
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0 || x->jacobi_x == 0) return -1;
+  if (n < 3 || n > x->part_max_line || (x->jacobi_x == 1 && n <= 512)) return -1;
+  if (host_ndom(x, 0) != 0 || x->njl_h[0] == 0) return -1;
+  double rho;
+  std::memcpy(&rho, x->rhoj_h, sizeof rho);
+  return jacobi_k(rho, x->jacobi_sweeps);
 }
 
 // Cell offsets of the partitioned solves of direction dir: the state in
@@ -675,9 +712,18 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     transpose_x_args(x, bounds, a);
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
     {
-      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for both kernels
+      ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for the kernels
+      const int kx = jacobi_x_iters(x, a.g.rmax - a.g.rmin + 1);
+      x->jacobi_last[0] = kx + 1;
+      if (kx >= 0) {
+        // the lines seven sweeps solve, then the partitioned solves of the
+        // others (which certainly run: every line dominant, n in range)
+        a.jline = x->lu_jl[0];
+        HIPCHK(launch_diff_cn_jacobi_x(x->stream, a, kx, d, x->ts0(), x->kap, x->sx, x->kx));
+      }
       const int tp = try_cn_partitioned(x, 0, a, &rc);
       if (rc) return rc;
+      if (kx >= 0 && tp != 2) return fail(ASH3D_TR_EDEVICE, "x-line Jacobi sweeps without the partitioned solves");
       if (tp != 2) HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
     }
     x->ts1_state = TS1_MIRROR;
@@ -1368,8 +1414,13 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     x->jacobi_sweeps = value;
     return 0;
   }
-  if (param == ASH3D_TUNE_JACOBI_LAST_Y || param == ASH3D_TUNE_JACOBI_LAST_Z)
+  if (param == ASH3D_TUNE_JACOBI_LAST_Y || param == ASH3D_TUNE_JACOBI_LAST_Z || param == ASH3D_TUNE_JACOBI_LAST_X)
     return fail(ASH3D_TR_EINVAL, "tuning parameter %d can only be read", param);
+  if (param == ASH3D_TUNE_JACOBI_X) {
+    if (value < 0 || value > 2) return fail(ASH3D_TR_EINVAL, "x-line Jacobi sweeps must be 0, 1 or 2 (got %d)", value);
+    x->jacobi_x = value;
+    return 0;
+  }
   if (param == ASH3D_TUNE_ADV_X_ROWS) {
     if (value != 0 && value != 1)
       return fail(ASH3D_TR_EINVAL, "x rows kernel must be 0 (workgroup rows) or 1 (wavefront rows) (got %d)", value);
@@ -1434,6 +1485,8 @@ int ash3d_tr_tune_get(ash3d_tr_ctx *x, int32_t param, int32_t *value) {
     case ASH3D_TUNE_JACOBI_SWEEPS: *value = x->jacobi_sweeps; return 0;
     case ASH3D_TUNE_JACOBI_LAST_Y: *value = x->jacobi_last[1]; return 0;
     case ASH3D_TUNE_JACOBI_LAST_Z: *value = x->jacobi_last[2]; return 0;
+    case ASH3D_TUNE_JACOBI_X: *value = x->jacobi_x; return 0;
+    case ASH3D_TUNE_JACOBI_LAST_X: *value = x->jacobi_last[0]; return 0;
     default: return fail(ASH3D_TR_EINVAL, "unknown tuning parameter %d", param);
   }
 }

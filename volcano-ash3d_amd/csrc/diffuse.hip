@@ -14,6 +14,7 @@
 #include <float.h>
 
 #include "common.hpp"
+#include "lanes.hpp"
 
 namespace ash3d {
 
@@ -69,7 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_line_lr(DiffArgs A, double *lr) {
 // solves, which eliminate without pivoting, are used only when it is zero.
 template <int DIR>
 __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint8_t *piv, int *sing, int *ndom,
-                                                   unsigned long long *rho) {
+                                                   unsigned long long *rho, uint8_t *jl, unsigned long long *rho_j,
+                                                   int *njl) {
   const Dims d = A.d;
   const LineGeom g = A.g;
   const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -167,9 +169,17 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   if (!dom) atomicAdd(ndom, 1);
   // the sweep's largest row ratio (non-negative doubles order as their bit
   // patterns); lines that are not dominant count as 1
-  if (rho) {
-    const double r = (dom && rho_line < 1.0) ? rho_line : 1.0;
-    atomicMax(rho, (unsigned long long)__double_as_longlong(r));
+  const double r = (dom && rho_line < 1.0) ? rho_line : 1.0;
+  if (rho) atomicMax(rho, (unsigned long long)__double_as_longlong(r));
+  // lines that seven Jacobi sweeps solve (rho_line^7 <= 2^-52 (1 - rho_line)),
+  // their number and the largest ratio among them
+  if (jl) {
+    const bool j = r <= kJacobiRhoMax;
+    jl[gid] = j ? 1 : 0;
+    if (j) {
+      atomicMax(rho_j, (unsigned long long)__double_as_longlong(r));
+      atomicAdd(njl, 1);
+    }
   }
 }
 
@@ -469,6 +479,8 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
   const int N = g.N, rmin = g.rmin, rmax = g.rmax;
   const int n = rmax - rmin + 1;
   const long long sL = line_stride(d, DIR), base = line_base(d, DIR, a, b);
+  // the state: cell r of the line at qb + qs*(r+1), bin stride qn
+  const long long qs = sL, qb = base, qn = d.ncell;
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   double *c = A.c0;
   bool exists[NB], on[NB];
@@ -479,16 +491,17 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
     exists[u] = (n0 + u) < d.ns;
     on[u] = exists[u] && line_on && A.aloft.on(n0 + u);
     any_on = any_on || on[u];
-    bofs[u] = d.ncell * (long long)min(n0 + u, d.ns - 1);
+    bofs[u] = qn * (long long)min(n0 + u, d.ns - 1);
   }
   if (any_on) {
     const double th = A.theta, th1 = 1.0 - th;
     const double *Lf = A.lr, *Rf = A.lr + d.ncell;
-    auto ofs = [&](int l) { return base + sL * (long long)(rmin + l); };  // row l: cell rmin-1+l
+    auto lofs = [&](int l) { return base + sL * (long long)(rmin + l); };  // row l: cell rmin-1+l
+    auto ofs = [&](int l) { return qb + qs * (long long)(rmin + l); };
     // rows ahead: slot p holds L, R of row s and q of row s+1
     double rL[PF], rR[PF], rq[PF][NB];
     auto fetch = [&](int slot, int l) {
-      const long long o = ofs(min(l, n));
+      const long long o = lofs(min(l, n));
       rL[slot] = Lf[o];
       rR[slot] = Rf[o];
       const long long oq = ofs(min(l + 1, n));
@@ -599,13 +612,175 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (!exists[u]) continue;
-    const long long bo = d.ncell * (long long)(n0 + u);
+    const long long bo = qn * (long long)(n0 + u);
     if (!on[u]) {
-      for (int r = 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
+      for (int r = 1; r <= N; ++r) c[qb + qs * (long long)(r + 1) + bo] = 0.0;
     } else {
-      for (int r = 1; r < rmin; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
-      for (int r = rmax + 1; r <= N; ++r) c[base + sL * (long long)(r + 1) + bo] = 0.0;
+      for (int r = 1; r < rmin; ++r) c[qb + qs * (long long)(r + 1) + bo] = 0.0;
+      for (int r = rmax + 1; r <= N; ++r) c[qb + qs * (long long)(r + 1) + bo] = 0.0;
     }
+  }
+}
+
+// x lines (contiguous in memory) by Jacobi sweeps, the lines that seven
+// sweeps solve (A.jline; the partitioned solves take the others).  One
+// workgroup of WV waves per line: lane l of wave w holds rows
+// i = 64 M w + 64 m + l (m < M rows per lane, 64 M per wave),
+// so that every load and store of a row block is one coalesced segment
+// and the neighbours of a row are in the neighbouring lanes (whole-wave
+// DPP rotations; lanes 0 and 63 take the neighbouring block's value, and
+// the first and last row of a wave go to the neighbouring waves through
+// LDS, one barrier per exchange).  The row factors L, R are formed from
+// kappa, sigma_nx and kx as k_line_lr forms them (same arithmetic, so the
+// same values as the factor cache) once per line, with the Jacobi
+// coefficients; then every aloft bin of the line in turn: its right-hand
+// side (the next bin's q already in flight), K sweeps in place, one store.
+// The arithmetic of a row is k_cn_jacobi's.  A is the x<->y transposed
+// description the other x-line solves use (lines a = j, b = k, rows along
+// x); X the original layout.
+struct XJac {
+  Dims o;                        // original dims
+  double *c;                     // concen ts0
+  const double *kap, *sig, *kk;  // kappa, sigma_nx (with ghosts) and kx
+};
+
+template <int WV, int M, int K>
+__global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
+  __shared__ double s_ex[2][WV][2];  // per parity: each wave's first and last value
+  const LineGeom g = A.g;
+  const long long line = blockIdx.x;
+  if (A.jline && !A.jline[line]) return;  // a line of the partitioned solves (uniform)
+  const int a = (int)(line % g.Na) + 1;   // j
+  const int b = (int)(line / g.Na) + 1;   // k
+  const Dims o = X.o;
+  const int rmin = g.rmin, rmax = g.rmax, n = rmax - rmin + 1, N = g.N;
+  const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const bool lane0 = l == 0, lane63 = l == 63;
+  const long long ob = o.p3(-1, a, b);       // cell r at ob + r + 1
+  const long long kb = kline_base(o, 0, a, b);  // K of cell r at kb + r
+  double *c = X.c;
+  const double th = A.theta, th1 = 1.0 - th, dt = A.dt;
+  auto row = [&](int m) { return w * 64 * M + m * 64 + l; };
+  // per row: L, R, 1/D and the Jacobi coefficients (zero on padding rows)
+  double rL[M], rR[M], rD[M], ca[M], cb[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = row(m);
+    const bool real = i < n;
+    const int r = rmin + (real ? i : 0);
+    auto kapat = [&](int q) { return X.kap[ob + q + 1]; };
+    auto Kat = [&](int q) { return X.kk[kb + q]; };
+    auto sigat = [&](int q) { return X.sig[ob + q + 1]; };
+    auto kds_at = [&](int I) {
+      const double vavg = 0.5 * (kapat(I - 1) + kapat(I));
+      const double kavg = 0.5 * (Kat(I - 1) + Kat(I));
+      const double ds = sigat(I) / vavg;
+      return kavg * ds;
+    };
+    const double kap_c = kapat(r);
+    const double L = dt * kds_at(r) * sigat(r) / kap_c;
+    const double R = dt * kds_at(r + 1) * sigat(r + 1) / kap_c;
+    const double C = L + R;
+    double D = 1.0 + th * C;
+    if (i == 0) D = D - th * L;
+    if (i == n - 1) D = D - th * R;
+    const double rd = 1.0 / D;
+    rL[m] = real ? L : 0.0;
+    rR[m] = real ? R : 0.0;
+    rD[m] = real ? rd : 0.0;
+    ca[m] = (real && i > 0) ? (th * L) * rd : 0.0;
+    cb[m] = (real && i < n - 1) ? (th * R) * rd : 0.0;
+  }
+  // the values of the rows before and after this wave's rows (the
+  // neighbouring waves' last and first; zero beyond the line)
+  int parity = 0;
+  auto exchange = [&](double first, double last, double &before, double &after) {
+    before = after = 0.0;
+    if constexpr (WV > 1) {
+      if (lane0) s_ex[parity][w][0] = first;
+      if (lane63) s_ex[parity][w][1] = last;
+      __syncthreads();
+      if (w > 0) before = s_ex[parity][w - 1][1];
+      if (w + 1 < WV) after = s_ex[parity][w + 1][0];
+      parity ^= 1;
+    }
+  };
+  // lo[m] = v at row i-1, hi[m] = v at row i+1, m ascending (all lanes)
+  auto neighbours = [&](const double (&v)[M], double before, double after, auto &&f) {
+    double ror_prev = 0.0;
+    double rol_cur = dpp_f64<kWaveRol1>(v[0]);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double ror_cur = dpp_f64<kWaveRor1>(v[m]);
+      const double rol_next = dpp_f64<kWaveRol1>(v[m + 1 < M ? m + 1 : m]);
+      const double lo = lane0 ? (m > 0 ? ror_prev : before) : ror_cur;
+      const double hi = lane63 ? (m + 1 < M ? rol_next : after) : rol_cur;
+      f(m, lo, hi);
+      ror_prev = ror_cur;
+      rol_cur = rol_next;
+    }
+  };
+  const long long ncell = o.ncell;
+  const int ns = o.ns;
+  auto on_bin = [&](int u) { return line_on && A.aloft.on(u); };
+  auto next_on = [&](int u) {
+    while (u < ns && !on_bin(u)) ++u;
+    return u;
+  };
+  double q[M];
+  auto load = [&](int u) {
+    const long long bo = ncell * (long long)u;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = row(m);
+      q[m] = c[ob + rmin + (i < n ? i : 0) + 1 + bo];
+    }
+  };
+  int u = next_on(0);
+  if (u < ns) load(u);
+  while (u < ns) {
+    // right-hand side (the reference's q_0 = q_1 and q_{n+1} = q_n)
+    double y[M], x[M];
+    {
+      double before, after;
+      exchange(q[0], q[M - 1], before, after);
+      neighbours(q, before, after, [&](int m, double ql, double qr) {
+        const int i = row(m);
+        if (i == 0) ql = q[m];
+        if (i == n - 1) qr = q[m];
+        const double L = rL[m], R = rR[m], C = L + R;
+        y[m] = ((th1 * L * ql + (1.0 - th1 * C) * q[m]) + th1 * R * qr) * rD[m];
+      });
+    }
+    const int un = next_on(u + 1);
+    if (un < ns) load(un);
+#pragma unroll
+    for (int m = 0; m < M; ++m) x[m] = y[m];
+    for (int k = 0; k < K; ++k) {
+      double before, after;
+      exchange(x[0], x[M - 1], before, after);
+      double xo[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) xo[m] = x[m];
+      neighbours(xo, before, after,
+                 [&](int m, double lo, double hi) { x[m] = (y[m] + ca[m] * lo) + cb[m] * hi; });
+    }
+    const long long bo = ncell * (long long)u;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = row(m);
+      if (i < n) c[ob + rmin + i + 1 + bo] = x[m];
+    }
+    u = un;
+  }
+  // cells the reference leaves at zero: outside [rmin, rmax], and every
+  // cell of bins that are not aloft and of lines outside the bounds
+  for (int v = 0; v < ns; ++v) {
+    const bool on = on_bin(v);
+    const long long bo = ncell * (long long)v;
+    for (int r = 1 + (int)threadIdx.x; r <= N; r += 64 * WV)
+      if (!on || r < rmin || r > rmax) c[ob + r + 1 + bo] = 0.0;
   }
 }
 
@@ -944,15 +1119,26 @@ hipError_t launch_line_lr(hipStream_t s, const DiffArgs &a, double *lr) {
 }
 
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom,
-                          unsigned long long *rho) {
+                          unsigned long long *rho, uint8_t *jl, unsigned long long *rho_j, int *njl) {
+  if ((jl == nullptr) != (rho_j == nullptr) || (jl == nullptr) != (njl == nullptr)) return hipErrorInvalidValue;
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)((lines + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, ndom);
   if (rho) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, rho);
+  if (rho_j) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, rho_j);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, njl);
+  }
   switch (a.g.dir) {
-    case 0: hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
-    case 1: hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
-    default: hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho); break;
+    case 0:
+      hipLaunchKernelGGL((k_line_lu<0>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho, jl, rho_j, njl);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_line_lu<1>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho, jl, rho_j, njl);
+      break;
+    default:
+      hipLaunchKernelGGL((k_line_lu<2>), grid, dim3(kBlock), 0, s, a, lu, piv, sing, ndom, rho, jl, rho_j, njl);
+      break;
   }
   return hipGetLastError();
 }
@@ -991,6 +1177,22 @@ hipError_t launch_jacobi_nb(hipStream_t s, const DiffArgs &a, int k) {
   return a.g.dir == 1 ? launch_jacobi_k<1, NB>(s, a, k, grid) : launch_jacobi_k<2, NB>(s, a, k, grid);
 }
 
+template <int WV, int M>
+hipError_t launch_jacobi_x_k(hipStream_t s, const DiffArgs &a, const XJac &x, int k) {
+  const dim3 grid((unsigned)((long long)a.g.Na * a.g.Nb)), block(64 * WV);
+  switch (k) {
+    case 0: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 0>), grid, block, 0, s, a, x); break;
+    case 1: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 1>), grid, block, 0, s, a, x); break;
+    case 2: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 2>), grid, block, 0, s, a, x); break;
+    case 3: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 3>), grid, block, 0, s, a, x); break;
+    case 4: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 4>), grid, block, 0, s, a, x); break;
+    case 5: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 5>), grid, block, 0, s, a, x); break;
+    case 6: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 6>), grid, block, 0, s, a, x); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k) {
@@ -1000,6 +1202,26 @@ hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k
     case 1: return launch_jacobi_nb<1>(s, a, k);
     case 2: return launch_jacobi_nb<2>(s, a, k);
     default: return launch_jacobi_nb<4>(s, a, k);
+  }
+}
+
+hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, int k, const Dims &orig, double *c_orig,
+                                   const double *kap, const double *sig, const double *kx) {
+  if (a.g.dir != 1 || k < 0 || k > kJacobiMaxIters) return hipErrorInvalidValue;
+  const int n = a.g.rmax - a.g.rmin + 1;
+  const XJac x{orig, c_orig, kap, sig, kx};
+  // 4 rows per lane, 256 per wave (98 VGPRs; 8 rows per lane: 196, and
+  // 1.6% slower on the 1440-cell lines of the global grid)
+  switch ((n + 255) / 256) {
+    case 1: return launch_jacobi_x_k<1, 4>(s, a, x, k);
+    case 2: return launch_jacobi_x_k<2, 4>(s, a, x, k);
+    case 3: return launch_jacobi_x_k<3, 4>(s, a, x, k);
+    case 4: return launch_jacobi_x_k<4, 4>(s, a, x, k);
+    case 5: return launch_jacobi_x_k<5, 4>(s, a, x, k);
+    case 6: return launch_jacobi_x_k<6, 4>(s, a, x, k);
+    case 7: return launch_jacobi_x_k<7, 4>(s, a, x, k);
+    case 8: return launch_jacobi_x_k<8, 4>(s, a, x, k);
+    default: return hipErrorInvalidValue;
   }
 }
 

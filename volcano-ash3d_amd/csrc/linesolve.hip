@@ -105,13 +105,25 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
   if (tg.tile >= (long long)tiles_a * g.Nb) return;  // padding workgroup (uniform)
   const int b = (int)(tg.tile / tiles_a) + 1;
   const int a_first = (int)(tg.tile % tiles_a) * T + 1;
+  // lines of the tile this kernel solves (bit l: line a_first + l): those
+  // of the grid, but not those of A.jline (the Jacobi sweeps'); a tile
+  // with none of them exits (uniform)
+  static_assert(T <= 32, "one bit per line of the tile");
+  unsigned own_mask = 0;
+  for (int l_ = 0; l_ < T; ++l_) {
+    const int al_ = a_first + l_;
+    if (al_ <= g.Na && !(A.jline && A.jline[(long long)(al_ - 1) + (long long)g.Na * (b - 1)]))
+      own_mask |= 1u << l_;
+  }
+  if (own_mask == 0) return;
+  auto owned = [&](int al_) { return ((own_mask >> (al_ - a_first)) & 1u) != 0; };
   const int tid = threadIdx.x;
   const int ln = tid / LPL;  // line of this lane within the tile
   const int p = tid % LPL;   // segment of this lane
   const int pw = p % LP;     // segment within the part (the PCR group)
   const int wv = p / LP;     // part of the line (wave)
   const int a = a_first + ln;
-  const bool has_line = a <= g.Na;
+  const bool has_line = owned(a);
   const int ac = has_line ? a : g.Na;  // an existing line for masked lanes' reads
   const int rmin = g.rmin, rmax = g.rmax, n = rmax - rmin + 1, N = g.N;
   const bool line_on = has_line && a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
@@ -479,7 +491,7 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
       const int l_ = CONTIG ? f / len : f % T;
       const int r_ = (CONTIG ? f % len : f / T) + r_lo;
       const int al_ = a_first + l_;
-      if (al_ <= g.Na) A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)r_ + ncell * (long long)w] = 0.0;
+      if (owned(al_)) A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)r_ + ncell * (long long)w] = 0.0;
     }
   };
   for (int w = u0; w < u1; ++w) {
@@ -494,7 +506,7 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
     auto one = [&](int f, bool in) {
       const int l_ = elem_l(f), i_ = elem_i(f);
       const int al_ = a_first + l_;
-      const bool ok = in && al_ <= g.Na;
+      const bool ok = in && owned(al_);
       const bool lon = al_ >= g.a0 && al_ <= g.a1 && b >= g.b0 && b <= g.b1;
       const double v = (ok && lon) ? s_tile[l_ * PITCH + i_] : 0.0;
       const long long o = Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)(rmin + i_) + ncell * (long long)w;
@@ -508,7 +520,7 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
       for (int f = tid; f < nel; f += NT) {
         const int l_ = elem_l(f), i_ = elem_i(f);
         const int al_ = a_first + l_;
-        if (al_ > g.Na) continue;
+        if (!owned(al_)) continue;
         const bool lon = al_ >= g.a0 && al_ <= g.a1 && b >= g.b0 && b <= g.b1;
         A.c0[Q.st0 + Q.sta * al_ + Q.stb * b + Q.str * (long long)(rmin + i_) + ncell * (long long)w] =
             lon ? s_tile[l_ * PITCH + i_] : 0.0;

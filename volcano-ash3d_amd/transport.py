@@ -571,7 +571,7 @@ class Transport:
                    "x_tile", "line_solver", "part_bins", "part_shape_x", "part_shape_y", "part_shape_z",
                    "adv_groups_y", "adv_groups_z", "adv_x_rows", "part_max_line",
                    "part_max_line_strided", "adv_fused", "jacobi_sweeps", "jacobi_last_y",
-                   "jacobi_last_z")
+                   "jacobi_last_z", "jacobi_x", "jacobi_last_x")
     LINE_SOLVERS = ("exact", "partitioned")
 
     def tune(self, **kw):
