@@ -253,14 +253,15 @@ enum {
    * 0 when it used another solver */
   ASH3D_TUNE_JACOBI_LAST_Y = 19,
   ASH3D_TUNE_JACOBI_LAST_Z = 20,
-  /* partitioned mode, eager steps, x lines: the lines whose rows have
-   * rho <= 5.79e-3 (seven sweeps suffice) take Jacobi sweeps (as many as
-   * the largest rho among them asks for), the partitioned solves the
-   * others; 1 (default) where the partitioned solves would run over
-   * several waves per line (lines longer than 512 cells), 2 at any
+  /* partitioned mode, eager steps, x lines: each line whose rows' rho
+   * needs at most ASH3D_TUNE_JACOBI_SWEEPS sweeps (as above: seven take
+   * rho up to 5.7998e-3) takes that many Jacobi sweeps, the partitioned
+   * solves the others; 1 (default) where the partitioned solves would run
+   * over several waves per line (lines longer than 512 cells), 2 at any
    * length, 0 never */
   ASH3D_TUNE_JACOBI_X = 21,
-  /* read only: the Jacobi sweeps of the last x theta-scheme sweep (0: none) */
+  /* read only: the most Jacobi sweeps a line of the last x theta-scheme
+   * sweep may have taken (0: none) */
   ASH3D_TUNE_JACOBI_LAST_X = 22
 };
 /* Theta-scheme line solves.  EXACT runs dgtsv's elimination (partial

@@ -96,14 +96,13 @@ def per_line_rho(st, dirn, rmin=1, rmax=None):
     return ((np.abs(DL) + np.abs(DU)) / np.abs(D)).max(axis=-1)
 
 
-RHO_LINE_MAX = 5.79e-3  # lines whose rho lets seven sweeps suffice
-
-
-def x_split(st, rmin=1, rmax=None):
-    """(lines taking the Jacobi sweeps, all lines, the sweeps they take)."""
-    r = per_line_rho(st, 0, rmin, rmax)
-    j = r <= RHO_LINE_MAX
-    return int(j.sum()), r.size, expected_sweeps(float(r[j].max())) if j.any() else 0
+def x_split(st, rmin=1, rmax=None, max_sweeps=MAX_SWEEPS):
+    """(lines taking Jacobi sweeps, all lines, the most sweeps a line takes)
+    for x lines: each line the sweeps its own rho needs, if at most
+    max_sweeps."""
+    sw = np.vectorize(expected_sweeps)(per_line_rho(st, 0, rmin, rmax))
+    j = sw > 0
+    return int(j.sum()), sw.size, int(sw[j].max()) if j.any() else 0
 
 
 def expected_sweeps(rho, max_sweeps=MAX_SWEEPS):
@@ -345,6 +344,24 @@ def test_long_x_lines(pkg, oracle, tr, theta, scale, nb):
     assert 0 < nj < nl
     assert seen == [0, want, want], (seen, want)
     assert_same(a, b, tol=TOL)
+
+
+def test_x_lines_sweep_counts_per_line(pkg, oracle, tr):
+    """Rows from 30 to 80 deg: lines needing 5, 6 and 7 sweeps and lines
+    needing more in one sweep (each takes its own number); with the cap at
+    5 the lines needing 6 or 7 go to the partitioned solves too."""
+    st = x_state(pkg, 24, 100, 4, 2, 30.0, seed=29)
+    sw = np.vectorize(expected_sweeps)(per_line_rho(st, 0))
+    counts = {int(v): int((sw == v).sum()) for v in np.unique(sw)}
+    print("lines per sweep count:", counts, flush=True)
+    assert set(counts) == {0, 5, 6, 7}, counts
+    for cap in (7, 5):
+        seen = []
+        a, b = run(pkg, oracle, tr, st, [("diffCN_x",)] * 3, jacobi_x=2, jacobi_sweeps=cap,
+                   after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
+        want = min(max(counts), cap)
+        assert seen == [0, want, want], (cap, seen, want)
+        assert_same(a, b, tol=TOL)
 
 
 def test_short_x_lines_keep_the_partitioned_solves(pkg, oracle, tr):

@@ -577,16 +577,8 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
 // at most by 1 / (1 - rho).  -1 while the factors' ratio is not known on
 // the host (the first sweep after a factorisation) and in graph captures.
 int jacobi_k(double rho, int max_sweeps) {
-  if (!(rho >= 0.0 && rho < 1.0)) return -1;
-  const double bound = std::ldexp(1.0, -52) * (1.0 - rho);
-  double e = rho;
-  int k = 0;
-  while (e > bound && k < kJacobiMaxIters) {
-    e *= rho;
-    ++k;
-  }
-  if (e > bound || k + 1 > max_sweeps) return -1;
-  return k;
+  const int sw = jacobi_sweeps_needed(rho);
+  return (sw == 0 || sw > max_sweeps) ? -1 : sw - 1;
 }
 
 int jacobi_iters(ash3d_tr_ctx *x, int dir) {
@@ -597,18 +589,22 @@ int jacobi_iters(ash3d_tr_ctx *x, int dir) {
   return jacobi_k(rho, x->jacobi_sweeps);
 }
 
-// x lines of n active cells: the K of the Jacobi sweeps for the lines that
-// seven sweeps solve (the factorisation's flags; rho over those lines),
-// the partitioned solves taking the others, or -1 (no Jacobi sweeps: the
-// x-line solves as before).  Only where the partitioned solves certainly
-// run for the sweep (every line dominant, n within their range).
-int jacobi_x_iters(ash3d_tr_ctx *x, int n) {
-  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0 || x->jacobi_x == 0) return -1;
-  if (n < 3 || n > x->part_max_line || (x->jacobi_x == 1 && n <= 512)) return -1;
-  if (host_ndom(x, 0) != 0 || x->njl_h[0] == 0) return -1;
+// x lines of n active cells: whether the lines that need at most
+// jacobi_sweeps Jacobi sweeps take them (each line its own number, from the
+// factorisation), the partitioned solves taking the others.  Returns the
+// most sweeps a line takes (at most the sweeps of the largest ratio among
+// the lines that need at most seven, capped by jacobi_sweeps), or 0 (no
+// Jacobi sweeps: the x-line solves as before).  Only where the partitioned
+// solves certainly run for the sweep (every line dominant, n within their
+// range).
+int jacobi_x_sweeps(ash3d_tr_ctx *x, int n) {
+  if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0 || x->jacobi_x == 0) return 0;
+  if (n < 3 || n > x->part_max_line || (x->jacobi_x == 1 && n <= 512)) return 0;
+  if (host_ndom(x, 0) != 0 || x->njl_h[0] == 0) return 0;
   double rho;
   std::memcpy(&rho, x->rhoj_h, sizeof rho);
-  return jacobi_k(rho, x->jacobi_sweeps);
+  const int sw = jacobi_sweeps_needed(rho);
+  return sw == 0 ? 0 : std::min(sw, x->jacobi_sweeps);
 }
 
 // Cell offsets of the partitioned solves of direction dir: the state in
@@ -713,17 +709,19 @@ int sweep_diffuse(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double imp_fa
     if ((rc = ensure_lr(x, 0, a)) || (rc = ensure_lu(x, 0, a))) return rc;
     {
       ProfScope ps(x, ASH3D_PROF_DIFFUSE_X);  // one record for the kernels
-      const int kx = jacobi_x_iters(x, a.g.rmax - a.g.rmin + 1);
-      x->jacobi_last[0] = kx + 1;
-      if (kx >= 0) {
-        // the lines seven sweeps solve, then the partitioned solves of the
-        // others (which certainly run: every line dominant, n in range)
+      const int sw = jacobi_x_sweeps(x, a.g.rmax - a.g.rmin + 1);
+      x->jacobi_last[0] = sw;
+      if (sw > 0) {
+        // the lines that need at most jacobi_sweeps sweeps, then the
+        // partitioned solves of the others (which certainly run: every
+        // line dominant, n in range)
         a.jline = x->lu_jl[0];
-        HIPCHK(launch_diff_cn_jacobi_x(x->stream, a, kx, d, x->ts0(), x->kap, x->sx, x->kx));
+        a.jcap = x->jacobi_sweeps;
+        HIPCHK(launch_diff_cn_jacobi_x(x->stream, a, d, x->ts0(), x->kap, x->sx, x->kx));
       }
       const int tp = try_cn_partitioned(x, 0, a, &rc);
       if (rc) return rc;
-      if (kx >= 0 && tp != 2) return fail(ASH3D_TR_EDEVICE, "x-line Jacobi sweeps without the partitioned solves");
+      if (sw > 0 && tp != 2) return fail(ASH3D_TR_EDEVICE, "x-line Jacobi sweeps without the partitioned solves");
       if (tp != 2) HIPCHK(launch_diff_cn_xrows(x->stream, a, d, x->ts0(), x->nb_diff[0], x->x_tile));
     }
     x->ts1_state = TS1_MIRROR;

@@ -181,9 +181,12 @@ struct DiffArgs {
   const int *ndom;
   int sel;
   double *junk;        // kJunk doubles that masked-off lanes store to
-  // per line of the sweep (k_line_lu's line index): 1 = solved by the
-  // Jacobi sweeps, which the partitioned solves then skip; null: none
+  // per line of the sweep (k_line_lu's line index): the Jacobi sweeps the
+  // line needs (0: more than seven); lines with 1 <= jline <= jcap are
+  // solved by the Jacobi sweeps and skipped by the partitioned solves.
+  // null: none
   const uint8_t *jline;
+  int jcap;
   double dt, theta;
   int *flag;           // set to the first singular pivot row (dgtsv INFO)
   AloftMask aloft;
@@ -294,24 +297,37 @@ hipError_t spec_diff_cn_part(const DiffArgs &a, const PartLayout &lay, int bins_
 // the LU factors of every line of the sweep (reads a.lr)
 // (*rho: the largest row ratio (|DL| + |DU|) / |D| over the sweep's lines as
 // double bits, 1.0 when a line is not diagonally dominant; may be null.
-// jl[line]: 1 when the line's ratio is at most kJacobiRhoMax, *rho_j the
-// largest ratio of those lines and *njl their number; all three null or
-// all given)
+// jl[line]: the Jacobi sweeps the line needs (jacobi_sweeps_needed of its
+// ratio; 0: more than seven), *rho_j the largest ratio of the lines that
+// need at most seven and *njl their number; all three null or all given)
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom,
                           unsigned long long *rho, uint8_t *jl, unsigned long long *rho_j, int *njl);
 // theta-scheme solves of y or z lines by K + 1 Jacobi sweeps (0 <= k <=
 // kJacobiMaxIters), in one pass, for sweeps whose rows are all diagonally
 // dominant (nb: bins per thread, 1, 2 or 4)
 constexpr int kJacobiMaxIters = 6;
-// the largest row ratio rho with rho^7 <= 2^-52 (1 - rho) (5.7998e-3),
-// rounded down: lines that seven sweeps solve
-constexpr double kJacobiRhoMax = 5.79e-3;
+// The least number of sweeps s <= kJacobiMaxIters + 1 with rho^s <= 2^-52
+// (1 - rho) -- after s sweeps from zero the error is at most rho^s of the
+// line's largest |x|, and the rounding grows at most by 1 / (1 - rho) --
+// or 0 when more are needed (rho not in [0, 1): 0).  Seven sweeps take
+// rho up to 5.7998e-3.
+__host__ __device__ inline int jacobi_sweeps_needed(double rho) {
+  if (!(rho >= 0.0 && rho < 1.0)) return 0;
+  const double bound = 0x1p-52 * (1.0 - rho);
+  double e = rho;
+  for (int s = 1; s <= kJacobiMaxIters + 1; ++s) {
+    if (e <= bound) return s;
+    e *= rho;
+  }
+  return 0;
+}
 hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k);
-// the same for the x lines a.jline names (null: all), which must be
-// dominant: a describes the x<->y transposed grid as for the other x-line
-// solves; the state c_orig, kappa, sigma_nx and kx in the original layout
-// (orig); lines of up to 2048 cells
-hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, int k, const Dims &orig, double *c_orig,
+// the same for x lines, each with the sweeps it needs: the lines with
+// 1 <= a.jline[line] <= a.jcap (the others are the partitioned solves');
+// a describes the x<->y transposed grid as for the other x-line solves;
+// the state c_orig, kappa, sigma_nx and kx in the original layout (orig);
+// lines of up to 2048 cells
+hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c_orig,
                                    const double *kap, const double *sig, const double *kx);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)

@@ -171,12 +171,12 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   // patterns); lines that are not dominant count as 1
   const double r = (dom && rho_line < 1.0) ? rho_line : 1.0;
   if (rho) atomicMax(rho, (unsigned long long)__double_as_longlong(r));
-  // lines that seven Jacobi sweeps solve (rho_line^7 <= 2^-52 (1 - rho_line)),
-  // their number and the largest ratio among them
+  // the Jacobi sweeps the line needs (0: more than seven); the number of
+  // lines that need at most seven and the largest ratio among them
   if (jl) {
-    const bool j = r <= kJacobiRhoMax;
-    jl[gid] = j ? 1 : 0;
-    if (j) {
+    const int sw = dom ? jacobi_sweeps_needed(r) : 0;
+    jl[gid] = (uint8_t)sw;
+    if (sw > 0) {
       atomicMax(rho_j, (unsigned long long)__double_as_longlong(r));
       atomicAdd(njl, 1);
     }
@@ -622,8 +622,9 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
   }
 }
 
-// x lines (contiguous in memory) by Jacobi sweeps, the lines that seven
-// sweeps solve (A.jline; the partitioned solves take the others).  One
+// x lines (contiguous in memory) by Jacobi sweeps, each line with the
+// sweeps it needs (A.jline, up to A.jcap; the partitioned solves take the
+// others).  One
 // workgroup of WV waves per line: lane l of wave w holds rows
 // i = 64 M w + 64 m + l (m < M rows per lane, 64 M per wave),
 // so that every load and store of a row block is one coalesced segment
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
 // kappa, sigma_nx and kx as k_line_lr forms them (same arithmetic, so the
 // same values as the factor cache) once per line, with the Jacobi
 // coefficients; then every aloft bin of the line in turn: its right-hand
-// side (the next bin's q already in flight), K sweeps in place, one store.
+// side (the next bin's q already in flight), the sweeps in place, one store.
 // The arithmetic of a row is k_cn_jacobi's.  A is the x<->y transposed
 // description the other x-line solves use (lines a = j, b = k, rows along
 // x); X the original layout.
@@ -644,12 +645,13 @@ struct XJac {
   const double *kap, *sig, *kk;  // kappa, sigma_nx (with ghosts) and kx
 };
 
-template <int WV, int M, int K>
+template <int WV, int M>
 __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
   __shared__ double s_ex[2][WV][2];  // per parity: each wave's first and last value
   const LineGeom g = A.g;
   const long long line = blockIdx.x;
-  if (A.jline && !A.jline[line]) return;  // a line of the partitioned solves (uniform)
+  const int sweeps = A.jline[line];
+  if (sweeps < 1 || sweeps > A.jcap) return;  // a line of the partitioned solves (uniform)
   const int a = (int)(line % g.Na) + 1;   // j
   const int b = (int)(line / g.Na) + 1;   // k
   const Dims o = X.o;
@@ -757,7 +759,7 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
     if (un < ns) load(un);
 #pragma unroll
     for (int m = 0; m < M; ++m) x[m] = y[m];
-    for (int k = 0; k < K; ++k) {
+    for (int k = 1; k < sweeps; ++k) {
       double before, after;
       exchange(x[0], x[M - 1], before, after);
       double xo[M];
@@ -1178,18 +1180,9 @@ hipError_t launch_jacobi_nb(hipStream_t s, const DiffArgs &a, int k) {
 }
 
 template <int WV, int M>
-hipError_t launch_jacobi_x_k(hipStream_t s, const DiffArgs &a, const XJac &x, int k) {
-  const dim3 grid((unsigned)((long long)a.g.Na * a.g.Nb)), block(64 * WV);
-  switch (k) {
-    case 0: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 0>), grid, block, 0, s, a, x); break;
-    case 1: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 1>), grid, block, 0, s, a, x); break;
-    case 2: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 2>), grid, block, 0, s, a, x); break;
-    case 3: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 3>), grid, block, 0, s, a, x); break;
-    case 4: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 4>), grid, block, 0, s, a, x); break;
-    case 5: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 5>), grid, block, 0, s, a, x); break;
-    case 6: hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, 6>), grid, block, 0, s, a, x); break;
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_jacobi_x_wv(hipStream_t s, const DiffArgs &a, const XJac &x) {
+  hipLaunchKernelGGL((k_cn_jacobi_x<WV, M>), dim3((unsigned)((long long)a.g.Na * a.g.Nb)), dim3(64 * WV), 0, s, a,
+                     x);
   return hipGetLastError();
 }
 
@@ -1205,22 +1198,22 @@ hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k
   }
 }
 
-hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, int k, const Dims &orig, double *c_orig,
+hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c_orig,
                                    const double *kap, const double *sig, const double *kx) {
-  if (a.g.dir != 1 || k < 0 || k > kJacobiMaxIters) return hipErrorInvalidValue;
+  if (a.g.dir != 1 || !a.jline) return hipErrorInvalidValue;
   const int n = a.g.rmax - a.g.rmin + 1;
   const XJac x{orig, c_orig, kap, sig, kx};
   // 4 rows per lane, 256 per wave (98 VGPRs; 8 rows per lane: 196, and
   // 1.6% slower on the 1440-cell lines of the global grid)
   switch ((n + 255) / 256) {
-    case 1: return launch_jacobi_x_k<1, 4>(s, a, x, k);
-    case 2: return launch_jacobi_x_k<2, 4>(s, a, x, k);
-    case 3: return launch_jacobi_x_k<3, 4>(s, a, x, k);
-    case 4: return launch_jacobi_x_k<4, 4>(s, a, x, k);
-    case 5: return launch_jacobi_x_k<5, 4>(s, a, x, k);
-    case 6: return launch_jacobi_x_k<6, 4>(s, a, x, k);
-    case 7: return launch_jacobi_x_k<7, 4>(s, a, x, k);
-    case 8: return launch_jacobi_x_k<8, 4>(s, a, x, k);
+    case 1: return launch_jacobi_x_wv<1, 4>(s, a, x);
+    case 2: return launch_jacobi_x_wv<2, 4>(s, a, x);
+    case 3: return launch_jacobi_x_wv<3, 4>(s, a, x);
+    case 4: return launch_jacobi_x_wv<4, 4>(s, a, x);
+    case 5: return launch_jacobi_x_wv<5, 4>(s, a, x);
+    case 6: return launch_jacobi_x_wv<6, 4>(s, a, x);
+    case 7: return launch_jacobi_x_wv<7, 4>(s, a, x);
+    case 8: return launch_jacobi_x_wv<8, 4>(s, a, x);
     default: return hipErrorInvalidValue;
   }
 }

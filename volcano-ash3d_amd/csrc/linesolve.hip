@@ -106,14 +106,14 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
   const int b = (int)(tg.tile / tiles_a) + 1;
   const int a_first = (int)(tg.tile % tiles_a) * T + 1;
   // lines of the tile this kernel solves (bit l: line a_first + l): those
-  // of the grid, but not those of A.jline (the Jacobi sweeps'); a tile
-  // with none of them exits (uniform)
+  // of the grid, but not those the Jacobi sweeps take (1 <= A.jline <=
+  // A.jcap); a tile with none of them exits (uniform)
   static_assert(T <= 32, "one bit per line of the tile");
   unsigned own_mask = 0;
   for (int l_ = 0; l_ < T; ++l_) {
     const int al_ = a_first + l_;
-    if (al_ <= g.Na && !(A.jline && A.jline[(long long)(al_ - 1) + (long long)g.Na * (b - 1)]))
-      own_mask |= 1u << l_;
+    const int sw = (A.jline && al_ <= g.Na) ? A.jline[(long long)(al_ - 1) + (long long)g.Na * (b - 1)] : 0;
+    if (al_ <= g.Na && !(sw >= 1 && sw <= A.jcap)) own_mask |= 1u << l_;
   }
   if (own_mask == 0) return;
   auto owned = [&](int al_) { return ((own_mask >> (al_ - a_first)) & 1u) != 0; };
