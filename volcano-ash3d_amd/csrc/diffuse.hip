@@ -624,21 +624,23 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
 
 // x lines (contiguous in memory) by Jacobi sweeps, each line with the
 // sweeps it needs (A.jline, up to A.jcap; the partitioned solves take the
-// others).  One
-// workgroup of WV waves per line: lane l of wave w holds rows
-// i = 64 M w + 64 m + l (m < M rows per lane, 64 M per wave),
-// so that every load and store of a row block is one coalesced segment
-// and the neighbours of a row are in the neighbouring lanes (whole-wave
-// DPP rotations; lanes 0 and 63 take the neighbouring block's value, and
-// the first and last row of a wave go to the neighbouring waves through
-// LDS, one barrier per exchange).  The row factors L, R are formed from
-// kappa, sigma_nx and kx as k_line_lr forms them (same arithmetic, so the
-// same values as the factor cache) once per line, with the Jacobi
-// coefficients; then every aloft bin of the line in turn: its right-hand
-// side (the next bin's q already in flight), the sweeps in place, one store.
-// The arithmetic of a row is k_cn_jacobi's.  A is the x<->y transposed
-// description the other x-line solves use (lines a = j, b = k, rows along
-// x); X the original layout.
+// others).  One workgroup of WV waves per line; wave w owns rows
+// 64 M w .. 64 M (w+1) - 1, and lane l of it the M consecutive rows
+// 64 M w + M l + m (m < M), so that a sweep finds every neighbour in the
+// lane's own registers but those of its first and last row (the
+// neighbouring lanes' by whole-wave DPP rotations; lanes 0 and 63 the
+// neighbouring waves' through LDS, one barrier per sweep).  Loads and
+// stores stay coalesced: a bin's q is read row block by row block (lane l
+// of block m: row 64 M w + 64 m + l) and turned into the lane's rows
+// through LDS, the solution the same way back.  The row factors L, R are
+// formed from kappa, sigma_nx and kx as k_line_lr forms them (same
+// arithmetic, so the same values as the factor cache) once per line, with
+// the Jacobi coefficients; then every aloft bin of the line in turn: its
+// right-hand side as k_cn_jacobi forms it (the next bin's q already in
+// flight), the sweeps (fused multiply-adds, alternating between two sets
+// of registers), one store.  A is the x<->y transposed description the
+// other x-line solves use (lines a = j, b = k, rows along x); X the
+// original layout.
 struct XJac {
   Dims o;                        // original dims
   double *c;                     // concen ts0
@@ -647,7 +649,10 @@ struct XJac {
 
 template <int WV, int M>
 __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
-  __shared__ double s_ex[2][WV][2];  // per parity: each wave's first and last value
+  constexpr int P = M + 1;  // LDS pitch of a lane's rows (bank spread)
+  __shared__ double s_ex[2][WV][2];       // per parity: each wave's first and last value
+  __shared__ double s_q[WV][64 * P];      // q, block order in, lane order out
+  __shared__ double s_x[WV][64 * P];      // the solution, lane order in, block order out
   const LineGeom g = A.g;
   const long long line = blockIdx.x;
   const int sweeps = A.jline[line];
@@ -659,16 +664,20 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
   const bool line_on = a >= g.a0 && a <= g.a1 && b >= g.b0 && b <= g.b1;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const bool lane0 = l == 0, lane63 = l == 63;
-  const long long ob = o.p3(-1, a, b);       // cell r at ob + r + 1
+  const long long ob = o.p3(-1, a, b);          // cell r at ob + r + 1
   const long long kb = kline_base(o, 0, a, b);  // K of cell r at kb + r
   double *c = X.c;
   const double th = A.theta, th1 = 1.0 - th, dt = A.dt;
-  auto row = [&](int m) { return w * 64 * M + m * 64 + l; };
+  const int w0 = w * 64 * M;
+  auto own = [&](int m) { return w0 + M * l + m; };   // the lane's rows
+  auto blk = [&](int m) { return w0 + 64 * m + l; };  // row of block m
+  // LDS slot of row w0 + t: lane t / M, element t % M
+  auto slot = [&](int t) { return (t / M) * P + t % M; };
   // per row: L, R, 1/D and the Jacobi coefficients (zero on padding rows)
   double rL[M], rR[M], rD[M], ca[M], cb[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const int i = row(m);
+    const int i = own(m);
     const bool real = i < n;
     const int r = rmin + (real ? i : 0);
     auto kapat = [&](int q) { return X.kap[ob + q + 1]; };
@@ -694,34 +703,26 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
     ca[m] = (real && i > 0) ? (th * L) * rd : 0.0;
     cb[m] = (real && i < n - 1) ? (th * R) * rd : 0.0;
   }
-  // the values of the rows before and after this wave's rows (the
-  // neighbouring waves' last and first; zero beyond the line)
+  // exchange slots: each wave writes its first and last value, then reads
+  // the neighbouring waves' (zero beyond the line)
   int parity = 0;
-  auto exchange = [&](double first, double last, double &before, double &after) {
-    before = after = 0.0;
-    if constexpr (WV > 1) {
+  auto put_ends = [&](double first, double last) {
+    if (WV > 1) {
       if (lane0) s_ex[parity][w][0] = first;
       if (lane63) s_ex[parity][w][1] = last;
-      __syncthreads();
-      if (w > 0) before = s_ex[parity][w - 1][1];
-      if (w + 1 < WV) after = s_ex[parity][w + 1][0];
-      parity ^= 1;
     }
   };
-  // lo[m] = v at row i-1, hi[m] = v at row i+1, m ascending (all lanes)
-  auto neighbours = [&](const double (&v)[M], double before, double after, auto &&f) {
-    double ror_prev = 0.0;
-    double rol_cur = dpp_f64<kWaveRol1>(v[0]);
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const double ror_cur = dpp_f64<kWaveRor1>(v[m]);
-      const double rol_next = dpp_f64<kWaveRol1>(v[m + 1 < M ? m + 1 : m]);
-      const double lo = lane0 ? (m > 0 ? ror_prev : before) : ror_cur;
-      const double hi = lane63 ? (m + 1 < M ? rol_next : after) : rol_cur;
-      f(m, lo, hi);
-      ror_prev = ror_cur;
-      rol_cur = rol_next;
-    }
+  auto get_ends = [&](double &before, double &after) {
+    before = (WV > 1 && w > 0) ? s_ex[parity][w - 1][1] : 0.0;
+    after = (WV > 1 && w + 1 < WV) ? s_ex[parity][w + 1][0] : 0.0;
+    parity ^= 1;
+  };
+  // the values at the lane's first row - 1 and last row + 1 (all lanes)
+  auto ends = [&](const double (&v)[M], double before, double after, double &lo, double &hi) {
+    const double p = dpp_f64<kWaveRor1>(v[M - 1]);  // lane l-1's last row
+    const double nx = dpp_f64<kWaveRol1>(v[0]);     // lane l+1's first row
+    lo = lane0 ? before : p;
+    hi = lane63 ? after : nx;
   };
   const long long ncell = o.ncell;
   const int ns = o.ns;
@@ -730,51 +731,92 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
     while (u < ns && !on_bin(u)) ++u;
     return u;
   };
-  double q[M];
+  double qb[M];  // q of the bin in flight, block order
   auto load = [&](int u) {
     const long long bo = ncell * (long long)u;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const int i = row(m);
-      q[m] = c[ob + rmin + (i < n ? i : 0) + 1 + bo];
+      const int i = blk(m);
+      qb[m] = c[ob + rmin + (i < n ? i : 0) + 1 + bo];
     }
   };
+  double xa[M], xb[M];  // sweeps alternate between these
   int u = next_on(0);
+  int u_prev = ns;      // the bin whose solution waits in s_x
   if (u < ns) load(u);
-  while (u < ns) {
-    // right-hand side (the reference's q_0 = q_1 and q_{n+1} = q_n)
-    double y[M], x[M];
-    {
-      double before, after;
-      exchange(q[0], q[M - 1], before, after);
-      neighbours(q, before, after, [&](int m, double ql, double qr) {
-        const int i = row(m);
-        if (i == 0) ql = q[m];
-        if (i == n - 1) qr = q[m];
-        const double L = rL[m], R = rR[m], C = L + R;
-        y[m] = ((th1 * L * ql + (1.0 - th1 * C) * q[m]) + th1 * R * qr) * rD[m];
-      });
-    }
-    const int un = next_on(u + 1);
-    if (un < ns) load(un);
-#pragma unroll
-    for (int m = 0; m < M; ++m) x[m] = y[m];
-    for (int k = 1; k < sweeps; ++k) {
-      double before, after;
-      exchange(x[0], x[M - 1], before, after);
-      double xo[M];
-#pragma unroll
-      for (int m = 0; m < M; ++m) xo[m] = x[m];
-      neighbours(xo, before, after,
-                 [&](int m, double lo, double hi) { x[m] = (y[m] + ca[m] * lo) + cb[m] * hi; });
-    }
-    const long long bo = ncell * (long long)u;
+  auto store_prev = [&]() {
+    // the previous bin's solution: s_x in lane order, stored by blocks
+    const long long bo = ncell * (long long)u_prev;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const int i = row(m);
-      if (i < n) c[ob + rmin + i + 1 + bo] = x[m];
+      const int i = blk(m);
+      if (i < n) c[ob + rmin + i + 1 + bo] = s_x[w][slot(64 * m + l)];
     }
+  };
+  while (u < ns) {
+    // q into the lane's rows (and the wave ends of q), with the previous
+    // bin's solution out of s_x: one barrier
+#pragma unroll
+    for (int m = 0; m < M; ++m) s_q[w][slot(64 * m + l)] = qb[m];
+    put_ends(qb[0], qb[M - 1]);  // block 0 lane 0 / block M-1 lane 63: the wave's first / last row
+    __syncthreads();
+    if (u_prev < ns) store_prev();
+    double q[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) q[m] = s_q[w][l * P + m];
+    double before, after, qlo, qhi;
+    get_ends(before, after);
+    const int un = next_on(u + 1);
+    if (un < ns) load(un);
+    ends(q, before, after, qlo, qhi);
+    // right-hand side (the reference's q_0 = q_1 and q_{n+1} = q_n)
+    double y[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = own(m);
+      double ql = m > 0 ? q[m > 0 ? m - 1 : 0] : qlo;
+      double qr = m + 1 < M ? q[m + 1 < M ? m + 1 : m] : qhi;
+      if (i == 0) ql = q[m];
+      if (i == n - 1) qr = q[m];
+      const double L = rL[m], R = rR[m], C = L + R;
+      y[m] = ((th1 * L * ql + (1.0 - th1 * C) * q[m]) + th1 * R * qr) * rD[m];
+    }
+    // sweep: dst = y + ca * src[row - 1] + cb * src[row + 1]
+    auto sweep = [&](const double (&src)[M], double (&dst)[M]) {
+      put_ends(src[0], src[M - 1]);
+      if (WV > 1) __syncthreads();
+      double bf, af, lo, hi;
+      get_ends(bf, af);
+      ends(src, bf, af, lo, hi);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const double xl = m > 0 ? src[m > 0 ? m - 1 : 0] : lo;
+        const double xr = m + 1 < M ? src[m + 1 < M ? m + 1 : m] : hi;
+        dst[m] = fma(cb[m], xr, fma(ca[m], xl, y[m]));
+      }
+    };
+    // sweep 1: x = y; then pairs of sweeps y -> xa -> xb -> xa ...
+    int done = 1;
+    bool in_a = false;  // the current iterate: y (false, before any sweep) or xa / xb
+#pragma unroll
+    for (int m = 0; m < M; ++m) xb[m] = y[m];
+    for (; done + 2 <= sweeps; done += 2) {
+      sweep(xb, xa);
+      sweep(xa, xb);
+    }
+    if (done < sweeps) {
+      sweep(xb, xa);
+      in_a = true;
+    }
+    // the solution into s_x in lane order; stored after the next barrier
+#pragma unroll
+    for (int m = 0; m < M; ++m) s_x[w][l * P + m] = in_a ? xa[m] : xb[m];
+    u_prev = u;
     u = un;
+  }
+  if (u_prev < ns) {
+    __syncthreads();
+    store_prev();
   }
   // cells the reference leaves at zero: outside [rmin, rmax], and every
   // cell of bins that are not aloft and of lines outside the bounds
