@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
 // bin-cell, the row factors L, R once per bin group.
 template <int DIR, int NB, int K>
 __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
-  constexpr int PF = 2;                 // rows loaded ahead
+  constexpr int PF = 3;  // rows loaded ahead (2: 7.12 ms, 3: 6.95, 4: 7.03 on the global y lines)
   constexpr int NY = K + 1 > 3 ? K + 1 : 3;  // y of rows s .. s-NY+1 (sweep 0 and the rows sweeps use)
   constexpr int NW = K > 1 ? K - 1 : 1;  // windows of sweeps 1 .. K-1
   const Dims d = A.d;
