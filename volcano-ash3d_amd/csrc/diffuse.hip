@@ -625,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
 // x lines (contiguous in memory) by Jacobi sweeps, each line with the
 // sweeps it needs (A.jline, up to A.jcap; the partitioned solves take the
 // others).  One workgroup of WV waves per line; wave w owns rows
-// 64 M w .. 64 M (w+1) - 1, and lane l of it the M consecutive rows
+// 64 M w .. 64 M (w+1) - 1 (M = 6), and lane l of it the M consecutive rows
 // 64 M w + M l + m (m < M), so that a sweep finds every neighbour in the
 // lane's own registers but those of its first and last row (the
 // neighbouring lanes' by whole-wave DPP rotations; lanes 0 and 63 the
@@ -1245,17 +1245,18 @@ hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims 
   if (a.g.dir != 1 || !a.jline) return hipErrorInvalidValue;
   const int n = a.g.rmax - a.g.rmin + 1;
   const XJac x{orig, c_orig, kap, sig, kx};
-  // 4 rows per lane, 256 per wave (98 VGPRs; 8 rows per lane: 196, and
-  // 1.6% slower on the 1440-cell lines of the global grid)
-  switch ((n + 255) / 256) {
-    case 1: return launch_jacobi_x_wv<1, 4>(s, a, x);
-    case 2: return launch_jacobi_x_wv<2, 4>(s, a, x);
-    case 3: return launch_jacobi_x_wv<3, 4>(s, a, x);
-    case 4: return launch_jacobi_x_wv<4, 4>(s, a, x);
-    case 5: return launch_jacobi_x_wv<5, 4>(s, a, x);
-    case 6: return launch_jacobi_x_wv<6, 4>(s, a, x);
-    case 7: return launch_jacobi_x_wv<7, 4>(s, a, x);
-    case 8: return launch_jacobi_x_wv<8, 4>(s, a, x);
+  // rows per lane (global grid, 1440-cell lines: M = 3 6.67 ms, 4 7.02,
+  // 6 6.63, 8 7.16)
+  constexpr int M = 6;
+  switch ((n + 64 * M - 1) / (64 * M)) {
+    case 1: return launch_jacobi_x_wv<1, M>(s, a, x);
+    case 2: return launch_jacobi_x_wv<2, M>(s, a, x);
+    case 3: return launch_jacobi_x_wv<3, M>(s, a, x);
+    case 4: return launch_jacobi_x_wv<4, M>(s, a, x);
+    case 5: return launch_jacobi_x_wv<5, M>(s, a, x);
+    case 6: return launch_jacobi_x_wv<6, M>(s, a, x);
+    case 7: return launch_jacobi_x_wv<7, M>(s, a, x);
+    case 8: return launch_jacobi_x_wv<8, M>(s, a, x);
     default: return hipErrorInvalidValue;
   }
 }
