@@ -29,6 +29,11 @@ dgtsv order.
   recomputed here; the sweeps those lines take pinned), periodic and not,
   lines of 600 cells over three waves with subgrid bounds and a mask;
   lines of one wave stay in the partitioned solves by default.
+* The adjoint (lambda <- K B p(A) K^-1 lambda, the transpose of the
+  sweeps' map p(A) B): <F q, w> = <q, F^T w> against the oracle's forward
+  map for y and z lines with 2-7 sweeps, theta 0 and 1, subgrids and a
+  mask, x lines split with the partitioned adjoint (one wave) and with
+  the transposed dgtsv-order adjoint (600 cells).
 """
 import math
 
@@ -373,3 +378,83 @@ def test_short_x_lines_keep_the_partitioned_solves(pkg, oracle, tr):
                after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
     assert seen == [0, 0, 0]
     assert_same(a, b, tol=TOL)
+
+
+# ---- the adjoint: the transpose of the Jacobi map (same sweeps on A^T =
+# K A K^-1: lambda <- K B p(A) K^-1 lambda), pinned by <F q, w> = <q, F^T w>
+# against the oracle's forward map (whose exact solves the sweeps match to
+# 2^-52 relative)
+
+def adjoint_jacobi(tr, st, lam, dirn, **tune):
+    """F^T lam through the library's adjoint after one warm-up call (which
+    factors the lines; the factors' ratios then reach the host), and the
+    sweeps the second call reports."""
+    T = tr.Transport(st.copy())
+    T.tune(line_solver="partitioned", **tune)
+    T.adj_init(0)
+    T.adj_set(lam)
+    T.adj_diffuse(dirn, True)
+    T.synchronize()
+    T.adj_set(lam)
+    T.adj_diffuse(dirn, True)
+    g = T.adj_get()
+    sweeps = T.tune_get("jacobi_last_" + "xyz"[dirn])
+    T.close()
+    return g, sweeps
+
+
+@pytest.mark.parametrize("dirn,scale", [(1, 1e-3), (1, 1.0), (1, 300.0), (2, 1e-4), (2, 0.3), (2, 0.6)])
+def test_adjoint_sweeps(pkg, oracle, tr, dirn, scale):
+    from test_gpu_adjoint import check_dot, forward, rand_lambda
+    st = small_state(pkg, scale)
+    want = expected_sweeps(line_rho(st, dirn))
+    Fq = forward(oracle, st, [("diffCN_" + "xyz"[dirn],)])
+    lam = rand_lambda(st, 70 + dirn)
+    g, sweeps = adjoint_jacobi(tr, st, lam, dirn, part_max_line_strided=3)
+    assert sweeps == want > 0, (sweeps, want)
+    check_dot(st, Fq, lam, g)
+
+
+def test_adjoint_subgrid_mask_and_theta(pkg, oracle, tr):
+    from test_gpu_adjoint import check_dot, forward, rand_lambda
+    for theta in (0.0, 1.0):
+        st = small_state(pkg, 0.1, ns=4, theta=theta, seed=13)
+        st.is_aloft = np.array([1, 0, 1, 1], dtype=np.int32)
+        st.bounds = (2, 10, 5, 33, 3, 26)
+        for dirn in (1, 2):
+            Fq = forward(oracle, st, [("diffCN_" + "xyz"[dirn],)])
+            lam = rand_lambda(st, 80 + dirn)
+            g, sweeps = adjoint_jacobi(tr, st, lam, dirn, part_max_line_strided=3)
+            assert sweeps > 0
+            check_dot(st, Fq, lam, g)
+
+
+@pytest.mark.parametrize("periodic,scale", [(False, 30.0), (False, 300.0), (True, 3e5)])
+def test_adjoint_x_lines_split(pkg, oracle, tr, periodic, scale):
+    """x lines of one wave (jacobi_x=2): the Jacobi lines' adjoint, the
+    others in the partitioned adjoint (which skips the Jacobi lines)."""
+    from test_gpu_adjoint import check_dot, forward, rand_lambda
+    st = x_state(pkg, 24, 40, 6, 3, scale, seed=29, periodic=periodic)
+    nj, nl, want = x_split(st)
+    Fq = forward(oracle, st, [("diffCN_x",)])
+    lam = rand_lambda(st, 90)
+    g, sweeps = adjoint_jacobi(tr, st, lam, 0, jacobi_x=2)
+    assert sweeps == want > 0, (sweeps, want)
+    check_dot(st, Fq, lam, g)
+
+
+def test_adjoint_long_x_lines(pkg, oracle, tr):
+    """600-cell x lines (default jacobi_x=1): 68 of 72 lines in the Jacobi
+    adjoint, the others in the transposed dgtsv-order solves on the
+    transposed lambda (which skip the Jacobi lines); subgrid and a mask."""
+    from test_gpu_adjoint import check_dot, forward, rand_lambda
+    st = x_state(pkg, 600, 24, 3, 5, 200.0, seed=31)
+    st.is_aloft = np.array([1, 1, 0, 1, 1], dtype=np.int32)
+    st.bounds = (3, 590, 2, 23, 1, 3)
+    nj, nl, want = x_split(st, 3, 590)
+    assert 0 < nj < nl
+    Fq = forward(oracle, st, [("diffCN_x",)])
+    lam = rand_lambda(st, 91)
+    g, sweeps = adjoint_jacobi(tr, st, lam, 0)
+    assert sweeps == want, (sweeps, want)
+    check_dot(st, Fq, lam, g)

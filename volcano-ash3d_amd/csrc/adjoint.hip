@@ -749,6 +749,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_cn(DiffArgs A, double *lam) {
   const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
   const long long gid = tg.tile * kBlock + threadIdx.x;
   if (gid >= (long long)g.Na * g.Nb) return;
+  if (A.jline) {  // a line of the Jacobi sweeps' adjoint, which ran before this kernel
+    const int sw = A.jline[gid];
+    if (sw >= 1 && sw <= A.jcap) return;
+  }
   const int a = (int)(gid % g.Na) + 1;
   const int b = (int)(gid / g.Na) + 1;
   const int n0 = tg.group * NB;

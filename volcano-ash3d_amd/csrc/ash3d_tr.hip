@@ -638,8 +638,12 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   *rc = 0;
   const int n = a.g.rmax - a.g.rmin + 1;
   a.sel = 0;
-  // (x lines are contiguous in memory; y and z lines strided)
-  const int nmax = adjoint ? 512 : (dir == 0 ? x->part_max_line : x->part_max_line_strided);
+  // (x lines are contiguous in memory; y and z lines strided; the adjoint's
+  // partitioned solves take lines of up to 512 cells, and only those the
+  // forward's took, so that the sweeps the forward ran as Jacobi sweeps
+  // get their transpose)
+  const int fmax = dir == 0 ? x->part_max_line : x->part_max_line_strided;
+  const int nmax = adjoint ? std::min(512, fmax) : fmax;
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > nmax) return 0;
   const int hn = host_ndom(x, dir);
   if (hn > 0) return 0;
@@ -828,10 +832,33 @@ int adj_diffuse_sweep(ash3d_tr_ctx *x, int dir, int use_cn, double dt, double im
     if ((rc = ensure_lr(x, dir, a)) || (rc = ensure_lu(x, dir, a))) return rc;
     a.c0 = x->lam;
     ProfScope ps(x, ASH3D_PROF_ADJ_DIFFUSE);
+    const int n = a.g.rmax - a.g.rmin + 1;
+    if (dir == 0) {
+      // the transpose of the forward's Jacobi sweeps for its x lines
+      // (lambda in place, original layout); the transposed dgtsv-order
+      // solves below skip those lines
+      const int sw = jacobi_x_sweeps(x, n);
+      x->jacobi_last[0] = sw;
+      if (sw > 0) {
+        a.jline = x->lu_jl[0];
+        a.jcap = x->jacobi_sweeps;
+        HIPCHK(launch_diff_cn_jacobi_x(x->stream, a, x->d, x->lam, x->kap, x->sx, x->kx, true));
+      }
+    }
     const int tp = try_cn_partitioned(x, dir, a, &rc, true);
     if (rc) return rc;
     if (tp == 2) return 0;  // known dominant: nothing else to run
     if (tp == 1) skip = a.ndom;
+    if (dir != 0 && tp == 0) {
+      // the transpose of the forward's Jacobi sweeps for its y or z lines
+      // (where the forward took them: lines outside the partitioned range)
+      const int k = (n < 3 || n > x->part_max_line_strided) ? jacobi_iters(x, dir) : -1;
+      x->jacobi_last[dir] = k + 1;
+      if (k >= 0) {
+        HIPCHK(launch_diff_cn_jacobi(x->stream, a, x->nb_diff[dir], k, true));
+        return 0;
+      }
+    }
   }
   if (transposed) {
     const Dims &d = x->d;

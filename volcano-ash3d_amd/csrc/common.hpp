@@ -321,14 +321,17 @@ __host__ __device__ inline int jacobi_sweeps_needed(double rho) {
   }
   return 0;
 }
-hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k);
+// adjoint: lambda <- K B p(A) K^-1 lambda in place of the solve's K B A^-1 K^-1
+// (p(A): the same sweeps; the transpose of the forward map, since
+// p(A)^T = p(A^T) = K p(A) K^-1 for K A symmetric)
+hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k, bool adjoint = false);
 // the same for x lines, each with the sweeps it needs: the lines with
 // 1 <= a.jline[line] <= a.jcap (the others are the partitioned solves');
 // a describes the x<->y transposed grid as for the other x-line solves;
 // the state c_orig, kappa, sigma_nx and kx in the original layout (orig);
 // lines of up to 2048 cells
 hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c_orig,
-                                   const double *kap, const double *sig, const double *kx);
+                                   const double *kap, const double *sig, const double *kx, bool adjoint = false);
 // x-line solves on the original state layout (geometry, K and scratch in
 // the x<->y transposed layout described by a)
 hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c0, int nb, int tx);

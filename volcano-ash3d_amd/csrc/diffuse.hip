@@ -463,11 +463,14 @@ __global__ __launch_bounds__(kBlock) void k_diff_cn(DiffArgs A) {
 // overwrites has been read).  One thread per line and NB bins, lanes along
 // the contiguous transverse index: q is read and written once per
 // bin-cell, the row factors L, R once per bin group.
-template <int DIR, int NB, int K>
+template <int DIR, int NB, int K, bool ADJ>
 __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
   constexpr int PF = 3;  // rows loaded ahead (2: 7.12 ms, 3: 6.95, 4: 7.03 on the global y lines)
   constexpr int NY = K + 1 > 3 ? K + 1 : 3;  // y of rows s .. s-NY+1 (sweep 0 and the rows sweeps use)
-  constexpr int NW = K > 1 ? K - 1 : 1;  // windows of sweeps 1 .. K-1
+  // windows of sweeps 1 .. K-1 (the adjoint: 1 .. K, its last sweep feeds B)
+  constexpr int NW = ADJ ? (K > 0 ? K : 1) : (K > 1 ? K - 1 : 1);
+  constexpr int KD = ADJ ? K + 2 : 1;  // the adjoint's rows s .. s-K-1 of L, R and kappa
+  constexpr int LAG = ADJ ? K + 1 : K;  // rows between a row's entry and its store
   const Dims d = A.d;
   const LineGeom g = A.g;
   const TileGroup tg = xcd_tile_group(blockIdx.x, (d.ns + NB - 1) / NB);
@@ -498,12 +501,14 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
     const double *Lf = A.lr, *Rf = A.lr + d.ncell;
     auto lofs = [&](int l) { return base + sL * (long long)(rmin + l); };  // row l: cell rmin-1+l
     auto ofs = [&](int l) { return qb + qs * (long long)(rmin + l); };
-    // rows ahead: slot p holds L, R of row s and q of row s+1
-    double rL[PF], rR[PF], rq[PF][NB];
+    // rows ahead: slot p holds L, R (the adjoint: and kappa) of row s and q
+    // of row s+1
+    double rL[PF], rR[PF], rK[PF], rq[PF][NB];
     auto fetch = [&](int slot, int l) {
       const long long o = lofs(min(l, n));
       rL[slot] = Lf[o];
       rR[slot] = Rf[o];
+      if (ADJ) rK[slot] = A.kap[o];
       const long long oq = ofs(min(l + 1, n));
 #pragma unroll
       for (int u = 0; u < NB; ++u) rq[slot][u] = c[oq + bofs[u]];
@@ -515,6 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
     for (int p = 0; p < PF; ++p) fetch(p, 1 + p);
     double y[NY][NB], ca[K + 1], cb[K + 1];  // index j: row s-j
     double xw[NW][3][NB];                    // sweep m: rows s-m, s-m-1, s-m-2
+    double dL[KD], dR[KD], dK[KD];           // the adjoint's row factors, index j: row s-j
 #pragma unroll
     for (int j = 0; j < NY; ++j)
 #pragma unroll
@@ -522,17 +528,19 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
 #pragma unroll
     for (int j = 0; j <= K; ++j) ca[j] = cb[j] = 0.0;
 #pragma unroll
+    for (int j = 0; j < KD; ++j) dL[j] = dR[j] = dK[j] = 0.0;
+#pragma unroll
     for (int m = 0; m < NW; ++m)
 #pragma unroll
       for (int e = 0; e < 3; ++e)
 #pragma unroll
         for (int u = 0; u < NB; ++u) xw[m][e][u] = 0.0;
-    for (int s0 = 1; s0 <= n + K; s0 += PF) {
+    for (int s0 = 1; s0 <= n + LAG; s0 += PF) {
 #pragma unroll
       for (int p = 0; p < PF; ++p) {
         const int s = s0 + p;
-        if (s > n + K) break;
-        const double L = rL[p], R = rR[p];
+        if (s > n + LAG) break;
+        const double L = rL[p], R = rR[p], kap = ADJ ? rK[p] : 0.0;
         double qn[NB];
 #pragma unroll
         for (int u = 0; u < NB; ++u) qn[u] = rq[p][u];
@@ -546,6 +554,14 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
           ca[j] = ca[j - 1];
           cb[j] = cb[j - 1];
         }
+        if (ADJ) {
+#pragma unroll
+          for (int j = KD - 1; j > 0; --j) {
+            dL[j] = dL[j - 1];
+            dR[j] = dR[j - 1];
+            dK[j] = dK[j - 1];
+          }
+        }
         if (s <= n) {
           const double C = L + R;
           double D = 1.0 + th * C;
@@ -554,14 +570,24 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
           const double rD = 1.0 / D;
           ca[0] = (s > 1) ? (th * L) * rD : 0.0;
           cb[0] = (s < n) ? (th * R) * rD : 0.0;
+          if (ADJ) {
+            // the adjoint solves A u = lambda / kappa (then lambda' = kappa B u)
+            dL[0] = L;
+            dR[0] = R;
+            dK[0] = kap;
 #pragma unroll
-          for (int u = 0; u < NB; ++u) {
-            const double ql = (s == 1) ? qc[u] : qp[u];
-            const double qr = (s == n) ? qc[u] : qn[u];
-            y[0][u] = ((th1 * L * ql + (1.0 - th1 * C) * qc[u]) + th1 * R * qr) * rD;
+            for (int u = 0; u < NB; ++u) y[0][u] = (qc[u] / kap) * rD;
+          } else {
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+              const double ql = (s == 1) ? qc[u] : qp[u];
+              const double qr = (s == n) ? qc[u] : qn[u];
+              y[0][u] = ((th1 * L * ql + (1.0 - th1 * C) * qc[u]) + th1 * R * qr) * rD;
+            }
           }
         } else {
           ca[0] = cb[0] = 0.0;
+          if (ADJ) dL[0] = dR[0] = dK[0] = 0.0;
 #pragma unroll
           for (int u = 0; u < NB; ++u) y[0][u] = 0.0;
         }
@@ -570,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
           qp[u] = qc[u];
           qc[u] = qn[u];
         }
-        if constexpr (K == 0) {
+        if constexpr (K == 0 && !ADJ) {
           if (s <= n) {
 #pragma unroll
             for (int u = 0; u < NB; ++u)
@@ -590,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
               const double hi = (m == 1) ? y[0][u] : xw[m > 1 ? m - 2 : 0][0][u];
               xm[u] = tin ? (y[m][u] + ca[m] * lo) + cb[m] * hi : 0.0;
             }
-            if (m < K) {
+            if (ADJ || m < K) {
               const int w = m - 1 < NW ? m - 1 : 0;
 #pragma unroll
               for (int u = 0; u < NB; ++u) {
@@ -604,11 +630,30 @@ __global__ __launch_bounds__(kBlock) void k_cn_jacobi(DiffArgs A) {
                 if (on[u]) c[ofs(t) + bofs[u]] = xm[u];
             }
           }
+          if constexpr (ADJ) {
+            // lambda'_t = kappa_t (B u)_t at t = s-K-1, B's line ends
+            // u_0 = u_1 and u_{n+1} = u_n (the partitioned adjoint's B)
+            const int t = s - K - 1;
+            if (t >= 1 && t <= n) {
+              const double Lt = dL[KD - 1], Rt = dR[KD - 1], kt = dK[KD - 1], Ct = Lt + Rt;
+#pragma unroll
+              for (int u = 0; u < NB; ++u) {
+                const double up = K == 0 ? y[0][u] : xw[NW - 1][0][u];
+                const double uc = K == 0 ? y[1][u] : xw[NW - 1][1][u];
+                const double um = K == 0 ? y[2][u] : xw[NW - 1][2][u];
+                const double xm_ = (t == 1) ? uc : um;
+                const double xp_ = (t == n) ? uc : up;
+                const double v = (th1 * Lt) * xm_ + (1.0 - th1 * Ct) * uc + (th1 * Rt) * xp_;
+                if (on[u]) c[ofs(t) + bofs[u]] = kt * v;
+              }
+            }
+          }
         }
       }
     }
   }
-  // cells the reference leaves at zero (zero fill of ts1, then copy)
+  // cells the reference leaves at zero (zero fill of ts1, then copy; the
+  // adjoint of those writes zeroes lambda there)
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (!exists[u]) continue;
@@ -647,7 +692,7 @@ struct XJac {
   const double *kap, *sig, *kk;  // kappa, sigma_nx (with ghosts) and kx
 };
 
-template <int WV, int M>
+template <int WV, int M, bool ADJ>
 __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
   constexpr int P = M + 1;  // LDS pitch of a lane's rows (bank spread)
   __shared__ double s_ex[2][WV][2];       // per parity: each wave's first and last value
@@ -673,8 +718,9 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
   auto blk = [&](int m) { return w0 + 64 * m + l; };  // row of block m
   // LDS slot of row w0 + t: lane t / M, element t % M
   auto slot = [&](int t) { return (t / M) * P + t % M; };
-  // per row: L, R, 1/D and the Jacobi coefficients (zero on padding rows)
-  double rL[M], rR[M], rD[M], ca[M], cb[M];
+  // per row: L, R, 1/D and the Jacobi coefficients (zero on padding rows);
+  // the adjoint also kappa
+  double rL[M], rR[M], rD[M], ca[M], cb[M], kc[ADJ ? M : 1];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = own(m);
@@ -702,6 +748,7 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
     rD[m] = real ? rd : 0.0;
     ca[m] = (real && i > 0) ? (th * L) * rd : 0.0;
     cb[m] = (real && i < n - 1) ? (th * R) * rd : 0.0;
+    if (ADJ) kc[ADJ ? m : 0] = kap_c;
   }
   // exchange slots: each wave writes its first and last value, then reads
   // the neighbouring waves' (zero beyond the line)
@@ -769,7 +816,8 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
     const int un = next_on(u + 1);
     if (un < ns) load(un);
     ends(q, before, after, qlo, qhi);
-    // right-hand side (the reference's q_0 = q_1 and q_{n+1} = q_n)
+    // right-hand side (the reference's q_0 = q_1 and q_{n+1} = q_n); the
+    // adjoint solves A u = lambda / kappa
     double y[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -779,7 +827,10 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
       if (i == 0) ql = q[m];
       if (i == n - 1) qr = q[m];
       const double L = rL[m], R = rR[m], C = L + R;
-      y[m] = ((th1 * L * ql + (1.0 - th1 * C) * q[m]) + th1 * R * qr) * rD[m];
+      if (ADJ)
+        y[m] = (q[m] / kc[ADJ ? m : 0]) * rD[m];
+      else
+        y[m] = ((th1 * L * ql + (1.0 - th1 * C) * q[m]) + th1 * R * qr) * rD[m];
     }
     // sweep: dst = y + ca * src[row - 1] + cb * src[row + 1]
     auto sweep = [&](const double (&src)[M], double (&dst)[M]) {
@@ -808,9 +859,32 @@ __global__ __launch_bounds__(64 * WV) void k_cn_jacobi_x(DiffArgs A, XJac X) {
       sweep(xb, xa);
       in_a = true;
     }
-    // the solution into s_x in lane order; stored after the next barrier
+    if constexpr (ADJ) {
+      // lambda' = kappa B u, B's line ends u_0 = u_1 and u_{n+1} = u_n
+      double uu[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) s_x[w][l * P + m] = in_a ? xa[m] : xb[m];
+      for (int m = 0; m < M; ++m) uu[m] = in_a ? xa[m] : xb[m];
+      put_ends(uu[0], uu[M - 1]);
+      if (WV > 1) __syncthreads();
+      double bf, af, lo, hi;
+      get_ends(bf, af);
+      ends(uu, bf, af, lo, hi);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int i = own(m);
+        double um = m > 0 ? uu[m > 0 ? m - 1 : 0] : lo;
+        double up = m + 1 < M ? uu[m + 1 < M ? m + 1 : m] : hi;
+        if (i == 0) um = uu[m];
+        if (i == n - 1) up = uu[m];
+        const double L = rL[m], R = rR[m];
+        const double v = (th1 * L) * um + (1.0 - th1 * (L + R)) * uu[m] + (th1 * R) * up;
+        s_x[w][l * P + m] = kc[ADJ ? m : 0] * v;
+      }
+    } else {
+      // the solution into s_x in lane order; stored after the next barrier
+#pragma unroll
+      for (int m = 0; m < M; ++m) s_x[w][l * P + m] = in_a ? xa[m] : xb[m];
+    }
     u_prev = u;
     u = un;
   }
@@ -1199,49 +1273,56 @@ hipError_t launch_diff_cn_xrows(hipStream_t s, const DiffArgs &a, const Dims &or
 
 namespace {
 
-template <int DIR, int NB>
+template <int DIR, int NB, bool ADJ>
 hipError_t launch_jacobi_k(hipStream_t s, const DiffArgs &a, int k, dim3 grid) {
   switch (k) {
-    case 0: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 0>), grid, dim3(kBlock), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 1>), grid, dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 2>), grid, dim3(kBlock), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 3>), grid, dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 4>), grid, dim3(kBlock), 0, s, a); break;
-    case 5: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 5>), grid, dim3(kBlock), 0, s, a); break;
-    case 6: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 6>), grid, dim3(kBlock), 0, s, a); break;
+    case 0: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 0, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 1, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 2, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 3, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 4, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 5, ADJ>), grid, dim3(kBlock), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_cn_jacobi<DIR, NB, 6, ADJ>), grid, dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int NB>
+template <int NB, bool ADJ>
 hipError_t launch_jacobi_nb(hipStream_t s, const DiffArgs &a, int k) {
   const long long lines = (long long)a.g.Na * a.g.Nb;
   dim3 grid((unsigned)(padded_tiles((lines + kBlock - 1) / kBlock) * ((a.d.ns + NB - 1) / NB)));
-  return a.g.dir == 1 ? launch_jacobi_k<1, NB>(s, a, k, grid) : launch_jacobi_k<2, NB>(s, a, k, grid);
+  return a.g.dir == 1 ? launch_jacobi_k<1, NB, ADJ>(s, a, k, grid) : launch_jacobi_k<2, NB, ADJ>(s, a, k, grid);
 }
 
 template <int WV, int M>
-hipError_t launch_jacobi_x_wv(hipStream_t s, const DiffArgs &a, const XJac &x) {
-  hipLaunchKernelGGL((k_cn_jacobi_x<WV, M>), dim3((unsigned)((long long)a.g.Na * a.g.Nb)), dim3(64 * WV), 0, s, a,
-                     x);
+hipError_t launch_jacobi_x_wv(hipStream_t s, const DiffArgs &a, const XJac &x, bool adjoint) {
+  const dim3 grid((unsigned)((long long)a.g.Na * a.g.Nb)), block(64 * WV);
+  if (adjoint)
+    hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, true>), grid, block, 0, s, a, x);
+  else
+    hipLaunchKernelGGL((k_cn_jacobi_x<WV, M, false>), grid, block, 0, s, a, x);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k) {
+hipError_t launch_diff_cn_jacobi(hipStream_t s, const DiffArgs &a, int nb, int k, bool adjoint) {
   if (a.g.dir != 1 && a.g.dir != 2) return hipErrorInvalidValue;
   if (k < 0 || k > kJacobiMaxIters) return hipErrorInvalidValue;
+  if (adjoint) {
+    // (the adjoint keeps more rows in flight: at most 2 bins per thread)
+    return nb == 1 ? launch_jacobi_nb<1, true>(s, a, k) : launch_jacobi_nb<2, true>(s, a, k);
+  }
   switch (nb) {
-    case 1: return launch_jacobi_nb<1>(s, a, k);
-    case 2: return launch_jacobi_nb<2>(s, a, k);
-    default: return launch_jacobi_nb<4>(s, a, k);
+    case 1: return launch_jacobi_nb<1, false>(s, a, k);
+    case 2: return launch_jacobi_nb<2, false>(s, a, k);
+    default: return launch_jacobi_nb<4, false>(s, a, k);
   }
 }
 
 hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims &orig, double *c_orig,
-                                   const double *kap, const double *sig, const double *kx) {
+                                   const double *kap, const double *sig, const double *kx, bool adjoint) {
   if (a.g.dir != 1 || !a.jline) return hipErrorInvalidValue;
   const int n = a.g.rmax - a.g.rmin + 1;
   const XJac x{orig, c_orig, kap, sig, kx};
@@ -1249,14 +1330,14 @@ hipError_t launch_diff_cn_jacobi_x(hipStream_t s, const DiffArgs &a, const Dims 
   // 6 6.63, 8 7.16)
   constexpr int M = 6;
   switch ((n + 64 * M - 1) / (64 * M)) {
-    case 1: return launch_jacobi_x_wv<1, M>(s, a, x);
-    case 2: return launch_jacobi_x_wv<2, M>(s, a, x);
-    case 3: return launch_jacobi_x_wv<3, M>(s, a, x);
-    case 4: return launch_jacobi_x_wv<4, M>(s, a, x);
-    case 5: return launch_jacobi_x_wv<5, M>(s, a, x);
-    case 6: return launch_jacobi_x_wv<6, M>(s, a, x);
-    case 7: return launch_jacobi_x_wv<7, M>(s, a, x);
-    case 8: return launch_jacobi_x_wv<8, M>(s, a, x);
+    case 1: return launch_jacobi_x_wv<1, M>(s, a, x, adjoint);
+    case 2: return launch_jacobi_x_wv<2, M>(s, a, x, adjoint);
+    case 3: return launch_jacobi_x_wv<3, M>(s, a, x, adjoint);
+    case 4: return launch_jacobi_x_wv<4, M>(s, a, x, adjoint);
+    case 5: return launch_jacobi_x_wv<5, M>(s, a, x, adjoint);
+    case 6: return launch_jacobi_x_wv<6, M>(s, a, x, adjoint);
+    case 7: return launch_jacobi_x_wv<7, M>(s, a, x, adjoint);
+    case 8: return launch_jacobi_x_wv<8, M>(s, a, x, adjoint);
     default: return hipErrorInvalidValue;
   }
 }
