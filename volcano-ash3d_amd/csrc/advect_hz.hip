@@ -88,14 +88,27 @@ __device__ inline void dcu_segment(int c0, int N, int lim, const double (&q)[R +
 // One column of the streamed vertical sweep (advect.hip, DIR = 2) for one
 // bin: cell r enters with its value q0 (zero outside the gathered range
 // [-1, nz+2] and for the ghost levels, which read as Set_BC(1)'s zeros),
-// sigma, dt/kappa, vz + vh and vf; the update of cell r-2 is formed.
+// sigma, vz + vh and vf; the update of cell r-2 is formed.  dt/kappa of
+// the column's cells r-1 and r-2 (bin-independent) is kept by the caller,
+// once for all bins of the column (ZDtv).  dq1 is q1 - q2, recomputed
+// (the same operation on the same values as the dq0 of the step before).
+struct ZDtv {
+  double d1 = 0.0, d2 = 0.0;  // dt / kappa of cells r-1, r-2
+  // dt / kappa of cell r as the sweep gathers it (zero outside [0, nz+1])
+  __device__ static inline double gather(int r, int nz, double dtvc) { return (r >= 0 && r <= nz + 1) ? dtvc : 0.0; }
+  __device__ inline void shift(double d0) {
+    d2 = d1;
+    d1 = d0;
+  }
+};
 struct ZWin {
-  double q1 = 0.0, q2 = 0.0, dq1 = 0.0, dq2 = 0.0, fsr2 = 0.0, fss2 = 0.0;
-  double vb1 = 0.0, ub1 = 0.0, ub2 = 0.0, dtv1 = 0.0, dtv2 = 0.0;
-  double upd_bot = 0.0, upd_top = 0.0;
-  // returns the new value of cell r-2 (meaningful for r-2 in [1, nz])
-  __device__ inline double step(int r, int nz, int lim, double q0, double sig0, double dtvc, double vzh0,
-                                double vfc) {
+  double q1 = 0.0, q2 = 0.0, dq2 = 0.0, fsr2 = 0.0, fss2 = 0.0;
+  double vb1 = 0.0, ub1 = 0.0, ub2 = 0.0;
+  // returns the new value of cell r-2 (meaningful for r-2 in [1, nz]); the
+  // update itself in *edge when r-2 is 0 or nz+1 (the bottom and top
+  // outflow increments)
+  __device__ inline double step(int r, int nz, int lim, double q0, double sig0, const ZDtv &dt, double vzh0,
+                                double vfc, double *edge) {
     const bool ld = r >= -1 && r <= nz + 2;
     if (!ld) {
       q0 = 0.0;
@@ -104,10 +117,11 @@ struct ZWin {
       vfc = 0.0;
     }
     const bool uin = r >= 0 && r <= nz + 1;
-    const double dtv0 = uin ? dtvc : 0.0;  // dt / kappa (cached)
+    const double dtv1 = dt.d1, dtv2 = dt.d2;
     const double vb0 = ld ? vzh0 + vfc : 0.0;
     const double ub0 = uin ? 0.5 * (vb1 + vb0) * sig0 : 0.0;
     const double dq0 = q0 - q1;
+    const double dq1 = q1 - q2;
     const int I = r - 1, cc = r - 2;
     double fssI = 0.0, fsrI = 0.0, fslI = 0.0;
     if (lim != 0 && I >= 1 && I <= nz + 1) fssI = limited_flux(lim, dq2, dq1, dq0, ub1, dtv1);
@@ -123,20 +137,16 @@ struct ZWin {
     if (cc >= 0 && cc <= nz + 1) {
       const double upd = -dtv2 * (fslI + fsr2 + fssI - fss2);
       out = q2 + upd;
-      if (cc == 0) upd_bot = upd;
-      if (cc == nz + 1) upd_top = upd;
+      if (cc == 0 || cc == nz + 1) *edge = upd;
     }
     fsr2 = fsrI;
     fss2 = fssI;
     q2 = q1;
     q1 = q0;
     dq2 = dq1;
-    dq1 = dq0;
     ub2 = ub1;
     ub1 = ub0;
     vb1 = vb0;
-    dtv2 = dtv1;
-    dtv1 = dtv0;
     return out;
   }
 };
@@ -237,19 +247,37 @@ __device__ __forceinline__ void advect_hz_body(const AdvArgs &H, const AdvArgs &
   }
 
   ZWin zw[NB][R];
+  ZDtv zd[R];
+  // the bottom and top outflow increments of an owned column of an aloft
+  // bin, added as the sweep forms them (src/AdvectionVert_DCU.f90:306-311);
+  // the deposit is the bottom outflow (:321)
+  auto edge = [&](int b, int t, int cc, double upd) {
+    if (!(on[b] && own[t])) return;
+    const int n = n0 + b;
+    const long long of = (long long)(i + 1) + (long long)(nx + 4) * (jr[t] + 1) + ofs_xy * n;
+    if (cc == 0) {
+      Z.of1[of] = Z.of1[of] + upd;
+      Z.dep[(long long)(i - 1) + (long long)nx * ((long long)(jr[t] - 1) + (long long)ny * n)] = Z.of1[of];
+    } else {
+      Z.of2[of] = Z.of2[of] + upd;
+    }
+  };
   // the vertical sweep steps of the ghost levels and past the top
   auto zghost = [&](int r) {
     const int rc = min(max(r, -1), nz + 2);  // clamped into the column
+    const int cc = r - 2;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const int c = p3(ic, jr[t], rc);
       const double zs = Z.sig[c], dv = dtvp[c], vzh = Z.vel[c] + Z.vh[c];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const double v = zw[b][t].step(r, nz, lim, 0.0, zs, dv, vzh, vfb[b][c]);
-        const int cc = r - 2;
+        double e = 0.0;
+        const double v = zw[b][t].step(r, nz, lim, 0.0, zs, zd[t], vzh, vfb[b][c], &e);
         if (on[b] && own[t] && cc >= 1 && cc <= nz) dst[b][p3(ic, jr[t], cc)] = v;
+        if (cc == 0 || cc == nz + 1) edge(b, t, cc, e);
       }
+      zd[t].shift(ZDtv::gather(r, nz, dv));
     }
   };
   zghost(-2);
@@ -351,28 +379,17 @@ __device__ __forceinline__ void advect_hz_body(const AdvArgs &H, const AdvArgs &
       const double dvz = DIR2 == 1 ? dv[t] : dv[0];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const double v = zw[b][t].step(k, nz, lim, hv[b][t], zs, dvz, vzh, vfb[b][c]);
+        double e = 0.0;
+        const double v = zw[b][t].step(k, nz, lim, hv[b][t], zs, zd[t], vzh, vfb[b][c], &e);
         if (on[b] && own[t] && k - 2 >= 1) dst[b][c - 2 * sz] = v;
+        if (k == 2) edge(b, t, 0, e);
       }
+      zd[t].shift(ZDtv::gather(k, nz, dvz));
     }
   }
   zghost(nz + 1);
   zghost(nz + 2);
   zghost(nz + 3);
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (!on[b]) continue;
-    const int n = n0 + b;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      if (!own[t]) continue;
-      // deposit and top outflow (src/AdvectionVert_DCU.f90:306-311, :321)
-      const long long of = (long long)(i + 1) + (long long)(nx + 4) * (jr[t] + 1) + ofs_xy * n;
-      Z.of1[of] = Z.of1[of] + zw[b][t].upd_bot;
-      Z.of2[of] = Z.of2[of] + zw[b][t].upd_top;
-      Z.dep[(long long)(i - 1) + (long long)nx * ((long long)(jr[t] - 1) + (long long)ny * n)] = Z.of1[of];
-    }
-  }
 }
 
 struct HzArgs {
