@@ -192,6 +192,22 @@ def test_adjoint_partitioned_line_lengths(pkg, oracle, tr, n, dirn, shape):
                                    **{knob: shape}))
 
 
+# lines over several waves (SPIKE coupling of the parts): 3, 6 and 8 parts,
+# contiguous (x) and strided (y) lines; the Jacobi sweeps switched off so
+# that every line takes the partitioned adjoint
+@pytest.mark.parametrize("n", [600, 1100, 2000])
+@pytest.mark.parametrize("dirn", [0, 1])
+def test_adjoint_partitioned_multi_wave(pkg, oracle, tr, n, dirn):
+    dims = [5, 4, 3]
+    dims[dirn] = n
+    st = pkg.workloads.random_state(*dims, 2, seed=n + dirn, latlon=True, periodic=dirn == 0, imp_fac=0.5)
+    st.dt *= 20.0
+    Fq = forward(oracle, st, [("diffCN_" + "xyz"[dirn],)])
+    lam = rand_lambda(st, 71 + dirn)
+    check_dot(st, Fq, lam, adjoint(tr, st, lam, "adj_diffuse", dirn, True, line_solver="partitioned",
+                                   part_max_line_strided=2048, jacobi_sweeps=0))
+
+
 def test_adjoint_partitioned_falls_back_on_negative_diffusivity(pkg, oracle, tr):
     """A line with a negative diffusivity is not an M-matrix row: the sweep
     runs the dgtsv-order adjoint (the device-side choice), whose result is

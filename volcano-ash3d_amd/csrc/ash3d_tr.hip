@@ -639,11 +639,9 @@ int try_cn_partitioned(ash3d_tr_ctx *x, int dir, DiffArgs &a, int *rc, bool adjo
   const int n = a.g.rmax - a.g.rmin + 1;
   a.sel = 0;
   // (x lines are contiguous in memory; y and z lines strided; the adjoint's
-  // partitioned solves take lines of up to 512 cells, and only those the
-  // forward's took, so that the sweeps the forward ran as Jacobi sweeps
-  // get their transpose)
-  const int fmax = dir == 0 ? x->part_max_line : x->part_max_line_strided;
-  const int nmax = adjoint ? std::min(512, fmax) : fmax;
+  // partitioned solves take the lines the forward's took, so that the
+  // sweeps the forward ran as Jacobi sweeps get their transpose)
+  const int nmax = dir == 0 ? x->part_max_line : x->part_max_line_strided;
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || n < 3 || n > nmax) return 0;
   const int hn = host_ndom(x, dir);
   if (hn > 0) return 0;

@@ -91,7 +91,7 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
   constexpr int LEV = Lev<LP>::value;
   constexpr int NR = 2 * WV;  // unknowns of the parts' reduced system
   static_assert(PD == 1 || PD == 2, "one or two bins in flight");
-  static_assert(WV == 1 || (LP == 64 && !ADJ), "several parts per line: whole waves, forward solves");
+  static_assert(WV == 1 || LP == 64, "several parts per line: whole waves");
   extern __shared__ double s_tile[];  // T lines x PITCH
   // a sweep with lines that are not diagonally dominant M-matrix rows runs
   // the dgtsv-order kernel instead (launched after this one; uniform exit)
@@ -428,6 +428,7 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
                  : (omt * rL[k]) * q[k] + (1.0 - omt * (rL[k] + rR[k])) * q[k + 1] + (omt * rR[k]) * q[k + 2];
     }
     local(d, xs);
+    double xlp = 0.0, xfn = 0.0;  // x_last of the previous part, x_first of the next
     if constexpr (WV > 1) {
       // every part's y end values, then the coupled correction
       double *yv = s_y + ln * NR;
@@ -437,7 +438,6 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
       // rows 2 wv - 1 and 2 wv + 2 of R^{-1} (zero rows at the line's ends)
       const double *Ri = s_Ri + ln * NR * NR;
       const double *rL_ = Ri + (wv > 0 ? 2 * wv - 1 : 0) * NR, *rF_ = Ri + (wv + 1 < WV ? 2 * wv + 2 : 0) * NR;
-      double xlp = 0.0, xfn = 0.0;
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const double y = yv[c];
@@ -456,6 +456,10 @@ __device__ __forceinline__ void cn_part_body(const DiffArgs &A, const PartArgs &
       double xprev, xnext, t;
       lane_updown<LP, 1>(xs[M - 1], xprev, t);
       lane_updown<LP, 1>(xs[0], t, xnext);
+      // several parts: the first and last lanes of a part take the
+      // neighbouring parts' end values of the solution
+      if (WV > 1 && pw == 0) xprev = xlp;
+      if (WV > 1 && pw == LP - 1) xnext = xfn;
 #pragma unroll
       for (int k = 0; k < M; ++k) {
         const int i = p * M + k;
@@ -625,21 +629,21 @@ hipError_t launch_cn_part_shapes(hipStream_t s, const DiffArgs &a, const PartArg
   // lanes per line LP, rows per lane M (LP*M >= n), lines per workgroup T;
   // shape 1: contiguous lines two per workgroup (fewer waves held at each
   // barrier), strided lines in wider tiles (longer coalesced row segments)
-  if constexpr (!ADJ) {
-    // lines over several waves (SPIKE coupling of the parts)
-    // (4 rows per lane, 256 per wave; contiguous lines one per workgroup,
-    // strided lines in tiles of 2-4 along the contiguous index)
+  {
+    // lines over several waves (SPIKE coupling of the parts; the adjoint
+    // too); 4 rows per lane, 256 per wave; contiguous lines one per
+    // workgroup, strided lines in tiles of 2-4 along the contiguous index
     if (n > 512) {
       if (contig) {
-        if (n <= 768) return launch_part<64, 4, 1, true, false, PDL, 3>(s, a, q, sp);
-        if (n <= 1024) return launch_part<64, 4, 1, true, false, PDL, 4>(s, a, q, sp);
-        if (n <= 1536) return launch_part<64, 4, 1, true, false, PDL, 6>(s, a, q, sp);
-        return launch_part<64, 4, 1, true, false, PDL, 8>(s, a, q, sp);
+        if (n <= 768) return launch_part<64, 4, 1, true, ADJ, PDL, 3>(s, a, q, sp);
+        if (n <= 1024) return launch_part<64, 4, 1, true, ADJ, PDL, 4>(s, a, q, sp);
+        if (n <= 1536) return launch_part<64, 4, 1, true, ADJ, PDL, 6>(s, a, q, sp);
+        return launch_part<64, 4, 1, true, ADJ, PDL, 8>(s, a, q, sp);
       }
-      if (n <= 768) return launch_part<64, 4, 2, false, false, PDL, 3>(s, a, q, sp);
-      if (n <= 1024) return launch_part<64, 4, 2, false, false, PDL, 4>(s, a, q, sp);
-      if (n <= 1536) return launch_part<64, 4, 1, false, false, PDL, 6>(s, a, q, sp);
-      return launch_part<64, 4, 1, false, false, PDL, 8>(s, a, q, sp);
+      if (n <= 768) return launch_part<64, 4, 2, false, ADJ, PDL, 3>(s, a, q, sp);
+      if (n <= 1024) return launch_part<64, 4, 2, false, ADJ, PDL, 4>(s, a, q, sp);
+      if (n <= 1536) return launch_part<64, 4, 1, false, ADJ, PDL, 6>(s, a, q, sp);
+      return launch_part<64, 4, 1, false, ADJ, PDL, 8>(s, a, q, sp);
     }
   }
   if (contig) {
@@ -681,7 +685,7 @@ static hipError_t diff_cn_part_any(hipStream_t s, const DiffArgs &a, const PartL
   q.lr0 = lay.lr0; q.lra = lay.lra; q.lrb = lay.lrb; q.lrr = lay.lrr; q.lr_ncell = lay.lr_ncell;
   q.kap = lay.kap;
   q.bins_wg = (bins_wg > 0 && bins_wg < a.d.ns) ? bins_wg : a.d.ns;
-  if (n < 3 || n > (adjoint ? 512 : 2048)) return hipErrorInvalidValue;
+  if (n < 3 || n > 2048) return hipErrorInvalidValue;
   if (adjoint && !q.kap) return hipErrorInvalidValue;
   return adjoint ? launch_cn_part_shapes<true>(s, a, q, lay.str == 1, n, shape, sp)
                  : launch_cn_part_shapes<false>(s, a, q, lay.str == 1, n, shape, sp);
