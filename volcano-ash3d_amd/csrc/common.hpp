@@ -289,7 +289,7 @@ struct PartLayout {
   const double *kap = nullptr;  // cell volumes in the state's layout (adjoint only)
 };
 // adjoint: lambda <- K B A^{-1} K^{-1} lambda (= B^T A^{-T} lambda, since
-// kappa_l R_l = kappa_{l+1} L_{l+1} makes K^{-1} A and K^{-1} B symmetric)
+// kappa_l R_l = kappa_{l+1} L_{l+1} makes K A and K B symmetric, K = diag kappa)
 hipError_t launch_diff_cn_part(hipStream_t s, const DiffArgs &a, const PartLayout &lay, int bins_wg, int shape,
                                bool adjoint = false);
 // the launch of the forward partitioned solves (launch_diff_cn_part)
