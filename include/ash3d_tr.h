@@ -241,13 +241,13 @@ enum {
    * sweeps), 0 three separate in-place sweeps; default 1 on grids of at
    * least 2^24 interior cells, 0 below */
   ASH3D_TUNE_ADV_FUSED = 17,
-  /* partitioned mode, eager steps: y and z theta-scheme sweeps whose lines
+  /* partitioned mode, eager steps: the most Jacobi sweeps a line may take
+   * (0..12; 0 never; default 12).  y and z theta-scheme sweeps whose lines
    * are too long for the partitioned solves are solved by Jacobi sweeps
-   * in one pass when at most this many (0..7; 0 never; default 7) bring
-   * the error under 2^-52 of the line's largest value (rows with
-   * rho = max (|DL| + |DU|) / |D| need the least K + 1 with
-   * rho^(K+1) <= 2^-52 (1 - rho)); otherwise dgtsv order.  Also the most
-   * sweeps of the x lines' Jacobi sweeps (ASH3D_TUNE_JACOBI_X) */
+   * in one pass when at most min(this, 7) bring the error under 2^-52 of
+   * the line's largest value (rows with rho = max (|DL| + |DU|) / |D|
+   * need the least K + 1 with rho^(K+1) <= 2^-52 (1 - rho)); otherwise
+   * dgtsv order.  x lines: see ASH3D_TUNE_JACOBI_X */
   ASH3D_TUNE_JACOBI_SWEEPS = 18,
   /* read only: the Jacobi sweeps the last y (z) theta-scheme sweep took,
    * 0 when it used another solver */
@@ -255,7 +255,7 @@ enum {
   ASH3D_TUNE_JACOBI_LAST_Z = 20,
   /* partitioned mode, eager steps, x lines: each line whose rows' rho
    * needs at most ASH3D_TUNE_JACOBI_SWEEPS sweeps (as above: seven take
-   * rho up to 5.7998e-3) takes that many Jacobi sweeps, the partitioned
+   * rho up to 5.80e-3, twelve up to 4.8e-2) takes that many Jacobi sweeps, the partitioned
    * solves the others; 1 (default) where the partitioned solves would run
    * over several waves per line (lines longer than 512 cells), 2 at any
    * length, 0 never */

@@ -34,7 +34,7 @@ PART_BINS = (0, 8, 4)
 # lines over several waves too (2048); only tried where a line is longer
 PART_MAX_LINE = (512, 2048)
 # most Jacobi sweeps for strided lines beyond the partitioned solves (0: never)
-JACOBI_SWEEPS = 7
+JACOBI_SWEEPS = 12
 
 
 def _key(st):

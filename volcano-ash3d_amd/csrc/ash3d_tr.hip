@@ -113,9 +113,10 @@ struct ash3d_tr_ctx {
   // LU factors of the theta-scheme lines per direction (one allocation:
   // fact, D, DU, DL (4 * ncell doubles), then ncell interchange flags, one
   // INFO per line, the count of lines that are not diagonally dominant,
-  // the largest row ratio (|DL| + |DU|) / |D| as double bits, per line a
-  // flag for the lines seven Jacobi sweeps solve and the largest ratio
-  // among those), valid for the key (dt, theta, rmin, rmax)
+  // the largest row ratio (|DL| + |DU|) / |D| as double bits, per line the
+  // Jacobi sweeps it needs (0: more than kJacobiMaxSweepsX), the number of
+  // lines that need at most that many and their largest ratio), valid for
+  // the key (dt, theta, rmin, rmax)
   double *lu[3] = {nullptr, nullptr, nullptr};
   uint8_t *lu_piv[3] = {nullptr, nullptr, nullptr};
   int *lu_sing[3] = {nullptr, nullptr, nullptr};
@@ -215,14 +216,14 @@ struct ash3d_tr_ctx {
   int adv_gy[3] = {0, 0, 4};
   int adv_xrows = 1;
   int adv_fused = 1;  // second horizontal + vertical advection in one pass (step with diffusion)
-  // most Jacobi sweeps a y or z theta-scheme sweep may take in place of
-  // the dgtsv-order solve (0: never), and the sweeps the last y and z
-  // theta-scheme sweeps took (0: another solver)
-  int jacobi_sweeps = kJacobiMaxIters + 1;
+  // most Jacobi sweeps a line may take (0: never; y and z lines at most
+  // kJacobiMaxIters + 1 in any case), and the sweeps the last sweep of
+  // each direction took (0: another solver)
+  int jacobi_sweeps = kJacobiMaxSweepsX;
   int jacobi_last[3] = {0, 0, 0};
-  // x lines: 1 the lines that seven sweeps solve take them and the
-  // partitioned solves the rest, when these run over several waves per
-  // line (longer than 512 cells); 2 at any length; 0 never
+  // x lines: 1 the lines that need at most jacobi_sweeps sweeps take them
+  // and the partitioned solves the rest, when these run over several waves
+  // per line (longer than 512 cells); 2 at any length; 0 never
   int jacobi_x = 1;
   // dt / kappa per cell for the fused pass, valid for dtv_dt
   double *dtvc = nullptr;
@@ -577,7 +578,8 @@ int host_ndom(ash3d_tr_ctx *x, int dir) {
 // at most by 1 / (1 - rho).  -1 while the factors' ratio is not known on
 // the host (the first sweep after a factorisation) and in graph captures.
 int jacobi_k(double rho, int max_sweeps) {
-  const int sw = jacobi_sweeps_needed(rho);
+  // y and z lines: at most kJacobiMaxIters + 1 sweeps (the kernels' windows)
+  const int sw = jacobi_sweeps_needed(rho, kJacobiMaxIters + 1);
   return (sw == 0 || sw > max_sweeps) ? -1 : sw - 1;
 }
 
@@ -593,7 +595,7 @@ int jacobi_iters(ash3d_tr_ctx *x, int dir) {
 // jacobi_sweeps Jacobi sweeps take them (each line its own number, from the
 // factorisation), the partitioned solves taking the others.  Returns the
 // most sweeps a line takes (at most the sweeps of the largest ratio among
-// the lines that need at most seven, capped by jacobi_sweeps), or 0 (no
+// the lines that need at most kJacobiMaxSweepsX, capped by jacobi_sweeps), or 0 (no
 // Jacobi sweeps: the x-line solves as before).  Only where the partitioned
 // solves certainly run for the sweep (every line dominant, n within their
 // range).
@@ -603,7 +605,7 @@ int jacobi_x_sweeps(ash3d_tr_ctx *x, int n) {
   if (host_ndom(x, 0) != 0 || x->njl_h[0] == 0) return 0;
   double rho;
   std::memcpy(&rho, x->rhoj_h, sizeof rho);
-  const int sw = jacobi_sweeps_needed(rho);
+  const int sw = jacobi_sweeps_needed(rho, kJacobiMaxSweepsX);
   return sw == 0 ? 0 : std::min(sw, x->jacobi_sweeps);
 }
 
@@ -1432,8 +1434,8 @@ int ash3d_tr_tune(ash3d_tr_ctx *x, int32_t param, int32_t value) {
     return 0;
   }
   if (param == ASH3D_TUNE_JACOBI_SWEEPS) {
-    if (value < 0 || value > kJacobiMaxIters + 1)
-      return fail(ASH3D_TR_EINVAL, "Jacobi sweeps must be 0..%d (got %d)", kJacobiMaxIters + 1, value);
+    if (value < 0 || value > kJacobiMaxSweepsX)
+      return fail(ASH3D_TR_EINVAL, "Jacobi sweeps must be 0..%d (got %d)", kJacobiMaxSweepsX, value);
     x->jacobi_sweeps = value;
     return 0;
   }

@@ -182,7 +182,7 @@ struct DiffArgs {
   int sel;
   double *junk;        // kJunk doubles that masked-off lanes store to
   // per line of the sweep (k_line_lu's line index): the Jacobi sweeps the
-  // line needs (0: more than seven); lines with 1 <= jline <= jcap are
+  // line needs (0: more than kJacobiMaxSweepsX); lines with 1 <= jline <= jcap are
   // solved by the Jacobi sweeps and skipped by the partitioned solves.
   // null: none
   const uint8_t *jline;
@@ -298,24 +298,27 @@ hipError_t spec_diff_cn_part(const DiffArgs &a, const PartLayout &lay, int bins_
 // (*rho: the largest row ratio (|DL| + |DU|) / |D| over the sweep's lines as
 // double bits, 1.0 when a line is not diagonally dominant; may be null.
 // jl[line]: the Jacobi sweeps the line needs (jacobi_sweeps_needed of its
-// ratio; 0: more than seven), *rho_j the largest ratio of the lines that
-// need at most seven and *njl their number; all three null or all given)
+// ratio; 0: more than kJacobiMaxSweepsX), *rho_j the largest ratio of the
+// lines that need at most that many and *njl their number; all three null
+// or all given)
 hipError_t launch_line_lu(hipStream_t s, const DiffArgs &a, double *lu, uint8_t *piv, int *sing, int *ndom,
                           unsigned long long *rho, uint8_t *jl, unsigned long long *rho_j, int *njl);
 // theta-scheme solves of y or z lines by K + 1 Jacobi sweeps (0 <= k <=
 // kJacobiMaxIters), in one pass, for sweeps whose rows are all diagonally
 // dominant (nb: bins per thread, 1, 2 or 4)
 constexpr int kJacobiMaxIters = 6;
-// The least number of sweeps s <= kJacobiMaxIters + 1 with rho^s <= 2^-52
-// (1 - rho) -- after s sweeps from zero the error is at most rho^s of the
-// line's largest |x|, and the rounding grows at most by 1 / (1 - rho) --
-// or 0 when more are needed (rho not in [0, 1): 0).  Seven sweeps take
-// rho up to 5.7998e-3.
-__host__ __device__ inline int jacobi_sweeps_needed(double rho) {
+// x lines take up to this many sweeps (a runtime count per line)
+constexpr int kJacobiMaxSweepsX = 12;
+// The least number of sweeps s <= max_sweeps with rho^s <= 2^-52 (1 - rho)
+// -- after s sweeps from zero the error is at most rho^s of the line's
+// largest |x|, and the rounding grows at most by 1 / (1 - rho) -- or 0
+// when more are needed (rho not in [0, 1): 0).  Seven sweeps take rho up
+// to 5.7998e-3, twelve up to 4.8e-2.
+__host__ __device__ inline int jacobi_sweeps_needed(double rho, int max_sweeps) {
   if (!(rho >= 0.0 && rho < 1.0)) return 0;
   const double bound = 0x1p-52 * (1.0 - rho);
   double e = rho;
-  for (int s = 1; s <= kJacobiMaxIters + 1; ++s) {
+  for (int s = 1; s <= max_sweeps; ++s) {
     if (e <= bound) return s;
     e *= rho;
   }

@@ -171,10 +171,11 @@ __global__ __launch_bounds__(kBlock) void k_line_lu(DiffArgs A, double *lu, uint
   // patterns); lines that are not dominant count as 1
   const double r = (dom && rho_line < 1.0) ? rho_line : 1.0;
   if (rho) atomicMax(rho, (unsigned long long)__double_as_longlong(r));
-  // the Jacobi sweeps the line needs (0: more than seven); the number of
-  // lines that need at most seven and the largest ratio among them
+  // the Jacobi sweeps the line needs (0: more than kJacobiMaxSweepsX); the
+  // number of lines that need at most that many and the largest ratio
+  // among them
   if (jl) {
-    const int sw = dom ? jacobi_sweeps_needed(r) : 0;
+    const int sw = dom ? jacobi_sweeps_needed(r, kJacobiMaxSweepsX) : 0;
     jl[gid] = (uint8_t)sw;
     if (sw > 0) {
       atomicMax(rho_j, (unsigned long long)__double_as_longlong(r));
