@@ -256,9 +256,8 @@ enum {
   /* partitioned mode, eager steps, x lines: each line whose rows' rho
    * needs at most ASH3D_TUNE_JACOBI_SWEEPS sweeps (as above: seven take
    * rho up to 5.80e-3, twelve up to 4.8e-2) takes that many Jacobi sweeps, the partitioned
-   * solves the others; 1 (default) where the partitioned solves would run
-   * over several waves per line (lines longer than 512 cells), 2 at any
-   * length, 0 never */
+   * solves the others; 1 (default) for lines of at least 256 cells, 2 at
+   * any length, 0 never */
   ASH3D_TUNE_JACOBI_X = 21,
   /* read only: the most Jacobi sweeps a line of the last x theta-scheme
    * sweep may have taken (0: none) */

@@ -42,11 +42,16 @@ def small_members(pkg):
     return sts
 
 
+# the members' own eager steps with the solvers an ensemble step runs: no
+# Jacobi sweeps (a captured or batched step keeps the partitioned solves)
+NO_JACOBI = {"jacobi_sweeps": 0}
+
+
 def contexts(tr, sts, **tune):
     Ts = []
     for st in sts:
         T = tr.Transport(st)
-        T.tune(line_solver="partitioned", **tune)
+        T.tune(line_solver="partitioned", **(NO_JACOBI | tune))
         T.transport_step(1, True)
         Ts.append(T)
     return Ts
@@ -55,7 +60,7 @@ def contexts(tr, sts, **tune):
 def run_individual(tr, st, steps, **tune):
     b = st.copy()
     T = tr.Transport(b)
-    T.tune(line_solver="partitioned", **tune)
+    T.tune(line_solver="partitioned", **(NO_JACOBI | tune))
     T.transport_step(1, True)
     for it in range(2, 2 + steps):
         T.step_glue()
@@ -144,7 +149,7 @@ def test_glue_and_source_between_steps(pkg, tr):
     for st in sts:
         b = st.copy()
         T = tr.Transport(b)
-        T.tune(line_solver="partitioned")
+        T.tune(line_solver="partitioned", **NO_JACOBI)
         T.transport_step(1, True)
         for it in range(2, 5):
             T.step_glue()

@@ -335,14 +335,16 @@ def test_x_lines_split_covers_both_kinds(pkg):
 
 @pytest.mark.parametrize("theta,scale", [(0.5, 2000.0), (1.0, 1000.0)])
 @pytest.mark.parametrize("nb", [1, 2, 4])
-def test_long_x_lines(pkg, oracle, tr, theta, scale, nb):
-    """Lines of 600 cells (default jacobi_x=1: beyond one wave), 51 of 72
-    lines in Jacobi sweeps, subgrid bounds and an IsAloft mask, 1, 2 and 4
-    bins per thread."""
-    st = x_state(pkg, 600, 24, 3, 5, scale, seed=31, theta=theta)
+@pytest.mark.parametrize("nx", [300, 600])
+def test_long_x_lines(pkg, oracle, tr, theta, scale, nb, nx):
+    """Lines of 300 and 600 cells (default jacobi_x=1: lines of 256 cells
+    and more; one and two waves of the Jacobi kernel, one and three parts
+    of the partitioned solves), 51 or 59 of 72 lines in Jacobi sweeps, subgrid
+    bounds and an IsAloft mask, 1, 2 and 4 bins per thread."""
+    st = x_state(pkg, nx, 24, 3, 5, scale, seed=31, theta=theta)
     st.is_aloft = np.array([1, 1, 0, 1, 1], dtype=np.int32)
-    st.bounds = (3, 590, 2, 23, 1, 3)
-    nj, nl, want = x_split(st, 3, 590)
+    st.bounds = (3, nx - 10, 2, 23, 1, 3)
+    nj, nl, want = x_split(st, 3, nx - 10)
     seen = []
     a, b = run(pkg, oracle, tr, st, [("diffCN_x",)] * 3, nb_diffuse_x=nb,
                after=lambda T, k: seen.append(T.tune_get("jacobi_last_x")))
@@ -371,8 +373,8 @@ def test_x_lines_sweep_counts_per_line(pkg, oracle, tr):
 
 
 def test_short_x_lines_keep_the_partitioned_solves(pkg, oracle, tr):
-    """jacobi_x=1 (default): x lines of one wave stay in the partitioned
-    solves."""
+    """jacobi_x=1 (default): x lines shorter than 256 cells stay in the
+    partitioned solves."""
     st = x_state(pkg, 24, 40, 6, 3, 30.0, seed=29)
     seen = []
     a, b = run(pkg, oracle, tr, st, [("diffCN_x",)] * 3,

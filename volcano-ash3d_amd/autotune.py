@@ -64,8 +64,8 @@ def autotune(T, use_diffusion: bool, steps: int = 2, candidates=(1, 2, 3, 4, 8),
             params.append(("part_max_line", "diffuse_x"))
         if st.ny > 512 or st.nz > 512:
             params.append(("part_max_line_strided", ("diffuse_y", "diffuse_z")))
-        if st.nx > 512:
-            # the lines that seven Jacobi sweeps solve off the multi-wave partitioned solves
+        if st.nx >= 256:
+            # the x lines that Jacobi sweeps solve off the partitioned solves
             params.append(("jacobi_x", "diffuse_x"))
         if max(st.nx, st.ny, st.nz) > 512:
             # Jacobi sweeps: strided lines left to dgtsv order, x lines as above

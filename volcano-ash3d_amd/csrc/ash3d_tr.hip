@@ -222,8 +222,9 @@ struct ash3d_tr_ctx {
   int jacobi_sweeps = kJacobiMaxSweepsX;
   int jacobi_last[3] = {0, 0, 0};
   // x lines: 1 the lines that need at most jacobi_sweeps sweeps take them
-  // and the partitioned solves the rest, when these run over several waves
-  // per line (longer than 512 cells); 2 at any length; 0 never
+  // and the partitioned solves the rest, for lines of at least 256 cells
+  // (shorter ones leave most of the kernel's lanes idle); 2 at any
+  // length; 0 never
   int jacobi_x = 1;
   // dt / kappa per cell for the fused pass, valid for dtv_dt
   double *dtvc = nullptr;
@@ -601,7 +602,7 @@ int jacobi_iters(ash3d_tr_ctx *x, int dir) {
 // range).
 int jacobi_x_sweeps(ash3d_tr_ctx *x, int n) {
   if (x->line_solver != ASH3D_LINE_SOLVER_PARTITIONED || x->jacobi_sweeps <= 0 || x->jacobi_x == 0) return 0;
-  if (n < 3 || n > x->part_max_line || (x->jacobi_x == 1 && n <= 512)) return 0;
+  if (n < 3 || n > x->part_max_line || (x->jacobi_x == 1 && n < 256)) return 0;
   if (host_ndom(x, 0) != 0 || x->njl_h[0] == 0) return 0;
   double rho;
   std::memcpy(&rho, x->rhoj_h, sizeof rho);
