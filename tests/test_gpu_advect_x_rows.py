@@ -7,7 +7,10 @@ staged through LDS).  Row lengths sit at and around the chunk edges
 even rows (nx > 380, > 252 with topography) take several wavefronts per
 row, a segment each with the halos exchanged through LDS, up to 8
 segments; odd ones, longer ones and more than 64 bins take the workgroup
-kernel whatever the setting."""
+kernel whatever the setting.  The adjoint sweep has the same three even-row
+kernels (segments of at most 248 cells) and is held to the workgroup
+kernel's bits and to the dot-product identity with the oracle's forward
+sweep."""
 import numpy as np
 import pytest
 
@@ -124,6 +127,44 @@ def test_adjoint_x_rows_kernels(pkg, oracle, tr, nx, variant):
         st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
     Fq = ta.forward(oracle, st, [("advect_x",)])
     lam = ta.rand_lambda(st, nx)
+    g1 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=1)
+    g0 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=0)
+    assert np.array_equal(g1, g0)
+    ta.check_dot(st, Fq, lam, g1)
+
+
+# adjoint rows in segments of one wavefront (two chunks of 128 cells, at
+# most 248 cells a segment): 2 segments, the global grid's 1444-cell rows
+# (6), 8 segments at the cap and the first row past it (workgroup kernel)
+NX_ADJ_SEG = [382, 762, 1440, 1980, 1982]
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("nx", NX_ADJ_SEG)
+def test_adjoint_x_segmented_rows(pkg, oracle, tr, nx, variant):
+    """Segmented adjoint rows give the bits of the workgroup kernel and
+    satisfy <F(q), w> = <q, J(q)^T w> with the oracle's forward sweep."""
+    import test_gpu_adjoint as ta
+    st = pkg.workloads.random_state(nx, 3, 3, 5, seed=nx + 11 * len(variant), **VARIANTS[variant])
+    if variant == "subgrid_aloft":
+        st.bounds = (3, nx - 2, 2, 3, 2, 3)
+        st.is_aloft = np.array([1, 0, 1, 1, 0], dtype=np.int32)
+    Fq = ta.forward(oracle, st, [("advect_x",)])
+    lam = ta.rand_lambda(st, nx + 1)
+    g1 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=1)
+    g0 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=0)
+    assert np.array_equal(g1, g0)
+    ta.check_dot(st, Fq, lam, g1)
+
+
+@pytest.mark.parametrize("ns", [1, 2, 7])
+def test_adjoint_x_segmented_rows_bin_counts(pkg, oracle, tr, ns):
+    import test_gpu_adjoint as ta
+    st = pkg.workloads.random_state(1440, 2, 3, ns, seed=200 + ns, latlon=True, periodic=True)
+    st.is_aloft = np.ones(ns, dtype=np.int32)
+    st.is_aloft[1::3] = 0 if ns > 2 else 1
+    Fq = ta.forward(oracle, st, [("advect_x",)])
+    lam = ta.rand_lambda(st, ns)
     g1 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=1)
     g0 = ta.adjoint(tr, st, lam, "adj_advect", 0, adv_x_rows=0)
     assert np.array_equal(g1, g0)

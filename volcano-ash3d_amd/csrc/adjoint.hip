@@ -733,6 +733,257 @@ __global__ __launch_bounds__(256) void k_adj_advect_x_wave(AdvArgs A, const doub
   if (n < d.ns) bin(sa, sq, n);
 }
 
+// ---- x rows longer than one wavefront's registers (nx even) -----------
+// WV waves per row in one workgroup, wave w owning the row positions
+// [S, S + L) with S = w L (L even) and holding the extended range
+// [S - 4, S + L + 4) in the pair layout of k_adj_advect_x_wave (lane l of
+// chunk m: positions S - 4 + 128 m + 2 l + e).  The adjoint of a cell
+// gathers the terms of the four interfaces around it, which read q three
+// cells and lambda two cells beyond it, so every cell a wave stores has
+// what it needs in its own registers and the arithmetic is the wavefront
+// kernel's.  The pre-sweep q and the per-cell fields of the extended
+// range come from memory (they are read only); lambda, which the
+// neighbouring waves update in place, is loaded at the wave's own
+// positions only, and the two halo pairs are the neighbours' first and
+// last pairs, exchanged through LDS (two slots by bin parity, one barrier
+// per bin).  On a periodic row the ghost adjoints of cells -1, 0 (wave 0)
+// and nx+1, nx+2 (the last wave) fold back across the row through LDS
+// after a second barrier.
+template <int NC, bool TOPO>
+__global__ __launch_bounds__(512) void k_adj_advect_x_seg(AdvArgs A, const double *__restrict__ qpre, double *lam,
+                                                          int L) {
+  constexpr int CS = 128;
+  const Dims d = A.d;
+  const LineGeom g = A.g;
+  const int nx = d.nx, W = nx + 4;
+  const int l = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6), WV = (int)(blockDim.x >> 6);
+  const long long row = blockIdx.x;
+  const int j = (int)(row % d.ny) + 1;
+  const int k = (int)(row / d.ny) + 1;
+  const int rmin = g.rmin, rmax = g.rmax;
+  const bool row_on = j >= g.a0 && j <= g.a1 && k >= g.b0 && k <= g.b1;
+  const long long base = d.p3(-1, j, k);
+  const int lim = A.limiter;
+  double *junk = A.junk + 2 * (long long)(threadIdx.x + blockDim.x * (blockIdx.x % (kJunk / 1024)));
+  const bool lane0 = l == 0, lane63 = l == 63;
+  const int S = w * L, Send = min(S + L, W), X = S - 4;
+  const bool has_lo = w > 0, has_hi = w + 1 < WV;
+  // halo pairs: positions S-2, S-1 (offset 2) and S+L, S+L+1 (offset L+4);
+  // own edge pairs: S, S+1 (offset 4) and S+L-2, S+L-1 (offset L+2)
+  const int m_hh = (L + 4) / CS, l_hh = ((L + 4) % CS) / 2;
+  const int m_le = (L + 2) / CS, l_le = ((L + 2) % CS) / 2;
+
+  int cc[NC][2];
+  bool valid[NC][2], ld[NC][2], uin[NC][2], act[NC][2];
+  int pf[NC], pl[NC];  // field / q and lambda offsets of the pair (clamped)
+  bool own[NC], store[NC], halo_lo[NC], halo_hi[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int p0 = X + CS * m + 2 * l;
+    own[m] = p0 >= S && p0 < Send;
+    pf[m] = min(max(p0, 0), W - 2);
+    pl[m] = own[m] ? p0 : S;
+    store[m] = own[m] && p0 >= 2 && p0 + 1 <= nx + 1;
+    halo_lo[m] = has_lo && m == 0 && l == 1;
+    halo_hi[m] = has_hi && m == m_hh && l == l_hh;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = p0 + e - 1;
+      cc[m][e] = c;
+      valid[m][e] = p0 + e >= 0 && p0 + e < W && p0 + e < Send + 4;
+      ld[m][e] = valid[m][e] && c >= rmin - 2 && c <= rmax + 2;
+      uin[m][e] = valid[m][e] && c >= rmin - 1 && c <= rmax + 1;
+      act[m][e] = valid[m][e] && c >= rmin && c <= rmax;
+    }
+  }
+  if (!row_on) {
+    for (int n = 0; n < d.ns; ++n) {
+      const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (store[m]) *(double2 *)(lam + base + pl[m] + bofs) = make_double2(0.0, 0.0);
+    }
+    return;  // the whole workgroup: row_on is the row's
+  }
+
+  // bin-independent terms, as the wavefront kernel
+  double u[NC][2], dtv[NC][2], tfD[NC][2], tfDm1[NC][2];
+  {
+    double vel[NC][2], sig[NC][2], kap[NC][2], D[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const long long o = base + pf[m];
+      const double2 v2 = *(const double2 *)(A.vel + o);
+      const double2 s2 = *(const double2 *)(A.sig + o);
+      const double2 k2 = *(const double2 *)(A.kap + o);
+      double2 d2 = make_double2(0.0, 0.0);
+      if (TOPO) d2 = *(const double2 *)(A.deld + (long long)pf[m] + (long long)(nx + 4) * (j + 1));
+      const double vv[2] = {v2.x, v2.y}, ss[2] = {s2.x, s2.y}, kk[2] = {k2.x, k2.y}, dd[2] = {d2.x, d2.y};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        vel[m][e] = ld[m][e] ? vv[e] : 0.0;
+        sig[m][e] = ld[m][e] ? ss[e] : 0.0;
+        kap[m][e] = ld[m][e] ? kk[e] : 1.0;
+        D[m][e] = (TOPO && uin[m][e]) ? dd[e] : 0.0;
+      }
+    }
+    double velm1[NC][2], sigp1[NC][2];
+    row_prev<NC>(vel, velm1, lane0);
+    if (TOPO) row_next<NC>(sig, sigp1, lane63);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        u[m][e] = uin[m][e] ? 0.5 * (velm1[m][e] + vel[m][e]) * sig[m][e] : 0.0;
+        dtv[m][e] = uin[m][e] ? A.dt / kap[m][e] : 0.0;
+        tfD[m][e] = (TOPO && valid[m][e]) ? 0.5 * vel[m][e] * (sig[m][e] + sigp1[m][e]) * D[m][e] : 0.0;
+      }
+    if (TOPO) row_prev<NC>(tfD, tfDm1, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        if (!TOPO) tfDm1[m][e] = 0.0;
+  }
+
+  // edge pairs s_edge[((slot * 8 + wave) * 2 + first/last) * 2 + e]; the
+  // periodic fold's ghost adjoints s_fold = {v(-1), v(0), v(nx+1), v(nx+2)}
+  __shared__ double s_edge[2 * 8 * 4];
+  __shared__ double s_fold[4];
+  // cells nx+1, nx+2 (positions nx+2, nx+3) in the last wave
+  const int o_g = nx + 2 - ((WV - 1) * L - 4);
+  const int m_g = o_g / CS, l_g = (o_g % CS) / 2;
+  const bool first_w = w == 0, last_w = w == WV - 1;
+  auto sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  };
+
+  double sa[NC][2], sq[NC][2], sb[NC][2], sr[NC][2];  // lambda and q of bins n, n+1
+  auto fetch = [&](double (&sl)[NC][2], double (&sqq)[NC][2], int n) {
+    const int nn = (n < d.ns && A.aloft.on(n)) ? n : max(0, min(n - 2, d.ns - 1));
+    const long long bofs = d.ncell * (long long)nn;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const double2 l2 = *(const double2 *)(lam + base + pl[m] + bofs);
+      const double2 q2 = *(const double2 *)(qpre + base + pf[m] + bofs);
+      sl[m][0] = l2.x; sl[m][1] = l2.y;
+      sqq[m][0] = q2.x; sqq[m][1] = q2.y;
+    }
+  };
+  auto bin = [&](double (&sl)[NC][2], double (&sqq)[NC][2], int n) {
+    const bool on = A.aloft.on(n);
+    double *edge = s_edge + (n & 1) * 32;
+    if (l == 2) {
+      edge[(w * 2 + 0) * 2 + 0] = sl[0][0];
+      edge[(w * 2 + 0) * 2 + 1] = sl[0][1];
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+      if (m == m_le && l == l_le) {
+        edge[(w * 2 + 1) * 2 + 0] = sl[m][0];
+        edge[(w * 2 + 1) * 2 + 1] = sl[m][1];
+      }
+    sync();
+    double hlo[2] = {0.0, 0.0}, hhi[2] = {0.0, 0.0};
+    if (has_lo) {
+      hlo[0] = edge[((w - 1) * 2 + 1) * 2 + 0];
+      hlo[1] = edge[((w - 1) * 2 + 1) * 2 + 1];
+    }
+    if (has_hi) {
+      hhi[0] = edge[((w + 1) * 2 + 0) * 2 + 0];
+      hhi[1] = edge[((w + 1) * 2 + 0) * 2 + 1];
+    }
+    double lm[NC][2], q[NC][2], mu[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const double lv = halo_lo[m] ? hlo[e] : (halo_hi[m] ? hhi[e] : (own[m] ? sl[m][e] : 0.0));
+        lm[m][e] = act[m][e] ? lv : 0.0;
+        q[m][e] = ld[m][e] ? sqq[m][e] : 0.0;
+      }
+    fetch(sl, sqq, n + 2);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) mu[m][e] = act[m][e] ? -dtv[m][e] * lm[m][e] : 0.0;
+    double um1[NC][2], up1[NC][2];
+    row_prev<NC>(u, um1, lane0);
+    row_next<NC>(u, up1, lane63);
+    double qm2[NC][2], qm1[NC][2], qp1[NC][2], mum1[NC][2];
+    row_prev2<NC>(q, qm2, lane0);
+    row_prev<NC>(q, qm1, lane0);
+    row_next<NC>(q, qp1, lane63);
+    row_prev<NC>(mu, mum1, lane0);
+    double e0[NC][2], e1[NC][2], e2[NC][2], e3[NC][2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int c = cc[m][e];
+        e0[m][e] = e1[m][e] = e2[m][e] = e3[m][e] = 0.0;
+        if (valid[m][e] && c >= rmin && c <= rmax + 1) {
+          const double divp = fmax(0.0, u[m][e]) - fmax(0.0, um1[m][e]);
+          const double divm = fmin(0.0, up1[m][e]) - fmin(0.0, u[m][e]);
+          const double aus = fabs(u[m][e]);
+          const double gI = 0.5 * aus * (1.0 - dtv[m][e] * aus);
+          interface_adjoint(lim, qm2[m][e], qm1[m][e], q[m][e], qp1[m][e], u[m][e], divp, divm, gI, tfD[m][e],
+                            tfDm1[m][e], mum1[m][e], mu[m][e], e0[m][e], e1[m][e], e2[m][e], e3[m][e]);
+        }
+      }
+    // interfaces r+2, r+1, r, r-1 reach cell r as their I-2, I-1, I, I+1
+    double e0n[NC][2], e1n[NC][2], e3p[NC][2], v[NC][2];
+    row_next2<NC>(e0, e0n, lane63);
+    row_next<NC>(e1, e1n, lane63);
+    row_prev<NC>(e3, e3p, lane0);
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) v[m][e] = e0n[m][e] + e1n[m][e] + e2[m][e] + e3p[m][e];
+    if (A.periodic) {
+      // cells -1, 0: element 0, 1 of lane 2, chunk 0 of wave 0
+      if (first_w && l == 2) {
+        s_fold[0] = v[0][0];
+        s_fold[1] = v[0][1];
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        if (last_w && m == m_g && l == l_g) {
+          s_fold[2] = v[m][0];
+          s_fold[3] = v[m][1];
+        }
+      sync();
+      const double gc = s_fold[0], gd = s_fold[1], ga = s_fold[2], gb = s_fold[3];
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int c = cc[m][e];
+          if (c == 1) v[m][e] += ga;
+          if (c == 2) v[m][e] += gb;
+          if (c == nx - 1) v[m][e] += gc;
+          if (c == nx) v[m][e] += gd;
+        }
+    }
+    const long long bofs = d.ncell * (long long)n;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      double *dst = store[m] ? lam + base + pl[m] + bofs : junk;
+      *(double2 *)dst = on ? make_double2(lm[m][0] + v[m][0], lm[m][1] + v[m][1]) : make_double2(0.0, 0.0);
+    }
+  };
+  fetch(sa, sq, 0);
+  fetch(sb, sr, 1);
+  int n = 0;
+  for (; n + 1 < d.ns; n += 2) {
+    bin(sa, sq, n);
+    bin(sb, sr, n + 1);
+  }
+  if (n < d.ns) bin(sa, sq, n);
+}
+
 // ---- theta-scheme lines: lambda' = B^T A^{-T} lambda -------------------
 struct RowC {
   double a, b, c;     // B row: a q_{l-1} + b q_l + c q_{l+1}
@@ -1000,6 +1251,28 @@ hipError_t adj_x_wave(hipStream_t s, const AdvArgs &a, const double *qpre, doubl
   return hipGetLastError();
 }
 
+// Segmented rows: WV = ceil(W / (128 NC - 8)) waves, L the even segment
+// length; hipErrorInvalidValue (nothing launched) for rows that need more
+// than 8 segments.
+template <int NC>
+hipError_t adj_x_seg(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam) {
+  const int W = a.d.nx + 4;
+  const int cap = 128 * NC - 8;
+  const int WV = (W + cap - 1) / cap;
+  if (WV > 8) return hipErrorInvalidValue;
+  int L = (W + WV - 1) / WV;
+  L += L & 1;
+  // the last segment holds cells nx-1 .. nx+2 (the periodic fold's targets
+  // and sources) and every segment its first and last pair
+  if (L > cap || L < 4 || (WV - 1) * L > W - 4) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((long long)a.d.ny * a.d.nz));
+  if (a.deld != nullptr)
+    hipLaunchKernelGGL((k_adj_advect_x_seg<NC, true>), grid, dim3(64 * WV), 0, s, a, qpre, lam, L);
+  else
+    hipLaunchKernelGGL((k_adj_advect_x_seg<NC, false>), grid, dim3(64 * WV), 0, s, a, qpre, lam, L);
+  return hipGetLastError();
+}
+
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 template <int DIR>
@@ -1076,22 +1349,30 @@ hipError_t adj_cn_nb(hipStream_t s, const DiffArgs &a, double *lam, int nb) {
 
 }  // namespace
 
-// x rows through LDS (rows of up to 8*256-4 cells, longer rows streamed
-// per lane), y and z lines in segments; a.periodic selects the periodic x
-// fold.
+// x rows: even rows in wavefront registers (one wavefront per row up to
+// 380 cells, 252 with topography; beyond, up to 8 segments of 248 cells),
+// other rows through LDS (rows of up to 8*256-4 cells), longer
+// rows streamed per lane; y and z lines in segments; a.periodic selects
+// the periodic x fold.
 hipError_t launch_adj_advect(hipStream_t s, const AdvArgs &a, const double *qpre, double *lam, int nb) {
   if (a.g.dir == 0) {
     const int cells = a.d.nx + 4;
-    // one wavefront per row under the forward's conditions
-    const bool wave = a.xrows == 1 && a.d.nx % 2 == 0 && cells <= (a.deld ? 256 : 384) && aligned16(lam) &&
-                      aligned16(qpre) && aligned16(a.vel) && aligned16(a.sig) && aligned16(a.kap) &&
-                      aligned16(a.junk) && (a.deld == nullptr || aligned16(a.deld));
-    if (wave) {
+    // one wavefront per row under the forward's conditions, longer rows in
+    // segments of one wavefront each
+    const bool pairs = a.xrows == 1 && a.d.nx % 2 == 0 && aligned16(lam) && aligned16(qpre) && aligned16(a.vel) &&
+                       aligned16(a.sig) && aligned16(a.kap) && aligned16(a.junk) &&
+                       (a.deld == nullptr || aligned16(a.deld));
+    if (pairs && cells <= (a.deld ? 256 : 384)) {
       switch ((cells + 127) / 128) {
         case 1: return adj_x_wave<1>(s, a, qpre, lam);
         case 2: return adj_x_wave<2>(s, a, qpre, lam);
         default: return adj_x_wave<3>(s, a, qpre, lam);
       }
+    }
+    if (pairs) {
+      // two chunks per wavefront: 188-200 VGPRs (three spill)
+      hipError_t e = adj_x_seg<2>(s, a, qpre, lam);
+      if (e != hipErrorInvalidValue) return e;
     }
     int cpt = (cells + 255) / 256;
     if (cpt > 8) return adj_lines<0>(s, a, qpre, lam, nb);
