@@ -369,7 +369,8 @@ def main():
                          "(lanes of a wave per line; agrees to rounding)")
     ap.add_argument("--adj-bins", type=int, default=2, help="bins of the global adjoint leg's state")
     ap.add_argument("--adj-steps", type=int, default=240, help="steps of the global adjoint leg")
-    ap.add_argument("--adj-slots", type=int, default=60, help="free checkpoint slots of the global adjoint leg")
+    ap.add_argument("--adj-slots", type=int, default=150,
+                    help="free checkpoint slots of the global adjoint leg (150 x 1.41 GB: 212 GB of the 288 GB of HBM)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transport step shards "
