@@ -6,20 +6,20 @@
 // diffusion, whose Set_BC(2) and Neumann passes rewrite every ghost
 // afterwards).
 //
-// A workgroup owns a tile of 64 x TY columns (lanes along x, one wave per
-// row of the tile) for one grain bin and streams it upward through the
-// levels.  At level k it stages the horizontal sweep's input plane -- the
-// first sweep's output, which that sweep wrote out of place to the ts1
-// slot -- with the two-cell halo along the sweep direction (rows j-2..j+2
-// for y lines; columns i-2..i+2 for x lines, wrapped on a periodic grid)
-// in LDS, evaluates the horizontal update of each owned cell from it, and
-// feeds the result into a register window that runs the vertical sweep of
-// the thread's column exactly as the streamed z kernel does (cell k-2 is
-// updated once level k is known).  The updated column goes to ts0.  So the
-// pair costs one read of the first sweep's output and one write of the
-// step's, instead of a read and a write for each sweep.  The planes are
-// double-buffered: the next level's plane is loaded into registers while
-// the current one is used, one barrier per level.
+// Each wave owns a set of columns -- 64 columns x TY rows for y lines,
+// 64 cells of one row for x lines -- for NB grain bins and streams them
+// upward through the levels.  At level k a lane loads the horizontal
+// sweep's input along its line -- the first sweep's output, which that
+// sweep wrote out of place to the ts1 slot -- with the two-cell halo
+// along the sweep direction (rows j-2..j+TY+1 for y lines; columns
+// i-2..i+2 for x lines, wrapped on a periodic grid; the neighbouring
+// waves' cells come through L1), evaluates the horizontal update of its
+// cells, and feeds the result into a register window that runs the
+// vertical sweep of the column exactly as the streamed z kernel does (cell
+// k-2 is updated once level k is known).  The updated column goes to ts0.
+// So the pair costs one read of the first sweep's output and one write of
+// the step's, instead of a read and a write for each sweep.  No LDS and
+// no barriers: every wave is independent.
 //
 // Every update is the reference expression evaluated in the same order on
 // the same inputs as the split sweeps (built with -ffp-contract=off), so
