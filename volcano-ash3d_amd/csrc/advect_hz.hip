@@ -296,8 +296,12 @@ __device__ __forceinline__ void advect_hz_body(const AdvArgs &H, const AdvArgs &
       for (int t = 0; t < QN; ++t) {
         const int r = jw - 2 + t;
         const bool v = lane_in && r >= 1 && r <= ny;
+        const int rc = min(r, ny + 2);  // rows -1..ny+2 exist: every lane loads
 #pragma unroll
-        for (int b = 0; b < NB; ++b) q[b][t] = v ? src[b][c0 + sy * (r + 1)] : 0.0;
+        for (int b = 0; b < NB; ++b) {
+          const double x = src[b][c0 + sy * (rc + 1)];
+          q[b][t] = v ? x : 0.0;
+        }
       }
 #pragma unroll
       for (int t = 0; t < VN; ++t) vel[t] = H.vel[c0 + sy * (min(jw - 1 + t, ny + 2) + 1)];
@@ -313,9 +317,12 @@ __device__ __forceinline__ void advect_hz_body(const AdvArgs &H, const AdvArgs &
       for (int t = 0; t < QN; ++t) {
         int ci = ic - 2 + t;
         if (xper) ci = ci < 1 ? ci + nx : (ci > nx ? ci - nx : ci);
-        const bool v = lane_in && ci >= 1 && ci <= nx;
+        const bool v = lane_in && ci >= 1 && ci <= nx;  // ci in -1..nx+2: every lane loads
 #pragma unroll
-        for (int b = 0; b < NB; ++b) q[b][t] = v ? src[b][c0 + ci + 1] : 0.0;
+        for (int b = 0; b < NB; ++b) {
+          const double x = src[b][c0 + ci + 1];
+          q[b][t] = v ? x : 0.0;
+        }
       }
 #pragma unroll
       for (int t = 0; t < VN; ++t) vel[t] = H.vel[c0 + ic + t];
