@@ -222,10 +222,13 @@ enum {
    * lines, so the per-cell fields are read once for all of them */
   ASH3D_TUNE_ADV_GROUPS_Y = 12,
   ASH3D_TUNE_ADV_GROUPS_Z = 13,
-  /* x advection rows: 1 (default) one wavefront per row, the row in
-   * registers (nx <= 380, or nx <= 252 with ZScaling_ID 2, at most 64
-   * bins; the adjoint for even nx only; other grids use 0), 0 one
-   * workgroup per row staged through LDS */
+  /* x advection rows: 1 (default) rows in wavefront registers: one
+   * wavefront per row for nx <= 380 (nx <= 252 with ZScaling_ID 2; the
+   * forward sweep at most 64 bins, the adjoint even nx only), longer even
+   * rows in up to 8 segments of one wavefront each (380 cells a segment
+   * in the forward sweep, 248 in the adjoint); other rows as with 0: one
+   * workgroup per row staged through LDS (rows of up to 2044 cells,
+   * longer ones streamed per lane) */
   ASH3D_TUNE_ADV_X_ROWS = 14,
   /* longest active line (3..2048 cells) that the partitioned solves take,
    * for x lines (contiguous in memory; default 2048) and for y and z lines
