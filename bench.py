@@ -215,10 +215,30 @@ def inversion_leg(pkg, line_solver, tuning, revolve_slots=40):
     return out
 
 
+def device_memory():
+    """(free, total) bytes of the current HIP device from the HIP runtime
+    that the library uses, or None."""
+    import ctypes
+    for name in ("/opt/rocm/lib/libamdhip64.so", "libamdhip64.so"):
+        try:
+            hip = ctypes.CDLL(name)
+            break
+        except OSError:
+            hip = None
+    if hip is None:
+        return None
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return None
+    return free.value, total.value
+
+
 def global_adjoint_leg(pkg, bins, n_steps, slots, line_solver, tuning):
     """Binomial checkpointing on the global grid (a bin subset of its full
     1440x720x80 grid): a source at 20-24 km over n_steps steps, loads every
-    n_steps/4 steps, reverse sweep in `slots` free slots."""
+    n_steps/4 steps, reverse sweep in `slots` free slots (0: as many as the
+    free HBM holds next to the context, measured on a probe inversion after
+    one forward run, less 8 GB)."""
     adj = importlib.import_module("volcano-ash3d_amd.adjoint")
     import numpy as np
     st = pkg.workloads.global_state(bins=bins)
@@ -227,18 +247,42 @@ def global_adjoint_leg(pkg, bins, n_steps, slots, line_solver, tuning):
     P = 4
     spp = n_steps // P
     H = min(60, st.nz)
-    inv = adj.SourceInversion(st, st.nx // 2, st.ny // 2, n_heights=H, n_periods=P, steps_per_period=spp,
-                              frac=np.full(st.ns, 1.0 / st.ns), dz=dz, obs_every=spp, use_diffusion=True,
-                              schedule="revolve", revolve_slots=slots)
-    inv.T.tune(line_solver=line_solver, **{k: v[0] for k, v in (tuning or {}).items()})
     rng = np.random.default_rng(0)
     s = 1e6 * rng.random((P, H))
+    knobs = {k: v[0] for k, v in (tuning or {}).items()}
+
+    def make(S):
+        inv = adj.SourceInversion(st, st.nx // 2, st.ny // 2, n_heights=H, n_periods=P, steps_per_period=spp,
+                                  frac=np.full(st.ns, 1.0 / st.ns), dz=dz, obs_every=spp, use_diffusion=True,
+                                  schedule="revolve", revolve_slots=S)
+        inv.T.tune(line_solver=line_solver, **knobs)
+        return inv
+
+    free_gb = None
+    if slots <= 0:
+        probe = make(0)
+        probe.forward(s)
+        probe.T.synchronize()
+        mem = device_memory()
+        state_bytes = probe.state_bytes
+        probe.close()
+        del probe
+        if mem is None:
+            slots = 150
+        else:
+            free_gb = mem[0] / 1e9
+            slots = int(max(1, min(n_steps - 1, (mem[0] - 8e9) // state_bytes)))
+        log(f"[bench] global adjoint: {slots} free checkpoint slots"
+            + (f" ({free_gb:.1f} GB free next to the context)" if free_gb is not None else ""))
+    inv = make(slots)
     w = rng.random((len(inv.obs_steps), st.nx, st.ny))
     t_f, t_g, g = timed_gradient(inv, s, w)
     out = {"workload": f"global 1440x720x80 grid, bins {bins} of 20, {P * spp} steps, {H} source heights x {P} periods, "
                        f"binomial checkpointing in {slots} free slots of {inv.state_bytes / 1e9:.2f} GB",
            "forward_s": round(t_f, 4), "gradient_s": round(t_g, 4), "ratio": round(t_g / t_f, 3),
            "recomputed_steps": inv.forward_steps, "steps": inv.n_steps}
+    if free_gb is not None:
+        out["free_hbm_gb_next_to_context"] = round(free_gb, 1)
     inv.close()
     return out
 
@@ -369,8 +413,8 @@ def main():
                          "(lanes of a wave per line; agrees to rounding)")
     ap.add_argument("--adj-bins", type=int, default=2, help="bins of the global adjoint leg's state")
     ap.add_argument("--adj-steps", type=int, default=240, help="steps of the global adjoint leg")
-    ap.add_argument("--adj-slots", type=int, default=150,
-                    help="free checkpoint slots of the global adjoint leg (150 x 1.41 GB: 212 GB of the 288 GB of HBM)")
+    ap.add_argument("--adj-slots", type=int, default=0,
+                    help="free checkpoint slots of the global adjoint leg (0: as many as the free HBM holds)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transport step shards "

@@ -71,3 +71,65 @@ def test_plan_store_all_and_errors():
         adj.plan_checkpoints(100, 3, 1.0, 1.5)
     with pytest.raises(ValueError, match="schedule"):
         adj.plan_checkpoints(10, 1, 1.0, 100.0, schedule="x")
+
+
+class _StateLog:
+    """Stand-in for the device context of the reverse sweep: the state is
+    ("input", t) after step t's glue and source insertion, ("output", t)
+    after its transport step and ("bc", t) after the adjoint step of t
+    rewrote its ghosts; slots hold copies.  Records every load."""
+
+    def __init__(self, itime0):
+        self.itime0, self.state, self.slots, self.loads, self.reversed = itime0, None, {}, 0, []
+
+    def checkpoint_save(self, k):
+        self.slots[k] = self.state
+
+    def checkpoint_load(self, k):
+        self.state, self.loads = self.slots[k], self.loads + 1
+
+    def transport_step(self, itime, use_diffusion):
+        t = itime - self.itime0
+        assert self.state == ("input", t), (self.state, t)
+        self.state = ("output", t)
+
+    def step_glue(self):
+        assert self.state[0] == "output"
+        self.state = ("glued", self.state[1] + 1)
+
+    def source_column_add(self, ivent, jvent, flux):
+        assert self.state[0] == "glued"
+        self.state = ("input", self.state[1])
+
+    def adj_step(self, itime, use_diffusion):
+        t = itime - self.itime0
+        assert self.state == ("input", t), (self.state, t)
+        self.state = ("bc", t)
+        self.reversed.append(t)
+
+    def adj_column_load(self, dz, w):
+        pass
+
+    def adj_source(self, ivent, jvent, slot):
+        pass
+
+
+@pytest.mark.parametrize("n,S", [(1, 0), (7, 0), (10, 2), (37, 5), (240, 60), (240, 150)])
+def test_revolve_executor_loads_only_what_it_needs(n, S):
+    """The executor's reverse sweep skips the load of a slot that the
+    previous action saved: every transport step and adjoint step still
+    starts from its own input, steps are reversed n-1..0, and the loads
+    are those of the schedule less one per save."""
+    inv = object.__new__(adj.SourceInversion)
+    inv.T = log = _StateLog(itime0=5)
+    inv.itime0, inv.n_steps, inv.S, inv.use_diffusion = 5, n, S, True
+    inv._s, inv.spp, inv.ivent, inv.jvent, inv.dz = None, 4, 1, 1, None
+    inv._obs_index = {}
+    inv.flux = lambda s, period: None
+    log.slots[1] = ("input", 0)  # the input of step 0, saved by the forward run
+    inv._reverse_revolve(w=None)
+    assert log.reversed == list(reversed(range(n)))
+    acts = list(adj.revolve_actions(n, S))
+    schedule_loads = sum(1 for a in acts if a[0] == "advance" or (a[0] == "reverse" and a[2] is not None))
+    saves = sum(1 for a in acts if a[0] == "save")
+    assert log.loads == schedule_loads - saves

@@ -266,23 +266,31 @@ class SourceInversion:
 
     def _reverse_revolve(self, w):
         """The reverse sweep under binomial checkpointing (slot 1 holds the
-        input of step 0, slots 2..S+1 are free)."""
+        input of step 0, slots 2..S+1 are free).  A slot saved by the
+        previous action still equals the state, so it is not loaded back
+        (half of the loads: every save is followed by an advance or a
+        reverse from that slot)."""
         T, s = self.T, self._s
         self.forward_steps = 0
+        held = None  # the slot that the state equals, while it does
         for act in revolve_actions(self.n_steps, self.S):
             if act[0] == "advance":
                 _, t_from, t_to, slot = act
-                T.checkpoint_load(1 + slot)
+                if held != slot:
+                    T.checkpoint_load(1 + slot)
                 for t in range(t_from, t_to):
                     T.transport_step(self.itime0 + t, self.use_diffusion)
                     self._begin_step(s, t + 1)
                 self.forward_steps += t_to - t_from
+                held = None
             elif act[0] == "save":
                 T.checkpoint_save(1 + act[1])
+                held = act[1]
             else:
                 _, t, slot = act
-                if slot is not None:
+                if slot is not None and held != slot:
                     T.checkpoint_load(1 + slot)
+                held = None  # the adjoint step rewrites the state's ghosts (Set_BC(1))
                 if t in self._obs_index:
                     T.adj_column_load(self.dz, w[self._obs_index[t]])
                 T.adj_step(self.itime0 + t, self.use_diffusion)
